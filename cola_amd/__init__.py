@@ -1,0 +1,12 @@
+"""cola_amd: CoLA's Krylov engine (CG, Lanczos, Arnoldi, SLQ, Hutchinson) on MI355X.
+
+Keeps cola's LinearOperator / annotation / cola.linalg API; the solvers run as
+hand-written gfx950 HIP kernels behind a C ABI (include/cola_krylov.h).
+"""
+from . import annotations, linalg, ops
+from .annotations import PSD, Hermitian, SelfAdjoint, Stiefel, Unitary
+from .fns import add, densify, dot, kron, lazify, mul
+from .ops import (Dense, DenseBF16, Diagonal, I_like, Identity, Kernel, Kronecker, LinearOperator, Product,
+                  RBFKernel, ScalarMul, Sparse, Sum, Tridiagonal, WoodburyInverse)
+
+__version__ = "0.1.0"

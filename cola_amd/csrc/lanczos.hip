@@ -1,0 +1,584 @@
+// Batched Lanczos (double classical Gram-Schmidt) and Arnoldi (modified or
+// double classical Gram-Schmidt) on basis blocks V[b][j][i].
+//
+// Re-orthogonalisation is a batched GEMV pair per pass: h = V^T w over the filled
+// columns only (the reference sweeps all m+2 preallocated columns, zeros
+// included; lanczos.py:287-296), then w -= V h.  The projection kernel stages
+// one 1024-element tile of w in LDS and lets each wavefront dot a subset of
+// basis vectors against it, writing one partial per (column, tile); a small
+// kernel adds the partials in index order.  All loop control (i, done, the
+// beta-relative stop rule of lanczos.py:263-268) stays on the device.
+#include "common.hpp"
+#include "ops_internal.hpp"
+
+using namespace ck;
+
+namespace {
+
+constexpr int kTile = 1024;  // vector elements per block in the sweeps
+
+struct LzScratch {
+  float* dpart;  // nb x P
+  float* hpart;  // nb x (m+2) x P
+  float* h;      // nb x (m+2)
+  float* h2;     // nb x (m+2)   (second pass; Arnoldi keeps both)
+  float* nrm;    // nb
+  int P;
+  int M2;        // m + 2
+};
+
+inline int lz_nparts(int64_t n) { return (int)cdiv(n, kTile); }
+
+inline LzScratch scratch_of(void* scratch, int64_t n, int nb, int m) {
+  LzScratch s;
+  s.P = lz_nparts(n);
+  s.M2 = m + 2;
+  float* f = static_cast<float*>(scratch);
+  s.dpart = f;
+  f += (size_t)nb * s.P * 2;  // two buffers (Arnoldi MGS alternates them)
+  s.hpart = f;
+  f += (size_t)nb * s.M2 * s.P;
+  s.h = f;
+  f += (size_t)nb * s.M2;
+  s.h2 = f;
+  f += (size_t)nb * s.M2;
+  s.nrm = f;
+  return s;
+}
+
+__device__ inline double col_total(const float* __restrict__ part, int P, int64_t base) {
+  double acc = 0.0;
+  for (int i = threadIdx.x & 63; i < P; i += 64) acc += (double)part[base + i];
+  return wave_sum(acc);
+}
+
+// sum of squares (MODE 1) or dot with Y (MODE 0) of one tile per block
+template <int MODE>
+__global__ __launch_bounds__(256) void tile_partials(int64_t n, const float* __restrict__ X, int64_t ldb_x,
+                                                     const float* __restrict__ Y, int64_t ldb_y,
+                                                     float* __restrict__ part, int P, const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ float red[4];
+  const int b = blockIdx.y, blk = blockIdx.x;
+  const int64_t i0 = (int64_t)blk * kTile;
+  const float* x = X + b * ldb_x;
+  const float* y = Y + b * ldb_y;
+  float acc = 0.f;
+  for (int e = threadIdx.x; e < kTile; e += 256) {
+    const int64_t i = i0 + e;
+    if (i < n) {
+      const float a = x[i];
+      acc = fmaf(a, MODE == 0 ? y[i] : a, acc);
+    }
+  }
+  acc = block_sum<256>(acc, red);
+  if (threadIdx.x == 0) part[(int64_t)b * P + blk] = acc;
+}
+
+// ---------------------------------------------------------------- Lanczos start
+__global__ __launch_bounds__(256) void lz_norm_final(int nb, const float* __restrict__ part, int P,
+                                                     float* __restrict__ nrm, const int* __restrict__ done) {
+  if (done && *done) return;
+  const int wave = threadIdx.x >> 6;
+  for (int b = wave; b < nb; b += 4) {
+    const double s = col_total(part, P, (int64_t)b * P);
+    if ((threadIdx.x & 63) == 0) nrm[b] = (float)sqrt(s);
+  }
+}
+
+// dst = src / nrm[b] (column 1 of V at start), and column 0 = 0
+__global__ __launch_bounds__(256) void lz_start_fill(int64_t n, const float* __restrict__ s, int64_t lds_,
+                                                     float* __restrict__ V, int64_t ldv, int64_t ldb,
+                                                     const float* __restrict__ nrm) {
+  const int b = blockIdx.y;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  V[b * ldb + i] = 0.f;
+  V[b * ldb + ldv + i] = s[b * lds_ + i] / nrm[b];
+}
+
+__global__ __launch_bounds__(256) void lz_start_ctrl(int nb, int m, float* __restrict__ diag,
+                                                     float* __restrict__ subdiag, void* state) {
+  ck_lz_ctrl* c = static_cast<ck_lz_ctrl*>(state);
+  float* errors = reinterpret_cast<float*>(c + 1);
+  for (int t = threadIdx.x; t < nb * m; t += 256) diag[t] = 0.f;
+  for (int t = threadIdx.x; t < nb * (m + 1); t += 256) subdiag[t] = 0.f;
+  if (threadIdx.x == 0) {
+    c->i = 1;
+    c->max_iters = m;
+    c->nb = nb;
+    c->done = (1 <= m) ? 0 : 1;  // lanczos.py:263-268 at i = 1
+    errors[0] = 1.0f;            // lanczos.py:256-261: 0/max(0,1e-30) + (i<=1)
+  }
+}
+
+// V[:,i] /= denom, denom = nrm[b] (first step) or subdiag[b][i-1]
+__global__ __launch_bounds__(256) void lz_scale_col(int64_t n, float* __restrict__ V, int64_t ldb, int64_t coloff,
+                                                    const float* __restrict__ denom, int64_t dstride,
+                                                    const int* __restrict__ done) {
+  if (done && *done) return;
+  const int b = blockIdx.y;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  V[b * ldb + coloff + i] /= denom[b * dstride];
+}
+
+// Projection partials h_j = V[:,j] . w over j in [j0, j1] for one tile.
+// FIRST (Lanczos): alpha = sum(dpart), w <- w - (alpha v_i + beta v_{i-1}) first
+// and alpha is written to diag[b][i-1].
+template <bool FIRST>
+__global__ __launch_bounds__(256) void lz_project(int64_t n, float* __restrict__ V, int64_t ldv, int64_t ldb, int i,
+                                                  int j0, int j1, const float* __restrict__ dpart, int P,
+                                                  float* __restrict__ hpart, int M2, float* __restrict__ diag,
+                                                  const float* __restrict__ subdiag, int m,
+                                                  const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ float wl[kTile];
+  __shared__ float coef[2];
+  const int b = blockIdx.y, blk = blockIdx.x;
+  const int64_t i0 = (int64_t)blk * kTile;
+  float* Vb = V + b * ldb;
+  float* w = Vb + (int64_t)(i + 1) * ldv;
+  if (FIRST) {
+    if (threadIdx.x < 64) {
+      const double a = col_total(dpart, P, (int64_t)b * P);
+      if (threadIdx.x == 0) {
+        coef[0] = (float)a;
+        coef[1] = subdiag[b * (m + 1) + (i - 1)];
+        if (blk == 0) diag[b * m + (i - 1)] = (float)a;
+      }
+    }
+    __syncthreads();
+  }
+  const float alpha = FIRST ? coef[0] : 0.f;
+  const float betap = FIRST ? coef[1] : 0.f;
+  for (int e = threadIdx.x; e < kTile; e += 256) {
+    const int64_t idx = i0 + e;
+    float wv = 0.f;
+    if (idx < n) {
+      wv = w[idx];
+      if (FIRST) {
+        // lanczos.py:247-248: aux = alpha v_i + beta v_{i-1}; w -= aux
+        const float aux = alpha * Vb[(int64_t)i * ldv + idx] + betap * Vb[(int64_t)(i - 1) * ldv + idx];
+        wv = wv - aux;
+        w[idx] = wv;
+      }
+    }
+    wl[e] = wv;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int j = j0 + wave; j <= j1; j += 4) {
+    const float* vj = Vb + (int64_t)j * ldv;
+    float acc = 0.f;
+    for (int e = lane; e < kTile; e += 64) {
+      const int64_t idx = i0 + e;
+      if (idx < n) acc = fmaf(vj[idx], wl[e], acc);
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) hpart[((int64_t)b * M2 + j) * P + blk] = acc;
+  }
+}
+
+// h[b][j] = sum_p hpart[b][j][p] for j in [j0, j1]; ACCUM adds into h2 (Arnoldi CGS2 keeps h1+h2)
+__global__ __launch_bounds__(256) void lz_reduce_h(const float* __restrict__ hpart, int P, int M2, int j0, int j1,
+                                                   float* __restrict__ h, const int* __restrict__ done) {
+  if (done && *done) return;
+  const int b = blockIdx.y;
+  const int j = j0 + blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j > j1) return;
+  const double s = col_total(hpart, P, ((int64_t)b * M2 + j) * P);
+  if ((threadIdx.x & 63) == 0) h[b * M2 + j] = (float)s;
+}
+
+// w -= sum_{j0<=j<=j1} V[:,j] h[j]  (the sum is formed first, as lanczos.py:295).
+// NORM: also writes partials of ||w||^2.
+template <bool NORM>
+__global__ __launch_bounds__(256) void lz_apply(int64_t n, float* __restrict__ V, int64_t ldv, int64_t ldb, int wcol,
+                                                int j0, int j1, const float* __restrict__ h, int M2,
+                                                float* __restrict__ npart, int P, const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ float hs[1056];
+  __shared__ float red[4];
+  const int b = blockIdx.y, blk = blockIdx.x;
+  for (int j = threadIdx.x; j < M2 && j < 1056; j += 256) hs[j] = h[b * M2 + j];
+  __syncthreads();
+  float* Vb = V + b * ldb;
+  float* w = Vb + (int64_t)wcol * ldv;
+  const int64_t i0 = (int64_t)blk * kTile;
+  float acc[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int j = j0; j <= j1; ++j) {
+    const float hj = hs[j];
+    const float* vj = Vb + (int64_t)j * ldv;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t idx = i0 + threadIdx.x + 256 * q;
+      if (idx < n) acc[q] = fmaf(vj[idx], hj, acc[q]);
+    }
+  }
+  float ss = 0.f;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int64_t idx = i0 + threadIdx.x + 256 * q;
+    if (idx < n) {
+      const float wv = w[idx] - acc[q];
+      w[idx] = wv;
+      ss = fmaf(wv, wv, ss);
+    }
+  }
+  if (NORM) {
+    ss = block_sum<256>(ss, red);
+    if (threadIdx.x == 0) npart[(int64_t)b * P + blk] = ss;
+  }
+}
+
+// beta_i = ||w||, errors and the loop condition (lanczos.py:251-268)
+__global__ __launch_bounds__(256) void lz_finalize(int nb, int m, const float* __restrict__ npart, int P,
+                                                   float* __restrict__ subdiag, void* state, float tol, int i) {
+  ck_lz_ctrl* c = static_cast<ck_lz_ctrl*>(state);
+  if (c->done) return;
+  float* errors = reinterpret_cast<float*>(c + 1);
+  const int wave = threadIdx.x >> 6;
+  for (int b = wave; b < nb; b += 4) {
+    const double s = col_total(npart, P, (int64_t)b * P);
+    if ((threadIdx.x & 63) == 0) subdiag[b * (m + 1) + i] = (float)sqrt(s);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int inext = i + 1;
+    float err = -INFINITY;
+    bool any = false;
+    for (int b = 0; b < nb; ++b) {
+      const float* sd = subdiag + b * (m + 1);
+      const float rel = sd[inext - 1] / fmaxf(sd[1], 1e-30f);
+      err = fmaxf(err, rel);
+      any = any || (sd[inext - 1] > tol * sd[1]) || (inext <= 1);
+    }
+    errors[inext - 1] = err + (inext <= 1 ? 1.f : 0.f);
+    c->i = inext;
+    if (!((inext <= m) && any)) c->done = 1;
+  }
+}
+
+// ---------------------------------------------------------------- Arnoldi (MGS)
+// One dependent Gram-Schmidt step j on w = Q[:, idx+1]: h_j = (partials of q_j.w),
+// w -= h_j q_j, then partials of the next product (q_{j+1}.w, or ||w||^2 when j == idx).
+__global__ __launch_bounds__(256) void ar_mgs_step(int64_t n, float* __restrict__ Q, int64_t ldv, int64_t ldb,
+                                                   int idx, int j, const float* __restrict__ pin,
+                                                   float* __restrict__ pout, int P, float* __restrict__ H, int m,
+                                                   const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ float red[4];
+  __shared__ float hj_s;
+  const int b = blockIdx.y, blk = blockIdx.x;
+  if (threadIdx.x < 64) {
+    const double hj = col_total(pin, P, (int64_t)b * P);
+    if (threadIdx.x == 0) {
+      hj_s = (float)hj;
+      if (blk == 0) H[(int64_t)b * (m + 1) * m + (int64_t)j * m + idx] = (float)hj;
+    }
+  }
+  __syncthreads();
+  const float hj = hj_s;
+  float* Qb = Q + b * ldb;
+  const float* qj = Qb + (int64_t)j * ldv;
+  float* w = Qb + (int64_t)(idx + 1) * ldv;
+  const float* qn = (j < idx) ? Qb + (int64_t)(j + 1) * ldv : nullptr;
+  const int64_t i0 = (int64_t)blk * kTile;
+  float acc = 0.f;
+  for (int e = threadIdx.x; e < kTile; e += 256) {
+    const int64_t i = i0 + e;
+    if (i < n) {
+      const float wv = w[i] - hj * qj[i];  // arnoldi.py:308
+      w[i] = wv;
+      acc = fmaf(qn ? qn[i] : wv, wv, acc);
+    }
+  }
+  acc = block_sum<256>(acc, red);
+  if (threadIdx.x == 0) pout[(int64_t)b * P + blk] = acc;
+}
+
+// norm = ||w||; H[idx+1, idx] = norm; w /= max(norm, tol/2); loop condition (arnoldi.py:293-319)
+__global__ __launch_bounds__(256) void ar_finalize(int nb, int m, const float* __restrict__ npart, int P,
+                                                   float* __restrict__ H, float* __restrict__ nrm, void* state,
+                                                   float tol, int idx) {
+  ck_ar_ctrl* c = static_cast<ck_ar_ctrl*>(state);
+  if (c->done) return;
+  float* errors = reinterpret_cast<float*>(c + 1);
+  const int wave = threadIdx.x >> 6;
+  for (int b = wave; b < nb; b += 4) {
+    const double s = col_total(npart, P, (int64_t)b * P);
+    if ((threadIdx.x & 63) == 0) {
+      const float nv = (float)sqrt(s);
+      nrm[b] = nv;
+      H[(int64_t)b * (m + 1) * m + (int64_t)(idx + 1) * m + idx] = nv;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int inext = idx + 1;
+    bool any = false;
+    for (int b = 0; b < nb; ++b) {
+      const float h10 = H[(int64_t)b * (m + 1) * m + 1 * m + 0];
+      any = any || (nrm[b] > tol * h10) || (inext <= 0);
+    }
+    errors[inext] = nrm[0];
+    c->idx = inext;
+    if (!((inext < m) && any)) c->done = 1;
+  }
+}
+
+// w /= max(norm, tol/2); runs for the step that just finished even if it set done.
+__global__ __launch_bounds__(256) void ar_normalize(int64_t n, float* __restrict__ Q, int64_t ldv, int64_t ldb,
+                                                    int col, const float* __restrict__ nrm, float clipv,
+                                                    const int* __restrict__ ctrl_idx, int expect_idx) {
+  if (*ctrl_idx != expect_idx) return;  // the step did not run
+  const int b = blockIdx.y;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Q[b * ldb + (int64_t)col * ldv + i] /= fmaxf(nrm[b], clipv);
+}
+
+// Arnoldi CGS2: copy h1 + h2 into H column idx.
+__global__ __launch_bounds__(256) void ar_store_h(int nb, int m, int M2, int idx, const float* __restrict__ h1,
+                                                  const float* __restrict__ h2, float* __restrict__ H,
+                                                  const int* __restrict__ done) {
+  if (done && *done) return;
+  const int b = blockIdx.x;
+  for (int j = threadIdx.x; j <= idx; j += 256)
+    H[(int64_t)b * (m + 1) * m + (int64_t)j * m + idx] = h1[b * M2 + j] + h2[b * M2 + j];
+}
+
+__global__ __launch_bounds__(256) void ar_start_ctrl(int nb, int m, float* __restrict__ H, const float* __restrict__ nrm,
+                                                     void* state) {
+  ck_ar_ctrl* c = static_cast<ck_ar_ctrl*>(state);
+  float* errors = reinterpret_cast<float*>(c + 1);
+  for (int64_t t = threadIdx.x; t < (int64_t)nb * (m + 1) * m; t += 256) H[t] = 0.f;
+  if (threadIdx.x == 0) {
+    c->idx = 0;
+    c->max_iters = m;
+    c->nb = nb;
+    c->done = (0 < m) ? 0 : 1;
+    errors[0] = nrm[0];
+  }
+}
+
+int check_basis(int64_t n, int nb, const float* V, int64_t ldv, int64_t ldb, int m, int extra) {
+  if (n < 1 || nb < 1 || nb > 4096 || !V || ldv < n || m < 1 || ldb < (int64_t)(m + extra) * ldv)
+    return CK_ERR_BAD_ARG;
+  if (m + 2 > 1056) return CK_ERR_UNSUPPORTED;
+  return CK_OK;
+}
+
+// Q[b][0][i] = s[b][i] / nrm[b]
+__global__ __launch_bounds__(256) void ar_start_fill(int64_t n, const float* __restrict__ s, int64_t lds_,
+                                                     float* __restrict__ Q, int64_t ldb,
+                                                     const float* __restrict__ nrm) {
+  const int b = blockIdx.y;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Q[b * ldb + i] = s[b * lds_ + i] / nrm[b];
+}
+
+int lz_pre_full(int64_t n, int nb, float* V, int64_t ldv, int64_t ldb, const float* subdiag, void* state, int m,
+                int step, void* scratch, hipStream_t s) {
+  const int i = step + 1;
+  const int* done = &static_cast<ck_lz_ctrl*>(state)->done;
+  LzScratch sc = scratch_of(scratch, n, nb, m);
+  if (step == 0) {
+    // the reference re-normalises column 1 at the first step (lanczos.py:240-241)
+    hipLaunchKernelGGL(tile_partials<1>, dim3(sc.P, nb), dim3(256), 0, s, n, V + ldv, ldb, V + ldv, ldb, sc.dpart,
+                       sc.P, done);
+    hipLaunchKernelGGL(lz_norm_final, dim3(1), dim3(256), 0, s, nb, sc.dpart, sc.P, sc.nrm, done);
+    hipLaunchKernelGGL(lz_scale_col, dim3((unsigned)cdiv(n, 256), nb), dim3(256), 0, s, n, V, ldb, ldv,
+                       (const float*)sc.nrm, (int64_t)1, done);
+  } else {
+    // V[:, i] was w_{i-1}; its norm is subdiag[i-1] (lanczos.py:252), the same reduction
+    hipLaunchKernelGGL(lz_scale_col, dim3((unsigned)cdiv(n, 256), nb), dim3(256), 0, s, n, V, ldb,
+                       (int64_t)i * ldv, subdiag + (i - 1), (int64_t)(m + 1), done);
+  }
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+int lz_post_full(int64_t n, int nb, float* V, int64_t ldv, int64_t ldb, float* diag, float* subdiag, void* state,
+                 int m, float tol, int step, void* scratch, hipStream_t s) {
+  const int i = step + 1;
+  if (i > m) return CK_ERR_BAD_ARG;
+  const int* done = &static_cast<ck_lz_ctrl*>(state)->done;
+  LzScratch sc = scratch_of(scratch, n, nb, m);
+  const float* w = V + (int64_t)(i + 1) * ldv;
+  const float* vi = V + (int64_t)i * ldv;
+  dim3 grid(sc.P, nb);
+  hipLaunchKernelGGL(tile_partials<0>, grid, dim3(256), 0, s, n, w, ldb, vi, ldb, sc.dpart, sc.P, done);
+  hipLaunchKernelGGL(lz_project<true>, grid, dim3(256), 0, s, n, V, ldv, ldb, i, 1, i, sc.dpart, sc.P, sc.hpart,
+                     sc.M2, diag, subdiag, m, done);
+  const dim3 rgrid((unsigned)cdiv(i, 4), nb);
+  hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, 1, i, sc.h, done);
+  hipLaunchKernelGGL(lz_apply<false>, grid, dim3(256), 0, s, n, V, ldv, ldb, i + 1, 1, i, sc.h, sc.M2, sc.dpart,
+                     sc.P, done);
+  hipLaunchKernelGGL(lz_project<false>, grid, dim3(256), 0, s, n, V, ldv, ldb, i, 1, i, sc.dpart, sc.P, sc.hpart,
+                     sc.M2, diag, subdiag, m, done);
+  hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, 1, i, sc.h, done);
+  hipLaunchKernelGGL(lz_apply<true>, grid, dim3(256), 0, s, n, V, ldv, ldb, i + 1, 1, i, sc.h, sc.M2, sc.dpart,
+                     sc.P, done);
+  hipLaunchKernelGGL(lz_finalize, dim3(1), dim3(256), 0, s, nb, m, sc.dpart, sc.P, subdiag, state, tol, i);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+int ar_post_full(int64_t n, int nb, float* Q, int64_t ldv, int64_t ldb, float* H, void* state, int m, float tol,
+                 int step, int mgs, void* scratch, hipStream_t s) {
+  const int idx = step;
+  if (idx >= m) return CK_ERR_BAD_ARG;
+  const int* done = &static_cast<ck_ar_ctrl*>(state)->done;
+  LzScratch sc = scratch_of(scratch, n, nb, m);
+  dim3 grid(sc.P, nb);
+  float* npart;
+  if (mgs) {
+    float* buf[2] = {sc.dpart, sc.dpart + (size_t)nb * sc.P};
+    const float* w = Q + (int64_t)(idx + 1) * ldv;
+    hipLaunchKernelGGL(tile_partials<0>, grid, dim3(256), 0, s, n, (const float*)Q, ldb, w, ldb, buf[0], sc.P, done);
+    for (int j = 0; j <= idx; ++j)
+      hipLaunchKernelGGL(ar_mgs_step, grid, dim3(256), 0, s, n, Q, ldv, ldb, idx, j, (const float*)buf[j & 1],
+                         buf[(j + 1) & 1], sc.P, H, m, done);
+    npart = buf[(idx + 1) & 1];
+  } else {
+    const dim3 rgrid((unsigned)cdiv(idx + 1, 4), nb);
+    hipLaunchKernelGGL(lz_project<false>, grid, dim3(256), 0, s, n, Q, ldv, ldb, idx, 0, idx, sc.dpart, sc.P,
+                       sc.hpart, sc.M2, (float*)nullptr, (const float*)nullptr, m, done);
+    hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, 0, idx, sc.h, done);
+    hipLaunchKernelGGL(lz_apply<false>, grid, dim3(256), 0, s, n, Q, ldv, ldb, idx + 1, 0, idx, sc.h, sc.M2,
+                       sc.dpart, sc.P, done);
+    hipLaunchKernelGGL(lz_project<false>, grid, dim3(256), 0, s, n, Q, ldv, ldb, idx, 0, idx, sc.dpart, sc.P,
+                       sc.hpart, sc.M2, (float*)nullptr, (const float*)nullptr, m, done);
+    hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, 0, idx, sc.h2, done);
+    hipLaunchKernelGGL(lz_apply<true>, grid, dim3(256), 0, s, n, Q, ldv, ldb, idx + 1, 0, idx, sc.h2, sc.M2,
+                       sc.dpart, sc.P, done);
+    hipLaunchKernelGGL(ar_store_h, dim3(nb), dim3(256), 0, s, nb, m, sc.M2, idx, (const float*)sc.h,
+                       (const float*)sc.h2, H, done);
+    npart = sc.dpart;
+  }
+  hipLaunchKernelGGL(ar_finalize, dim3(1), dim3(256), 0, s, nb, m, (const float*)npart, sc.P, H, sc.nrm, state, tol,
+                     idx);
+  hipLaunchKernelGGL(ar_normalize, dim3((unsigned)cdiv(n, 256), nb), dim3(256), 0, s, n, Q, ldv, ldb, idx + 1,
+                     (const float*)sc.nrm, tol / 2.f, &static_cast<ck_ar_ctrl*>(state)->idx, idx + 1);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t ck_lanczos_state_bytes(int32_t nb, int32_t max_iters) {
+  return sizeof(ck_lz_ctrl) + sizeof(float) * ((size_t)max_iters + 2);
+}
+
+size_t ck_lanczos_scratch_bytes(int64_t n, int32_t nb, int32_t max_iters) {
+  const size_t P = lz_nparts(n), M2 = max_iters + 2;
+  return sizeof(float) * ((size_t)nb * P * 2 + (size_t)nb * M2 * P + 2 * (size_t)nb * M2 + (size_t)nb + 64);
+}
+
+int ck_lanczos_start(int64_t n, int32_t nb, const float* s, int64_t lds_, float* V, int64_t ldv, int64_t ldb,
+                     float* diag, float* subdiag, void* state, int32_t max_iters, void* scratch, void* stream) {
+  if (int st = check_basis(n, nb, V, ldv, ldb, max_iters, 2)) return st;
+  if (!s || lds_ < n || !diag || !subdiag || !state || !scratch) return CK_ERR_BAD_ARG;
+  hipStream_t st_ = (hipStream_t)stream;
+  LzScratch sc = scratch_of(scratch, n, nb, max_iters);
+  hipLaunchKernelGGL(tile_partials<1>, dim3(sc.P, nb), dim3(256), 0, st_, n, s, lds_, s, lds_, sc.dpart, sc.P,
+                     (const int*)nullptr);
+  hipLaunchKernelGGL(lz_norm_final, dim3(1), dim3(256), 0, st_, nb, sc.dpart, sc.P, sc.nrm, (const int*)nullptr);
+  hipLaunchKernelGGL(lz_start_fill, dim3((unsigned)cdiv(n, 256), nb), dim3(256), 0, st_, n, s, lds_, V, ldv, ldb,
+                     (const float*)sc.nrm);
+  hipLaunchKernelGGL(lz_start_ctrl, dim3(1), dim3(256), 0, st_, nb, max_iters, diag, subdiag, state);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+int ck_lanczos_pre(int64_t n, int32_t nb, int32_t max_iters, float* V, int64_t ldv, int64_t ldb,
+                   const float* subdiag, void* state, int32_t step, void* scratch, void* stream) {
+  if (int st = check_basis(n, nb, V, ldv, ldb, max_iters, 2)) return st;
+  if (!subdiag || !state || !scratch || step < 0 || step >= max_iters) return CK_ERR_BAD_ARG;
+  return lz_pre_full(n, nb, V, ldv, ldb, subdiag, state, max_iters, step, scratch, (hipStream_t)stream);
+}
+
+int ck_lanczos_post(int64_t n, int32_t nb, int32_t max_iters, float* V, int64_t ldv, int64_t ldb, float* diag,
+                    float* subdiag, void* state, float tol, int32_t step, void* scratch, void* stream) {
+  if (int st = check_basis(n, nb, V, ldv, ldb, max_iters, 2)) return st;
+  if (!diag || !subdiag || !state || !scratch || step < 0 || step >= max_iters) return CK_ERR_BAD_ARG;
+  return lz_post_full(n, nb, V, ldv, ldb, diag, subdiag, state, max_iters, tol, step, scratch, (hipStream_t)stream);
+}
+
+int ck_lanczos_iterate(const ck_op* A, int64_t n, int32_t nb, int32_t max_iters, float* V, int64_t ldv, int64_t ldb,
+                       float* diag, float* subdiag, void* state, float tol, int32_t first_step, int32_t nsteps,
+                       void* scratch, void* work, size_t work_bytes, void* stream) {
+  if (!A || A->n != n || first_step < 0 || nsteps < 0) return CK_ERR_BAD_ARG;
+  if (int st = check_basis(n, nb, V, ldv, ldb, max_iters, 2)) return st;
+  if (!diag || !subdiag || !state || !scratch) return CK_ERR_BAD_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const int* done = &static_cast<ck_lz_ctrl*>(state)->done;
+  for (int step = first_step; step < first_step + nsteps && step < max_iters; ++step) {
+    const int i = step + 1;
+    int st = lz_pre_full(n, nb, V, ldv, ldb, subdiag, state, max_iters, step, scratch, s);
+    if (st) return st;
+    // w = A v_i into column i+1, batched over the nb probes (vector stride ldb)
+    st = op_apply_internal(A, nb, V + (int64_t)i * ldv, ldb, V + (int64_t)(i + 1) * ldv, ldb, work, work_bytes, done,
+                           s);
+    if (st) return st;
+    st = lz_post_full(n, nb, V, ldv, ldb, diag, subdiag, state, max_iters, tol, step, scratch, s);
+    if (st) return st;
+  }
+  return CK_OK;
+}
+
+size_t ck_arnoldi_state_bytes(int32_t nb, int32_t max_iters) {
+  return sizeof(ck_ar_ctrl) + sizeof(float) * ((size_t)max_iters + 2);
+}
+
+size_t ck_arnoldi_scratch_bytes(int64_t n, int32_t nb, int32_t max_iters) {
+  return ck_lanczos_scratch_bytes(n, nb, max_iters);
+}
+
+int ck_arnoldi_start(int64_t n, int32_t nb, const float* s, int64_t lds_, float* Q, int64_t ldv, int64_t ldb, float* H,
+                     void* state, int32_t max_iters, void* scratch, void* stream) {
+  if (int st = check_basis(n, nb, Q, ldv, ldb, max_iters, 1)) return st;
+  if (!s || lds_ < n || !H || !state || !scratch) return CK_ERR_BAD_ARG;
+  hipStream_t st_ = (hipStream_t)stream;
+  LzScratch sc = scratch_of(scratch, n, nb, max_iters);
+  hipLaunchKernelGGL(tile_partials<1>, dim3(sc.P, nb), dim3(256), 0, st_, n, s, lds_, s, lds_, sc.dpart, sc.P,
+                     (const int*)nullptr);
+  hipLaunchKernelGGL(lz_norm_final, dim3(1), dim3(256), 0, st_, nb, sc.dpart, sc.P, sc.nrm, (const int*)nullptr);
+  hipLaunchKernelGGL(ar_start_fill, dim3((unsigned)cdiv(n, 256), nb), dim3(256), 0, st_, n, s, lds_, Q, ldb,
+                     (const float*)sc.nrm);
+  hipLaunchKernelGGL(ar_start_ctrl, dim3(1), dim3(256), 0, st_, nb, max_iters, H, (const float*)sc.nrm, state);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+int ck_arnoldi_post(int64_t n, int32_t nb, int32_t max_iters, float* Q, int64_t ldv, int64_t ldb, float* H,
+                    void* state, float tol, int32_t step, int32_t mgs, void* scratch, void* stream) {
+  if (int st = check_basis(n, nb, Q, ldv, ldb, max_iters, 1)) return st;
+  if (!H || !state || !scratch || step < 0 || step >= max_iters) return CK_ERR_BAD_ARG;
+  return ar_post_full(n, nb, Q, ldv, ldb, H, state, max_iters, tol, step, mgs, scratch, (hipStream_t)stream);
+}
+
+int ck_arnoldi_iterate(const ck_op* A, int64_t n, int32_t nb, int32_t max_iters, float* Q, int64_t ldv, int64_t ldb,
+                       float* H, void* state, float tol, int32_t first_step, int32_t nsteps, int32_t mgs,
+                       void* scratch, void* work, size_t work_bytes, void* stream) {
+  if (!A || A->n != n || first_step < 0 || nsteps < 0) return CK_ERR_BAD_ARG;
+  if (int st = check_basis(n, nb, Q, ldv, ldb, max_iters, 1)) return st;
+  if (!H || !state || !scratch) return CK_ERR_BAD_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const int* done = &static_cast<ck_ar_ctrl*>(state)->done;
+  for (int step = first_step; step < first_step + nsteps && step < max_iters; ++step) {
+    int st = op_apply_internal(A, nb, Q + (int64_t)step * ldv, ldb, Q + (int64_t)(step + 1) * ldv, ldb, work,
+                               work_bytes, done, s);
+    if (st) return st;
+    st = ar_post_full(n, nb, Q, ldv, ldb, H, state, max_iters, tol, step, mgs, scratch, s);
+    if (st) return st;
+  }
+  return CK_OK;
+}
+
+}  // extern "C"

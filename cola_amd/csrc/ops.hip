@@ -1,0 +1,578 @@
+// Operator kernels: y = scale * S x + shift * x on column blocks.
+//
+//   dense  : each wavefront owns ROWS rows; lanes stride the row in float4 chunks,
+//            one DPP wave reduction per (row, rhs).            [HBM / L2 bound]
+//   csr    : one lane per row (short rows) or one wavefront per row (long rows);
+//            int32 column indices; coalesced rowptr/value streams. [HBM bound]
+//   kron   : factors of order <= 64; the last two modes are fused per slab
+//            (U = K2 S K3^T with S the 64x64 slab) and the first mode is a
+//            64-row GEMM over 64-column tiles, both on v_mfma_f32_16x16x4f32 with
+//            LDS-staged, bank-padded factors.                    [MFMA bound]
+#include "common.hpp"
+#include "ops_internal.hpp"
+
+using namespace ck;
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+// ============================================================== dense fp32
+template <int NB, int ROWS>
+__global__ __launch_bounds__(256) void dense_f32_kernel(int n, const float* __restrict__ A, int64_t lda,
+                                                        const float* __restrict__ X, int64_t ldx, float* __restrict__ Y,
+                                                        int64_t ldy, int nbr, float scale, float shift,
+                                                        const int* __restrict__ done, bool vec) {
+  if (done && *done) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int row0 = wave * ROWS;
+  if (row0 >= n) return;
+  float acc[ROWS][NB];
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[r][b] = 0.f;
+
+  if (vec) {
+    for (int j = lane * 4; j < n; j += 256) {
+      float4 xv[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+        xv[b] = (b < nbr) ? *reinterpret_cast<const float4*>(X + b * ldx + j) : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) {
+        if (row0 + r < n) {
+          const float4 a = *reinterpret_cast<const float4*>(A + (int64_t)(row0 + r) * lda + j);
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            acc[r][b] = fmaf(a.x, xv[b].x, acc[r][b]);
+            acc[r][b] = fmaf(a.y, xv[b].y, acc[r][b]);
+            acc[r][b] = fmaf(a.z, xv[b].z, acc[r][b]);
+            acc[r][b] = fmaf(a.w, xv[b].w, acc[r][b]);
+          }
+        }
+      }
+    }
+  } else {
+    for (int j = lane; j < n; j += 64) {
+      float xv[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) xv[b] = (b < nbr) ? X[b * ldx + j] : 0.f;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) {
+        if (row0 + r < n) {
+          const float a = A[(int64_t)(row0 + r) * lda + j];
+#pragma unroll
+          for (int b = 0; b < NB; ++b) acc[r][b] = fmaf(a, xv[b], acc[r][b]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const float s = wave_sum(acc[r][b]);
+      const int row = row0 + r;
+      if (lane == 0 && b < nbr && row < n) {
+        float y = scale * s;
+        if (shift != 0.f) y = fmaf(shift, X[b * ldx + row], y);
+        Y[b * ldy + row] = y;
+      }
+    }
+  }
+}
+
+// bf16 stored matrix, fp32 vectors and accumulation (vector path; the MFMA
+// path for nrhs >= 4 is in mixed.hip).
+__device__ __forceinline__ float bf16_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
+
+template <int NB, int ROWS>
+__global__ __launch_bounds__(256) void dense_bf16_kernel(int n, const uint16_t* __restrict__ A, int64_t lda,
+                                                         const float* __restrict__ X, int64_t ldx, float* __restrict__ Y,
+                                                         int64_t ldy, int nbr, float scale, float shift,
+                                                         const int* __restrict__ done) {
+  if (done && *done) return;
+  const int lane = threadIdx.x & 63;
+  const int wave = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int row0 = wave * ROWS;
+  if (row0 >= n) return;
+  float acc[ROWS][NB];
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[r][b] = 0.f;
+  // 8 bf16 (16 bytes) per lane per chunk; requires n % 8 == 0 and lda % 8 == 0 (checked by host)
+  for (int j = lane * 8; j < n; j += 512) {
+    float4 x0[NB], x1[NB];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      if (b < nbr) {
+        x0[b] = *reinterpret_cast<const float4*>(X + b * ldx + j);
+        x1[b] = *reinterpret_cast<const float4*>(X + b * ldx + j + 4);
+      } else {
+        x0[b] = x1[b] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      if (row0 + r < n) {
+        const uint4 w = *reinterpret_cast<const uint4*>(A + (int64_t)(row0 + r) * lda + j);
+        const float a0 = bf16_lo(w.x), a1 = bf16_hi(w.x), a2 = bf16_lo(w.y), a3 = bf16_hi(w.y);
+        const float a4 = bf16_lo(w.z), a5 = bf16_hi(w.z), a6 = bf16_lo(w.w), a7 = bf16_hi(w.w);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          float s = acc[r][b];
+          s = fmaf(a0, x0[b].x, s);
+          s = fmaf(a1, x0[b].y, s);
+          s = fmaf(a2, x0[b].z, s);
+          s = fmaf(a3, x0[b].w, s);
+          s = fmaf(a4, x1[b].x, s);
+          s = fmaf(a5, x1[b].y, s);
+          s = fmaf(a6, x1[b].z, s);
+          s = fmaf(a7, x1[b].w, s);
+          acc[r][b] = s;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const float s = wave_sum(acc[r][b]);
+      const int row = row0 + r;
+      if (lane == 0 && b < nbr && row < n) {
+        float y = scale * s;
+        if (shift != 0.f) y = fmaf(shift, X[b * ldx + row], y);
+        Y[b * ldy + row] = y;
+      }
+    }
+  }
+}
+
+int dense_dispatch(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, const int* done,
+                   hipStream_t s) {
+  const int n = op->n;
+  const bool bf16 = op->kind == CK_OP_DENSE_BF16;
+  auto aligned16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
+  // float4 / uint4 loads need 16-byte aligned rows and vector starts
+  const bool vec = (n % 4 == 0) && (op->lda % 4 == 0) && (ldx % 4 == 0) && aligned16(op->a) && aligned16(X);
+  if (bf16 && ((n % 8) || (op->lda % 8) || (ldx % 4) || !aligned16(op->a) || !aligned16(X)))
+    return CK_ERR_UNSUPPORTED;
+  for (int b0 = 0; b0 < nrhs; b0 += 16) {
+    const int nb = nrhs - b0 < 16 ? nrhs - b0 : 16;
+    const float* Xb = X + b0 * ldx;
+    float* Yb = Y + b0 * ldy;
+    // ROWS rows per wave: more rows reuse each loaded x chunk more often.
+    constexpr int WAVES = 4;
+#define CK_DENSE_LAUNCH(NB_, ROWS_)                                                                               \
+  do {                                                                                                            \
+    const int nwaves = (n + (ROWS_)-1) / (ROWS_);                                                                 \
+    const int grid = (nwaves + WAVES - 1) / WAVES;                                                                \
+    if (bf16)                                                                                                     \
+      hipLaunchKernelGGL((dense_bf16_kernel<NB_, ROWS_>), dim3(grid), dim3(64 * WAVES), 0, s, n,                  \
+                         (const uint16_t*)op->a, op->lda, Xb, ldx, Yb, ldy, nb, op->scale, op->shift, done);      \
+    else                                                                                                          \
+      hipLaunchKernelGGL((dense_f32_kernel<NB_, ROWS_>), dim3(grid), dim3(64 * WAVES), 0, s, n,                   \
+                         (const float*)op->a, op->lda, Xb, ldx, Yb, ldy, nb, op->scale, op->shift, done, vec);    \
+  } while (0)
+    if (nb == 1)
+      CK_DENSE_LAUNCH(1, 1);
+    else if (nb == 2)
+      CK_DENSE_LAUNCH(2, 2);
+    else if (nb <= 4)
+      CK_DENSE_LAUNCH(4, 4);
+    else if (nb <= 8)
+      CK_DENSE_LAUNCH(8, 4);
+    else
+      CK_DENSE_LAUNCH(16, 2);
+#undef CK_DENSE_LAUNCH
+  }
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+// ============================================================== CSR
+__global__ __launch_bounds__(256) void csr_lane_per_row(int n, const int32_t* __restrict__ rowptr,
+                                                        const int32_t* __restrict__ colind,
+                                                        const float* __restrict__ val, const float* __restrict__ X,
+                                                        int64_t ldx, float* __restrict__ Y, int64_t ldy, int nrhs,
+                                                        float scale, float shift, const int* __restrict__ done) {
+  if (done && *done) return;
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  const int k0 = rowptr[row], k1 = rowptr[row + 1];
+  for (int b = 0; b < nrhs; ++b) {
+    const float* xb = X + b * ldx;
+    float acc = 0.f;
+    for (int k = k0; k < k1; ++k) acc = fmaf(val[k], xb[colind[k]], acc);
+    float y = scale * acc;
+    if (shift != 0.f) y = fmaf(shift, xb[row], y);
+    Y[b * ldy + row] = y;
+  }
+}
+
+__global__ __launch_bounds__(256) void csr_wave_per_row(int n, const int32_t* __restrict__ rowptr,
+                                                        const int32_t* __restrict__ colind,
+                                                        const float* __restrict__ val, const float* __restrict__ X,
+                                                        int64_t ldx, float* __restrict__ Y, int64_t ldy, int nrhs,
+                                                        float scale, float shift, const int* __restrict__ done) {
+  if (done && *done) return;
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (row >= n) return;
+  const int k0 = rowptr[row], k1 = rowptr[row + 1];
+  for (int b = 0; b < nrhs; ++b) {
+    const float* xb = X + b * ldx;
+    float acc = 0.f;
+    for (int k = k0 + lane; k < k1; k += 64) acc = fmaf(val[k], xb[colind[k]], acc);
+    acc = wave_sum(acc);
+    if (lane == 0) {
+      float y = scale * acc;
+      if (shift != 0.f) y = fmaf(shift, xb[row], y);
+      Y[b * ldy + row] = y;
+    }
+  }
+}
+
+int csr_dispatch(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, const int* done,
+                 hipStream_t s) {
+  const int n = op->n;
+  if (!op->rowptr || !op->colind || !op->values) return CK_ERR_BAD_ARG;
+  const double avg = n > 0 ? (double)op->nnz / n : 0.0;
+  if (avg > 32.0) {
+    const int grid = (n + 3) / 4;
+    hipLaunchKernelGGL(csr_wave_per_row, dim3(grid), dim3(256), 0, s, n, op->rowptr, op->colind, op->values, X, ldx,
+                       Y, ldy, nrhs, op->scale, op->shift, done);
+  } else {
+    const int grid = (n + 255) / 256;
+    hipLaunchKernelGGL(csr_lane_per_row, dim3(grid), dim3(256), 0, s, n, op->rowptr, op->colind, op->values, X, ldx,
+                       Y, ldy, nrhs, op->scale, op->shift, done);
+  }
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+// ============================================================== diagonal / identity
+__global__ __launch_bounds__(256) void diag_kernel(int64_t n, int nrhs, const float* __restrict__ d,
+                                                   const float* __restrict__ X, int64_t ldx, float* __restrict__ Y,
+                                                   int64_t ldy, float scale, float shift,
+                                                   const int* __restrict__ done) {
+  if (done && *done) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (i >= n || b >= nrhs) return;
+  const float x = X[b * ldx + i];
+  const float s = d ? d[i] : 1.f;
+  Y[b * ldy + i] = fmaf(scale * s, x, shift * x);
+}
+
+// ============================================================== Kronecker (MFMA)
+constexpr int KP = 66;  // LDS row pitch (floats): conflict-free 16x4 operand reads
+
+__device__ __forceinline__ floatx4 mfma16x16x4(float a, float b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Load an r x c row-major global matrix (leading dim ld) into LDS [64][KP], zero padded.
+__device__ __forceinline__ void load_pad64(float* __restrict__ dst, const float* __restrict__ src, int r, int c,
+                                           int64_t ld) {
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += blockDim.x) {
+    const int i = idx >> 6, j = idx & 63;
+    dst[i * KP + j] = (i < r && j < c) ? src[(int64_t)i * ld + j] : 0.f;
+  }
+}
+
+// Wave w computes rows [16w, 16w+16) x cols [0,64) of C = A * B^T where A, Bt are
+// LDS [64][KP] (Bt holds B transposed, i.e. Bt[col][k]); k runs over kdim (<= 64).
+__device__ __forceinline__ void wave_gemm_abt(const float* __restrict__ A, const float* __restrict__ Bt, int kdim,
+                                              floatx4 acc[4]) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int ra = (wave * 16 + (lane & 15)) * KP;
+  const int kk = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = floatx4{0.f, 0.f, 0.f, 0.f};
+  for (int k = 0; k < kdim; k += 4) {
+    const float a = A[ra + k + kk];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const float b = Bt[(t * 16 + (lane & 15)) * KP + k + kk];
+      acc[t] = mfma16x16x4(a, b, acc[t]);
+    }
+  }
+}
+
+// Pass A: for slab (b, i1): U = K2 * S * K3^T, S = X[b][i1] viewed as n2 x n3.
+// LAST: U is the operator output (two-factor case): Y = scale*U + shift*S.
+template <bool LAST>
+__global__ __launch_bounds__(256) void kron_pass_a(int n1, int n2, int n3, const float* __restrict__ K2,
+                                                   const float* __restrict__ K3, const float* __restrict__ X,
+                                                   int64_t ldx, float* __restrict__ Out, int64_t ldo, float scale,
+                                                   float shift, const int* __restrict__ done) {
+  if (done && *done) return;
+  extern __shared__ float lds[];
+  float* S = lds;           // [64][KP] slab
+  float* Kx = S + 64 * KP;  // K3 for the first product, then K2
+  float* Tt = Kx + 64 * KP; // (S K3^T)^T
+  const int i1 = blockIdx.x, b = blockIdx.y;
+  const int64_t slab = (int64_t)b * ldx + (int64_t)i1 * n2 * n3;
+  const int64_t oslab = (int64_t)b * ldo + (int64_t)i1 * n2 * n3;
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += blockDim.x) {
+    const int i = idx >> 6, j = idx & 63;
+    S[i * KP + j] = (i < n2 && j < n3) ? X[slab + i * n3 + j] : 0.f;
+  }
+  load_pad64(Kx, K3, n3, n3, n3);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  floatx4 acc[4];
+  // T = S * K3^T  (T[r][c] = sum_k S[r][k] K3[c][k]); store transposed.
+  wave_gemm_abt(S, Kx, n3, acc);
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      const int r = wave * 16 + 4 * (lane >> 4) + it;
+      const int c = t * 16 + (lane & 15);
+      Tt[c * KP + r] = acc[t][it];
+    }
+  __syncthreads();
+  load_pad64(Kx, K2, n2, n2, n2);
+  __syncthreads();
+  // U = K2 * T  (U[r][c] = sum_k K2[r][k] T[k][c] = sum_k K2[r][k] Tt[c][k]).
+  wave_gemm_abt(Kx, Tt, n2, acc);
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      const int r = wave * 16 + 4 * (lane >> 4) + it;
+      const int c = t * 16 + (lane & 15);
+      if (r < n2 && c < n3) {
+        float u = acc[t][it];
+        if (LAST) {
+          u = scale * u;
+          if (shift != 0.f) u = fmaf(shift, S[r * KP + c], u);
+        }
+        Out[oslab + r * n3 + c] = u;
+      }
+    }
+}
+
+// Pass B: Y[b][j1][c] = scale * sum_k K1[j1][k] U[b][k][c] + shift * X[b][j1][c]
+// for a 64-column tile c in [64*blockIdx.x, +64).  Optionally accumulates the
+// per-block partial sum of X*Y into partials[b * nparts + blockIdx.x].
+__global__ __launch_bounds__(256) void kron_pass_b(int n1, int plane, const float* __restrict__ K1,
+                                                   const float* __restrict__ U, int64_t ldu,
+                                                   const float* __restrict__ X, int64_t ldx, float* __restrict__ Y,
+                                                   int64_t ldy, float scale, float shift, float* __restrict__ partials,
+                                                   int nparts, const int* __restrict__ done) {
+  if (done && *done) return;
+  extern __shared__ float lds[];
+  float* Ka = lds;           // K1 [64][KP]
+  float* Ut = Ka + 64 * KP;  // U tile transposed: Ut[c][k]
+  __shared__ float red[4];
+  const int b = blockIdx.y;
+  const int c0 = blockIdx.x * 64;
+  load_pad64(Ka, K1, n1, n1, n1);
+  for (int idx = threadIdx.x; idx < 64 * 64; idx += blockDim.x) {
+    const int k = idx >> 6, c = idx & 63;
+    Ut[c * KP + k] = (k < n1 && c0 + c < plane) ? U[(int64_t)b * ldu + (int64_t)k * plane + c0 + c] : 0.f;
+  }
+  __syncthreads();
+  floatx4 acc[4];
+  wave_gemm_abt(Ka, Ut, n1, acc);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float dot = 0.f;
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      const int r = wave * 16 + 4 * (lane >> 4) + it;
+      const int c = c0 + t * 16 + (lane & 15);
+      if (r < n1 && c < plane) {
+        const int64_t off = (int64_t)r * plane + c;
+        const float x = X[(int64_t)b * ldx + off];
+        float y = scale * acc[t][it];
+        if (shift != 0.f) y = fmaf(shift, x, y);
+        Y[(int64_t)b * ldy + off] = y;
+        dot = fmaf(x, y, dot);
+      }
+    }
+  if (partials) {
+    const float s = block_sum<256>(dot, red);
+    if (threadIdx.x == 0) partials[(int64_t)b * nparts + blockIdx.x] = s;
+  }
+}
+
+}  // namespace
+
+// ============================================================== internal API
+namespace ck {
+
+size_t kron_workspace_bytes(const ck_op* op, int nrhs) {
+  if (op->nfactors == 3) return (size_t)nrhs * op->n * sizeof(float);
+  return 0;
+}
+
+int kron_dispatch(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,
+                  size_t work_bytes, const int* done, float* partials, int nparts, hipStream_t s) {
+  const int d = op->nfactors;
+  if (d != 2 && d != 3) return CK_ERR_UNSUPPORTED;
+  int64_t prod = 1;
+  for (int i = 0; i < d; ++i) {
+    if (op->fdim[i] < 1 || op->fdim[i] > 64 || !op->factor[i]) return CK_ERR_UNSUPPORTED;
+    prod *= op->fdim[i];
+  }
+  if (prod != op->n) return CK_ERR_BAD_ARG;
+  const size_t ldsA = 3 * 64 * KP * sizeof(float);
+  const size_t ldsB = 2 * 64 * KP * sizeof(float);
+  if (d == 2) {
+    if (partials) return CK_ERR_UNSUPPORTED;
+    hipLaunchKernelGGL((kron_pass_a<true>), dim3(1, nrhs), dim3(256), ldsA, s, 1, op->fdim[0], op->fdim[1],
+                       op->factor[0], op->factor[1], X, ldx, Y, ldy, op->scale, op->shift, done);
+    CK_CHECK_LAUNCH();
+    return CK_OK;
+  }
+  if (work_bytes < kron_workspace_bytes(op, nrhs) || !work) return CK_ERR_WORKSPACE;
+  float* U = static_cast<float*>(work);
+  const int n1 = op->fdim[0], n2 = op->fdim[1], n3 = op->fdim[2];
+  const int plane = n2 * n3;
+  hipLaunchKernelGGL((kron_pass_a<false>), dim3(n1, nrhs), dim3(256), ldsA, s, n1, n2, n3, op->factor[1],
+                     op->factor[2], X, ldx, U, (int64_t)op->n, 1.f, 0.f, done);
+  const int tiles = (plane + 63) / 64;
+  if (partials && nparts != tiles) return CK_ERR_BAD_ARG;
+  hipLaunchKernelGGL(kron_pass_b, dim3(tiles, nrhs), dim3(256), ldsB, s, n1, plane, op->factor[0], U,
+                     (int64_t)op->n, X, ldx, Y, ldy, op->scale, op->shift, partials, nparts, done);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+int kron_nparts(const ck_op* op) {
+  if (op->nfactors != 3) return 0;
+  return (op->fdim[1] * op->fdim[2] + 63) / 64;
+}
+
+int op_apply_internal(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,
+                      size_t work_bytes, const int* done, hipStream_t s) {
+  if (!op || !X || !Y || nrhs < 1 || op->n < 1 || ldx < op->n || ldy < op->n) return CK_ERR_BAD_ARG;
+  switch (op->kind) {
+    case CK_OP_IDENTITY:
+    case CK_OP_DIAG_F32: {
+      const float* d = op->kind == CK_OP_DIAG_F32 ? (const float*)op->a : nullptr;
+      if (op->kind == CK_OP_DIAG_F32 && !d) return CK_ERR_BAD_ARG;
+      dim3 grid((unsigned)cdiv(op->n, 256), nrhs);
+      hipLaunchKernelGGL(diag_kernel, grid, dim3(256), 0, s, (int64_t)op->n, nrhs, d, X, ldx, Y, ldy, op->scale,
+                         op->shift, done);
+      CK_CHECK_LAUNCH();
+      return CK_OK;
+    }
+    case CK_OP_DENSE_F32:
+    case CK_OP_DENSE_BF16:
+      if (!op->a || op->lda < op->n) return CK_ERR_BAD_ARG;
+      return dense_dispatch(op, nrhs, X, ldx, Y, ldy, done, s);
+    case CK_OP_CSR_F32:
+      return csr_dispatch(op, nrhs, X, ldx, Y, ldy, done, s);
+    case CK_OP_KRON_F32:
+      return kron_dispatch(op, nrhs, X, ldx, Y, ldy, work, work_bytes, done, nullptr, 0, s);
+    case CK_OP_WOODBURY_INV:
+      return woodbury_apply(op, nrhs, X, ldx, Y, ldy, work, work_bytes, done, s);
+    case CK_OP_RBF_F32:
+      return rbf_apply(op, nrhs, X, ldx, Y, ldy, done, s);
+    default:
+      return CK_ERR_UNSUPPORTED;
+  }
+}
+
+size_t op_workspace_internal(const ck_op* op, int nrhs) {
+  if (!op) return 0;
+  switch (op->kind) {
+    case CK_OP_KRON_F32:
+      return kron_workspace_bytes(op, nrhs);
+    case CK_OP_WOODBURY_INV:
+      return woodbury_workspace_bytes(op, nrhs);
+    default:
+      return 0;
+  }
+}
+
+}  // namespace ck
+
+// ============================================================== transpose / dot
+namespace {
+
+__global__ __launch_bounds__(256) void transpose_kernel(int64_t n, int nrhs, const float* __restrict__ src,
+                                                        int64_t lds_, float* __restrict__ dst, int64_t ldd,
+                                                        int to_blocks) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (to_blocks) {
+    for (int b = 0; b < nrhs; ++b) dst[b * ldd + i] = src[i * lds_ + b];
+  } else {
+    for (int b = 0; b < nrhs; ++b) dst[i * ldd + b] = src[b * lds_ + i];
+  }
+}
+
+__global__ __launch_bounds__(256) void dot_partials_kernel(int64_t n, const float* __restrict__ X, int64_t ldx,
+                                                           const float* __restrict__ Y, int64_t ldy,
+                                                           double* __restrict__ part, int nparts) {
+  __shared__ double red[4];
+  const int b = blockIdx.y;
+  const int64_t per = cdiv(n, nparts);
+  const int64_t i0 = blockIdx.x * per, i1 = i0 + per < n ? i0 + per : n;
+  double acc = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x)
+    acc += (double)X[b * ldx + i] * (double)Y[b * ldy + i];
+  acc = block_sum<256>(acc, red);
+  if (threadIdx.x == 0) part[b * nparts + blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(64) void dot_final_kernel(int nparts, const double* __restrict__ part,
+                                                       double* __restrict__ out) {
+  const int b = blockIdx.x;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += 64) acc += part[b * nparts + i];
+  acc = wave_sum(acc);
+  if (threadIdx.x == 0) out[b] = acc;
+}
+
+constexpr int kDotParts = 256;
+
+}  // namespace
+
+extern "C" {
+
+size_t ck_op_workspace_bytes(const ck_op* op, int32_t nrhs) { return ck::op_workspace_internal(op, nrhs); }
+
+int ck_op_apply(const ck_op* op, int32_t nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,
+                size_t work_bytes, void* stream) {
+  return ck::op_apply_internal(op, nrhs, X, ldx, Y, ldy, work, work_bytes, nullptr, (hipStream_t)stream);
+}
+
+int ck_transpose_f32(int64_t n, int32_t nrhs, const float* src, int64_t lds_, float* dst, int64_t ldd,
+                     int32_t to_blocks, void* stream) {
+  if (n < 1 || nrhs < 1 || !src || !dst) return CK_ERR_BAD_ARG;
+  if (to_blocks ? (lds_ < nrhs || ldd < n) : (lds_ < n || ldd < nrhs)) return CK_ERR_BAD_ARG;
+  hipLaunchKernelGGL(transpose_kernel, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, n, nrhs, src,
+                     lds_, dst, ldd, to_blocks);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+size_t ck_dot_workspace_bytes(int64_t n, int32_t nrhs) { return (size_t)nrhs * kDotParts * sizeof(double); }
+
+int ck_dot_f32(int64_t n, int32_t nrhs, const float* X, int64_t ldx, const float* Y, int64_t ldy, double* out,
+               void* work, size_t work_bytes, void* stream) {
+  if (n < 1 || nrhs < 1 || !X || !Y || !out || ldx < n || ldy < n) return CK_ERR_BAD_ARG;
+  if (!work || work_bytes < ck_dot_workspace_bytes(n, nrhs)) return CK_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  double* part = static_cast<double*>(work);
+  hipLaunchKernelGGL(dot_partials_kernel, dim3(kDotParts, nrhs), dim3(256), 0, s, n, X, ldx, Y, ldy, part, kDotParts);
+  hipLaunchKernelGGL(dot_final_kernel, dim3(nrhs), dim3(64), 0, s, kDotParts, part, out);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,28 @@
+// Internal (non-ABI) entry points shared between the engine's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/cola_krylov.h"
+
+namespace ck {
+
+// op(X) -> Y; kernels return immediately when `done` (device flag) is nonzero.
+int op_apply_internal(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,
+                      size_t work_bytes, const int* done, hipStream_t s);
+size_t op_workspace_internal(const ck_op* op, int nrhs);
+
+// Kronecker apply that can also emit per-column partial sums of X*Y
+// (nparts = kron_nparts(op) per column); used to fuse p.Ap into CG's matvec.
+int kron_dispatch(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,
+                  size_t work_bytes, const int* done, float* partials, int nparts, hipStream_t s);
+int kron_nparts(const ck_op* op);
+size_t kron_workspace_bytes(const ck_op* op, int nrhs);
+
+int woodbury_apply(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,
+                   size_t work_bytes, const int* done, hipStream_t s);
+size_t woodbury_workspace_bytes(const ck_op* op, int nrhs);
+int rbf_apply(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, const int* done,
+              hipStream_t s);
+
+}  // namespace ck

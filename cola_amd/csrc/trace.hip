@@ -1,0 +1,288 @@
+// Probe streams, Hutchinson accumulation, the quadrature reduction of
+// stochastic Lanczos quadrature, and Ritz-vector assembly.
+//
+// CSR Hutchinson with packed probes: 64 Rademacher probes of a row live in one
+// 64-bit word, so a wavefront takes one row with lane p = probe p; the row's
+// values and column indices are wave-uniform loads and each nonzero costs one
+// 8-byte gather of the column's word.  The trace-only variant never forms a
+// per-probe value at all: sum_p z_p[i] z_p[j] = 64 - 2 popcount(w_i ^ w_j).
+#include "common.hpp"
+#include "ops_internal.hpp"
+
+using namespace ck;
+
+namespace {
+
+// ---------------------------------------------------------------- probes
+__global__ __launch_bounds__(256) void rng_rademacher_kernel(int64_t n, int nprobes, uint64_t seed,
+                                                             float* __restrict__ Z, int64_t ldz) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.y;
+  if (i >= n) return;
+  const uint64_t w = stream_word(stream_key(seed, (uint64_t)g), (uint64_t)i);
+  const int p0 = 64 * g;
+  const int p1 = p0 + 64 < nprobes ? p0 + 64 : nprobes;
+  for (int p = p0; p < p1; ++p) Z[(int64_t)p * ldz + i] = ((w >> (p - p0)) & 1ull) ? -1.f : 1.f;
+}
+
+__global__ __launch_bounds__(256) void rng_bits_kernel(int64_t n, uint64_t seed, uint64_t* __restrict__ W) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.y;
+  if (i >= n) return;
+  W[(int64_t)g * n + i] = stream_word(stream_key(seed, (uint64_t)g), (uint64_t)i);
+}
+
+__global__ __launch_bounds__(256) void rng_gaussian_kernel(int64_t n, uint64_t seed, float* __restrict__ Z,
+                                                           int64_t ldz) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = blockIdx.y;
+  if (i >= n) return;
+  const uint64_t w = stream_word(stream_key(seed ^ kGaussSalt, (uint64_t)p), (uint64_t)i);
+  Z[(int64_t)p * ldz + i] = gauss_from_word(w);
+}
+
+// ---------------------------------------------------------------- Hutchinson
+__global__ __launch_bounds__(256) void hutch_accum_kernel(int64_t n, int nprobes, const float* __restrict__ Z,
+                                                          int64_t ldz, const float* __restrict__ AZ, int64_t ldaz,
+                                                          float* __restrict__ sum, float* __restrict__ sumsq) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f, s2 = 0.f;
+  for (int p = 0; p < nprobes; ++p) {
+    const float est = AZ[(int64_t)p * ldaz + i] * Z[(int64_t)p * ldz + i];
+    s += est;
+    s2 = fmaf(est, est, s2);
+  }
+  sum[i] += s;
+  sumsq[i] += s2;
+}
+
+__device__ __forceinline__ float sign_bit(uint64_t w, int lane) { return ((w >> lane) & 1ull) ? -1.f : 1.f; }
+
+__global__ __launch_bounds__(256) void hutch_csr_bits_kernel(int n, const int32_t* __restrict__ rowptr,
+                                                             const int32_t* __restrict__ colind,
+                                                             const float* __restrict__ val, float scale, float shift,
+                                                             int ngroups, const uint64_t* __restrict__ W,
+                                                             float* __restrict__ sum, float* __restrict__ sumsq) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (row >= n) return;
+  const int k0 = rowptr[row], k1 = rowptr[row + 1];
+  float s = 0.f, s2 = 0.f;
+  for (int g = 0; g < ngroups; ++g) {
+    const uint64_t* Wg = W + (int64_t)g * n;
+    const float zi = sign_bit(Wg[row], lane);
+    float acc = 0.f;
+    for (int k = k0; k < k1; ++k) acc = fmaf(val[k], sign_bit(Wg[colind[k]], lane), acc);
+    const float az = fmaf(shift, zi, scale * acc);
+    const float est = zi * az;
+    s += wave_sum(est);
+    s2 += wave_sum(est * est);
+  }
+  if (lane == 0) {
+    sum[row] += s;
+    sumsq[row] += s2;
+  }
+}
+
+__global__ __launch_bounds__(256) void trace_bits_partials(int n, const int32_t* __restrict__ rowptr,
+                                                           const int32_t* __restrict__ colind,
+                                                           const float* __restrict__ val, int ngroups,
+                                                           const uint64_t* __restrict__ W, double* __restrict__ part) {
+  __shared__ double red[4];
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (row < n) {
+    const int k0 = rowptr[row], k1 = rowptr[row + 1];
+    for (int k = k0; k < k1; ++k) {
+      const int c = colind[k];
+      int agree = 0;
+      for (int g = 0; g < ngroups; ++g) agree += 64 - 2 * __popcll(W[(int64_t)g * n + row] ^ W[(int64_t)g * n + c]);
+      acc += (double)val[k] * (double)agree;
+    }
+  }
+  acc = block_sum<256>(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(256) void trace_bits_final(int nparts, const double* __restrict__ part, double scale,
+                                                        double shift_n, double inv_probes, double* __restrict__ out) {
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += 256) acc += part[i];
+  acc = block_sum<256>(acc, red);
+  if (threadIdx.x == 0) out[0] = scale * acc * inv_probes + shift_n;
+}
+
+// ---------------------------------------------------------------- quadrature
+__device__ double apply_f(int kind, double x) {
+  switch (kind) {
+    case 0:
+      return log(x);
+    case 1:
+      return x;
+    case 2:
+      return exp(x);
+    case 3:
+      return sqrt(x);
+    default:
+      return 1.0 / x;
+  }
+}
+
+__global__ __launch_bounds__(64) void slq_kernel(int m, const double* __restrict__ evals,
+                                                 const double* __restrict__ first_row,
+                                                 const double* __restrict__ scale, int f_kind, double eps,
+                                                 int max_mode, double* __restrict__ out) {
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  double mx = -INFINITY;
+  for (int c = lane; c < m; c += 64) {
+    const double t = evals[(int64_t)b * m + c];
+    mx = fmax(mx, max_mode == 0 ? fabs(t) : t);
+  }
+  // wave max in fp64 via two passes of readlane (m is small)
+  for (int off = 32; off >= 1; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
+  const double thresh = 10.0 * eps * mx;
+  double acc = 0.0;
+  for (int c = lane; c < m; c += 64) {
+    const double t = evals[(int64_t)b * m + c];
+    const double tau = first_row[(int64_t)b * m + c];
+    const double fv = fabs(t) > thresh ? apply_f(f_kind, t) : 0.0;
+    acc += tau * tau * fv;
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) out[b] = (scale ? scale[b] : 1.0) * acc;
+}
+
+// ---------------------------------------------------------------- Ritz vectors
+constexpr int kCombine = 16;
+
+__global__ __launch_bounds__(256) void basis_combine_kernel(int64_t n, int m, int k, const float* __restrict__ V,
+                                                            int64_t ldv, const double* __restrict__ S, int64_t lds_,
+                                                            float* __restrict__ out, int64_t ldo) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c0 = blockIdx.y * kCombine;
+  if (i >= n) return;
+  float acc[kCombine];
+#pragma unroll
+  for (int c = 0; c < kCombine; ++c) acc[c] = 0.f;
+  for (int j = 0; j < m; ++j) {
+    const float v = V[(int64_t)j * ldv + i];
+#pragma unroll
+    for (int c = 0; c < kCombine; ++c)
+      if (c0 + c < k) acc[c] = fmaf((float)S[(int64_t)j * lds_ + c0 + c], v, acc[c]);
+  }
+#pragma unroll
+  for (int c = 0; c < kCombine; ++c)
+    if (c0 + c < k) out[(int64_t)(c0 + c) * ldo + i] = acc[c];
+}
+
+// out[b][i] = sum_j C[b][j] V[b][j][i]: one combination per probe of its own basis.
+__global__ __launch_bounds__(256) void basis_combine_batched_kernel(int64_t n, int k, const float* __restrict__ V,
+                                                                    int64_t ldv, int64_t ldb,
+                                                                    const double* __restrict__ C, int64_t ldc,
+                                                                    float* __restrict__ out, int64_t ldo) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (i >= n) return;
+  const float* Vb = V + (int64_t)b * ldb;
+  float acc = 0.f;
+  for (int j = 0; j < k; ++j) acc = fmaf((float)C[(int64_t)b * ldc + j], Vb[(int64_t)j * ldv + i], acc);
+  out[(int64_t)b * ldo + i] = acc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ck_basis_combine_batched(int64_t n, int32_t nb, int32_t k, const float* V, int64_t ldv, int64_t ldb,
+                             const double* C, int64_t ldc, float* out, int64_t ldo, void* stream) {
+  if (n < 1 || nb < 1 || k < 1 || !V || !C || !out || ldv < n || ldo < n || ldc < k || ldb < (int64_t)k * ldv)
+    return CK_ERR_BAD_ARG;
+  hipLaunchKernelGGL(basis_combine_batched_kernel, dim3((unsigned)cdiv(n, 256), nb), dim3(256), 0,
+                     (hipStream_t)stream, n, k, V, ldv, ldb, C, ldc, out, ldo);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+int ck_rng_rademacher_f32(int64_t n, int32_t nprobes, uint64_t seed, float* Z, int64_t ldz, void* stream) {
+  if (n < 1 || nprobes < 1 || !Z || ldz < n) return CK_ERR_BAD_ARG;
+  const int groups = (nprobes + 63) / 64;
+  hipLaunchKernelGGL(rng_rademacher_kernel, dim3((unsigned)cdiv(n, 256), groups), dim3(256), 0, (hipStream_t)stream, n,
+                     nprobes, seed, Z, ldz);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+int ck_rng_gaussian_f32(int64_t n, int32_t nprobes, uint64_t seed, float* Z, int64_t ldz, void* stream) {
+  if (n < 1 || nprobes < 1 || !Z || ldz < n) return CK_ERR_BAD_ARG;
+  hipLaunchKernelGGL(rng_gaussian_kernel, dim3((unsigned)cdiv(n, 256), nprobes), dim3(256), 0, (hipStream_t)stream, n,
+                     seed, Z, ldz);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+int ck_rng_rademacher_bits(int64_t n, int32_t ngroups, uint64_t seed, uint64_t* W, void* stream) {
+  if (n < 1 || ngroups < 1 || !W) return CK_ERR_BAD_ARG;
+  hipLaunchKernelGGL(rng_bits_kernel, dim3((unsigned)cdiv(n, 256), ngroups), dim3(256), 0, (hipStream_t)stream, n, seed,
+                     W);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+int ck_hutch_accumulate(int64_t n, int32_t nprobes, const float* Z, int64_t ldz, const float* AZ, int64_t ldaz,
+                        float* sum, float* sumsq, void* stream) {
+  if (n < 1 || nprobes < 1 || !Z || !AZ || !sum || !sumsq || ldz < n || ldaz < n) return CK_ERR_BAD_ARG;
+  hipLaunchKernelGGL(hutch_accum_kernel, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, (hipStream_t)stream, n, nprobes,
+                     Z, ldz, AZ, ldaz, sum, sumsq);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+int ck_hutch_csr_bits(const ck_op* A, int32_t ngroups, const uint64_t* W, float* sum, float* sumsq, void* stream) {
+  if (!A || A->kind != CK_OP_CSR_F32 || !A->rowptr || !A->colind || !A->values || ngroups < 1 || !W || !sum || !sumsq)
+    return CK_ERR_BAD_ARG;
+  hipLaunchKernelGGL(hutch_csr_bits_kernel, dim3((unsigned)cdiv(A->n, 4)), dim3(256), 0, (hipStream_t)stream, A->n,
+                     A->rowptr, A->colind, A->values, A->scale, A->shift, ngroups, W, sum, sumsq);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+size_t ck_trace_workspace_bytes(int64_t n) { return sizeof(double) * (size_t)cdiv(n, 256); }
+
+int ck_trace_csr_rademacher_bits(const ck_op* A, int32_t ngroups, const uint64_t* W, double* out, void* work,
+                                 size_t work_bytes, void* stream) {
+  if (!A || A->kind != CK_OP_CSR_F32 || !A->rowptr || !A->colind || !A->values || ngroups < 1 || !W || !out)
+    return CK_ERR_BAD_ARG;
+  if (!work || work_bytes < ck_trace_workspace_bytes(A->n)) return CK_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  const int nparts = (int)cdiv(A->n, 256);
+  double* part = static_cast<double*>(work);
+  hipLaunchKernelGGL(trace_bits_partials, dim3(nparts), dim3(256), 0, s, A->n, A->rowptr, A->colind, A->values,
+                     ngroups, W, part);
+  hipLaunchKernelGGL(trace_bits_final, dim3(1), dim3(256), 0, s, nparts, (const double*)part, (double)A->scale,
+                     (double)A->shift * (double)A->n, 1.0 / (64.0 * ngroups), out);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+int ck_slq_quadrature(int32_t nb, int32_t m, const double* evals, const double* first_row, const double* scale,
+                      int32_t f_kind, double eps, int32_t max_mode, double* out, void* stream) {
+  if (nb < 1 || m < 1 || !evals || !first_row || !out || f_kind < 0 || f_kind > 4) return CK_ERR_BAD_ARG;
+  hipLaunchKernelGGL(slq_kernel, dim3(nb), dim3(64), 0, (hipStream_t)stream, m, evals, first_row, scale, f_kind, eps,
+                     max_mode, out);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+int ck_basis_combine(int64_t n, int32_t m, int32_t k, const float* V, int64_t ldv, const double* S, int64_t lds_,
+                     float* out, int64_t ldo, void* stream) {
+  if (n < 1 || m < 1 || k < 1 || !V || !S || !out || ldv < n || ldo < n || lds_ < k) return CK_ERR_BAD_ARG;
+  hipLaunchKernelGGL(basis_combine_kernel, dim3((unsigned)cdiv(n, 256), (unsigned)cdiv(k, kCombine)), dim3(256), 0,
+                     (hipStream_t)stream, n, m, k, V, ldv, S, lds_, out, ldo);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+}  // extern "C"

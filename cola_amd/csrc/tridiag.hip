@@ -1,0 +1,188 @@
+// Batched FP64 symmetric tridiagonal eigensolver: implicit QL with Wilkinson
+// shifts (Bowdler, Martin, Reinsch & Wilkinson, "The QR and QL algorithms for
+// symmetric matrices", Numer. Math. 11 (1968); EISPACK tql2).
+//
+// One workgroup per problem.  Lane 0 runs the scalar QL recurrence on d/e in
+// LDS and logs each sweep's Givens rotations; after a barrier every thread
+// applies the sweep to the eigenvector rows it owns (row-parallel, so the
+// O(m^3) accumulation is spread over the workgroup).  With first-row-only
+// output (stochastic Lanczos quadrature needs tau = Z[0, :]) lane 0 tracks the
+// single row itself.
+#include <float.h>
+
+#include "common.hpp"
+
+using namespace ck;
+
+namespace {
+
+constexpr int kMaxSweeps = 60;
+
+__global__ __launch_bounds__(256) void tridiag_ql_kernel(int m, const float* __restrict__ din, int64_t ldd,
+                                                         const float* __restrict__ ein, int64_t lde,
+                                                         double* __restrict__ evals, double* __restrict__ evecs,
+                                                         double* __restrict__ first_row,
+                                                         int32_t* __restrict__ status) {
+  extern __shared__ double sh[];
+  double* d = sh;        // m
+  double* e = d + m;     // m
+  double* cs = e + m;    // m  (rotation cosines of the current sweep)
+  double* sn = cs + m;   // m
+  double* z0 = sn + m;   // m  (first row of Z when only it is tracked)
+  int* perm = reinterpret_cast<int*>(z0 + m);  // m
+  __shared__ int ctl_mm, ctl_lo, ctl_fail;
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x;
+  double* Z = evecs ? evecs + (int64_t)b * m * m : nullptr;  // row-major Z[j][c]
+
+  for (int i = tid; i < m; i += blockDim.x) {
+    d[i] = (double)din[(int64_t)b * ldd + i];
+    e[i] = (i < m - 1) ? (double)ein[(int64_t)b * lde + i] : 0.0;
+    z0[i] = (i == 0) ? 1.0 : 0.0;
+  }
+  if (Z)
+    for (int64_t t = tid; t < (int64_t)m * m; t += blockDim.x) Z[t] = ((t / m) == (t % m)) ? 1.0 : 0.0;
+  if (tid == 0) ctl_fail = 0;
+  __syncthreads();
+
+  for (int l = 0; l < m; ++l) {
+    for (int iter = 0;; ++iter) {
+      if (tid == 0) {
+        int mm = l;
+        for (; mm < m - 1; ++mm) {
+          const double dd = fabs(d[mm]) + fabs(d[mm + 1]);
+          if (e[mm] == 0.0 || fabs(e[mm]) <= DBL_EPSILON * dd) break;
+        }
+        ctl_mm = mm;
+        ctl_lo = mm;
+        if (mm != l) {
+          if (iter >= kMaxSweeps) {
+            ctl_fail = 1;
+            ctl_mm = l;  // give up on this eigenvalue
+          } else {
+            // Wilkinson shift from the leading 2x2 block
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = hypot(g, 1.0);
+            g = d[mm] - d[l] + e[l] / (g + copysign(r, g));
+            double s = 1.0, c = 1.0, p = 0.0;
+            bool early = false;
+            int i = mm - 1;
+            for (; i >= l; --i) {
+              const double f = s * e[i];
+              const double bb = c * e[i];
+              r = hypot(f, g);
+              e[i + 1] = r;
+              if (r == 0.0) {  // deflation inside the sweep
+                d[i + 1] -= p;
+                e[mm] = 0.0;
+                early = true;
+                break;
+              }
+              s = f / r;
+              c = g / r;
+              g = d[i + 1] - p;
+              r = (d[i] - g) * s + 2.0 * c * bb;
+              p = s * r;
+              d[i + 1] = g + p;
+              g = c * r - bb;
+              cs[i] = c;
+              sn[i] = s;
+              if (!Z) {  // track the first row only
+                const double zf = z0[i + 1];
+                z0[i + 1] = s * z0[i] + c * zf;
+                z0[i] = c * z0[i] - s * zf;
+              }
+            }
+            ctl_lo = early ? i + 1 : l;
+            if (!early) {
+              d[l] -= p;
+              e[l] = g;
+              e[mm] = 0.0;
+            }
+          }
+        }
+      }
+      __syncthreads();
+      const int mm = ctl_mm, lo = ctl_lo;
+      __syncthreads();  // every lane has read the control words before lane 0 may rewrite them
+      if (mm == l) break;
+      if (Z) {
+        // rotations (i, i+1) for i = mm-1 down to lo, applied to every row
+        for (int row = tid; row < m; row += blockDim.x) {
+          double* zr = Z + (int64_t)row * m;
+          for (int i = mm - 1; i >= lo; --i) {
+            const double c = cs[i], s = sn[i];
+            const double zf = zr[i + 1];
+            zr[i + 1] = s * zr[i] + c * zf;
+            zr[i] = c * zr[i] - s * zf;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  // ascending order (selection sort of the permutation on lane 0)
+  if (tid == 0) {
+    for (int i = 0; i < m; ++i) perm[i] = i;
+    for (int i = 0; i < m; ++i) {
+      int k = i;
+      for (int j = i + 1; j < m; ++j)
+        if (d[perm[j]] < d[perm[k]]) k = j;
+      const int t = perm[i];
+      perm[i] = perm[k];
+      perm[k] = t;
+    }
+    if (status) status[b] = ctl_fail ? CK_ERR_NOCONV : CK_OK;
+  }
+  __syncthreads();
+  for (int c = tid; c < m; c += blockDim.x) {
+    evals[(int64_t)b * m + c] = d[perm[c]];
+    if (first_row) first_row[(int64_t)b * m + c] = Z ? Z[perm[c]] : z0[perm[c]];
+  }
+  if (Z) {
+    // permute each row in place: new[c] = old[perm[c]], following cycles (same walk on every row)
+    __syncthreads();
+    for (int row = tid; row < m; row += blockDim.x) {
+      double* zr = Z + (int64_t)row * m;
+      for (int c0 = 0; c0 < m; ++c0) {
+        // c0 leads its cycle iff it is the smallest index on it
+        int c = perm[c0];
+        bool leader = true;
+        while (c != c0) {
+          if (c < c0) {
+            leader = false;
+            break;
+          }
+          c = perm[c];
+        }
+        if (!leader) continue;
+        const double keep = zr[c0];
+        c = c0;
+        for (;;) {
+          const int nx = perm[c];
+          if (nx == c0) {
+            zr[c] = keep;
+            break;
+          }
+          zr[c] = zr[nx];
+          c = nx;
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int ck_tridiag_eigh_f64(int32_t nb, int32_t m, const float* d, int64_t ldd, const float* e, int64_t lde,
+                                   double* evals, double* evecs, double* first_row, int32_t* status_out,
+                                   void* stream) {
+  if (nb < 1 || m < 1 || m > 4096 || !d || !evals || ldd < m || (m > 1 && (!e || lde < m - 1))) return CK_ERR_BAD_ARG;
+  const size_t lds = (size_t)m * (5 * sizeof(double) + sizeof(int));
+  if (lds > 64 * 1024) return CK_ERR_UNSUPPORTED;  // m <= 1489
+  hipLaunchKernelGGL(tridiag_ql_kernel, dim3(nb), dim3(256), lds, (hipStream_t)stream, m, d, ldd, e ? e : d, lde,
+                     evals, evecs, first_row, status_out);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}

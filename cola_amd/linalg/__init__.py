@@ -1,0 +1,26 @@
+"""Linear algebra on the engine: inv/solve, eig, logdet, trace/diag, unary functions.
+
+``CG``, ``Lanczos``, ``Arnoldi`` and ``Hutch`` name the engine's algorithms, so
+code written against cola's names runs on the HIP kernels unchanged."""
+from .algorithm_base import Algorithm, Auto, IterativeOperatorWInfo
+from .arnoldi import HipArnoldi, arnoldi, arnoldi_eigs
+from .cg import HipCG, cg, run_batched_cg
+from .eig import eig, eigmax, eigmin, get_slice
+from .inv import inv, solve
+from .keys import PRNGKey, next_key
+from .lanczos import HipLanczos, lanczos, lanczos_eigs
+from .logdet import logdet, slogdet
+from .slq import HipSLQ, stochastic_lanczos_quad
+from .trace import Exact, HipHutch, diag, hutch_trace_csr, hutchinson_diag_estimate, trace
+from .unary import LanczosUnary, apply_unary, exp, isqrt, log, pow, sqrt
+
+CG = HipCG
+Lanczos = HipLanczos
+Arnoldi = HipArnoldi
+Hutch = HipHutch
+
+__all__ = ["Algorithm", "Auto", "IterativeOperatorWInfo", "HipCG", "CG", "cg", "run_batched_cg", "inv", "solve",
+           "HipLanczos", "Lanczos", "lanczos", "lanczos_eigs", "HipArnoldi", "Arnoldi", "arnoldi", "arnoldi_eigs",
+           "eig", "eigmax", "eigmin", "get_slice", "logdet", "slogdet", "HipSLQ", "stochastic_lanczos_quad",
+           "HipHutch", "Hutch", "Exact", "diag", "trace", "hutch_trace_csr", "hutchinson_diag_estimate",
+           "LanczosUnary", "apply_unary", "log", "exp", "sqrt", "isqrt", "pow", "PRNGKey", "next_key"]

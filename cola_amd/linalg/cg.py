@@ -1,0 +1,140 @@
+"""HipCG: batched (preconditioned) conjugate gradients on the engine.
+
+Drop-in for cola's ``CG`` (cola/linalg/inverse/cg.py:11-36): same fields, same
+``(soln, info)`` return, same arithmetic and stopping rule (run_batched_cg,
+cg.py:94-178).  The loop condition lives on the device: the host enqueues
+``chunk`` steps at a time (through ``ck_cg_iterate`` when the operator lowers
+to an engine kernel, otherwise step by step around ``A._matmat``) and reads
+the 8-byte control block once per chunk instead of syncing twice per step
+(cola/utils/torch_tqdm.py:42,88).
+"""
+import time
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .. import _native as N
+from .._blocks import from_blocks, ld_of, to_blocks
+from ._engine import apply_blocks, lower, read_ints, workspace
+from .algorithm_base import Algorithm
+
+
+@dataclass
+class HipCG(Algorithm):
+    """Conjugate gradients on the HIP engine.
+
+    Args:
+        tol (float): relative tolerance (threshold tol*||r0|| + tol on the
+            normalised residual, cg.py:101).
+        max_iters (int): step cap.
+        pbar (bool): accepted for compatibility; no progress bar is drawn.
+        x0 (Tensor): (n,) or (n, b) initial guess.
+        P (LinearOperator): preconditioner, applied as z = P r (an approximation of A^-1).
+        chunk (int): steps enqueued between two reads of the device loop flag.
+    """
+    tol: float = 1e-6
+    max_iters: int = 1000
+    pbar: bool = False
+    x0: object = None
+    P: object = None
+    chunk: int = 16
+
+    def __call__(self, A, b):
+        return cg(A, b, x0=self.x0, P=self.P, tol=self.tol, max_iters=self.max_iters, chunk=self.chunk)
+
+
+def cg(A, rhs, x0=None, P=None, tol=1e-6, max_iters=1000, pbar=False, chunk=16):
+    """cg.py:39-69 on the engine. rhs (n,) or (n, b) on the HIP device; returns (soln, info)."""
+    is_vector = rhs.dim() == 1
+    if is_vector:
+        rhs = rhs[:, None]
+        if x0 is not None:
+            x0 = x0[:, None]
+    X, _, _, info = run_batched_cg(A, rhs, x0, max_iters, tol, P, chunk=chunk)
+    return (X.reshape(-1) if is_vector else X), info
+
+
+class CGState:
+    """Device buffers of one batched solve (column blocks, control block, partials)."""
+
+    def __init__(self, n, B, max_iters, device, precond):
+        self.n, self.B, self.device = n, B, device
+        self.ld = ld_of(n)
+        mk = lambda: torch.zeros((B, self.ld), dtype=torch.float32, device=device)  # noqa: E731
+        self.bn, self.x, self.r, self.p, self.Ap = mk(), mk(), mk(), mk(), mk()
+        self.z = mk() if precond else self.r
+        self.state = torch.zeros(N.q("ck_cg_state_bytes", B, max_iters), dtype=torch.uint8, device=device)
+        self.partials = torch.empty(N.q("ck_cg_partials_bytes", n, B) // 4, dtype=torch.float32, device=device)
+        self.stride = N.q("ck_cg_stride", B)
+
+    def floats(self, index, count):
+        off = N.CTRL_BYTES + 4 * self.stride * index
+        return self.state[off:off + 4 * count].view(torch.float32)
+
+
+def run_batched_cg(A, b, x0=None, max_iters=1000, tol=1e-6, P=None, chunk=16, return_state=False):
+    """cg.py:94-119. b is (n, B) on the HIP device. Returns (x, r, k, info)."""
+    N.require_device(b, "right-hand side")
+    if b.dim() == 1:
+        b = b[:, None]
+    n, B = b.shape
+    if B > 1024:
+        raise ValueError("at most 1024 right-hand sides per solve")
+    dev = b.device
+    aop = lower(A)
+    pop = lower(P) if P is not None else None
+    st = CGState(n, B, max_iters, dev, P is not None)
+    work, wb = workspace([aop[0] if aop else None, pop[0] if pop else None], B, dev)
+    s = N.stream_ptr(dev)
+    ld = st.ld
+    t0 = time.time()
+    Bb = to_blocks(b, ld)
+    N.call("ck_cg_normalize", n, B, N.ptr(Bb), ld, N.ptr(st.bn), ld, N.ptr(st.state), N.ptr(st.partials), s)
+    if x0 is None:
+        st.r.copy_(st.bn)
+    else:
+        # cg.py:123: r0 = b_norm - A x0 (x0 taken as given, not normalised)
+        st.x.copy_(to_blocks(x0.reshape(n, B), ld))
+        apply_blocks(A, aop, st.x, st.Ap, n, B, work, wb)
+        torch.sub(st.bn, st.Ap, out=st.r)
+    if P is not None:
+        apply_blocks(P, pop, st.r, st.z, n, B, work, wb)
+    N.call("ck_cg_start", n, B, N.ptr(st.r), N.ptr(st.z), N.ptr(st.p), ld, N.ptr(st.state), float(tol),
+           int(max_iters), N.ptr(st.partials), s)
+    fused = aop is not None and (P is None or pop is not None)
+    steps = 0
+    k, done = read_ints(st.state, 2, dev)
+    while not done and steps < max_iters:
+        nst = min(int(chunk), max_iters - steps)
+        if fused:
+            N.call("ck_cg_iterate", aop[0], pop[0] if pop else None, B, N.ptr(st.x), N.ptr(st.r), N.ptr(st.p),
+                   N.ptr(st.Ap), N.ptr(st.z), ld, N.ptr(st.state), N.ptr(st.partials), N.ptr(work), wb, nst, s)
+        else:
+            for _ in range(nst):
+                apply_blocks(A, aop, st.p, st.Ap, n, B, work, wb)
+                N.call("ck_cg_step_xr", n, B, N.ptr(st.p), N.ptr(st.Ap), N.ptr(st.x), N.ptr(st.r), ld,
+                       N.ptr(st.state), N.ptr(st.partials), s)
+                if P is not None:
+                    apply_blocks(P, pop, st.r, st.z, n, B, work, wb)
+                N.call("ck_cg_step_p", n, B, N.ptr(st.r), N.ptr(st.z), N.ptr(st.p), ld, N.ptr(st.state),
+                       N.ptr(st.partials), s)
+        steps += nst
+        k, done = read_ints(st.state, 2, dev)
+    N.call("ck_cg_finish", n, B, N.ptr(st.x), N.ptr(st.r), ld, N.ptr(st.state), s)
+    errs = st.floats(N.CK_CG_ERRORS, k + 1).cpu().numpy().astype(np.float64)
+    elapsed = time.time() - t0
+    # torch_tqdm.py:38-62: one error per cond evaluation (k+1) plus the final one, first two dropped
+    info = {"iterations": k + 1, "errors": np.concatenate([errs, errs[-1:]])[2:],
+            "iteration_time": elapsed / (k + 1), "converged": bool(done and k < max_iters) or _all_below(st, B)}
+    x = from_blocks(st.x, n)
+    r = from_blocks(st.r, n)
+    if return_state:
+        info["state"] = st
+    return x, r, k, info
+
+
+def _all_below(st, B):
+    rn = st.floats(N.CK_CG_RNORM, B).cpu()
+    tl = st.floats(N.CK_CG_TOL, B).cpu()
+    return bool((rn <= tl).all())

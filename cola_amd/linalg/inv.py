@@ -1,0 +1,76 @@
+"""inv / solve dispatch (cola/linalg/inverse/inv.py:24-165) onto the engine."""
+import numpy as np
+
+from ..annotations import PSD, Unitary
+from ..dispatch import dispatch
+from ..ops import Diagonal, Identity, Kronecker, LinearOperator, Product, ScalarMul
+from .algorithm_base import Algorithm, Auto, IterativeOperatorWInfo
+from .cg import HipCG
+
+
+def solve(A, b, alg=Auto()):
+    """inv.py:24-39: inv(A, alg) @ b."""
+    return inv(A, alg) @ b
+
+
+@dispatch.abstract
+def inv(A: LinearOperator, alg: Algorithm = Auto()):
+    """(Lazily) the inverse of A; ``inv(A, alg) @ b`` solves A x = b."""
+
+
+@dispatch(precedence=-1)
+def inv(A: LinearOperator, alg: HipCG):
+    """inv.py:66-69."""
+    assert A.isa(PSD), "CG only valid for PSD matrices, wrap in cola_amd.PSD if desired"
+    return IterativeOperatorWInfo(A, alg)
+
+
+@dispatch(precedence=-1)
+def inv(A: LinearOperator, alg: Auto):
+    """inv.py:73-92 restricted to the engine: PSD operators go to CG at every size.
+
+    The reference's small-size dense factorisations (Cholesky/LU) and GMRES for
+    non-PSD operators are outside the engine; those cases raise."""
+    if A.isa(PSD):
+        return inv(A, HipCG(**alg.__dict__))
+    raise NotImplementedError("the engine solves PSD operators with CG; wrap A in PSD(...) or use a dense solver")
+
+
+@dispatch(cond=lambda A, *_: A.isa(Unitary))
+def inv(A: LinearOperator, alg: Algorithm):
+    """inv.py:108-110."""
+    from ..fns import lazify
+    return lazify(A.to_dense().T.conj())
+
+
+@dispatch
+def inv(A: Identity, alg: Algorithm):
+    """inv.py:113-115."""
+    return A
+
+
+@dispatch
+def inv(A: ScalarMul, alg: Algorithm):
+    """inv.py:118-120."""
+    return ScalarMul(1 / A.c, shape=A.shape, dtype=A.dtype, device=A.device)
+
+
+@dispatch(cond=lambda A, *_: all([M.shape[-2] == M.shape[-1] for M in A.Ms]))
+def inv(A: Product, alg: Algorithm):
+    """inv.py:128-131."""
+    return Product(*reversed([inv(M, alg) for M in A.Ms]))
+
+
+@dispatch
+def inv(A: Kronecker, alg: Algorithm):
+    """inv.py:139-141: factor-wise."""
+    return Kronecker(*[inv(M, alg) for M in A.Ms])
+
+
+@dispatch
+def inv(A: Diagonal, alg: Algorithm):
+    """inv.py:144-146."""
+    return Diagonal(1. / A.diag)
+
+
+_ = np

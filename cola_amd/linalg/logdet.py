@@ -1,0 +1,96 @@
+"""logdet / slogdet on the engine (cola/linalg/logdet/logdet.py:29-184).
+
+Iterative rule (logdet.py:111-117): log(A) through Lanczos, then its trace by
+Hutchinson; plus the structural rules for Kronecker, Diagonal, Identity,
+ScalarMul and square Products.  ``HipSLQ`` is the engine's direct stochastic
+Lanczos quadrature log-determinant (explicit probe count, FP64 quadrature)."""
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+import torch
+
+from ..annotations import PSD
+from ..dispatch import dispatch
+from ..ops import Diagonal, Identity, Kronecker, LinearOperator, Product, ScalarMul
+from .algorithm_base import Algorithm, Auto
+from .lanczos import HipLanczos
+from .slq import HipSLQ, slq_logdet
+from .trace import HipHutch, trace
+from .unary import log
+
+
+def logdet(A, log_alg=Auto(), trace_alg=Auto()):
+    """logdet.py:29-49."""
+    _, ld = slogdet(A, log_alg, trace_alg)
+    return ld
+
+
+@dispatch.abstract
+def slogdet(A: LinearOperator, log_alg: Algorithm = Auto(), trace_alg: Algorithm = Auto()):
+    """(sign, log|det|) with det(A) = sign * exp(logdet) (logdet.py:52-78)."""
+
+
+@dispatch(precedence=-1)
+def slogdet(A: LinearOperator, log_alg: Auto, trace_alg: Algorithm):
+    """logdet.py:82-94 restricted to the engine: PSD operators use Lanczos."""
+    if A.isa(PSD):
+        return slogdet(A, HipLanczos(**log_alg.__dict__), trace_alg)
+    raise NotImplementedError("the engine computes log-determinants of PSD operators")
+
+
+@dispatch(precedence=-1)
+def slogdet(A: LinearOperator, log_alg: HipLanczos, trace_alg: Algorithm):
+    """logdet.py:111-117: trace(log(A))."""
+    logA = log(A, log_alg)
+    tr = trace(logA, trace_alg if not isinstance(trace_alg, Auto) else HipHutch(**trace_alg.__dict__))
+    mag = torch.abs(tr)
+    return tr / mag, mag
+
+
+@dispatch(precedence=-1)
+def slogdet(A: LinearOperator, log_alg: HipSLQ, trace_alg: Algorithm):
+    """Stochastic Lanczos quadrature in one batched Lanczos run (engine algorithm)."""
+    est, _ = slq_logdet(A, log_alg)
+    mag = torch.abs(est)
+    return est / mag, mag
+
+
+@dispatch(cond=lambda A, *_: all([(Ai.shape[-2] == Ai.shape[-1]) for Ai in A.Ms]))
+def slogdet(A: Product, log_alg: Algorithm, trace_alg: Algorithm):
+    """logdet.py:121-124."""
+    signs, lds = zip(*[slogdet(Ai, log_alg, trace_alg) for Ai in A.Ms])
+    out_s = 1.0
+    for s_ in signs:
+        out_s = out_s * s_
+    return out_s, sum(lds)
+
+
+@dispatch
+def slogdet(A: Identity, log_alg: Algorithm, trace_alg: Algorithm):
+    zero = torch.zeros((), dtype=A.dtype, device=A.device)
+    return 1. + zero, zero
+
+
+@dispatch
+def slogdet(A: ScalarMul, log_alg: Algorithm, trace_alg: Algorithm):
+    c = A.c
+    return c / torch.abs(c), torch.log(torch.abs(c))
+
+
+@dispatch
+def slogdet(A: Diagonal, log_alg: Algorithm, trace_alg: Algorithm):
+    mag = torch.abs(A.diag)
+    return torch.prod(A.diag / mag), torch.sum(torch.log(mag))
+
+
+@dispatch
+def slogdet(A: Kronecker, log_alg: Algorithm, trace_alg: Algorithm):
+    """logdet.py:150-158: logdet(A1 x ... x Ak) = sum_i (N / n_i) logdet(A_i)."""
+    signs, lds = zip(*[slogdet(Ai, log_alg, trace_alg) for Ai in A.Ms])
+    sizes = [Ai.shape[-1] for Ai in A.Ms]
+    prod = int(np.prod(sizes))
+    out_s = 1.0
+    for s_, n_ in zip(signs, sizes):
+        out_s = out_s * s_**(prod / n_)
+    return out_s, sum(ld_ * prod / n_ for ld_, n_ in zip(lds, sizes))

@@ -1,0 +1,572 @@
+"""Linear operators (cola/ops/operator_base.py, cola/ops/operators.py) for the engine.
+
+Same classes and contract as cola: ``LinearOperator(dtype, shape, matmat)``,
+``_matmat(X)`` on an (n, k) block, ``A @ x`` with 1-D promotion
+(operator_base.py:97-106), ``A.isa(PSD)`` (operator_base.py:52-54).
+
+On the HIP device the structures the engine knows -- Dense (fp32 or bf16
+storage), Sparse CSR, Kronecker of dense factors, Diagonal, Identity,
+ScalarMul, an on-the-fly RBF kernel, a Woodbury inverse, and any of these
+scaled by a scalar and/or shifted by c*I -- lower to one ``ck_op`` descriptor
+(``hip_op()``) and run as the engine's own kernels.  Operators on CPU tensors
+keep plain torch arithmetic; the Krylov algorithms themselves refuse CPU input.
+"""
+from functools import reduce
+
+import numpy as np
+import torch
+
+from . import _native as N
+from . import annotations as ann
+from ._blocks import from_blocks, ld_of, to_blocks
+
+
+def _find_device(objs):
+    for o in objs:
+        if isinstance(o, torch.Tensor):
+            return o.device
+        if isinstance(o, LinearOperator):
+            return o.device
+        if isinstance(o, (list, tuple)):
+            d = _find_device(o)
+            if d is not None:
+                return d
+    return None
+
+
+class LinearOperator:
+    """Base class (operator_base.py:15-219)."""
+    __array_ufunc__ = None
+
+    def __init__(self, dtype, shape, matmat=None, annotations=frozenset(), device=None):
+        self.dtype = dtype
+        self.shape = tuple(int(s) for s in shape)
+        if matmat is not None:
+            self._matmat = matmat
+        if not hasattr(self, "device") or self.device is None:
+            self.device = device if device is not None else torch.device("cpu")
+        self.annotations = ann.get_annotations(self)
+        self.annotations.update(annotations)
+
+    def isa(self, annotation):
+        return any(issubclass(a, annotation) for a in self.annotations)
+
+    def _matmat(self, X):
+        raise NotImplementedError
+
+    # ---- structure recognition --------------------------------------------
+    def hip_op(self):
+        """(CkOp, keepalive list) when the engine has a kernel for this operator, else None."""
+        return None
+
+    def _hip_apply(self, X):
+        op, keep = self.hip_op()
+        n = self.shape[0]
+        Xb = to_blocks(X)
+        ld = ld_of(n)
+        B = Xb.shape[0]
+        Yb = torch.empty_like(Xb)
+        wb = N.q("ck_op_workspace_bytes", op, B)
+        work = torch.empty(max(1, (wb + 3) // 4), dtype=torch.float32, device=Xb.device) if wb else None
+        N.call("ck_op_apply", op, B, N.ptr(Xb), ld, N.ptr(Yb), ld, N.ptr(work), wb, N.stream_ptr(Xb.device))
+        del keep
+        return from_blocks(Yb, n).to(X.dtype)
+
+    def _use_hip(self, X):
+        return isinstance(X, torch.Tensor) and X.is_cuda and X.dtype == torch.float32 and self.hip_op() is not None
+
+    # ---- algebra ------------------------------------------------------------
+    def __matmul__(self, X):
+        if isinstance(X, LinearOperator):
+            from .fns import dot
+            return dot(self, X)
+        assert X.shape[0] == self.shape[-1], f"dimension mismatch {self.shape} vs {tuple(X.shape)}"
+        if X.dim() == 1:
+            return self._matmat(X.reshape(-1, 1)).reshape(-1)
+        return self._matmat(X)
+
+    def __rmatmul__(self, X):
+        if isinstance(X, LinearOperator):
+            from .fns import dot
+            return dot(X, self)
+        if self.isa(ann.SelfAdjoint):
+            return (self @ X.T).T
+        return X @ self.to_dense()
+
+    def to_dense(self):
+        n = self.shape[-1]
+        return self @ torch.eye(n, dtype=self.dtype, device=self.device)
+
+    @property
+    def T(self):
+        if self.isa(ann.SelfAdjoint):
+            return self
+        return Dense(self.to_dense().T.contiguous())
+
+    @property
+    def H(self):
+        return self.T
+
+    def __add__(self, other):
+        if isinstance(other, (int, float)) and other == 0:
+            return self
+        from .fns import add
+        return add(self, other)
+
+    def __radd__(self, other):
+        return self.__add__(other)
+
+    def __mul__(self, c):
+        from .fns import mul
+        return mul(self, c)
+
+    def __rmul__(self, c):
+        return self * c
+
+    def __neg__(self):
+        return -1 * self
+
+    def __sub__(self, x):
+        return self.__add__(-x)
+
+    def to(self, device):
+        raise NotImplementedError(f"{type(self).__name__}.to")
+
+    def __repr__(self):
+        return f"<{type(self).__name__} {self.shape[0]}x{self.shape[1]} {self.dtype}>"
+
+
+def _op(kind, n, scale=1.0, shift=0.0):
+    op = N.CkOp()
+    op.kind, op.n, op.scale, op.shift = kind, n, scale, shift
+    return op
+
+
+class Dense(LinearOperator):
+    """operators.py:12-38."""
+
+    def __init__(self, A):
+        self.A = A
+        self.device = A.device
+        super().__init__(A.dtype, A.shape)
+
+    def _matmat(self, X):
+        if self._use_hip(X):
+            return self._hip_apply(X)
+        dtype = torch.promote_types(self.dtype, X.dtype)
+        return self.A.to(dtype) @ X.to(dtype)
+
+    def hip_op(self):
+        A = self.A
+        if not (A.is_cuda and A.dtype == torch.float32 and A.dim() == 2 and A.shape[0] == A.shape[1]
+                and A.stride(1) == 1):
+            return None
+        op = _op(N.CK_OP_DENSE_F32, A.shape[0])
+        op.a, op.lda = A.data_ptr(), A.stride(0)
+        return op, [A]
+
+    def to_dense(self):
+        return self.A
+
+
+class DenseBF16(LinearOperator):
+    """A dense matrix stored in bfloat16; products accumulate in fp32 (engine-only)."""
+
+    def __init__(self, A):
+        self.A = A.to(torch.bfloat16).contiguous()
+        self.device = A.device
+        super().__init__(torch.float32, A.shape)
+
+    def hip_op(self):
+        A = self.A
+        if not (A.is_cuda and A.shape[0] % 8 == 0):
+            return None
+        op = _op(N.CK_OP_DENSE_BF16, A.shape[0])
+        op.a, op.lda = A.data_ptr(), A.stride(0)
+        return op, [A]
+
+    def _matmat(self, X):
+        if self._use_hip(X):
+            return self._hip_apply(X)
+        return self.A.to(torch.float32) @ X.to(torch.float32)
+
+    def to_dense(self):
+        return self.A.to(torch.float32)
+
+
+def coo_to_csr(data, rows, cols, shape):
+    """Device-side COO -> CSR: sort by (row, col), sum duplicates, int32 indices.
+
+    The reference does argsort + scipy ``coo_array(...).tocsr()`` on the host
+    (operators.py:64-75); the resulting arrays are the same."""
+    dev = data.device
+    rows = rows.to(device=dev, dtype=torch.int64)
+    cols = cols.to(device=dev, dtype=torch.int64)
+    key = rows * int(shape[1]) + cols
+    key, order = torch.sort(key, stable=True)
+    vals = data[order]
+    uniq, inverse = torch.unique_consecutive(key, return_inverse=True)
+    summed = torch.zeros(uniq.numel(), dtype=data.dtype, device=dev).index_add_(0, inverse, vals)
+    r = uniq // int(shape[1])
+    c = (uniq % int(shape[1])).to(torch.int32)
+    counts = torch.bincount(r, minlength=int(shape[0]))
+    indptr = torch.zeros(int(shape[0]) + 1, dtype=torch.int64, device=dev)
+    indptr[1:] = torch.cumsum(counts, 0)
+    return indptr.to(torch.int32), c, summed
+
+
+def _check_coo(rows, cols, shape):
+    # kernels index x[col] and y[row] without bounds checks: validate once here
+    if rows.numel() != cols.numel():
+        raise ValueError("row and column index arrays differ in length")
+    if rows.numel() and (int(rows.min()) < 0 or int(rows.max()) >= shape[0] or int(cols.min()) < 0
+                         or int(cols.max()) >= shape[1]):
+        raise ValueError("sparse indices out of range for shape")
+    if shape[0] >= 2**31 or shape[1] >= 2**31 or rows.numel() >= 2**31:
+        raise ValueError("CSR with int32 indices needs n, nnz < 2**31")
+
+
+def _check_csr(indptr, indices, data, shape):
+    if indptr.numel() != shape[0] + 1 or int(indptr[0]) != 0 or int(indptr[-1]) != data.numel():
+        raise ValueError("indptr must have n+1 entries from 0 to nnz")
+    if indices.numel() != data.numel():
+        raise ValueError("indices and data differ in length")
+    if shape[0] > 0 and bool((indptr[1:] < indptr[:-1]).any()):
+        raise ValueError("indptr must be non-decreasing")
+    if indices.numel() and (int(indices.min()) < 0 or int(indices.max()) >= shape[1]):
+        raise ValueError("column indices out of range for shape")
+
+
+class Sparse(LinearOperator):
+    """operators.py:48-81 (COO input, CSR storage with int32 indices)."""
+
+    def __init__(self, data, row_indices, col_indices, shape):
+        self.device = data.device
+        super().__init__(data.dtype, shape)
+        _check_coo(row_indices, col_indices, shape)
+        self.indptr, self.indices, self.data = coo_to_csr(data, row_indices, col_indices, shape)
+
+    @classmethod
+    def from_csr(cls, indptr, indices, data, shape):
+        obj = cls.__new__(cls)
+        obj.device = data.device
+        LinearOperator.__init__(obj, data.dtype, shape)
+        obj.indptr = indptr.to(device=data.device, dtype=torch.int32)
+        obj.indices = indices.to(device=data.device, dtype=torch.int32)
+        obj.data = data
+        _check_csr(obj.indptr, obj.indices, data, shape)
+        return obj
+
+    def hip_op(self):
+        if not (self.data.is_cuda and self.data.dtype == torch.float32 and self.shape[0] == self.shape[1]):
+            return None
+        op = _op(N.CK_OP_CSR_F32, self.shape[0])
+        op.rowptr, op.colind, op.values = self.indptr.data_ptr(), self.indices.data_ptr(), self.data.data_ptr()
+        op.nnz = self.data.numel()
+        return op, [self.indptr, self.indices, self.data]
+
+    def _matmat(self, V):
+        if self._use_hip(V):
+            return self._hip_apply(V)
+        n = self.shape[0]
+        counts = (self.indptr[1:] - self.indptr[:-1]).to(torch.int64)
+        row_of = torch.repeat_interleave(torch.arange(n, device=V.device), counts)
+        out = torch.zeros((n, V.shape[1]), dtype=torch.promote_types(self.dtype, V.dtype), device=V.device)
+        out.index_add_(0, row_of, self.data[:, None].to(out.dtype) * V[self.indices.long()].to(out.dtype))
+        return out
+
+    def to_dense(self):
+        n, m = self.shape
+        D = torch.zeros((n, m), dtype=self.dtype, device=self.device)
+        counts = (self.indptr[1:] - self.indptr[:-1]).to(torch.int64)
+        row_of = torch.repeat_interleave(torch.arange(n, device=self.device), counts)
+        D.index_put_((row_of, self.indices.long()), self.data, accumulate=True)
+        return D
+
+
+class ScalarMul(LinearOperator):
+    """c * I (operators.py:84-101)."""
+
+    def __init__(self, c, shape, dtype=None, device=None):
+        self.device = device if device is not None else torch.device("cpu")
+        self.c = c if isinstance(c, torch.Tensor) else torch.tensor(c, dtype=dtype or torch.float32,
+                                                                    device=self.device)
+        super().__init__(dtype or self.c.dtype, shape)
+
+    def _matmat(self, v):
+        return self.c.to(v.device) * v
+
+    def hip_op(self):
+        if self.device.type != "cuda":
+            return None
+        return _op(N.CK_OP_IDENTITY, self.shape[0], scale=float(self.c)), []
+
+
+class Identity(LinearOperator):
+    """operators.py:104-128."""
+
+    def __init__(self, shape, dtype, device=None):
+        self.device = device if device is not None else torch.device("cpu")
+        super().__init__(dtype, shape)
+
+    def _matmat(self, X):
+        return X
+
+    def hip_op(self):
+        if self.device.type != "cuda":
+            return None
+        return _op(N.CK_OP_IDENTITY, self.shape[0]), []
+
+
+def I_like(A):
+    """operators.py:130-135."""
+    return Identity(A.shape, A.dtype, device=A.device)
+
+
+class Product(LinearOperator):
+    """operators.py:138-164."""
+
+    def __init__(self, *Ms):
+        from .fns import lazify
+        self.Ms = tuple(lazify(M) for M in Ms)
+        for M1, M2 in zip(self.Ms[:-1], self.Ms[1:]):
+            if M1.shape[-1] != M2.shape[-2]:
+                raise ValueError(f"dimension mismatch {M1.shape} vs {M2.shape}")
+        self.device = self.Ms[0].device
+        shape = (self.Ms[0].shape[-2], self.Ms[-1].shape[-1])
+        dtype = reduce(torch.promote_types, (M.dtype for M in self.Ms))
+        super().__init__(dtype, shape)
+
+    def _matmat(self, v):
+        for M in self.Ms[::-1]:
+            v = M @ v
+        return v
+
+    def hip_op(self):
+        # scalar * structured: fold the scalars into the structured op's scale
+        scal = 1.0
+        rest = []
+        for M in self.Ms:
+            if isinstance(M, ScalarMul):
+                scal *= float(M.c)
+            elif isinstance(M, Identity):
+                continue
+            else:
+                rest.append(M)
+        if len(rest) > 1:
+            return None
+        if not rest:
+            if self.device.type != "cuda":
+                return None
+            return _op(N.CK_OP_IDENTITY, self.shape[0], scale=scal), []
+        got = rest[0].hip_op()
+        if got is None:
+            return None
+        op, keep = got
+        op.scale *= scal
+        op.shift *= scal
+        return op, keep
+
+
+class Sum(LinearOperator):
+    """operators.py:167-191."""
+
+    def __init__(self, *Ms):
+        from .fns import lazify
+        self.Ms = tuple(lazify(M) for M in Ms)
+        shape = self.Ms[0].shape
+        for M in self.Ms:
+            if M.shape != shape:
+                raise ValueError(f"dimension mismatch {M.shape} vs {shape}")
+        self.device = self.Ms[0].device
+        super().__init__(self.Ms[0].dtype, shape)
+
+    def _matmat(self, v):
+        if self._use_hip(v):
+            return self._hip_apply(v)
+        return sum(M @ v for M in self.Ms)
+
+    def hip_op(self):
+        # one structured member plus any number of c*I members -> shift epilogue
+        shift = 0.0
+        structured = []
+        for M in self.Ms:
+            got = M.hip_op()
+            if got is None:
+                return None
+            op, keep = got
+            if op.kind == N.CK_OP_IDENTITY:
+                shift += op.scale + op.shift
+            else:
+                structured.append((op, keep))
+        if len(structured) > 1:
+            return None
+        if not structured:
+            return _op(N.CK_OP_IDENTITY, self.shape[0], scale=shift), []
+        op, keep = structured[0]
+        op.shift += shift
+        return op, keep
+
+
+class Kronecker(LinearOperator):
+    """operators.py:198-230."""
+
+    def __init__(self, *Ms):
+        from .fns import lazify
+        self.Ms = tuple(lazify(M) for M in Ms)
+        self.device = self.Ms[0].device
+        shape = (int(np.prod([M.shape[-2] for M in self.Ms])), int(np.prod([M.shape[-1] for M in self.Ms])))
+        dtype = reduce(torch.promote_types, (M.dtype for M in self.Ms))
+        super().__init__(dtype, shape)
+
+    def _matmat(self, v):
+        if self._use_hip(v):
+            return self._hip_apply(v)
+        ev = v.reshape(*[M.shape[-1] for M in self.Ms], -1)
+        for i, M in enumerate(self.Ms):
+            ev_front = torch.movedim(ev, i, 0)
+            shape = (M.shape[0], *ev_front.shape[1:])
+            Mev_front = (M @ ev_front.reshape(M.shape[-1], -1)).reshape(shape)
+            ev = torch.movedim(Mev_front, 0, i)
+        return ev.reshape(self.shape[-2], ev.shape[-1])
+
+    def hip_op(self):
+        if len(self.Ms) not in (2, 3):
+            return None
+        facs = []
+        for M in self.Ms:
+            if not isinstance(M, Dense):
+                return None
+            A = M.A
+            if not (A.is_cuda and A.dtype == torch.float32 and A.shape[0] == A.shape[1] and A.shape[0] <= 64):
+                return None
+            facs.append(A.contiguous())
+        op = _op(N.CK_OP_KRON_F32, self.shape[0])
+        op.nfactors = len(facs)
+        for i, A in enumerate(facs):
+            op.fdim[i] = A.shape[0]
+            op.factor[i] = A.data_ptr()
+        return op, facs
+
+    def to_dense(self):
+        return reduce(torch.kron, [M.to_dense() for M in self.Ms])
+
+
+class Diagonal(LinearOperator):
+    """operators.py:323-348."""
+
+    def __init__(self, diag):
+        assert diag.dim() == 1
+        self.diag = diag
+        self.device = diag.device
+        super().__init__(diag.dtype, (diag.numel(), diag.numel()))
+
+    def _matmat(self, X):
+        if self._use_hip(X):
+            return self._hip_apply(X)
+        return self.diag[:, None] * X
+
+    def hip_op(self):
+        d = self.diag
+        if not (d.is_cuda and d.dtype == torch.float32 and d.is_contiguous()):
+            return None
+        op = _op(N.CK_OP_DIAG_F32, d.numel())
+        op.a = d.data_ptr()
+        return op, [d]
+
+    def to_dense(self):
+        return torch.diag(self.diag)
+
+
+class Tridiagonal(LinearOperator):
+    """operators.py:351-380 (alpha lower band, beta diagonal, gamma upper band)."""
+
+    def __init__(self, alpha, beta, gamma):
+        self.alpha, self.beta, self.gamma = alpha, beta, gamma
+        self.device = beta.device
+        super().__init__(beta.dtype, (beta.shape[-1], beta.shape[-1]))
+
+    def to_dense(self):
+        T = torch.diag(self.beta)
+        if self.alpha.numel():
+            T = T + torch.diag(self.alpha, -1) + torch.diag(self.gamma, 1)
+        return T
+
+    def _matmat(self, X):
+        return self.to_dense().to(X.dtype) @ X
+
+
+class Kernel(LinearOperator):
+    """Blocked on-the-fly kernel matvec [Kv]_i = sum_j f(x1_i, x2_j) v_j (operators.py:609-646)."""
+
+    def __init__(self, x1, x2, fn, block_size1, block_size2):
+        self.x1, self.x2, self.fn = x1, x2, fn
+        self.block_size1, self.block_size2 = block_size1, block_size2
+        self.device = x1.device
+        super().__init__(x1.dtype, (x1.shape[0], x2.shape[0]))
+
+    def _matmat(self, V):
+        out = torch.zeros((self.shape[0], V.shape[1]), dtype=V.dtype, device=V.device)
+        for i0 in range(0, self.shape[0], self.block_size1):
+            blk = slice(i0, i0 + self.block_size1)
+            acc = torch.zeros_like(out[blk])
+            for j0 in range(0, self.shape[1], self.block_size2):
+                blk2 = slice(j0, j0 + self.block_size2)
+                acc += self.fn(self.x1[blk], self.x2[blk2]) @ V[blk2]
+            out[blk] = acc
+        return out
+
+
+class RBFKernel(LinearOperator):
+    """exp(-|x_i - x_j|^2 / (2 l^2)) on points x (n x d, d <= 8), never stored.
+
+    The engine builds each entry inside the matvec kernel (CK_OP_RBF_F32)."""
+
+    def __init__(self, points, lengthscale):
+        self.points = points.contiguous()
+        self.lengthscale = float(lengthscale)
+        self.device = points.device
+        super().__init__(points.dtype, (points.shape[0], points.shape[0]), annotations={ann.PSD})
+
+    def hip_op(self):
+        P = self.points
+        if not (P.is_cuda and P.dtype == torch.float32 and P.shape[1] <= 8):
+            return None
+        op = _op(N.CK_OP_RBF_F32, P.shape[0])
+        op.a, op.lda, op.dim, op.aux_f = P.data_ptr(), P.stride(0), P.shape[1], self.lengthscale
+        return op, [P]
+
+    def _matmat(self, V):
+        if self._use_hip(V):
+            return self._hip_apply(V)
+        return self.to_dense().to(V.dtype) @ V
+
+    def to_dense(self):
+        d2 = torch.cdist(self.points, self.points) ** 2
+        return torch.exp(-d2 / (2 * self.lengthscale**2))
+
+
+class WoodburyInverse(LinearOperator):
+    """(s I + F F^T)^{-1} applied by the Woodbury identity; F is (n, k), k <= 128."""
+
+    def __init__(self, F, s):
+        self.F = F.contiguous()
+        self.s = float(s)
+        k = F.shape[1]
+        core = self.s * torch.eye(k, dtype=torch.float64, device=F.device) + (F.T.double() @ F.double())
+        self.core_inv = torch.linalg.inv(core).to(torch.float32).contiguous()
+        self.device = F.device
+        super().__init__(F.dtype, (F.shape[0], F.shape[0]), annotations={ann.PSD})
+
+    def hip_op(self):
+        if not (self.F.is_cuda and self.F.dtype == torch.float32 and self.F.shape[1] <= 128):
+            return None
+        op = _op(N.CK_OP_WOODBURY_INV, self.F.shape[0], scale=1.0 / self.s)
+        op.a, op.lda, op.aux, op.rank = self.F.data_ptr(), self.F.stride(0), self.core_inv.data_ptr(), self.F.shape[1]
+        return op, [self.F, self.core_inv]
+
+    def _matmat(self, V):
+        if self._use_hip(V):
+            return self._hip_apply(V)
+        t = self.F.T @ V
+        return (V - self.F @ (self.core_inv @ t)) / self.s

@@ -1,0 +1,291 @@
+/*
+ * cola_krylov.h -- C ABI of the MI355X (gfx950) Krylov engine behind cola_amd.
+ *
+ * Plain pointers and sizes only.  Every pointer argument named X, Y, V, x, r, p,
+ * ... is DEVICE memory owned by the caller; the library never allocates or frees
+ * user memory.  Every function returns an int status (CK_OK == 0) and never
+ * throws.  `stream` is a hipStream_t passed as void*; all work is enqueued on it
+ * and no function blocks the host unless its comment says so.
+ *
+ * Vector blocks ("column blocks"): B vectors of length n are stored one after
+ * another, vector b at X + b*ldx (ldx >= n).  The reference keeps the same data
+ * as an (n, B) row-major array (cola/ops/operator_base.py:56-60); the Python
+ * layer transposes at the boundary.
+ *
+ * Reference citations are relative to the wilson-labs/cola source tree.
+ */
+#ifndef COLA_KRYLOV_H
+#define COLA_KRYLOV_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+enum {
+  CK_OK = 0,
+  CK_ERR_BAD_ARG = 1,     /* shapes, null pointers, sizes out of range          */
+  CK_ERR_HIP = 2,         /* a HIP runtime call or launch failed                */
+  CK_ERR_UNSUPPORTED = 3, /* operator structure this entry point does not cover */
+  CK_ERR_WORKSPACE = 4,   /* workspace smaller than the *_workspace_bytes query */
+  CK_ERR_NOCONV = 5       /* tridiagonal QL did not converge in 60 sweeps/value  */
+};
+
+/* Library version string, e.g. "cola_krylov 0.1.0 gfx950". */
+const char* ck_version(void);
+/* Human-readable text for a status code. */
+const char* ck_status_string(int status);
+
+/* ---------------------------------------------------------------- operators
+ * A square operator y = scale * S x + shift * x, where S is the structured part:
+ *   CK_OP_IDENTITY   S = I
+ *   CK_OP_DENSE_F32  S = A, row-major fp32 with leading dimension lda
+ *                    (replaces Dense._matmat, cola/ops/operators.py:26-28)
+ *   CK_OP_DENSE_BF16 S = A stored as bf16 (uint16 bit patterns), fp32 accumulate
+ *   CK_OP_CSR_F32    S = CSR(rowptr, colind, values), int32 indices
+ *                    (replaces Sparse._matmat, cola/ops/operators.py:64-78)
+ *   CK_OP_KRON_F32   S = F0 (x) F1 (x) ... , 2 or 3 square row-major fp32 factors,
+ *                    each of order <= 64 (replaces Kronecker._matmat,
+ *                    cola/ops/operators.py:216-223)
+ *   CK_OP_DIAG_F32   S = diag(a)  (cola/ops/operators.py:338-339)
+ *   CK_OP_WOODBURY_INV  y = scale * (x - F (C^-1 (F^T x))): the inverse of
+ *                    s I + F F^T when scale = 1/s and C^-1 = (s I + F^T F)^-1;
+ *                    a = F (n x rank, row-major fp32, lda), aux = C^-1 (rank x rank fp32)
+ *   CK_OP_RBF_F32    S_ij = exp(-|p_i - p_j|^2 / (2 l^2)) built on the fly from
+ *                    points a (n x dim fp32, row-major, lda) with l = aux_f
+ *                    (replaces Kernel._matmat, cola/ops/operators.py:631-643)
+ * The shift term is how Sum(S, ScalarMul(c) @ I) (cola/ops/operators.py:182-183,
+ * 97-98) reaches one kernel epilogue.
+ */
+enum {
+  CK_OP_IDENTITY = 0,
+  CK_OP_DENSE_F32 = 1,
+  CK_OP_DENSE_BF16 = 2,
+  CK_OP_CSR_F32 = 3,
+  CK_OP_KRON_F32 = 4,
+  CK_OP_DIAG_F32 = 5,
+  CK_OP_WOODBURY_INV = 6,
+  CK_OP_RBF_F32 = 7
+};
+
+typedef struct ck_op {
+  int32_t kind;
+  int32_t n;
+  float scale;
+  float shift;
+  const void* a;
+  int64_t lda;
+  const int32_t* rowptr;
+  const int32_t* colind;
+  const float* values;
+  int64_t nnz;
+  int32_t nfactors;
+  int32_t fdim[4];
+  const float* factor[4];
+  const float* aux;
+  int32_t rank;
+  int32_t dim;
+  float aux_f;
+  int32_t reserved[5];
+} ck_op;
+
+/* Bytes of device scratch ck_op_apply needs for `nrhs` vectors. */
+size_t ck_op_workspace_bytes(const ck_op* op, int32_t nrhs);
+
+/* Y = op(X) for nrhs vectors.  X and Y must not overlap.
+ * Replaces LinearOperator.__matmul__ -> _matmat (cola/ops/operator_base.py:97-106). */
+int ck_op_apply(const ck_op* op, int32_t nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,
+                size_t work_bytes, void* stream);
+
+/* ---------------------------------------------------------------- vector kernels */
+/* Per-column dot products out[b] = sum_i X[b,i] Y[b,i], fp64 result (a two-pass
+ * reduction with fixed order).  Used for residual checks. */
+int ck_dot_f32(int64_t n, int32_t nrhs, const float* X, int64_t ldx, const float* Y, int64_t ldy, double* out,
+               void* work, size_t work_bytes, void* stream);
+size_t ck_dot_workspace_bytes(int64_t n, int32_t nrhs);
+
+/* Transpose between the (n, B) row-major layout and column blocks:
+ * dst[b*ldd + i] = src[i*lds + b] (to_blocks=1) or dst[i*ldd + b] = src[b*lds + i]. */
+int ck_transpose_f32(int64_t n, int32_t nrhs, const float* src, int64_t lds, float* dst, int64_t ldd, int32_t to_blocks,
+                     void* stream);
+
+/* ---------------------------------------------------------------- probes
+ * Counter-based streams (SplitMix64 output function applied to (seed, stream, row)).
+ * Rademacher probe p at row i is 1 - 2*bit(p % 64) of word(seed, p / 64, i).
+ * Gaussian probe p at row i is Box-Muller of word(seed ^ 0x5851F42D4C957F2D, p, i).
+ * Replaces the keyed torch.randn / sign draws of cola/linalg/trace/diagonal_estimation.py:187-190
+ * and cola/linalg/tbd/slq.py:41. */
+int ck_rng_rademacher_f32(int64_t n, int32_t nprobes, uint64_t seed, float* Z, int64_t ldz, void* stream);
+int ck_rng_gaussian_f32(int64_t n, int32_t nprobes, uint64_t seed, float* Z, int64_t ldz, void* stream);
+/* Packed Rademacher words: W[g*n + i] = word(seed, g, i) for g < ceil(nprobes/64). */
+int ck_rng_rademacher_bits(int64_t n, int32_t ngroups, uint64_t seed, uint64_t* W, void* stream);
+
+/* ---------------------------------------------------------------- CG
+ * Batched (preconditioned) CG with the reference's arithmetic and stopping rule
+ * (cola/linalg/inverse/cg.py:94-178): columns normalised by ||b||, per-column
+ * threshold tol*||r0|| + tol, continue while ANY column has ||r|| > threshold and
+ * k < max_iters, safe division at 1e-40, columns with ||r|| < 1e-40 frozen.
+ * The loop condition is evaluated on the device; kernels of steps enqueued after
+ * it turns false return immediately, so the host checks `done` only once per
+ * chunk of steps.
+ *
+ * The state buffer (ck_cg_state_bytes) starts with this control block: */
+typedef struct ck_cg_ctrl {
+  int32_t k;         /* CG steps taken                                   */
+  int32_t done;      /* 1 once the loop condition is false               */
+  int32_t max_iters;
+  int32_t nrhs;
+  int32_t nparts;    /* partial sums per column                          */
+  int32_t pad[11];
+} ck_cg_ctrl;
+/* followed by fp32 arrays, each of ck_cg_stride(nrhs) floats, at these indices:
+ * 0 mult(||b||) 1 tol 2 gamma 3 alpha 4 beta 5 rnorm, then errors[max_iters + 2]:
+ * errors[k] = mean_b ||r_k|| (the value cola/utils/torch_tqdm.py:38-50 records). */
+enum { CK_CG_MULT = 0, CK_CG_TOL = 1, CK_CG_GAMMA = 2, CK_CG_ALPHA = 3, CK_CG_BETA = 4, CK_CG_RNORM = 5, CK_CG_ERRORS = 6 };
+int32_t ck_cg_stride(int32_t nrhs);
+size_t ck_cg_state_bytes(int32_t nrhs, int32_t max_iters);
+/* Device scratch for partial sums: 3 * nrhs * ck_cg_nparts(n) floats. */
+int32_t ck_cg_nparts(int64_t n);
+size_t ck_cg_partials_bytes(int64_t n, int32_t nrhs);
+
+/* mult = ||b|| per column; bn = b / mult (safe division) -- cg.py:96-97. */
+int ck_cg_normalize(int64_t n, int32_t nrhs, const float* b, int64_t ldb, float* bn, int64_t ldbn, void* state,
+                    float* partials, void* stream);
+/* p = z; gamma = r.z; rnorm = ||r||; tol = tol*rnorm + tol; errors[0]; k = 0;
+ * done = !(any(rnorm > tol) && 0 < max_iters)  -- cg.py:98-101,122-138. */
+int ck_cg_start(int64_t n, int32_t nrhs, const float* r, const float* z, float* p, int64_t ld, void* state, float tol,
+                int32_t max_iters, float* partials, void* stream);
+/* First half of take_cg_step (cg.py:141-149): alpha = gamma / p.Ap (safe, 0 when
+ * frozen); x += alpha p; r -= alpha Ap. */
+int ck_cg_step_xr(int64_t n, int32_t nrhs, const float* p, const float* Ap, float* x, float* r, int64_t ld, void* state,
+                  float* partials, void* stream);
+/* Second half (cg.py:150-154, 133-138): gamma1 = r.z, beta = gamma1/gamma (safe, 0
+ * when frozen), p = z + beta p, k += 1, loop condition -> done.  z == r means no
+ * preconditioner (r.r was already accumulated by ck_cg_step_xr). */
+int ck_cg_step_p(int64_t n, int32_t nrhs, const float* r, const float* z, float* p, int64_t ld, void* state,
+                 float* partials, void* stream);
+/* Enqueue `nsteps` whole steps with library operators A and (optional) P.
+ * z may equal r when P is NULL.  Ap and z are caller scratch blocks. */
+int ck_cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r, float* p, float* Ap, float* z,
+                  int64_t ld, void* state, float* partials, void* work, size_t work_bytes, int32_t nsteps,
+                  void* stream);
+/* x *= mult; r *= mult  (cg.py:119). */
+int ck_cg_finish(int64_t n, int32_t nrhs, float* x, float* r, int64_t ld, const void* state, void* stream);
+
+/* ---------------------------------------------------------------- Lanczos
+ * Batched Lanczos with double classical Gram-Schmidt against every filled basis
+ * vector (cola/linalg/decompositions/lanczos.py:235-296).  For nb start vectors
+ * the basis is V[b][j][i] at V + b*ldb + j*ldv + i, j = 0..max_iters+1; column 0
+ * is zero and column 1 the normalised start vector (lanczos.py:275-284).
+ * diag (nb x max_iters) and subdiag (nb x (max_iters+1)) are fp32 like the
+ * reference's alpha/beta arrays.  The state buffer starts with: */
+typedef struct ck_lz_ctrl {
+  int32_t i;         /* next step index (starts at 1)            */
+  int32_t done;
+  int32_t max_iters;
+  int32_t nb;
+  int32_t nparts;
+  int32_t pad[11];
+} ck_lz_ctrl;
+/* followed by fp32 errors[max_iters + 2] (lanczos.py:256-261 values). */
+size_t ck_lanczos_state_bytes(int32_t nb, int32_t max_iters);
+size_t ck_lanczos_scratch_bytes(int64_t n, int32_t nb, int32_t max_iters);
+
+/* V[:,0] = 0; V[:,1] = s / ||s||; diag = subdiag = 0; i = 1 (lanczos.py:275-284). */
+int ck_lanczos_start(int64_t n, int32_t nb, const float* s, int64_t lds, float* V, int64_t ldv, int64_t ldb,
+                     float* diag, float* subdiag, void* state, int32_t max_iters, void* scratch, void* stream);
+/* Step `step` (0-based; uses column i = step+1), before the matvec:
+ * V[:,i] /= ||V[:,i]|| (lanczos.py:240-241; for i > 1 the norm is subdiag[i-1]). */
+int ck_lanczos_pre(int64_t n, int32_t nb, int32_t max_iters, float* V, int64_t ldv, int64_t ldb,
+                   const float* subdiag, void* state, int32_t step, void* scratch, void* stream);
+/* After the matvec wrote w = A v_i into column i+1: alpha_i = w.v_i; w -= alpha v_i +
+ * beta_{i-1} v_{i-1}; w -= V(V^T w) twice over columns 1..i; beta_i = ||w||;
+ * loop condition (lanczos.py:242-268) -> done. */
+int ck_lanczos_post(int64_t n, int32_t nb, int32_t max_iters, float* V, int64_t ldv, int64_t ldb, float* diag,
+                    float* subdiag, void* state, float tol, int32_t step, void* scratch, void* stream);
+/* `nsteps` whole steps from `first_step` with library operator A. */
+int ck_lanczos_iterate(const ck_op* A, int64_t n, int32_t nb, int32_t max_iters, float* V, int64_t ldv, int64_t ldb,
+                       float* diag, float* subdiag, void* state, float tol, int32_t first_step, int32_t nsteps,
+                       void* scratch, void* work, size_t work_bytes, void* stream);
+
+/* ---------------------------------------------------------------- Arnoldi
+ * Arnoldi with Gram-Schmidt against columns 0..idx (cola/linalg/decompositions/
+ * arnoldi.py:289-324).  Q[b][j][i] at Q + b*ldb + j*ldv + i, j = 0..max_iters;
+ * H[b][r][c] at H + b*(max_iters+1)*max_iters + r*max_iters + c (fp32).
+ * mgs = 1 reproduces the reference's modified Gram-Schmidt order (one dependent
+ * dot/axpy pair per column); mgs = 0 uses two classical passes (CGS2). */
+typedef struct ck_ar_ctrl {
+  int32_t idx;
+  int32_t done;
+  int32_t max_iters;
+  int32_t nb;
+  int32_t nparts;
+  int32_t pad[11];
+} ck_ar_ctrl;
+size_t ck_arnoldi_state_bytes(int32_t nb, int32_t max_iters);
+size_t ck_arnoldi_scratch_bytes(int64_t n, int32_t nb, int32_t max_iters);
+int ck_arnoldi_start(int64_t n, int32_t nb, const float* s, int64_t lds, float* Q, int64_t ldv, int64_t ldb, float* H,
+                     void* state, int32_t max_iters, void* scratch, void* stream);
+/* After the matvec wrote w = A q_idx into column idx+1 (idx = step): Gram-Schmidt
+ * against columns 0..idx into H[:, idx], H[idx+1, idx] = ||w||, w /= max(||w||, tol/2),
+ * loop condition idx+1 < max_iters && any(||w|| > tol * H[1,0]) (arnoldi.py:293-319). */
+int ck_arnoldi_post(int64_t n, int32_t nb, int32_t max_iters, float* Q, int64_t ldv, int64_t ldb, float* H,
+                    void* state, float tol, int32_t step, int32_t mgs, void* scratch, void* stream);
+int ck_arnoldi_iterate(const ck_op* A, int64_t n, int32_t nb, int32_t max_iters, float* Q, int64_t ldv, int64_t ldb,
+                       float* H, void* state, float tol, int32_t first_step, int32_t nsteps, int32_t mgs,
+                       void* scratch, void* work, size_t work_bytes, void* stream);
+
+/* ---------------------------------------------------------------- small FP64 eigensolves
+ * Symmetric tridiagonal eigensolve by implicit QL with Wilkinson shifts
+ * (Bowdler, Martin, Reinsch & Wilkinson, Numer. Math. 11 (1968); EISPACK tql2),
+ * one workgroup per problem.  Replaces xnp.eigh(T.to_dense()) of
+ * lanczos.py:57, unary.py:52, slq.py:44, in FP64.
+ * Problem b has diagonal d[b*ldd + 0..m-1] and off-diagonal e[b*lde + 0..m-2]
+ * (fp32 inputs).  Outputs: ascending evals[b*m + c]; if evecs != NULL the
+ * eigenvectors evecs[b*m*m + j*m + c] (component j of vector c); if first_row != NULL
+ * the first components first_row[b*m + c]. status_out[b] = CK_OK or CK_ERR_NOCONV. */
+int ck_tridiag_eigh_f64(int32_t nb, int32_t m, const float* d, int64_t ldd, const float* e, int64_t lde,
+                        double* evals, double* evecs, double* first_row, int32_t* status_out, void* stream);
+
+/* out[c][i] = sum_j S[j][c] V[j][i] for c < k: Ritz vectors from a basis block
+ * (lanczos.py:59, arnoldi.py:61).  V rows j = 0..m-1 at V + j*ldv; S (m x k) fp64
+ * row-major with leading dimension lds; out vector c at out + c*ldo. */
+int ck_basis_combine(int64_t n, int32_t m, int32_t k, const float* V, int64_t ldv, const double* S, int64_t lds,
+                     float* out, int64_t ldo, void* stream);
+
+/* out[b][i] = sum_j C[b][j] V[b][j][i] for j < k: one vector per probe from its own
+ * basis block (f(A) z = Q_b c_b in cola/linalg/unary/unary.py:58-59).  C (nb x k) fp64. */
+int ck_basis_combine_batched(int64_t n, int32_t nb, int32_t k, const float* V, int64_t ldv, int64_t ldb,
+                             const double* C, int64_t ldc, float* out, int64_t ldo, void* stream);
+
+/* ---------------------------------------------------------------- quadrature / trace
+ * Stochastic Lanczos quadrature: per probe b, q_b = scale_b * sum_c tau_bc^2 f(theta_bc)
+ * with f(theta) = 0 where |theta| <= 10*eps*max|theta| (cola/linalg/unary/unary.py:55-57,
+ * cola/linalg/tbd/slq.py:46-51).  f_kind: 0 log, 1 identity, 2 exp, 3 sqrt, 4 reciprocal.
+ * eps is the machine epsilon the threshold uses; max_mode 0 = max|theta|, 1 = max theta. */
+int ck_slq_quadrature(int32_t nb, int32_t m, const double* evals, const double* first_row, const double* scale,
+                      int32_t f_kind, double eps, int32_t max_mode, double* out, void* stream);
+
+/* Hutchinson accumulation (cola/linalg/trace/diagonal_estimation.py:191-195, k = 0):
+ * sum[i] += sum_b Z[b,i] AZ[b,i]; sumsq[i] += sum_b (Z[b,i] AZ[b,i])^2. */
+int ck_hutch_accumulate(int64_t n, int32_t nprobes, const float* Z, int64_t ldz, const float* AZ, int64_t ldaz,
+                        float* sum, float* sumsq, void* stream);
+/* Fused CSR Hutchinson on packed Rademacher probes W (ck_rng_rademacher_bits):
+ * one wavefront per row, lane = probe, so a probe block is never materialised;
+ * the same sum/sumsq update as ck_hutch_accumulate for the 64*ngroups probes,
+ * including the op's scale and shift. */
+int ck_hutch_csr_bits(const ck_op* A, int32_t ngroups, const uint64_t* W, float* sum, float* sumsq, void* stream);
+/* Trace-only CSR estimate from packed Rademacher words W (ck_rng_rademacher_bits):
+ * out = (1/(64*ngroups)) sum_i sum_k val_k * sum_g (64 - 2 popcount(W[g,i] ^ W[g,col_k]))
+ *       + shift * n, fp64.  work >= ck_trace_workspace_bytes(n). */
+int ck_trace_csr_rademacher_bits(const ck_op* A, int32_t ngroups, const uint64_t* W, double* out, void* work,
+                                 size_t work_bytes, void* stream);
+size_t ck_trace_workspace_bytes(int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* COLA_KRYLOV_H */

@@ -1,0 +1,99 @@
+"""The C ABI library loads and exports every function include/cola_krylov.h declares;
+host-side size queries and status strings work without a GPU."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+import cola_amd._native as N
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "cola_krylov.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\**\s+\**(ck_[a-z_0-9]+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(N.lib_path()):
+        pytest.fail("libcolakrylov.so is not built (run python -m cola_amd.csrc.build)")
+    return N.load()
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for must in ["ck_op_apply", "ck_cg_iterate", "ck_lanczos_iterate", "ck_arnoldi_post", "ck_tridiag_eigh_f64",
+                 "ck_hutch_csr_bits", "ck_slq_quadrature", "ck_rng_rademacher_bits"]:
+        assert must in names
+    assert len(names) >= 39
+
+
+def test_every_declared_symbol_is_exported(lib):
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_every_bound_prototype_exists(lib):
+    for name in N._PROTOS:
+        assert hasattr(lib, name), name
+
+
+def test_version_and_status(lib):
+    assert N.version().startswith("cola_krylov")
+    assert lib.ck_status_string(0) == b"ok"
+    assert b"workspace" in lib.ck_status_string(4)
+    assert lib.ck_status_string(99) == b"unknown status"
+
+
+def test_size_queries(lib):
+    assert N.q("ck_cg_stride", 1) == 64 and N.q("ck_cg_stride", 65) == 128
+    assert N.q("ck_cg_state_bytes", 16, 1000) == 64 + 4 * (6 * 64 + 1002)
+    assert N.q("ck_cg_nparts", 262144) == 64
+    assert N.q("ck_cg_nparts", 1 << 30) == 1024
+    assert N.q("ck_cg_partials_bytes", 262144, 16) == 3 * 4 * 16 * 64
+    assert N.q("ck_lanczos_state_bytes", 1, 120) == 64 + 4 * 122
+    assert N.q("ck_trace_workspace_bytes", 1000) == 8 * 4
+    assert N.q("ck_dot_workspace_bytes", 10, 3) == 3 * 256 * 8
+
+
+def test_op_workspace_queries(lib):
+    op = N.CkOp()
+    op.kind, op.n, op.nfactors = N.CK_OP_KRON_F32, 64**3, 3
+    assert N.q("ck_op_workspace_bytes", op, 16) == 16 * 64**3 * 4
+    op2 = N.CkOp()
+    op2.kind, op2.n, op2.rank = N.CK_OP_WOODBURY_INV, 16384, 64
+    assert N.q("ck_op_workspace_bytes", op2, 8) == 4 * (8 * 64 * 16 + 8 * 64)
+
+
+def test_bad_arguments_rejected_without_launch(lib):
+    # null pointers / bad shapes return CK_ERR_BAD_ARG before any device work
+    assert lib.ck_cg_start(0, 1, None, None, None, 1, None, ctypes.c_float(1e-6), 10, None, None) == 1
+    assert lib.ck_tridiag_eigh_f64(0, 5, None, 5, None, 4, None, None, None, None, None) == 1
+    assert lib.ck_op_apply(None, 1, None, 1, None, 1, None, 0, None) == 1
+    op = N.CkOp()
+    op.kind, op.n = 99, 4
+    buf = (ctypes.c_float * 4)()
+    assert lib.ck_op_apply(ctypes.byref(op), 1, buf, 4, buf, 4, None, 0, None) == N.CK_ERR_UNSUPPORTED
+
+
+def test_struct_layout_matches_header(tmp_path):
+    """ctypes' ck_op layout equals what the C compiler makes of include/cola_krylov.h."""
+    fields = [f[0] for f in N.CkOp._fields_]
+    src = tmp_path / "layout.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {",
+             '  printf("%zu\\n", sizeof(ck_op));']
+    lines += [f'  printf("%zu\\n", offsetof(ck_op, {f}));' for f in fields]
+    lines += ["  return 0;", "}"]
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    out = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(N.CkOp)
+    assert out[1:] == [getattr(N.CkOp, f).offset for f in fields]

@@ -1,0 +1,125 @@
+"""Host logic that needs no GPU: the dispatch registry, annotations, operator
+algebra and lowering decisions, and the engine's refusal of CPU tensors."""
+import numpy as np
+import pytest
+import torch
+
+import cola_amd as ca
+from cola_amd.dispatch import dispatch
+from cola_amd.linalg import Auto, HipCG, HipHutch, HipLanczos, inv
+from cola_amd.linalg.algorithm_base import Algorithm, IterativeOperatorWInfo
+
+
+class _Base:
+    pass
+
+
+class _Child(_Base):
+    pass
+
+
+@dispatch.abstract
+def _route(a: _Base, b: int = 0):
+    pass
+
+
+@dispatch(precedence=-1)
+def _route(a: _Base, b: int):
+    return "base"
+
+
+@dispatch
+def _route(a: _Child, b: int):
+    return "child"
+
+
+@dispatch(cond=lambda a, b: b > 10)
+def _route(a: _Base, b: int):
+    return "cond"
+
+
+def test_dispatch_specificity_precedence_cond():
+    assert _route(_Base(), 1) == "base"
+    assert _route(_Child(), 1) == "child"
+    assert _route(_Base(), 20) == "cond"  # equal signatures: precedence 0 beats -1
+    assert _route(_Base()) == "base"  # default argument from the abstract signature
+    with pytest.raises(NotImplementedError):
+        _route("x", 1)
+
+
+def test_annotations_inference():
+    A = ca.PSD(ca.Dense(torch.eye(3)))
+    assert A.isa(ca.PSD) and A.isa(ca.SelfAdjoint)
+    B = ca.Dense(torch.eye(3))
+    assert not B.isa(ca.PSD)
+    K = ca.Kronecker(A, A)
+    assert K.isa(ca.PSD)
+    S = K + 0.1 * ca.I_like(K)
+    assert S.isa(ca.PSD) and not S.isa(ca.Unitary)
+    assert ca.I_like(B).isa(ca.Unitary)
+
+
+def test_operator_algebra_cpu():
+    g = torch.Generator().manual_seed(0)
+    A = torch.randn(6, 6, generator=g)
+    B = torch.randn(6, 6, generator=g)
+    X = torch.randn(6, 2, generator=g)
+    opA, opB = ca.Dense(A), ca.Dense(B)
+    assert torch.allclose((opA + opB) @ X, (A + B) @ X, atol=1e-6)
+    assert torch.allclose((opA @ opB) @ X, A @ B @ X, atol=1e-5)
+    assert torch.allclose((2.0 * opA) @ X, 2 * A @ X, atol=1e-6)
+    assert torch.allclose((opA - opB) @ X, (A - B) @ X, atol=1e-6)
+    K = ca.kron(ca.Dense(A[:2, :2]), ca.Dense(B[:3, :3]))
+    assert torch.allclose(K.to_dense(), torch.kron(A[:2, :2], B[:3, :3]), atol=1e-6)
+    x1 = torch.randn(6, generator=g)
+    assert (opA @ x1).shape == (6, )
+
+
+def test_sparse_cpu_and_validation():
+    data = torch.tensor([4., 5., 6., 1., 2., 3.])
+    rows = torch.tensor([2, 2, 2, 0, 0, 1])
+    cols = torch.tensor([0, 1, 2, 1, 3, 3])
+    A = ca.Sparse(data, rows, cols, (3, 4))
+    assert torch.allclose(A @ torch.tensor([1., 0., -1., 1.]), torch.tensor([2., 3., -2.]))
+    assert A.indptr.tolist() == [0, 2, 3, 6] and A.indices.dtype == torch.int32
+    with pytest.raises(ValueError):
+        ca.Sparse(data, rows, cols + 10, (3, 4))
+    with pytest.raises(ValueError):
+        ca.Sparse.from_csr(torch.tensor([0, 2, 1, 6]), A.indices, A.data, (3, 4))
+
+
+def test_lowering_needs_device_tensors():
+    A = ca.Dense(torch.eye(4))
+    assert A.hip_op() is None  # CPU storage never lowers to a kernel
+    assert ca.Sparse(torch.ones(2), torch.tensor([0, 1]), torch.tensor([0, 1]), (2, 2)).hip_op() is None
+
+
+def test_inv_routing_and_cpu_refusal():
+    A = ca.PSD(ca.Dense(torch.eye(4) * 2))
+    Ainv = inv(A, HipCG())
+    assert isinstance(Ainv, IterativeOperatorWInfo)
+    assert isinstance(inv(A, Auto(tol=1e-3)).alg, HipCG) and inv(A, Auto(tol=1e-3)).alg.tol == 1e-3
+    with pytest.raises(ValueError, match="HIP"):
+        Ainv @ torch.ones(4)
+    with pytest.raises(AssertionError):
+        inv(ca.Dense(torch.eye(4)), HipCG())
+    # structural rules win over the algorithm base case (inv.py:139-146)
+    D = ca.Diagonal(torch.tensor([2., 4.]))
+    assert torch.allclose(inv(D, HipCG()).diag, torch.tensor([0.5, 0.25]))
+    K = ca.Kronecker(ca.Diagonal(torch.tensor([2., 4.])), ca.Diagonal(torch.tensor([1., 2.])))
+    assert isinstance(inv(K, HipCG()), ca.Kronecker)
+
+
+def test_algorithm_fields_match_reference_defaults():
+    cg = HipCG()
+    assert (cg.tol, cg.max_iters, cg.pbar, cg.x0, cg.P) == (1e-6, 1000, False, None, None)
+    lz = HipLanczos()
+    assert (lz.start_vector, lz.max_iters, lz.tol, lz.key) == (None, 1000, 1e-6, None)
+    h = HipHutch()
+    assert (h.tol, h.max_iters, h.bs, h.rand) == (3e-2, 10_000, 100, "normal")
+    assert issubclass(HipCG, Algorithm) and ca.linalg.CG is HipCG and ca.linalg.Lanczos is HipLanczos
+
+
+def test_keys_match_reference_chain():
+    from cola_amd.linalg.keys import PRNGKey, next_key
+    assert PRNGKey(42) == 3300923923 and next_key(3300923923) == 1419105812

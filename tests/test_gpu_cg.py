@@ -1,0 +1,141 @@
+"""HipCG against the oracle's restatement of cola's run_batched_cg (cg.py:94-178)
+and against FP64 closed forms.  Tolerances: FP32 iterates agree with the FP32
+oracle to rounding; iteration counts may differ by reduction order (+-2 steps)."""
+import numpy as np
+import pytest
+import torch
+
+import cola_amd as ca
+from cola_amd.linalg import HipCG, inv, run_batched_cg
+from oracle import cg as ocg
+from oracle import operators as oop
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = a.double().cpu()
+    b = b.double().cpu()
+    return float((a - b).norm() / max(b.norm(), 1e-30))
+
+
+def spd_from_spectrum(diag, seed):
+    rs = np.random.RandomState(seed)
+    Q, _ = np.linalg.qr(rs.normal(size=(len(diag), len(diag))))
+    return torch.from_numpy(Q.T @ np.diag(diag) @ Q).float()
+
+
+def test_cg_diag_closed_form(dev):
+    """tests/algorithms/test_cg.py:170-190: diag(3,4,5) x = 1 -> x = (1/3, 1/4, 1/5)."""
+    A = ca.PSD(ca.Diagonal(torch.tensor([3., 4., 5.], device=dev)))
+    x = inv(A, HipCG(tol=1e-7, max_iters=5)) @ torch.ones(3, device=dev)
+    assert rel(x, torch.tensor([1 / 3, 1 / 4, 1 / 5])) < 1e-6
+
+
+def test_cg_repeated_eigenvalues(dev):
+    """tests/algorithms/test_cg.py:148-167: three distinct eigenvalues -> k < 7."""
+    diag = np.array([1.] * 10 + [0.5] * 10 + [0.25] * 10)
+    A = spd_from_spectrum(diag, 0)
+    b = torch.ones(30, 1)
+    x, r, k, info = run_batched_cg(ca.Dense(A.to(dev)), b.to(dev), None, 100, 1e-11, None)
+    assert k < 7
+    assert rel(x, torch.linalg.solve(A.double(), b.double())) < 1e-5
+
+
+def test_cg_config1_dense(dev):
+    """A = X X^T/256 + 0.1 I, N=1024, X 1024x256 N(0,1) (seed 0), one RHS, tol 1e-6."""
+    g = torch.Generator().manual_seed(0)
+    X = torch.randn(1024, 256, generator=g)
+    b = torch.randn(1024, generator=g)
+    A = X @ X.T / 256 + 0.1 * torch.eye(1024)
+    xo, io = ocg.cg(lambda V: A @ V, b[:, None], tol=1e-6, max_iters=1000)
+    op = ca.PSD(ca.Dense(A.to(dev)))
+    solver = inv(op, HipCG(tol=1e-6, max_iters=1000))
+    x = solver @ b.to(dev)
+    info = solver.info
+    print("iters gpu", info["iterations"], "oracle", io["iterations"], flush=True)
+    assert abs(info["iterations"] - io["iterations"]) <= 2
+    assert rel(x, xo[:, 0]) < 1e-5
+    xd = torch.linalg.solve(A.double(), b.double())
+    assert rel(x, xd) < 1e-5
+    # the reference stops on the recursive residual of the normalised rhs (<= 2*tol);
+    # the true FP64 residual of its FP32 iterate is the bar (3.42e-6 for the oracle here)
+    res = (A.double() @ x.double().cpu() - b.double()).norm() / b.double().norm()
+    res_o = (A.double() @ xo[:, 0].double() - b.double()).norm() / b.double().norm()
+    assert res < 1.25 * res_o and res < 1e-5
+    n = min(len(info["errors"]), len(io["errors"]))
+    assert np.allclose(info["errors"][:n], io["errors"][:n], rtol=1e-2, atol=1e-7)
+
+
+@pytest.mark.parametrize("n,B", [(100, 5), (777, 3), (2048, 16)])
+def test_cg_batched_matches_oracle(dev, n, B):
+    g = torch.Generator().manual_seed(n + B)
+    M = torch.randn(n, n // 2, generator=g)
+    A = M @ M.T / n + 0.05 * torch.eye(n)
+    b = torch.randn(n, B, generator=g)
+    xo, ro, ko, io = ocg.run_batched_cg(lambda V: A @ V, b, None, 500, 1e-6, None)
+    x, r, k, info = run_batched_cg(ca.Dense(A.to(dev)), b.to(dev), None, 500, 1e-6, None)
+    assert abs(k - ko) <= 2
+    assert rel(x, xo) < 1e-5
+    assert rel(x, torch.linalg.solve(A.double(), b.double())) < 2e-5
+
+
+def test_cg_kronecker(dev):
+    """Kronecker + shift (the 64^3 workload's structure at 16^3) with 4 RHS."""
+    g = torch.Generator().manual_seed(1)
+    facs = []
+    for _ in range(3):
+        t = torch.sort(torch.rand(16, generator=g)).values
+        facs.append(torch.exp(-(t[:, None] - t[None, :])**2 / (2 * 0.1**2)))
+    b = torch.randn(16**3, 4, generator=g)
+
+    def mm(V):
+        return oop.kron_matmat(facs, V) + 0.01 * V
+
+    xo, _, ko, _ = ocg.run_batched_cg(mm, b, None, 1000, 1e-6, None)
+    K = ca.Kronecker(*[ca.Dense(f.to(dev)) for f in facs])
+    op = ca.PSD(K + 0.01 * ca.I_like(K))
+    x, r, k, info = run_batched_cg(op, b.to(dev), None, 1000, 1e-6, None)
+    print("kron iters", k, ko, flush=True)
+    assert abs(k - ko) <= max(2, ko // 20)
+    assert rel(x, xo) < 1e-4
+
+
+def test_cg_generic_operator(dev):
+    """An operator the engine does not recognise goes through A._matmat between kernel launches."""
+    g = torch.Generator().manual_seed(2)
+    M = torch.randn(300, 300, generator=g)
+    A = (M @ M.T / 300 + 0.5 * torch.eye(300)).to(dev)
+    op = ca.PSD(ca.LinearOperator(torch.float32, (300, 300), matmat=lambda V: A @ V, device=dev))
+    b = torch.randn(300, 2, generator=g)
+    xo, _, ko, _ = ocg.run_batched_cg(lambda V: A.cpu() @ V, b, None, 500, 1e-6, None)
+    x, r, k, info = run_batched_cg(op, b.to(dev), None, 500, 1e-6, None)
+    assert abs(k - ko) <= 2
+    assert rel(x, xo) < 1e-5
+
+
+def test_cg_preconditioned(dev):
+    g = torch.Generator().manual_seed(3)
+    n = 500
+    d = torch.logspace(-2, 2, n)
+    A = spd_from_spectrum(d.numpy(), 4)
+    Pinv = torch.diag(1 / torch.diag(A))
+    b = torch.randn(n, 3, generator=g)
+    xo, _, ko, _ = ocg.run_batched_cg(lambda V: A @ V, b, None, 1000, 1e-6, lambda V: Pinv @ V)
+    P = ca.Diagonal((1 / torch.diag(A)).to(dev))
+    x, r, k, info = run_batched_cg(ca.Dense(A.to(dev)), b.to(dev), None, 1000, 1e-6, P)
+    # kappa = 1e4: FP32 rounding order moves the step count by ~1% on either side
+    assert abs(k - ko) <= max(3, ko // 50)
+    xd = torch.linalg.solve(A.double(), b.double())
+    assert rel(x, xd) < 3 * rel(xo, xd) + 1e-6
+
+
+def test_cg_zero_rhs_and_maxiter(dev):
+    A = spd_from_spectrum(np.linspace(1, 2, 50), 5)
+    b = torch.randn(50, 3)
+    b[:, 1] = 0.0
+    x, r, k, info = run_batched_cg(ca.Dense(A.to(dev)), b.to(dev), None, 100, 1e-6, None)
+    assert torch.all(x[:, 1] == 0)
+    assert rel(x[:, 0], torch.linalg.solve(A.double(), b[:, 0].double())) < 1e-5
+    x, r, k, info = run_batched_cg(ca.Dense(A.to(dev)), b.to(dev), None, 1, 1e-6, None)
+    assert k == 1 and info["iterations"] == 2

@@ -1,0 +1,130 @@
+"""Operator kernels (dense fp32/bf16, CSR, Kronecker on MFMA, diagonal, Woodbury,
+on-the-fly RBF) against the oracle's restatement of the reference matvecs."""
+import numpy as np
+import pytest
+import torch
+
+import cola_amd as ca
+from oracle import operators as oop
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    a = a.double().cpu()
+    b = b.double().cpu()
+    return float((a - b).norm() / max(b.norm(), 1e-30))
+
+
+@pytest.mark.parametrize("n,B", [(1, 1), (7, 3), (64, 1), (1024, 1), (1000, 5), (4096, 16), (513, 17)])
+def test_dense_matmat(dev, n, B):
+    print(f"dense n={n} B={B}", flush=True)
+    g = torch.Generator().manual_seed(n * 31 + B)
+    A = torch.randn(n, n, generator=g)
+    X = torch.randn(n, B, generator=g)
+    ref = oop.dense_matmat(A.double(), X.double())
+    out = ca.Dense(A.to(dev)) @ X.to(dev)
+    assert rel(out, ref) < 2e-6
+
+
+def test_dense_shift_scale(dev):
+    g = torch.Generator().manual_seed(3)
+    A = torch.randn(300, 300, generator=g)
+    X = torch.randn(300, 4, generator=g)
+    op = 2.5 * ca.Dense(A.to(dev)) + 0.1 * ca.I_like(ca.Dense(A.to(dev)))
+    ref = oop.shifted(lambda V: 2.5 * (A.double() @ V), 0.1)(X.double())
+    assert rel(op @ X.to(dev), ref) < 2e-6
+
+
+def test_dense_bf16(dev):
+    g = torch.Generator().manual_seed(4)
+    A = torch.randn(512, 512, generator=g)
+    X = torch.randn(512, 6, generator=g)
+    Ab = A.to(torch.bfloat16).to(torch.float32)
+    out = ca.DenseBF16(A.to(dev)) @ X.to(dev)
+    assert rel(out, Ab.double() @ X.double()) < 2e-6
+
+
+def test_csr_known_answer(dev):
+    """tests/test_operators.py:170-204 (3x4 matrix) -- square 4x4 padding of the same data."""
+    data = torch.tensor([4., 5., 6., 1., 2., 3.])
+    rows = torch.tensor([2, 2, 2, 0, 0, 1])
+    cols = torch.tensor([0, 1, 2, 1, 3, 3])
+    A = ca.Sparse(data.to(dev), rows.to(dev), cols.to(dev), (4, 4))
+    x1 = torch.tensor([1., 0., -1., 1.], device=dev)
+    out = A @ x1
+    assert torch.allclose(out[:3].cpu(), torch.tensor([2., 3., -2.]))
+
+
+@pytest.mark.parametrize("m,B", [(5, 1), (33, 3), (128, 2)])
+def test_csr_laplacian(dev, m, B):
+    ip, ix, v = oop.laplacian_2d_csr(m, 0.0)
+    g = torch.Generator().manual_seed(m)
+    X = torch.randn(m * m, B, generator=g)
+    ref = oop.csr_matmat(ip, ix, torch.from_numpy(v).double(), X.double())
+    A = ca.Sparse.from_csr(torch.from_numpy(ip).to(dev), torch.from_numpy(ix).to(dev), torch.from_numpy(v).to(dev),
+                           (m * m, m * m))
+    op = A + 1e-3 * ca.I_like(A)
+    out = op @ X.to(dev)
+    assert rel(out, ref + 1e-3 * X.double()) < 2e-6
+
+
+def test_csr_long_rows(dev):
+    g = torch.Generator().manual_seed(9)
+    n = 300
+    D = torch.randn(n, n, generator=g) * (torch.rand(n, n, generator=g) < 0.3)
+    r, c = D.nonzero(as_tuple=True)
+    A = ca.Sparse(D[r, c].to(dev), r.to(dev), c.to(dev), (n, n))
+    X = torch.randn(n, 2, generator=g)
+    assert rel(A @ X.to(dev), D.double() @ X.double()) < 2e-6
+
+
+@pytest.mark.parametrize("dims,B", [((4, 5, 6), 1), ((64, 64, 64), 2), ((8, 16), 3), ((17, 33, 9), 4),
+                                    ((64, 64), 1), ((3, 64, 20), 16)])
+def test_kron_matmat(dev, dims, B):
+    print(f"kron dims={dims} B={B}", flush=True)
+    g = torch.Generator().manual_seed(sum(dims) + B)
+    facs = [torch.randn(d, d, generator=g) for d in dims]
+    n = int(np.prod(dims))
+    X = torch.randn(n, B, generator=g)
+    ref = oop.kron_matmat([f.double() for f in facs], X.double())
+    K = ca.Kronecker(*[ca.Dense(f.to(dev)) for f in facs])
+    op = K + 0.01 * ca.I_like(K)
+    out = op @ X.to(dev)
+    assert rel(out, ref + 0.01 * X.double()) < 5e-6
+
+
+def test_kron_vs_torch_kron(dev):
+    """tests/test_operators.py:527-541 pattern (square factors): dense form equals torch.kron."""
+    A = torch.tensor([[1., 2., 7], [3., 4., 5], [0., 1., 2.]])
+    B = torch.tensor([[-1, 4.], [3., 3.]])
+    K = ca.Kronecker(ca.Dense(A.to(dev)), ca.Dense(B.to(dev)))
+    Id = torch.eye(6, device=dev)
+    assert rel(K @ Id, torch.kron(A, B)) < 1e-6
+
+
+def test_diag_identity(dev):
+    d = torch.arange(1., 11.)
+    X = torch.randn(10, 3)
+    assert rel(ca.Diagonal(d.to(dev)) @ X.to(dev), d[:, None] * X) < 1e-7
+
+
+def test_woodbury_inverse(dev):
+    g = torch.Generator().manual_seed(5)
+    n, k, s = 2000, 64, 0.05
+    F = torch.randn(n, k, generator=g) / 8
+    X = torch.randn(n, 8, generator=g)
+    Pm = s * torch.eye(n, dtype=torch.float64) + F.double() @ F.double().T
+    ref = torch.linalg.solve(Pm, X.double())
+    out = ca.WoodburyInverse(F.to(dev), s) @ X.to(dev)
+    assert rel(out, ref) < 1e-4
+
+
+def test_rbf_on_the_fly(dev):
+    g = torch.Generator().manual_seed(6)
+    P = torch.rand(700, 4, generator=g)
+    X = torch.randn(700, 3, generator=g)
+    K = torch.exp(-torch.cdist(P.double(), P.double())**2 / (2 * 0.3**2))
+    op = ca.RBFKernel(P.to(dev), 0.3)
+    out = (op + 0.01 * ca.I_like(op)) @ X.to(dev)
+    assert rel(out, K @ X.double() + 0.01 * X.double()) < 2e-5

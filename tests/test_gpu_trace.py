@@ -1,0 +1,152 @@
+"""Probe streams, Hutchinson and stochastic Lanczos quadrature against the oracle
+(diagonal_estimation.py:158-210, slq.py:37-75, unary.py:46-60) and closed forms."""
+import ctypes
+
+import numpy as np
+import pytest
+import scipy.linalg
+import torch
+
+import cola_amd as ca
+import cola_amd._native as N
+from cola_amd.linalg import HipHutch, HipLanczos, HipSLQ, diag, hutch_trace_csr, logdet
+from cola_amd.linalg._engine import probes
+from cola_amd.linalg.slq import slq_quadratures
+from cola_amd.linalg.unary import LanczosUnary
+from oracle import hutch as ohu
+from oracle import operators as oop
+from oracle import rng as orng
+from oracle import unary as oun
+
+pytestmark = pytest.mark.gpu
+
+
+def spd(diag, seed):
+    rs = np.random.RandomState(seed)
+    Q, _ = np.linalg.qr(rs.normal(size=(len(diag), len(diag))))
+    return torch.from_numpy(Q.T @ np.diag(diag) @ Q).float()
+
+
+@pytest.mark.parametrize("n,B,seed", [(1000, 64, 0), (777, 100, 3300923923), (5, 130, 2**63 + 5)])
+def test_probe_streams_bitexact(dev, n, B, seed):
+    Z, _ = probes("rademacher", n, B, seed, dev)
+    assert torch.equal(Z[:, :n].cpu(), orng.rademacher(seed, n, B))
+    G, _ = probes("normal", n, B, seed, dev)
+    assert torch.allclose(G[:, :n].cpu().double(), orng.gaussian(seed, n, B, torch.float64), atol=2e-5)
+    W = torch.empty(((B + 63) // 64, n), dtype=torch.int64, device=dev)
+    N.call("ck_rng_rademacher_bits", n, (B + 63) // 64, ctypes.c_uint64(seed), N.ptr(W), N.stream_ptr(dev))
+    assert np.array_equal(W.cpu().numpy().view(np.uint64), orng.rademacher_bits(seed, n, B))
+
+
+def test_hutch_hadamard_exact(dev):
+    """Rows of a Hadamard matrix satisfy sum_b z_b z_b^T = n I: the estimate is exact."""
+    n = 64
+    H = torch.from_numpy(scipy.linalg.hadamard(n)).float()
+    A = torch.randn(n, n, generator=torch.Generator().manual_seed(1))
+    Z = H.to(dev).contiguous()  # (B=n, n) probe block
+    AZ = (A.to(dev) @ Z.T).T.contiguous()
+    s = torch.zeros(n, device=dev)
+    s2 = torch.zeros(n, device=dev)
+    N.call("ck_hutch_accumulate", n, n, N.ptr(Z), n, N.ptr(AZ), n, N.ptr(s), N.ptr(s2), N.stream_ptr(dev))
+    assert torch.allclose((s / n).cpu(), torch.diag(A), atol=1e-4)
+
+
+def test_hutch_csr_packed_vs_oracle(dev):
+    m = 64
+    ip, ix, v = oop.laplacian_2d_csr(m, 0.0)
+    n = m * m
+    A = ca.Sparse.from_csr(torch.from_numpy(ip).to(dev), torch.from_numpy(ix).to(dev), torch.from_numpy(v).to(dev),
+                           (n, n))
+    op = A + 1e-3 * ca.I_like(A)
+    tr = float(hutch_trace_csr(op, 128, seed=7))
+    Z = orng.rademacher(7, n, 128, torch.float64).T
+    mm = oop.shifted(lambda X: oop.csr_matmat(ip, ix, torch.from_numpy(v).double(), X), 1e-3)
+    ref = float(ohu.trace_from_probes(mm, Z))
+    assert abs(tr - ref) <= 1e-9 * abs(ref)
+    # exact trace 4.001 * n; Rademacher error is O(sqrt(2 * sum offdiag^2 / B))
+    assert abs(tr - 4.001 * n) < 6 * np.sqrt(2 * 4 * n / 128)
+    # per-row sums and sums of squares of the fused kernel
+    W = torch.empty((2, n), dtype=torch.int64, device=dev)
+    N.call("ck_rng_rademacher_bits", n, 2, ctypes.c_uint64(7), N.ptr(W), N.stream_ptr(dev))
+    s = torch.zeros(n, device=dev)
+    s2 = torch.zeros(n, device=dev)
+    N.call("ck_hutch_csr_bits", op.hip_op()[0], 2, N.ptr(W), N.ptr(s), N.ptr(s2), N.stream_ptr(dev))
+    ds, dq = ohu.hutch_with_probes(mm, Z)
+    assert torch.allclose(s.cpu().double(), ds, rtol=1e-5, atol=1e-4)
+    assert torch.allclose(s2.cpu().double(), dq, rtol=1e-5, atol=1e-3)
+
+
+def test_hipHutch_matches_oracle_with_same_probes(dev):
+    n = 300
+    A = spd(np.linspace(0.5, 4, n), 11)
+    op = ca.Dense(A.to(dev))
+    d, info = ca.linalg.hutchinson_diag_estimate(op, 0, tol=5e-2, rand="rademacher")
+    probe_fn = lambda key, n_, bs: orng.rademacher(key, n_, bs).T  # noqa: E731
+    do, io = ohu.hutchinson_diag_estimate(lambda V: A @ V, n, k=0, tol=5e-2, rand="rademacher", probe_fn=probe_fn)
+    assert info["iterations"] == io["iterations"]
+    assert torch.allclose(d.cpu(), do, rtol=1e-4, atol=1e-4)
+
+
+def test_lanczos_unary_matches_oracle(dev):
+    n = 60
+    A = spd(np.linspace(0.2, 5, n), 5)
+    V = torch.randn(n, 3, generator=torch.Generator().manual_seed(9))
+    op = ca.SelfAdjoint(ca.Dense(A.to(dev)))
+    got = LanczosUnary(op, torch.log, max_iters=n, tol=1e-7) @ V.to(dev)
+    ref, _ = oun.lanczos_unary_matmat(lambda X: A.double() @ X, n, torch.log, V.double(), n, 1e-7, torch.float64,
+                                      torch.float64)
+    assert float((got.cpu().double() - ref).norm() / ref.norm()) < 2e-4
+
+
+def test_slq_exact_with_unit_probes(dev):
+    """With probes sqrt(n) e_i and full Lanczos, n * mean_i tau^2 log(theta) = sum log(lambda)
+    (tests/algorithms/test_stochastic_lanczos_quad.py:48-64 spectrum)."""
+    n = 10
+    diag = 1 - np.linspace(0, 1, n + 1)[:-1]**0.5 + 0.05
+    A = spd(diag, 4)
+    op = ca.PSD(ca.Dense(A.to(dev)))
+    Z = (np.sqrt(n) * torch.eye(n)).to(dev)
+    sc = torch.full((n, ), float(n), dtype=torch.float64, device=dev)
+    q, _ = slq_quadratures(op, Z, n, 1e-10, 0, None, 1, sc)
+    assert abs(float(q.mean()) - float(np.sum(np.log(diag)))) < 1e-4
+
+
+def test_slq_kron_vs_oracle(dev):
+    """SLQ log-determinant of K1 x K2 x K3 + 0.01 I (16^3) with 32 Rademacher probes x 30
+    steps: engine vs the oracle on identical probes; closed form from factor spectra."""
+    g = torch.Generator().manual_seed(0)
+    facs = []
+    for _ in range(3):
+        t = torch.sort(torch.rand(16, generator=g)).values
+        facs.append(torch.exp(-(t[:, None] - t[None, :])**2 / (2 * 0.1**2)))
+    K = ca.Kronecker(*[ca.Dense(f.to(dev)) for f in facs])
+    op = ca.PSD(K + 0.01 * ca.I_like(K))
+    alg = HipSLQ(num_probes=32, max_iters=30, key=5)
+    est = float(logdet(op, alg))
+    n = 16**3
+    Z = orng.rademacher(5, n, 32, torch.float64).T
+    mm = lambda X: oop.kron_matmat([f.double() for f in facs], X) + 0.01 * X  # noqa: E731
+    from oracle.lanczos import lanczos as olz_l
+    _, al, be, _ = olz_l(mm, n, Z, 30, 1e-6, torch.float64)
+    q, _, _ = oun.slq_quadratures(al, be, (Z**2).sum(0), torch.log)
+    ref = float(q.mean())
+    assert abs(est - ref) < 1e-4 * abs(ref)
+    lam = [torch.linalg.eigvalsh(f.double()) for f in facs]
+    exact = float(torch.log((lam[0][:, None, None] * lam[1][None, :, None] * lam[2][None, None, :]) + 0.01).sum())
+    print("slq vs exact rel err", abs(est - exact) / abs(exact), flush=True)
+    assert abs(est - exact) < 0.05 * abs(exact)
+
+
+def test_logdet_lanczos_hutch(dev):
+    n = 50
+    diag = np.linspace(0.5, 3, n)
+    A = spd(diag, 12)
+    op = ca.PSD(ca.Dense(A.to(dev)))
+    ld = float(logdet(op, HipLanczos(max_iters=n, tol=1e-7), HipHutch(tol=1e-2, rand="rademacher")))
+    assert abs(ld - np.sum(np.log(diag))) < 0.1 * abs(np.sum(np.log(diag)))
+
+
+def test_diag_structural(dev):
+    d = torch.arange(1., 5.)
+    K = ca.Kronecker(ca.Diagonal(d.to(dev)), ca.Dense(torch.diag(d).to(dev)))
+    assert torch.allclose(diag(K, 0).cpu(), torch.kron(d, d))
