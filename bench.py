@@ -79,68 +79,84 @@ def cpu_threads():
 
 
 def bench_kron_cg(args, dev):
+    from cola_amd.linalg.mixed import HipMixedCG, KroneckerEigInverse, MixedSolver
     A, facs, b = kron_workload(dev)
     n, B = b.shape
     bd = b.to(dev)
-    tol = args.tol
-    # warmup (includes first-touch allocation)
+    s = N.stream_ptr(dev)
+
+    # ---- engine: FP64-residual refinement around FP32 PCG (Kronecker eigenbasis preconditioner)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter()
+    P = KroneckerEigInverse([f.to(dev) for f in facs], 1.0, 0.01)
+    alg = HipMixedCG(tol=1e-6, inner_tol=args.inner_tol, P=P, chunk=args.chunk)
+    solver = MixedSolver(A, B, alg)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_setup
     for _ in range(args.warmup):
-        x, r, k, info = run_batched_cg(A, bd, None, 1000, tol, None, chunk=args.chunk)
+        x, info = solver.solve(bd)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        x, r, k, info = run_batched_cg(A, bd, None, 1000, tol, None, chunk=args.chunk)
+        x, info = solver.solve(bd)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
-    # FP64 check of every right-hand side
+    # independent FP64 check of the returned solution (fp32 x as returned to the caller)
     res = (kron_apply_f64(facs, x) - bd.double()).norm(dim=0) / bd.double().norm(dim=0)
-    max_res = float(res.max())
+    res_x64 = float(info["rel_residual_f64"].max())
 
-    # roofline of the Kronecker matvec kernels, timed alone on the solve's block shape
-    op, keep = A.hip_op()
+    # ---- plain FP32 CG (the drop-in CG with the reference's defaults: tol 1e-6, 1000-step cap)
+    run_batched_cg(A, bd, None, 1000, 1e-6, None, chunk=16)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    xp, _, kp, _ = run_batched_cg(A, bd, None, 1000, 1e-6, None, chunk=16)
+    torch.cuda.synchronize()
+    dtp = time.perf_counter() - t1
+    resp = (kron_apply_f64(facs, xp) - bd.double()).norm(dim=0) / bd.double().norm(dim=0)
+    plain = {"solves_per_s": B / dtp, "ms_per_solve_block": dtp * 1e3, "cg_steps": int(kp),
+             "max_fp64_rel_residual": float(resp.max()), "reaches_1e-6": bool(resp.max() <= 1e-6)}
+
+    # ---- roofline: the fused mode-2/3 Kronecker pass (one kernel), timed alone on the solve's block shape
     ld = (n + 63) // 64 * 64
     Xb = torch.randn(B, ld, device=dev)
     Yb = torch.empty_like(Xb)
+    K2, K3 = [f.to(dev).contiguous() for f in facs[1:]]
+    t_a = time_kernel(lambda: N.call("ck_kron_apply_last2_f32", N.ptr(K2), N.ptr(K3), B, N.ptr(Xb), ld, N.ptr(Yb),
+                                      ld, s))
+    flops_a = 2 * 2 * 64 * n * B  # two mode products
+    bytes_a = 2 * 4 * n * B  # read the slab, write the result
+    op, keep = A.hip_op()
     wb = N.q("ck_op_workspace_bytes", op, B)
     work = torch.empty(wb // 4, device=dev)
-    s = N.stream_ptr(dev)
     t_mv = time_kernel(lambda: N.call("ck_op_apply", op, B, N.ptr(Xb), ld, N.ptr(Yb), ld, N.ptr(work), wb, s))
-    flops_mv = 3 * 2 * 64 * n * B  # three mode products
-    bytes_mv = 4 * n * B * 5  # pass A: read X, write U; pass B: read U, read X, write Y
-    # CG vector update kernel (x += a p, r -= a Ap with partial sums), same block shape
-    from cola_amd.linalg.cg import CGState
-    st = CGState(n, B, 10, dev, False)
-    st.p.normal_()
-    st.Ap.normal_()
-    st.r.normal_()
-    N.call("ck_cg_start", n, B, N.ptr(st.r), N.ptr(st.r), N.ptr(st.p), ld, N.ptr(st.state), 1e-30, 10**9,
-           N.ptr(st.partials), s)
-    t_xr = time_kernel(lambda: N.call("ck_cg_step_xr", n, B, N.ptr(st.p), N.ptr(st.Ap), N.ptr(st.x), N.ptr(st.r), ld,
-                                      N.ptr(st.state), N.ptr(st.partials), s))
-    bytes_xr = 4 * n * B * 7  # partial p.Ap (2 reads) + update (4 reads, 2 writes) -- two launches
-    roof = {"bound": "mfma", "kernel": "kron matvec (kron_pass_a + kron_pass_b)",
-            "achieved": flops_mv / t_mv / 1e12, "peak": FP32_MFMA_PEAK, "unit": "TFLOP/s",
-            "frac": flops_mv / t_mv / 1e12 / FP32_MFMA_PEAK, "traffic": None,
-            "launch_us": t_mv * 1e6, "algorithmic_flops_per_launch": flops_mv,
-            "algorithmic_bytes_per_launch": bytes_mv, "hbm_gbs_equiv": bytes_mv / t_mv / 1e9,
-            "cg_update_xr_us": t_xr * 1e6, "cg_update_xr_gbs": bytes_xr / t_xr / 1e9}
+    roof = {"bound": "mfma", "kernel": "kron64_pass_a<float> (fused mode-2/3 Kronecker pass)",
+            "achieved": flops_a / t_a / 1e12, "peak": FP32_MFMA_PEAK, "unit": "TFLOP/s",
+            "frac": flops_a / t_a / 1e12 / FP32_MFMA_PEAK, "traffic": None,
+            "launch_us": t_a * 1e6, "algorithmic_flops_per_launch": flops_a,
+            "algorithmic_bytes_per_launch": bytes_a, "achieved_gbs": bytes_a / t_a / 1e9,
+            "full_kron_matvec_us": t_mv * 1e6, "full_kron_matvec_tflops": 3 * 2 * 64 * n * B / t_mv / 1e12}
 
     out = {"metric": METRIC, "value": B / dt, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded RBF Kronecker factors, N(0,1) rhs)",
-           "config": {"workload": "kron_cg: K1xK2xK3+0.01I, N=64^3=262144, 16 rhs, plain FP32 CG",
-                      "global_batch": B, "seq_len": n, "parallelism": "single GPU", "tol": tol,
-                      "cg_iterations": int(k), "max_fp64_rel_residual": max_res},
+           "vs_baseline": None, "dtype": "f32 (inner CG, MFMA) + f64 (refinement residuals)",
+           "data": "synthetic (seeded RBF Kronecker factors, N(0,1) rhs)",
+           "config": {"workload": "kron_cg: K1xK2xK3+0.01I, N=64^3=262144, 16 rhs, to FP64-checked 1e-6",
+                      "method": "FP64-residual iterative refinement around FP32 PCG, Kronecker eigenbasis preconditioner",
+                      "global_batch": B, "seq_len": n, "parallelism": "single GPU",
+                      "refinements": info["refinements"], "inner_cg_steps": info["inner_iterations"],
+                      "max_fp64_rel_residual_x64": res_x64, "max_fp64_rel_residual_returned_f32_x": float(res.max()),
+                      "setup_s": t_setup, "plain_fp32_cg": plain,
+                      "speedup_vs_plain_per_solve": (dtp / dt)},
            "roofline": roof}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline_kron(facs, b, k, tol)
+        out["cpu_baseline"] = cpu_baseline_kron(facs, b, 1000, 1e-6)
     return out
 
 
-def cpu_baseline_kron(facs, b, k_gpu, tol, iters=20):
-    """The CPU torch restatement of cola's run_batched_cg with Kronecker._matmat, timed
-    for `iters` CG steps on the full 16-rhs system and scaled to the k_gpu steps a
-    solve takes."""
+def cpu_baseline_kron(facs, b, k_steps, tol, iters=20):
+    """The CPU torch restatement of cola's run_batched_cg with Kronecker._matmat (what
+    cola.linalg.inv(PSD(A)) @ B runs), timed for `iters` CG steps on the full 16-rhs
+    system and scaled to the k_steps (the 1000-step default cap) of one call."""
     from oracle import cg as ocg
     from oracle import operators as oop
     torch.set_num_threads(cpu_threads())
@@ -149,10 +165,11 @@ def cpu_baseline_kron(facs, b, k_gpu, tol, iters=20):
     t0 = time.perf_counter()
     _, _, kk, _ = ocg.run_batched_cg(mm, b, None, iters, tol, None)
     dt = (time.perf_counter() - t0) / max(kk, 1)
-    solve_s = dt * k_gpu
+    solve_s = dt * k_steps
     return {"value": b.shape[1] / solve_s, "unit": "solves/s", "cores": cpu_threads(), "kind": "port",
-            "sample": f"{kk} CG steps of the 16-rhs 64^3 Kronecker solve ({dt * 1e3:.1f} ms/step, fp32 torch), "
-                      f"scaled to the {int(k_gpu)} steps of a full solve"}
+            "sample": f"{kk} CG steps of the 16-rhs 64^3 Kronecker system ({dt * 1e3:.1f} ms/step, fp32 torch), "
+                      f"scaled to the {int(k_steps)}-step cap of one reference CG call (which does not reach 1e-6 "
+                      "on this system)"}
 
 
 def main():
@@ -162,7 +179,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="kron_cg")
     ap.add_argument("--tol", type=float, default=1e-6)
-    ap.add_argument("--chunk", type=int, default=16)
+    ap.add_argument("--chunk", type=int, default=8)
+    ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

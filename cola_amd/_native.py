@@ -13,7 +13,7 @@ _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "lib
 
 CK_OK, CK_ERR_BAD_ARG, CK_ERR_HIP, CK_ERR_UNSUPPORTED, CK_ERR_WORKSPACE, CK_ERR_NOCONV = range(6)
 (CK_OP_IDENTITY, CK_OP_DENSE_F32, CK_OP_DENSE_BF16, CK_OP_CSR_F32, CK_OP_KRON_F32, CK_OP_DIAG_F32,
- CK_OP_WOODBURY_INV, CK_OP_RBF_F32) = range(8)
+ CK_OP_WOODBURY_INV, CK_OP_RBF_F32, CK_OP_KRON_EIG_F32) = range(9)
 CK_CG_MULT, CK_CG_TOL, CK_CG_GAMMA, CK_CG_ALPHA, CK_CG_BETA, CK_CG_RNORM, CK_CG_ERRORS = range(7)
 CTRL_BYTES = 64
 
@@ -45,6 +45,7 @@ class CkOp(ctypes.Structure):
         ("nfactors", ctypes.c_int32),
         ("fdim", ctypes.c_int32 * 4),
         ("factor", ctypes.c_void_p * 4),
+        ("factor2", ctypes.c_void_p * 4),
         ("aux", ctypes.c_void_p),
         ("rank", ctypes.c_int32),
         ("dim", ctypes.c_int32),
@@ -63,6 +64,12 @@ _PROTOS = {
     "ck_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "ck_op_workspace_bytes": (_sz, [_op, _i32]),
     "ck_op_apply": (ctypes.c_int, [_op, _i32, _p, _i64, _p, _i64, _p, _sz, _p]),
+    "ck_kron_apply_f64": (ctypes.c_int, [_i32, _p, _p, _p, _p, _f64, _f64, _i32, _p, _i64, _p, _i64, _p, _sz, _p]),
+    "ck_kron_apply_last2_f32": (ctypes.c_int, [_p, _p, _i32, _p, _i64, _p, _i64, _p]),
+    "ck_ir_workspace_bytes": (_sz, [_i32]),
+    "ck_ir_residual_f64": (ctypes.c_int, [_i64, _i32, _p, _i64, _p, _i64, _p, _i64, _p, _i64, _p, _p, _sz, _p]),
+    "ck_ir_update_f64": (ctypes.c_int, [_i64, _i32, _p, _i64, _p, _i64, _p]),
+    "ck_f32_to_bf16": (ctypes.c_int, [_i64, _p, _p, _p]),
     "ck_dot_f32": (ctypes.c_int, [_i64, _i32, _p, _i64, _p, _i64, _p, _p, _sz, _p]),
     "ck_dot_workspace_bytes": (_sz, [_i64, _i32]),
     "ck_transpose_f32": (ctypes.c_int, [_i64, _i32, _p, _i64, _p, _i64, _i32, _p]),

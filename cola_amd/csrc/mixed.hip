@@ -1,0 +1,104 @@
+// Mixed precision: iterative refinement with FP64 residuals around FP32 (or
+// BF16-matrix) inner solves, and bf16 storage conversion.
+//
+//   ck_ir_residual_f64: r = b - Ax in fp64, its fp32 copy for the inner solve,
+//                       and per-column ||r||^2, ||b||^2 partials -> fp64 norms
+//   ck_ir_update_f64:   x += d (fp32 correction accumulated in fp64)
+#include "common.hpp"
+
+using namespace ck;
+
+namespace {
+
+constexpr int kIrParts = 256;
+
+__global__ __launch_bounds__(256) void ir_residual_kernel(int64_t n, const double* __restrict__ b, int64_t ldb,
+                                                          const double* __restrict__ Ax, int64_t ldax,
+                                                          double* __restrict__ r, int64_t ldr,
+                                                          float* __restrict__ r32, int64_t ldr32,
+                                                          double* __restrict__ part) {
+  __shared__ double red[4];
+  const int col = blockIdx.y, blk = blockIdx.x;
+  const int64_t per = cdiv(n, kIrParts);
+  const int64_t i0 = blk * per, i1 = i0 + per < n ? i0 + per : n;
+  double rr = 0.0, bb = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+    const double bv = b[col * ldb + i];
+    const double rv = bv - (Ax ? Ax[col * ldax + i] : 0.0);
+    if (r) r[col * ldr + i] = rv;
+    if (r32) r32[col * ldr32 + i] = (float)rv;
+    rr += rv * rv;
+    bb += bv * bv;
+  }
+  rr = block_sum<256>(rr, red);
+  bb = block_sum<256>(bb, red);
+  if (threadIdx.x == 0) {
+    part[(2 * col) * kIrParts + blk] = rr;
+    part[(2 * col + 1) * kIrParts + blk] = bb;
+  }
+}
+
+__global__ __launch_bounds__(64) void ir_norms_kernel(const double* __restrict__ part, double* __restrict__ out) {
+  const int col = blockIdx.x, which = blockIdx.y;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < kIrParts; i += 64) acc += part[(2 * col + which) * kIrParts + i];
+  acc = wave_sum(acc);
+  if (threadIdx.x == 0) out[2 * col + which] = sqrt(acc);
+}
+
+__global__ __launch_bounds__(256) void ir_update_kernel(int64_t n, double* __restrict__ x, int64_t ldx,
+                                                        const float* __restrict__ d, int64_t ldd) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int col = blockIdx.y;
+  if (i >= n) return;
+  x[col * ldx + i] += (double)d[col * ldd + i];
+}
+
+__global__ __launch_bounds__(256) void to_bf16_kernel(int64_t count, const float* __restrict__ src,
+                                                      uint16_t* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  // round to nearest even (finite inputs)
+  const uint32_t u = __float_as_uint(src[i]);
+  const uint32_t rounded = u + 0x7FFFu + ((u >> 16) & 1u);
+  dst[i] = (uint16_t)(rounded >> 16);
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t ck_ir_workspace_bytes(int32_t nrhs) { return sizeof(double) * 2 * (size_t)nrhs * kIrParts; }
+
+int ck_ir_residual_f64(int64_t n, int32_t nrhs, const double* b, int64_t ldb, const double* Ax, int64_t ldax,
+                       double* r, int64_t ldr, float* r32, int64_t ldr32, double* norms, void* work,
+                       size_t work_bytes, void* stream) {
+  if (n < 1 || nrhs < 1 || !b || !norms || ldb < n || (Ax && ldax < n) || (r && ldr < n) || (r32 && ldr32 < n))
+    return CK_ERR_BAD_ARG;
+  if (!work || work_bytes < ck_ir_workspace_bytes(nrhs)) return CK_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  double* part = static_cast<double*>(work);
+  hipLaunchKernelGGL(ir_residual_kernel, dim3(kIrParts, nrhs), dim3(256), 0, s, n, b, ldb, Ax, ldax, r, ldr, r32,
+                     ldr32, part);
+  hipLaunchKernelGGL(ir_norms_kernel, dim3(nrhs, 2), dim3(64), 0, s, (const double*)part, norms);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+int ck_ir_update_f64(int64_t n, int32_t nrhs, double* x, int64_t ldx, const float* d, int64_t ldd, void* stream) {
+  if (n < 1 || nrhs < 1 || !x || !d || ldx < n || ldd < n) return CK_ERR_BAD_ARG;
+  hipLaunchKernelGGL(ir_update_kernel, dim3((unsigned)cdiv(n, 256), nrhs), dim3(256), 0, (hipStream_t)stream, n, x,
+                     ldx, d, ldd);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+int ck_f32_to_bf16(int64_t count, const float* src, uint16_t* dst, void* stream) {
+  if (count < 1 || !src || !dst) return CK_ERR_BAD_ARG;
+  hipLaunchKernelGGL(to_bf16_kernel, dim3((unsigned)cdiv(count, 256)), dim3(256), 0, (hipStream_t)stream, count, src,
+                     dst);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+}  // extern "C"

@@ -437,6 +437,8 @@ int kron_dispatch(const ck_op* op, int nrhs, const float* X, int64_t ldx, float*
     return CK_OK;
   }
   if (work_bytes < kron_workspace_bytes(op, nrhs) || !work) return CK_ERR_WORKSPACE;
+  if (kron64_eligible(op, X, ldx, Y, ldy))
+    return kron64_apply_f32(op, nrhs, X, ldx, Y, ldy, work, work_bytes, done, partials, nparts, s);
   float* U = static_cast<float*>(work);
   const int n1 = op->fdim[0], n2 = op->fdim[1], n3 = op->fdim[2];
   const int plane = n2 * n3;
@@ -481,6 +483,9 @@ int op_apply_internal(const ck_op* op, int nrhs, const float* X, int64_t ldx, fl
       return woodbury_apply(op, nrhs, X, ldx, Y, ldy, work, work_bytes, done, s);
     case CK_OP_RBF_F32:
       return rbf_apply(op, nrhs, X, ldx, Y, ldy, done, s);
+    case CK_OP_KRON_EIG_F32:
+      if (!kron64_eligible(op, X, ldx, Y, ldy)) return CK_ERR_UNSUPPORTED;
+      return kron64_eig_apply_f32(op, nrhs, X, ldx, Y, ldy, work, work_bytes, done, s);
     default:
       return CK_ERR_UNSUPPORTED;
   }
@@ -493,6 +498,8 @@ size_t op_workspace_internal(const ck_op* op, int nrhs) {
       return kron_workspace_bytes(op, nrhs);
     case CK_OP_WOODBURY_INV:
       return woodbury_workspace_bytes(op, nrhs);
+    case CK_OP_KRON_EIG_F32:
+      return 2 * (size_t)nrhs * op->n * sizeof(float);
     default:
       return 0;
   }

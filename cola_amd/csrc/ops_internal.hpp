@@ -19,6 +19,12 @@ int kron_dispatch(const ck_op* op, int nrhs, const float* X, int64_t ldx, float*
 int kron_nparts(const ck_op* op);
 size_t kron_workspace_bytes(const ck_op* op, int nrhs);
 
+bool kron64_eligible(const ck_op* op, const void* X, int64_t ldx, const void* Y, int64_t ldy);
+int kron64_apply_f32(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,
+                     size_t work_bytes, const int* done, float* partials, int nparts, hipStream_t s);
+int kron64_eig_apply_f32(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,
+                         size_t work_bytes, const int* done, hipStream_t s);
+
 int woodbury_apply(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,
                    size_t work_bytes, const int* done, hipStream_t s);
 size_t woodbury_workspace_bytes(const ck_op* op, int nrhs);

@@ -10,6 +10,7 @@ from .inv import inv, solve
 from .keys import PRNGKey, next_key
 from .lanczos import HipLanczos, lanczos, lanczos_eigs
 from .logdet import logdet, slogdet
+from .mixed import HipMixedCG, KroneckerEigInverse, mixed_cg
 from .slq import HipSLQ, stochastic_lanczos_quad
 from .trace import Exact, HipHutch, diag, hutch_trace_csr, hutchinson_diag_estimate, trace
 from .unary import LanczosUnary, apply_unary, exp, isqrt, log, pow, sqrt
@@ -23,4 +24,4 @@ __all__ = ["Algorithm", "Auto", "IterativeOperatorWInfo", "HipCG", "CG", "cg", "
            "HipLanczos", "Lanczos", "lanczos", "lanczos_eigs", "HipArnoldi", "Arnoldi", "arnoldi", "arnoldi_eigs",
            "eig", "eigmax", "eigmin", "get_slice", "logdet", "slogdet", "HipSLQ", "stochastic_lanczos_quad",
            "HipHutch", "Hutch", "Exact", "diag", "trace", "hutch_trace_csr", "hutchinson_diag_estimate",
-           "LanczosUnary", "apply_unary", "log", "exp", "sqrt", "isqrt", "pow", "PRNGKey", "next_key"]
+           "HipMixedCG", "KroneckerEigInverse", "mixed_cg", "LanczosUnary", "apply_unary", "log", "exp", "sqrt", "isqrt", "pow", "PRNGKey", "next_key"]

@@ -81,21 +81,38 @@ def run_batched_cg(A, b, x0=None, max_iters=1000, tol=1e-6, P=None, chunk=16, re
     n, B = b.shape
     if B > 1024:
         raise ValueError("at most 1024 right-hand sides per solve")
-    dev = b.device
-    aop = lower(A)
-    pop = lower(P) if P is not None else None
-    st = CGState(n, B, max_iters, dev, P is not None)
-    work, wb = workspace([aop[0] if aop else None, pop[0] if pop else None], B, dev)
+    st = CGState(n, B, max_iters, b.device, P is not None)
+    Bb = to_blocks(b, st.ld)
+    x0b = to_blocks(x0.reshape(n, B), st.ld) if x0 is not None else None
+    k, done, info = cg_blocks(A, Bb, st, x0b, max_iters, tol, P, chunk)
+    x = from_blocks(st.x, n)
+    r = from_blocks(st.r, n)
+    if return_state:
+        info["state"] = st
+    return x, r, k, info
+
+
+def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, lowered=None):
+    """Batched CG on column blocks: solves A X = Bb[:, :n] into st.x (st.r the residual).
+
+    ``st`` (a CGState) is reused across calls; ``lowered`` may carry precomputed
+    (aop, pop, work, wb). Returns (k, done, info)."""
+    n, B, dev, ld = st.n, st.B, st.device, st.ld
+    if lowered is None:
+        aop = lower(A)
+        pop = lower(P) if P is not None else None
+        work, wb = workspace([aop[0] if aop else None, pop[0] if pop else None], B, dev)
+    else:
+        aop, pop, work, wb = lowered
     s = N.stream_ptr(dev)
-    ld = st.ld
     t0 = time.time()
-    Bb = to_blocks(b, ld)
-    N.call("ck_cg_normalize", n, B, N.ptr(Bb), ld, N.ptr(st.bn), ld, N.ptr(st.state), N.ptr(st.partials), s)
-    if x0 is None:
+    N.call("ck_cg_normalize", n, B, N.ptr(Bb), Bb.stride(0), N.ptr(st.bn), ld, N.ptr(st.state), N.ptr(st.partials), s)
+    if x0b is None:
+        st.x.zero_()
         st.r.copy_(st.bn)
     else:
         # cg.py:123: r0 = b_norm - A x0 (x0 taken as given, not normalised)
-        st.x.copy_(to_blocks(x0.reshape(n, B), ld))
+        st.x.copy_(x0b)
         apply_blocks(A, aop, st.x, st.Ap, n, B, work, wb)
         torch.sub(st.bn, st.Ap, out=st.r)
     if P is not None:
@@ -127,11 +144,7 @@ def run_batched_cg(A, b, x0=None, max_iters=1000, tol=1e-6, P=None, chunk=16, re
     # torch_tqdm.py:38-62: one error per cond evaluation (k+1) plus the final one, first two dropped
     info = {"iterations": k + 1, "errors": np.concatenate([errs, errs[-1:]])[2:],
             "iteration_time": elapsed / (k + 1), "converged": bool(done and k < max_iters) or _all_below(st, B)}
-    x = from_blocks(st.x, n)
-    r = from_blocks(st.r, n)
-    if return_state:
-        info["state"] = st
-    return x, r, k, info
+    return k, done, info
 
 
 def _all_below(st, B):

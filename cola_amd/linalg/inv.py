@@ -6,6 +6,7 @@ from ..dispatch import dispatch
 from ..ops import Diagonal, Identity, Kronecker, LinearOperator, Product, ScalarMul
 from .algorithm_base import Algorithm, Auto, IterativeOperatorWInfo
 from .cg import HipCG
+from .mixed import HipMixedCG
 
 
 def solve(A, b, alg=Auto()):
@@ -21,6 +22,13 @@ def inv(A: LinearOperator, alg: Algorithm = Auto()):
 @dispatch(precedence=-1)
 def inv(A: LinearOperator, alg: HipCG):
     """inv.py:66-69."""
+    assert A.isa(PSD), "CG only valid for PSD matrices, wrap in cola_amd.PSD if desired"
+    return IterativeOperatorWInfo(A, alg)
+
+
+@dispatch(precedence=-1)
+def inv(A: LinearOperator, alg: HipMixedCG):
+    """Engine algorithm: FP64-residual refinement around FP32 (P)CG."""
     assert A.isa(PSD), "CG only valid for PSD matrices, wrap in cola_amd.PSD if desired"
     return IterativeOperatorWInfo(A, alg)
 

@@ -1,0 +1,216 @@
+"""Mixed-precision CG: FP64-residual iterative refinement around FP32 inner CG.
+
+No reference counterpart -- cola's CG runs in the operator's dtype
+(cola/linalg/inverse/cg.py:94-119).  ``HipMixedCG`` keeps the inner solver the
+engine's CG (same kernels, optional preconditioner ``P``) and wraps it in
+
+    r = b - A x   (FP64, the matrix as stored)      x += CG_fp32(A, r)
+
+until every column satisfies ||b - A x|| / ||b|| <= tol in FP64.  The FP32
+recurrence's attainable accuracy limit therefore no longer bounds the answer.
+
+``KroneckerEigInverse`` is the structured preconditioner for K1 x K2 x K3 + s I
+with 64x64 symmetric factors: (Q1 x Q2 x Q3) diag(1/(lambda + s)) (Q1 x Q2 x Q3)^T
+applied with the same MFMA Kronecker passes as the operator (CK_OP_KRON_EIG_F32).
+"""
+import time
+from dataclasses import dataclass
+
+import torch
+
+from .. import _native as N
+from .._blocks import from_blocks, ld_of, to_blocks
+from ..annotations import PSD
+from ..ops import Dense, Identity, Kronecker, LinearOperator, Product, ScalarMul, Sum, _op
+from ._engine import lower, workspace
+from .algorithm_base import Algorithm
+from .cg import CGState, cg_blocks
+
+
+def kron_structure(A):
+    """(factors, scale, shift) when A is scale*(K1 x K2 x K3) + shift*I with dense factors, else None."""
+    got = A.hip_op()
+    if got is None or got[0].kind != N.CK_OP_KRON_F32:
+        return None
+    op = got[0]
+    facs = [M for M in _find_kron(A).Ms]
+    return [M.A for M in facs], float(op.scale), float(op.shift)
+
+
+def _find_kron(A):
+    if isinstance(A, Kronecker):
+        return A
+    if isinstance(A, (Sum, Product)):
+        for M in A.Ms:
+            k = _find_kron(M)
+            if k is not None:
+                return k
+    return None
+
+
+class KroneckerEigInverse(LinearOperator):
+    """(scale*(K1 x K2 x K3) + shift*I)^{-1} through the factors' eigendecompositions.
+
+    Built once (three 64x64 symmetric eigensolves in FP64); applied as two MFMA
+    Kronecker passes pairs and a diagonal scaling."""
+
+    def __init__(self, factors, scale=1.0, shift=0.0):
+        dev = factors[0].device
+        self.device = dev
+        n = 1
+        for f in factors:
+            n *= f.shape[0]
+        super().__init__(torch.float32, (n, n), annotations={PSD})
+        lams, Qs = [], []
+        for f in factors:
+            w, Q = torch.linalg.eigh(f.double())
+            lams.append(w)
+            Qs.append(Q)
+        lam = lams[0]
+        for w in lams[1:]:
+            lam = (lam[:, None] * w[None, :]).reshape(-1)
+        self.lam = scale * lam + shift
+        self.dinv = (1.0 / self.lam).to(torch.float32).contiguous()
+        self.Q = [Q.to(torch.float32).contiguous() for Q in Qs]
+        self.Qt = [Q.T.contiguous() for Q in self.Q]
+
+    def hip_op(self):
+        if len(self.Q) != 3 or any(Q.shape[0] != 64 for Q in self.Q):
+            return None
+        op = _op(N.CK_OP_KRON_EIG_F32, self.shape[0])
+        op.nfactors = 3
+        for i in range(3):
+            op.fdim[i] = 64
+            op.factor[i] = self.Q[i].data_ptr()
+            op.factor2[i] = self.Qt[i].data_ptr()
+        op.aux = self.dinv.data_ptr()
+        return op, self.Q + self.Qt + [self.dinv]
+
+    def _matmat(self, X):
+        if self._use_hip(X):
+            return self._hip_apply(X)
+        raise NotImplementedError("KroneckerEigInverse runs on the engine (HIP device, 64x64x64)")
+
+
+class _F64Apply:
+    """FP64 application of A for refinement residuals (the matrix as stored)."""
+
+    def __init__(self, A, nrhs):
+        ks = kron_structure(A)
+        self.kind = None
+        if ks is not None and all(f.shape[0] == 64 for f in ks[0]) and len(ks[0]) == 3:
+            self.kind = "kron64"
+            self.f64 = [f.double().contiguous() for f in ks[0]]
+            self.scale, self.shift = ks[1], ks[2]
+            self.dims = torch.tensor([64, 64, 64], dtype=torch.int32)
+            n = 64**3
+            self.work = torch.empty(nrhs * n, dtype=torch.float64, device=A.device)
+        else:
+            raise N.UnsupportedOperator("mixed-precision refinement", N.CK_ERR_UNSUPPORTED,
+                                        "FP64 residuals need a 64^3 Kronecker operator (plus shift)")
+
+    def __call__(self, X, Y, nrhs):
+        ld = X.stride(0)
+        dims = (ctypes_int32_array(self.dims))
+        N.call("ck_kron_apply_f64", 3, dims, N.ptr(self.f64[0]), N.ptr(self.f64[1]), N.ptr(self.f64[2]),
+               self.scale, self.shift, nrhs, N.ptr(X), ld, N.ptr(Y), Y.stride(0), N.ptr(self.work),
+               self.work.numel() * 8, N.stream_ptr(X.device))
+
+
+def ctypes_int32_array(t):
+    import ctypes
+    vals = [int(v) for v in t.tolist()]
+    return (ctypes.c_int32 * len(vals))(*vals)
+
+
+@dataclass
+class HipMixedCG(Algorithm):
+    """Iterative refinement with FP64 residuals around the engine's FP32 (P)CG.
+
+    Args:
+        tol: target FP64 relative residual ||b - A x|| / ||b|| for every column.
+        inner_tol: CG tolerance of each correction solve (cg.py:101 semantics).
+        inner_max_iters: step cap of each correction solve.
+        max_refinements: outer step cap.
+        P: preconditioner of the inner CG (approximation of A^-1).
+        chunk: CG steps between reads of the device loop flag.
+    """
+    tol: float = 1e-6
+    inner_tol: float = 1e-4
+    inner_max_iters: int = 1000
+    max_refinements: int = 20
+    P: object = None
+    chunk: int = 8
+
+    def __call__(self, A, b):
+        is_vector = b.dim() == 1
+        x, info = mixed_cg(A, b[:, None] if is_vector else b, self)
+        return (x[:, 0] if is_vector else x), info
+
+
+class MixedSolver:
+    """Reusable buffers for repeated refinement solves with the same operator and block shape."""
+
+    def __init__(self, A, nrhs, alg):
+        self.A, self.alg = A, alg
+        n = A.shape[0]
+        self.n, self.B = n, nrhs
+        dev = A.device
+        self.ld = ld_of(n)
+        self.f64 = _F64Apply(A, nrhs)
+        self.st = CGState(n, nrhs, alg.inner_max_iters, dev, alg.P is not None)
+        aop = lower(A)
+        pop = lower(alg.P) if alg.P is not None else None
+        work, wb = workspace([aop[0] if aop else None, pop[0] if pop else None], nrhs, dev)
+        self.lowered = (aop, pop, work, wb)
+        z64 = lambda: torch.zeros((nrhs, self.ld), dtype=torch.float64, device=dev)  # noqa: E731
+        self.b64, self.x64, self.Ax64, self.r64 = z64(), z64(), z64(), z64()
+        self.r32 = torch.zeros((nrhs, self.ld), dtype=torch.float32, device=dev)
+        self.norms = torch.zeros(2 * nrhs, dtype=torch.float64, device=dev)
+        wbytes = N.q("ck_ir_workspace_bytes", nrhs)
+        self.irwork = torch.empty(wbytes // 8, dtype=torch.float64, device=dev)
+
+    def residual(self, with_ax=True):
+        n, B, ld = self.n, self.B, self.ld
+        s = N.stream_ptr(self.b64.device)
+        if with_ax:
+            self.f64(self.x64, self.Ax64, B)
+        N.call("ck_ir_residual_f64", n, B, N.ptr(self.b64), ld, N.ptr(self.Ax64) if with_ax else None, ld,
+               N.ptr(self.r64), ld, N.ptr(self.r32), ld, N.ptr(self.norms), N.ptr(self.irwork),
+               self.irwork.numel() * 8, s)
+        nr = self.norms.view(B, 2).cpu()
+        return (nr[:, 0] / torch.clamp(nr[:, 1], min=1e-300)).numpy()
+
+    def solve(self, b):
+        alg = self.alg
+        n, B, ld = self.n, self.B, self.ld
+        t0 = time.time()
+        self.b64.zero_()
+        self.b64[:, :n].copy_(b.T)
+        self.x64.zero_()
+        rel = self.residual(with_ax=False)
+        history = [float(rel.max())]
+        inner_steps = []
+        s = N.stream_ptr(b.device)
+        it = 0
+        while rel.max() > alg.tol and it < alg.max_refinements:
+            k, _, _ = cg_blocks(self.A, self.r32, self.st, None, alg.inner_max_iters, alg.inner_tol, alg.P,
+                                alg.chunk, self.lowered)
+            inner_steps.append(int(k))
+            N.call("ck_ir_update_f64", n, B, N.ptr(self.x64), ld, N.ptr(self.st.x), ld, s)
+            rel = self.residual(with_ax=True)
+            history.append(float(rel.max()))
+            it += 1
+        x = self.x64[:, :n].T.to(torch.float32)
+        info = {"refinements": it, "inner_iterations": inner_steps, "rel_residual_f64": rel,
+                "history": history, "converged": bool(rel.max() <= alg.tol), "time": time.time() - t0}
+        return x, info
+
+
+def mixed_cg(A, b, alg):
+    N.require_device(b, "right-hand side")
+    solver = MixedSolver(A, b.shape[1], alg)
+    return solver.solve(b)
+
+
+_ = (from_blocks, to_blocks, Dense, Identity, ScalarMul)

@@ -57,6 +57,12 @@ const char* ck_status_string(int status);
  *   CK_OP_RBF_F32    S_ij = exp(-|p_i - p_j|^2 / (2 l^2)) built on the fly from
  *                    points a (n x dim fp32, row-major, lda) with l = aux_f
  *                    (replaces Kernel._matmat, cola/ops/operators.py:631-643)
+ *   CK_OP_KRON_EIG_F32  S = Q diag(aux) Q^T with Q = factor[0] (x) factor[1] (x) factor[2]
+ *                    (three 64x64 orthogonal factors) and factor2[i] = factor[i]^T:
+ *                    a function of a Kronecker operator in its eigenbasis, e.g.
+ *                    (K1 (x) K2 (x) K3 + s I)^-1 with aux = 1/(lambda + s); no engine
+ *                    counterpart in the reference (it inverts bare Kronecker products
+ *                    factor-wise, cola/linalg/inverse/inv.py:139-141); shift must be 0
  * The shift term is how Sum(S, ScalarMul(c) @ I) (cola/ops/operators.py:182-183,
  * 97-98) reaches one kernel epilogue.
  */
@@ -68,7 +74,8 @@ enum {
   CK_OP_KRON_F32 = 4,
   CK_OP_DIAG_F32 = 5,
   CK_OP_WOODBURY_INV = 6,
-  CK_OP_RBF_F32 = 7
+  CK_OP_RBF_F32 = 7,
+  CK_OP_KRON_EIG_F32 = 8
 };
 
 typedef struct ck_op {
@@ -85,6 +92,7 @@ typedef struct ck_op {
   int32_t nfactors;
   int32_t fdim[4];
   const float* factor[4];
+  const float* factor2[4];
   const float* aux;
   int32_t rank;
   int32_t dim;
@@ -99,6 +107,32 @@ size_t ck_op_workspace_bytes(const ck_op* op, int32_t nrhs);
  * Replaces LinearOperator.__matmul__ -> _matmat (cola/ops/operator_base.py:97-106). */
 int ck_op_apply(const ck_op* op, int32_t nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,
                 size_t work_bytes, void* stream);
+
+/* FP64 Kronecker matvec Y = scale * (F0 (x) F1 (x) F2) X + shift * X for three 64x64
+ * row-major fp64 factors (v_mfma_f64_16x16x4f64); used for refinement residuals.
+ * work >= nrhs * 64^3 doubles.  (cola/ops/operators.py:216-223 in FP64.) */
+int ck_kron_apply_f64(int32_t nfactors, const int32_t* fdim, const double* f0, const double* f1, const double* f2,
+                      double scale, double shift, int32_t nrhs, const double* X, int64_t ldx, double* Y, int64_t ldy,
+                      void* work, size_t work_bytes, void* stream);
+
+/* ---------------------------------------------------------------- mixed precision
+ * Iterative refinement with FP64 residuals around fp32 inner solves (no
+ * reference counterpart: cola's CG runs in the operator's dtype).
+ * ck_ir_residual_f64: r = b - Ax (Ax may be NULL: r = b) in fp64, an optional fp32
+ * copy r32, and norms[2*col] = ||r_col||, norms[2*col+1] = ||b_col|| (fp64).
+ * ck_ir_update_f64: x += d.  ck_f32_to_bf16: round-to-nearest-even storage copy. */
+size_t ck_ir_workspace_bytes(int32_t nrhs);
+int ck_ir_residual_f64(int64_t n, int32_t nrhs, const double* b, int64_t ldb, const double* Ax, int64_t ldax,
+                       double* r, int64_t ldr, float* r32, int64_t ldr32, double* norms, void* work,
+                       size_t work_bytes, void* stream);
+int ck_ir_update_f64(int64_t n, int32_t nrhs, double* x, int64_t ldx, const float* d, int64_t ldd, void* stream);
+int ck_f32_to_bf16(int64_t count, const float* src, uint16_t* dst, void* stream);
+
+/* Y = (I (x) K2 (x) K3) X for 64x64 fp32 factors: the fused mode-2/3 pass of the
+ * 64^3 Kronecker matvec (one kernel; the per-mode loop of cola/ops/operators.py:218-222
+ * for its last two modes). */
+int ck_kron_apply_last2_f32(const float* K2, const float* K3, int32_t nrhs, const float* X, int64_t ldx, float* Y,
+                            int64_t ldy, void* stream);
 
 /* ---------------------------------------------------------------- vector kernels */
 /* Per-column dot products out[b] = sum_i X[b,i] Y[b,i], fp64 result (a two-pass

@@ -63,8 +63,9 @@ def test_cg_config1_dense(dev):
     res = (A.double() @ x.double().cpu() - b.double()).norm() / b.double().norm()
     res_o = (A.double() @ xo[:, 0].double() - b.double()).norm() / b.double().norm()
     assert res < 1.25 * res_o and res < 1e-5
+    # the mean-residual history of cola/utils/torch_tqdm.py agrees step by step (log scale)
     n = min(len(info["errors"]), len(io["errors"]))
-    assert np.allclose(info["errors"][:n], io["errors"][:n], rtol=1e-2, atol=1e-7)
+    assert np.allclose(np.log10(info["errors"][:n]), np.log10(io["errors"][:n]), atol=0.05)
 
 
 @pytest.mark.parametrize("n,B", [(100, 5), (777, 3), (2048, 16)])

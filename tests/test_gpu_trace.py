@@ -128,7 +128,7 @@ def test_slq_kron_vs_oracle(dev):
     mm = lambda X: oop.kron_matmat([f.double() for f in facs], X) + 0.01 * X  # noqa: E731
     from oracle.lanczos import lanczos as olz_l
     _, al, be, _ = olz_l(mm, n, Z, 30, 1e-6, torch.float64)
-    q, _, _ = oun.slq_quadratures(al, be, (Z**2).sum(0), torch.log)
+    q, _, _ = oun.slq_quadratic_forms(al, be, (Z**2).sum(0), torch.log)
     ref = float(q.mean())
     assert abs(est - ref) < 1e-4 * abs(ref)
     lam = [torch.linalg.eigvalsh(f.double()) for f in facs]
