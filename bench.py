@@ -172,6 +172,22 @@ def cpu_baseline_kron(facs, b, k_steps, tol, iters=20):
                       "on this system)"}
 
 
+def bench_copy(args, dev):
+    """STREAM-style copy of 1 GiB buffers (4x the 256 MB Infinity Cache): the measured HBM peak."""
+    n = 1 << 28
+    src = torch.ones(n, device=dev)
+    dst = torch.empty_like(src)
+    s = N.stream_ptr(dev)
+    t = time_kernel(lambda: N.call("ck_stream_copy_f32", n, N.ptr(src), N.ptr(dst), s), reps=max(args.steps, 10))
+    gbs = 2 * 4 * n / t / 1e9
+    return {"metric": "HBM copy bandwidth (read+write)", "value": gbs, "unit": "GB/s", "n_gpus": 1,
+            "steps": max(args.steps, 10), "warmup": 1, "ms_per_step": t * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "copy 1 GiB -> 1 GiB (ck_stream_copy_f32)", "bytes_per_launch": 8 * n},
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK, "unit": "GB/s", "frac": gbs / HBM_PEAK,
+                         "traffic": None}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,7 +204,7 @@ def main():
         sys.exit(2)
     dev = torch.device("cuda:0")
     N.load()
-    fn = {"kron_cg": bench_kron_cg}[args.workload]
+    fn = {"kron_cg": bench_kron_cg, "copy": bench_copy}[args.workload]
     out = fn(args, dev)
     print(json.dumps(out), flush=True)
 

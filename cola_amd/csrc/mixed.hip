@@ -64,9 +64,29 @@ __global__ __launch_bounds__(256) void to_bf16_kernel(int64_t count, const float
   dst[i] = (uint16_t)(rounded >> 16);
 }
 
+// grid-stride float4 copy: the bandwidth probe the roofline peak and the PMC byte
+// counters are calibrated with
+__global__ __launch_bounds__(256) void stream_copy_kernel(int64_t n4, const float4* __restrict__ src,
+                                                          float4* __restrict__ dst) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 }  // namespace
 
 extern "C" {
+
+int ck_stream_copy_f32(int64_t n, const float* src, float* dst, void* stream) {
+  if (n < 4 || (n % 4) || !src || !dst || ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15))
+    return CK_ERR_BAD_ARG;
+  const int64_t n4 = n / 4;
+  int64_t blocks = cdiv(n4, 256);
+  if (blocks > 256 * 64) blocks = 256 * 64;  // 64 blocks per CU, grid-stride for the rest
+  hipLaunchKernelGGL(stream_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, n4,
+                     reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst));
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
 
 size_t ck_ir_workspace_bytes(int32_t nrhs) { return sizeof(double) * 2 * (size_t)nrhs * kIrParts; }
 

@@ -16,6 +16,8 @@ applied with the same MFMA Kronecker passes as the operator (CK_OP_KRON_EIG_F32)
 import time
 from dataclasses import dataclass
 
+import numpy as np
+
 import torch
 
 from .. import _native as N
@@ -193,7 +195,9 @@ class MixedSolver:
         inner_steps = []
         s = N.stream_ptr(b.device)
         it = 0
-        while rel.max() > alg.tol and it < alg.max_refinements:
+        while not (np.all(np.isfinite(rel)) and rel.max() <= alg.tol) and it < alg.max_refinements:
+            if not np.all(np.isfinite(rel)):
+                raise FloatingPointError("non-finite FP64 residual in iterative refinement")
             k, _, _ = cg_blocks(self.A, self.r32, self.st, None, alg.inner_max_iters, alg.inner_tol, alg.P,
                                 alg.chunk, self.lowered)
             inner_steps.append(int(k))
@@ -203,7 +207,7 @@ class MixedSolver:
             it += 1
         x = self.x64[:, :n].T.to(torch.float32)
         info = {"refinements": it, "inner_iterations": inner_steps, "rel_residual_f64": rel,
-                "history": history, "converged": bool(rel.max() <= alg.tol), "time": time.time() - t0}
+                "history": history, "converged": bool(np.all(np.isfinite(rel)) and rel.max() <= alg.tol), "time": time.time() - t0}
         return x, info
 
 

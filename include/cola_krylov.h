@@ -127,6 +127,9 @@ int ck_ir_residual_f64(int64_t n, int32_t nrhs, const double* b, int64_t ldb, co
                        size_t work_bytes, void* stream);
 int ck_ir_update_f64(int64_t n, int32_t nrhs, double* x, int64_t ldx, const float* d, int64_t ldd, void* stream);
 int ck_f32_to_bf16(int64_t count, const float* src, uint16_t* dst, void* stream);
+/* dst = src for n floats (n % 4 == 0, 16-byte aligned): the bandwidth probe that sets the
+ * measured HBM peak and calibrates byte counters; no reference counterpart. */
+int ck_stream_copy_f32(int64_t n, const float* src, float* dst, void* stream);
 
 /* Y = (I (x) K2 (x) K3) X for 64x64 fp32 factors: the fused mode-2/3 pass of the
  * 64^3 Kronecker matvec (one kernel; the per-mode loop of cola/ops/operators.py:218-222

@@ -64,8 +64,12 @@ def test_cg_config1_dense(dev):
     res_o = (A.double() @ xo[:, 0].double() - b.double()).norm() / b.double().norm()
     assert res < 1.25 * res_o and res < 1e-5
     # the mean-residual history of cola/utils/torch_tqdm.py agrees step by step (log scale)
+    # until the FP32 rounding floor (~1e-5) the histories agree closely; below it both wander
     n = min(len(info["errors"]), len(io["errors"]))
-    assert np.allclose(np.log10(info["errors"][:n]), np.log10(io["errors"][:n]), atol=0.05)
+    ge, oe = np.log10(info["errors"][:n]), np.log10(io["errors"][:n])
+    above = oe > -4.5
+    assert np.allclose(ge[above], oe[above], atol=0.02)
+    assert np.allclose(ge, oe, atol=0.3)
 
 
 @pytest.mark.parametrize("n,B", [(100, 5), (777, 3), (2048, 16)])
