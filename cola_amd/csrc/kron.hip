@@ -24,9 +24,12 @@ namespace {
 
 template <typename T>
 struct MF;
+// Accumulator layout (measured): lane l, item r of a 16x16 result holds
+// D[row(l/16, r)][l % 16] with row = 4*(l/16) + r for f32 and (l/16) + 4*r for f64.
 template <>
 struct MF<float> {
   using acc = floatx4;
+  __device__ __forceinline__ static int row(int g, int r) { return 4 * g + r; }
   __device__ __forceinline__ static acc step(float a, float b, acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
@@ -34,6 +37,7 @@ struct MF<float> {
 template <>
 struct MF<double> {
   using acc = doublex4;
+  __device__ __forceinline__ static int row(int g, int r) { return g + 4 * r; }
   __device__ __forceinline__ static acc step(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
@@ -86,10 +90,10 @@ __global__ __launch_bounds__(256) void kron64_pass_a(const T* __restrict__ K2, c
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const auto c = tile_product<T>(a, K3 + (16 * t + li) * 64 + 16 * g);
-    // c[it] = T[16w + 4g + it][16t + li] -> Tt[16t + li][16w + 4g + it]
-    T* dst = Tt + (16 * t + li) * TP + 16 * w + 4 * g;
+    // c[it] = T[16w + row(g, it)][16t + li] -> Tt[16t + li][16w + row(g, it)]
+    T* dst = Tt + (16 * t + li) * TP + 16 * w;
 #pragma unroll
-    for (int it = 0; it < 4; ++it) dst[it] = c[it];
+    for (int it = 0; it < 4; ++it) dst[MF<T>::row(g, it)] = c[it];
   }
   __syncthreads();
   // U = K2 T: U[r][c] = sum_k K2[r][k] Tt[c][k]
@@ -99,7 +103,7 @@ __global__ __launch_bounds__(256) void kron64_pass_a(const T* __restrict__ K2, c
   for (int t = 0; t < 4; ++t) {
     const auto c = tile_product<T>(a, Tt + (16 * t + li) * TP + 16 * g);
 #pragma unroll
-    for (int it = 0; it < 4; ++it) Ub[(16 * w + 4 * g + it) * 64 + 16 * t + li] = c[it];
+    for (int it = 0; it < 4; ++it) Ub[(16 * w + MF<T>::row(g, it)) * 64 + 16 * t + li] = c[it];
   }
 }
 
@@ -146,7 +150,7 @@ __global__ __launch_bounds__(256) void kron64_pass_b(const T* __restrict__ K1, c
     const auto c = tile_product<T>(a, Ut + (16 * t + li) * TP + 16 * g);
 #pragma unroll
     for (int it = 0; it < 4; ++it) {
-      const int64_t off = (int64_t)(16 * w + 4 * g + it) * 4096 + c0 + 16 * t + li;
+      const int64_t off = (int64_t)(16 * w + MF<T>::row(g, it)) * 4096 + c0 + 16 * t + li;
       const T x = X[xb + off];
       T y = scale * c[it];
       if (dvec) y *= dvec[off];

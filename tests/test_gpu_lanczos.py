@@ -138,5 +138,7 @@ def test_arnoldi_eigs_extremes_vs_oracle(dev):
     got = np.sort(vals.cpu().numpy().real)
     ev_o, _, _ = oar.arnoldi_eigs(lambda V: A @ V, n, torch.ones(n), n - 2, 1e-7)
     ref = np.sort(ev_o.real.numpy())
-    assert np.allclose(got, ref, atol=2e-3)
+    # the converged extremes agree with the oracle and the true spectrum; interior Ritz
+    # values after n-2 FP32 MGS steps are not converged and differ with evaluation order
+    assert abs(got[0] - ref[0]) < 2e-3 and abs(got[-1] - ref[-1]) < 2e-3
     assert abs(got[-1] - 3.0) < 1e-3 and abs(got[0] - 1.0) < 1e-3
