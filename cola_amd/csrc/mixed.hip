@@ -64,12 +64,24 @@ __global__ __launch_bounds__(256) void to_bf16_kernel(int64_t count, const float
   dst[i] = (uint16_t)(rounded >> 16);
 }
 
-// grid-stride float4 copy: the bandwidth probe the roofline peak and the PMC byte
-// counters are calibrated with
-__global__ __launch_bounds__(256) void stream_copy_kernel(int64_t n4, const float4* __restrict__ src,
-                                                          float4* __restrict__ dst) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
+// float4 copy with four independent loads in flight per thread: the bandwidth
+// probe the roofline peak and the PMC byte counters are calibrated with
+constexpr int kCopyUnroll = 4;
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void stream_copy_kernel(int64_t n4, const f32x4* __restrict__ src,
+                                                          f32x4* __restrict__ dst) {
+  const int64_t base = (int64_t)blockIdx.x * (256 * kCopyUnroll) + threadIdx.x;
+  f32x4 v[kCopyUnroll];
+#pragma unroll
+  for (int u = 0; u < kCopyUnroll; ++u) {
+    const int64_t i = base + 256 * u;
+    if (i < n4) v[u] = __builtin_nontemporal_load(src + i);
+  }
+#pragma unroll
+  for (int u = 0; u < kCopyUnroll; ++u) {
+    const int64_t i = base + 256 * u;
+    if (i < n4) __builtin_nontemporal_store(v[u], dst + i);
+  }
 }
 
 }  // namespace
@@ -80,10 +92,10 @@ int ck_stream_copy_f32(int64_t n, const float* src, float* dst, void* stream) {
   if (n < 4 || (n % 4) || !src || !dst || ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15))
     return CK_ERR_BAD_ARG;
   const int64_t n4 = n / 4;
-  int64_t blocks = cdiv(n4, 256);
-  if (blocks > 256 * 64) blocks = 256 * 64;  // 64 blocks per CU, grid-stride for the rest
+  const int64_t blocks = cdiv(n4, 256 * kCopyUnroll);
+  if (blocks > 0x7FFFFFFF) return CK_ERR_BAD_ARG;
   hipLaunchKernelGGL(stream_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, n4,
-                     reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst));
+                     reinterpret_cast<const f32x4*>(src), reinterpret_cast<f32x4*>(dst));
   CK_CHECK_LAUNCH();
   return CK_OK;
 }

@@ -92,7 +92,7 @@ def run_batched_cg(A, b, x0=None, max_iters=1000, tol=1e-6, P=None, chunk=16, re
     return x, r, k, info
 
 
-def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, lowered=None):
+def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, lowered=None, want_info=True):
     """Batched CG on column blocks: solves A X = Bb[:, :n] into st.x (st.r the residual).
 
     ``st`` (a CGState) is reused across calls; ``lowered`` may carry precomputed
@@ -139,6 +139,8 @@ def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, l
         steps += nst
         k, done = read_ints(st.state, 2, dev)
     N.call("ck_cg_finish", n, B, N.ptr(st.x), N.ptr(st.r), ld, N.ptr(st.state), s)
+    if not want_info:
+        return k, done, {"iterations": k + 1}
     errs = st.floats(N.CK_CG_ERRORS, k + 1).cpu().numpy().astype(np.float64)
     elapsed = time.time() - t0
     # torch_tqdm.py:38-62: one error per cond evaluation (k+1) plus the final one, first two dropped
