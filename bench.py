@@ -30,6 +30,9 @@ from cola_amd.linalg import run_batched_cg  # noqa: E402
 
 METRIC = "CG solves/sec to 1e-6 rel. residual: learned+mixed-prec vs plain; HBM GB/s"
 HBM_PEAK = 8000.0  # GB/s
+# HBM bytes per kron64_pass_a<float> launch at 64^3 x 16 rhs, from rocprofv3 --pmc FETCH_SIZE (x2.0) plus
+# WRITE_SIZE (x1.0), the factors a 1 GiB stream copy reads on the same counters (profiles/kron64_cg_64cubed_hbm_traffic.md)
+PASS_A_TRAFFIC_BYTES = (16407.4 + 16110.2) * 1024
 FP32_MFMA_PEAK = 157.3  # TFLOP/s
 
 
@@ -133,7 +136,7 @@ def bench_kron_cg(args, dev):
     t_mv = time_kernel(lambda: N.call("ck_op_apply", op, B, N.ptr(Xb), ld, N.ptr(Yb), ld, N.ptr(work), wb, s))
     roof = {"bound": "mfma", "kernel": "kron64_pass_a<float> (fused mode-2/3 Kronecker pass)",
             "achieved": flops_a / t_a / 1e12, "peak": FP32_MFMA_PEAK, "unit": "TFLOP/s",
-            "frac": flops_a / t_a / 1e12 / FP32_MFMA_PEAK, "traffic": None,
+            "frac": flops_a / t_a / 1e12 / FP32_MFMA_PEAK, "traffic": PASS_A_TRAFFIC_BYTES,
             "launch_us": t_a * 1e6, "algorithmic_flops_per_launch": flops_a,
             "algorithmic_bytes_per_launch": bytes_a, "achieved_gbs": bytes_a / t_a / 1e9,
             "full_kron_matvec_us": t_mv * 1e6, "full_kron_matvec_tflops": 3 * 2 * 64 * n * B / t_mv / 1e12}
