@@ -108,6 +108,22 @@ _PROTOS = {
     "ck_hutch_csr_bits": (ctypes.c_int, [_op, _i32, _p, _p, _p, _p]),
     "ck_trace_csr_rademacher_bits": (ctypes.c_int, [_op, _i32, _p, _p, _p, _sz, _p]),
     "ck_trace_workspace_bytes": (_sz, [_i64]),
+    # learned preconditioner, pivoted Cholesky, dense helpers
+    "ck_mlp_param_count": (_i64, [_i32, _i32]),
+    "ck_mlp_forward": (ctypes.c_int, [_i64, _i32, _i32, _p, _i64, _p, _p, _i64, _i64, _p, _p, _i64, _p]),
+    "ck_mlp_grad_workspace_bytes": (_sz, [_i64, _i32, _i32]),
+    "ck_mlp_backward": (ctypes.c_int, [_i64, _i32, _i32, _p, _i64, _p, _p, _p, _i64, _p, _i64, _i64, _f32, _p, _p,
+                                       _sz, _p]),
+    "ck_adam_step": (ctypes.c_int, [_i64, _p, _p, _p, _p, _f32, _f32, _f32, _f32, _i32, _p]),
+    "ck_gram_workspace_bytes": (_sz, [_i64, _i32, _i32]),
+    "ck_gram_f64": (ctypes.c_int, [_i64, _i32, _i32, _p, _i64, _i64, _p, _i64, _i64, _p, _p, _sz, _p]),
+    "ck_rows_gemm_f32": (ctypes.c_int, [_i64, _i32, _i32, _p, _i64, _i64, _p, _f32, _f32, _p, _i64, _i64, _p]),
+    "ck_spd_inverse_f64": (ctypes.c_int, [_i32, _i32, _p, _i64, _i64, _f64, _f64, _p, _p, _p, _p, _p, _p]),
+    "ck_energy_loss_f64": (ctypes.c_int, [_i32, _i32, _p, _p, _p, _p, _p, _p, _p]),
+    "ck_pivchol_workspace_bytes": (_sz, [_i64]),
+    "ck_pivchol_dense_f32": (ctypes.c_int, [_i64, _p, _i64, _i32, _p, _i64, _p, _p, _p, _sz, _p]),
+    "ck_rbf_dense_f32": (ctypes.c_int, [_i64, _i32, _p, _i64, _f32, _p, _i64, _p, _i64, _p]),
+    "ck_dense_apply_f64acc": (ctypes.c_int, [_i64, _p, _i64, _f64, _f64, _i32, _p, _i64, _p, _i64, _p]),
 }
 
 _lib = None
@@ -129,7 +145,7 @@ def load():
     for name, (res, args) in _PROTOS.items():
         fn = getattr(lib, name, None)
         if fn is None:
-            continue  # optional symbols (added by later sources) are bound when present
+            raise ImportError(f"{_LIB_PATH} does not export {name}: rebuild it with `python -m cola_amd.csrc.build`")
         fn.restype = res
         fn.argtypes = args
     _lib = lib

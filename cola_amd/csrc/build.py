@@ -53,7 +53,10 @@ def _compile(src, verbose):
 def build(jobs=8, verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIBDIR, exist_ok=True)
-    srcs = [s for s in SOURCES if os.path.exists(os.path.join(HERE, s))]
+    missing = [s for s in SOURCES if not os.path.exists(os.path.join(HERE, s))]
+    if missing:
+        raise FileNotFoundError(f"missing sources: {missing}")
+    srcs = list(SOURCES)
     with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
     out = lib_path()

@@ -1,8 +1,11 @@
 // Low-rank (Woodbury) inverse application and the on-the-fly RBF kernel operator.
 //
 // Woodbury: (s I + F F^T)^{-1} x = (x - F (s I + F^T F)^{-1} F^T x) / s with the
-// small core inverse precomputed (rank x rank).  F^T x is a tall-skinny
-// reduction (one partial per 1024-row tile, added in index order), the core
+// small core inverse precomputed (rank x rank, fp64).  All arithmetic after the
+// fp32 loads is fp64: for x in span(F) the subtraction cancels to a relative size
+// s / (s + lambda), so fp32 arithmetic would leave errors of u * lambda / s there
+// (0.06 at lambda / s = 1e6), enough to stall PCG.  F^T x is a tall-skinny
+// reduction (one partial per 1024-row tile, added in index order); the core
 // product and the rank-k update are fused into one pass over F.
 //
 // RBF: S_ij = exp(-|p_i - p_j|^2 / (2 l^2)) is never stored; each block stages
@@ -20,7 +23,7 @@ constexpr int kMaxRank = 128;
 
 __global__ __launch_bounds__(256) void wb_ftx_partials(int64_t n, int k, const float* __restrict__ F, int64_t lda,
                                                        const float* __restrict__ X, int64_t ldx,
-                                                       float* __restrict__ part, int P,
+                                                       double* __restrict__ part, int P,
                                                        const int* __restrict__ done) {
   if (done && *done) return;
   const int b = blockIdx.y, blk = blockIdx.x;
@@ -33,51 +36,51 @@ __global__ __launch_bounds__(256) void wb_ftx_partials(int64_t n, int k, const f
     xv[q] = i < n ? X[b * ldx + i] : 0.f;
   }
   for (int c = wave; c < k; c += 4) {
-    float acc = 0.f;
+    double acc = 0.0;
 #pragma unroll
     for (int q = 0; q < kWTile / 64; ++q) {
       const int64_t i = i0 + lane + 64 * q;
-      if (i < n) acc = fmaf(F[i * lda + c], xv[q], acc);
+      if (i < n) acc = fma((double)F[i * lda + c], (double)xv[q], acc);
     }
     acc = wave_sum(acc);
     if (lane == 0) part[((int64_t)b * k + c) * P + blk] = acc;
   }
 }
 
-__global__ __launch_bounds__(256) void wb_reduce(int k, const float* __restrict__ part, int P, float* __restrict__ t,
+__global__ __launch_bounds__(256) void wb_reduce(int k, const double* __restrict__ part, int P, double* __restrict__ t,
                                                  const int* __restrict__ done) {
   if (done && *done) return;
   const int b = blockIdx.y;
   const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (c >= k) return;
   double acc = 0.0;
-  for (int p = threadIdx.x & 63; p < P; p += 64) acc += (double)part[((int64_t)b * k + c) * P + p];
+  for (int p = threadIdx.x & 63; p < P; p += 64) acc += part[((int64_t)b * k + c) * P + p];
   acc = wave_sum(acc);
-  if ((threadIdx.x & 63) == 0) t[b * k + c] = (float)acc;
+  if ((threadIdx.x & 63) == 0) t[b * k + c] = acc;
 }
 
 __global__ __launch_bounds__(256) void wb_apply(int64_t n, int k, const float* __restrict__ F, int64_t lda,
-                                                const float* __restrict__ Cinv, const float* __restrict__ t,
+                                                const double* __restrict__ Cinv, const double* __restrict__ t,
                                                 const float* __restrict__ X, int64_t ldx, float* __restrict__ Y,
-                                                int64_t ldy, float scale, float shift, const int* __restrict__ done) {
+                                                int64_t ldy, double scale, double shift, const int* __restrict__ done) {
   if (done && *done) return;
-  __shared__ float ts[kMaxRank], us[kMaxRank];
+  __shared__ double ts[kMaxRank], us[kMaxRank];
   const int b = blockIdx.y;
   for (int c = threadIdx.x; c < k; c += 256) ts[c] = t[b * k + c];
   __syncthreads();
   for (int c = threadIdx.x; c < k; c += 256) {
-    float acc = 0.f;
-    for (int j = 0; j < k; ++j) acc = fmaf(Cinv[c * k + j], ts[j], acc);
+    double acc = 0.0;
+    for (int j = 0; j < k; ++j) acc = fma(Cinv[c * k + j], ts[j], acc);
     us[c] = acc;
   }
   __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  float acc = 0.f;
+  double acc = 0.0;
   const float* Fi = F + i * lda;
-  for (int c = 0; c < k; ++c) acc = fmaf(Fi[c], us[c], acc);
-  const float x = X[b * ldx + i];
-  Y[b * ldy + i] = fmaf(scale, x - acc, shift * x);
+  for (int c = 0; c < k; ++c) acc = fma((double)Fi[c], us[c], acc);
+  const double x = (double)X[b * ldx + i];
+  Y[b * ldy + i] = (float)(scale * (x - acc) + shift * x);
 }
 
 constexpr int kRbfTile = 256;
@@ -136,7 +139,7 @@ namespace ck {
 
 size_t woodbury_workspace_bytes(const ck_op* op, int nrhs) {
   const size_t P = cdiv(op->n, kWTile);
-  return sizeof(float) * ((size_t)nrhs * op->rank * P + (size_t)nrhs * op->rank);
+  return sizeof(double) * ((size_t)nrhs * op->rank * P + (size_t)nrhs * op->rank);
 }
 
 int woodbury_apply(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,
@@ -146,13 +149,15 @@ int woodbury_apply(const ck_op* op, int nrhs, const float* X, int64_t ldx, float
   if (!work || work_bytes < woodbury_workspace_bytes(op, nrhs)) return CK_ERR_WORKSPACE;
   const int64_t n = op->n;
   const int P = (int)cdiv(n, kWTile);
-  float* part = static_cast<float*>(work);
-  float* t = part + (size_t)nrhs * k * P;
+  double* part = static_cast<double*>(work);
+  double* t = part + (size_t)nrhs * k * P;
   const float* F = static_cast<const float*>(op->a);
   hipLaunchKernelGGL(wb_ftx_partials, dim3(P, nrhs), dim3(256), 0, s, n, k, F, op->lda, X, ldx, part, P, done);
-  hipLaunchKernelGGL(wb_reduce, dim3((unsigned)cdiv(k, 4), nrhs), dim3(256), 0, s, k, (const float*)part, P, t, done);
-  hipLaunchKernelGGL(wb_apply, dim3((unsigned)cdiv(n, 256), nrhs), dim3(256), 0, s, n, k, F, op->lda, op->aux,
-                     (const float*)t, X, ldx, Y, ldy, op->scale, op->shift, done);
+  hipLaunchKernelGGL(wb_reduce, dim3((unsigned)cdiv(k, 4), nrhs), dim3(256), 0, s, k, (const double*)part, P, t,
+                     done);
+  hipLaunchKernelGGL(wb_apply, dim3((unsigned)cdiv(n, 256), nrhs), dim3(256), 0, s, n, k, F, op->lda,
+                     reinterpret_cast<const double*>(op->aux), (const double*)t, X, ldx, Y, ldy, (double)op->scale,
+                     (double)op->shift, done);
   CK_CHECK_LAUNCH();
   return CK_OK;
 }

@@ -13,6 +13,7 @@ recurrence's attainable accuracy limit therefore no longer bounds the answer.
 with 64x64 symmetric factors: (Q1 x Q2 x Q3) diag(1/(lambda + s)) (Q1 x Q2 x Q3)^T
 applied with the same MFMA Kronecker passes as the operator (CK_OP_KRON_EIG_F32).
 """
+import ctypes
 import time
 from dataclasses import dataclass
 
@@ -95,11 +96,14 @@ class KroneckerEigInverse(LinearOperator):
 
 
 class _F64Apply:
-    """FP64 application of A for refinement residuals (the matrix as stored)."""
+    """FP64 application of A for refinement residuals (the matrix as stored):
+    64^3 Kronecker operators through ck_kron_apply_f64, dense fp32 matrices (with
+    scale and shift) through ck_dense_apply_f64acc."""
 
     def __init__(self, A, nrhs):
         ks = kron_structure(A)
         self.kind = None
+        got = A.hip_op()
         if ks is not None and all(f.shape[0] == 64 for f in ks[0]) and len(ks[0]) == 3:
             self.kind = "kron64"
             self.f64 = [f.double().contiguous() for f in ks[0]]
@@ -107,12 +111,21 @@ class _F64Apply:
             self.dims = torch.tensor([64, 64, 64], dtype=torch.int32)
             n = 64**3
             self.work = torch.empty(nrhs * n, dtype=torch.float64, device=A.device)
+        elif got is not None and got[0].kind == N.CK_OP_DENSE_F32:
+            self.kind = "dense"
+            self.op, self.keep = got
         else:
             raise N.UnsupportedOperator("mixed-precision refinement", N.CK_ERR_UNSUPPORTED,
-                                        "FP64 residuals need a 64^3 Kronecker operator (plus shift)")
+                                        "FP64 residuals need a dense fp32 matrix or a 64^3 Kronecker operator "
+                                        "(plus shift)")
 
     def __call__(self, X, Y, nrhs):
         ld = X.stride(0)
+        if self.kind == "dense":
+            op = self.op
+            N.call("ck_dense_apply_f64acc", op.n, ctypes.c_void_p(op.a), op.lda, float(op.scale), float(op.shift),
+                   nrhs, N.ptr(X), ld, N.ptr(Y), Y.stride(0), N.stream_ptr(X.device))
+            return
         dims = (ctypes_int32_array(self.dims))
         N.call("ck_kron_apply_f64", 3, dims, N.ptr(self.f64[0]), N.ptr(self.f64[1]), N.ptr(self.f64[2]),
                self.scale, self.shift, nrhs, N.ptr(X), ld, N.ptr(Y), Y.stride(0), N.ptr(self.work),
@@ -120,7 +133,6 @@ class _F64Apply:
 
 
 def ctypes_int32_array(t):
-    import ctypes
     vals = [int(v) for v in t.tolist()]
     return (ctypes.c_int32 * len(vals))(*vals)
 
@@ -138,6 +150,12 @@ class HipMixedCG(Algorithm):
         max_refinements: outer step cap.
         P: preconditioner of the inner CG (approximation of A^-1).
         chunk: CG steps between reads of the device loop flag.
+        inner: operator the correction solves use in place of A (e.g. a bf16 copy of a
+            dense matrix, see ``bf16_inner``); the residuals always use A as stored.
+        contraction: with a lower-precision ``inner``, a refinement that does not cut
+            the FP64 residual by this factor switches the remaining corrections to A
+            itself (the perturbation A - inner is then too large for refinement to
+            converge on this system).
     """
     tol: float = 1e-6
     inner_tol: float = 1e-4
@@ -145,6 +163,8 @@ class HipMixedCG(Algorithm):
     max_refinements: int = 20
     P: object = None
     chunk: int = 8
+    inner: object = None
+    contraction: float = 0.5
 
     def __call__(self, A, b):
         is_vector = b.dim() == 1
@@ -163,10 +183,13 @@ class MixedSolver:
         self.ld = ld_of(n)
         self.f64 = _F64Apply(A, nrhs)
         self.st = CGState(n, nrhs, alg.inner_max_iters, dev, alg.P is not None)
-        aop = lower(A)
-        pop = lower(alg.P) if alg.P is not None else None
-        work, wb = workspace([aop[0] if aop else None, pop[0] if pop else None], nrhs, dev)
-        self.lowered = (aop, pop, work, wb)
+        self.lowered_for = {}
+        self.inner = alg.inner if alg.inner is not None else A
+        for op in {id(A): A, id(self.inner): self.inner}.values():
+            aop = lower(op)
+            pop = lower(alg.P) if alg.P is not None else None
+            work, wb = workspace([aop[0] if aop else None, pop[0] if pop else None], nrhs, dev)
+            self.lowered_for[id(op)] = (aop, pop, work, wb)
         z64 = lambda: torch.zeros((nrhs, self.ld), dtype=torch.float64, device=dev)  # noqa: E731
         self.b64, self.x64, self.Ax64, self.r64 = z64(), z64(), z64(), z64()
         self.r32 = torch.zeros((nrhs, self.ld), dtype=torch.float32, device=dev)
@@ -198,21 +221,28 @@ class MixedSolver:
         s = N.stream_ptr(b.device)
         it = 0
         chunk = alg.chunk
+        inner = self.inner
+        fallback_at = None
         while not (np.all(np.isfinite(rel)) and rel.max() <= alg.tol) and it < alg.max_refinements:
             if not np.all(np.isfinite(rel)):
                 raise FloatingPointError("non-finite FP64 residual in iterative refinement")
-            k, _, _ = cg_blocks(self.A, self.r32, self.st, None, alg.inner_max_iters, alg.inner_tol, alg.P,
-                                chunk, self.lowered, want_info=False)
+            k, _, _ = cg_blocks(inner, self.r32, self.st, None, alg.inner_max_iters, alg.inner_tol, alg.P,
+                                chunk, self.lowered_for[id(inner)], want_info=False)
             inner_steps.append(int(k))
             # the next correction solve usually needs as many steps: enqueue that many before the first check
             chunk = max(1, min(alg.chunk, int(k)))
             N.call("ck_ir_update_f64", n, B, N.ptr(self.x64), ld, N.ptr(self.st.x), ld, s)
+            prev = rel.max()
             rel = self.residual(with_ax=True)
             history.append(float(rel.max()))
             it += 1
+            if inner is not self.A and not (rel.max() <= alg.contraction * prev):
+                inner = self.A
+                fallback_at = it
         # the refined iterate is returned in FP64: rounding it to FP32 would cost ~kappa * 6e-8 of residual
         x = self.x64[:, :n].T
         info = {"refinements": it, "inner_iterations": inner_steps, "rel_residual_f64": rel,
+                "precision_fallback_at": fallback_at,
                 "history": history, "converged": bool(np.all(np.isfinite(rel)) and rel.max() <= alg.tol), "time": time.time() - t0}
         return x, info
 

@@ -547,14 +547,23 @@ class RBFKernel(LinearOperator):
 
 
 class WoodburyInverse(LinearOperator):
-    """(s I + F F^T)^{-1} applied by the Woodbury identity; F is (n, k), k <= 128."""
+    """(s I + F F^T)^{-1} applied by the Woodbury identity; F is (n, k), k <= 128.
 
-    def __init__(self, F, s):
+    The core (s I + F^T F)^{-1} is kept in FP64 and the engine applies it with FP64
+    arithmetic (see CK_OP_WOODBURY_INV)."""
+
+    def __init__(self, F, s, core_inv=None):
         self.F = F.contiguous()
         self.s = float(s)
         k = F.shape[1]
-        core = self.s * torch.eye(k, dtype=torch.float64, device=F.device) + (F.T.double() @ F.double())
-        self.core_inv = torch.linalg.inv(core).to(torch.float32).contiguous()
+        if core_inv is not None:
+            self.core_inv = core_inv.to(torch.float64).contiguous()
+        elif self.F.is_cuda and self.F.dtype == torch.float32 and k <= 64:
+            from .linalg.learned import woodbury_core
+            self.core_inv = woodbury_core(self.F, self.s)[0]
+        else:
+            core = self.s * torch.eye(k, dtype=torch.float64, device=F.device) + (F.T.double() @ F.double())
+            self.core_inv = torch.linalg.inv(core).contiguous()
         self.device = F.device
         super().__init__(F.dtype, (F.shape[0], F.shape[0]), annotations={ann.PSD})
 
@@ -568,5 +577,6 @@ class WoodburyInverse(LinearOperator):
     def _matmat(self, V):
         if self._use_hip(V):
             return self._hip_apply(V)
-        t = self.F.T @ V
-        return (V - self.F @ (self.core_inv @ t)) / self.s
+        Fd = self.F.double()
+        t = Fd.T @ V.double()
+        return ((V.double() - Fd @ (self.core_inv @ t)) / self.s).to(V.dtype)

@@ -53,7 +53,8 @@ const char* ck_status_string(int status);
  *   CK_OP_DIAG_F32   S = diag(a)  (cola/ops/operators.py:338-339)
  *   CK_OP_WOODBURY_INV  y = scale * (x - F (C^-1 (F^T x))): the inverse of
  *                    s I + F F^T when scale = 1/s and C^-1 = (s I + F^T F)^-1;
- *                    a = F (n x rank, row-major fp32, lda), aux = C^-1 (rank x rank fp32)
+ *                    a = F (n x rank, row-major fp32, lda), aux = C^-1 (rank x rank
+ *                    fp64); arithmetic after the loads in fp64, rank <= 128
  *   CK_OP_RBF_F32    S_ij = exp(-|p_i - p_j|^2 / (2 l^2)) built on the fly from
  *                    points a (n x dim fp32, row-major, lda) with l = aux_f
  *                    (replaces Kernel._matmat, cola/ops/operators.py:631-643)
@@ -321,6 +322,68 @@ int ck_hutch_csr_bits(const ck_op* A, int32_t ngroups, const uint64_t* W, float*
 int ck_trace_csr_rademacher_bits(const ck_op* A, int32_t ngroups, const uint64_t* W, double* out, void* work,
                                  size_t work_bytes, void* stream);
 size_t ck_trace_workspace_bytes(int64_t n);
+
+/* ---------------------------------------------------------------- learned preconditioner
+ * No reference counterpart: the preconditioner these build enters the solver as
+ * CG(P=...) (cola/linalg/inverse/cg.py:39-69, applied as z = P r at cg.py:150) in the
+ * form P^-1 = (s I + F F^T) through CK_OP_WOODBURY_INV.
+ *
+ * Per-point MLP din -> 128 -> 128 -> dout with exact (erf) GELU after the two hidden
+ * layers.  Parameters are one flat fp32 array: W1 (din x 128, row-major, y = x W + b),
+ * b1, W2 (128 x 128), b2, W3 (128 x dout), b3.  din <= 8, dout <= 128.
+ * Points X are row-major (n x din, leading dimension ldx).  Output G[i*gsr + c*gsc].
+ * With Z1, Z2 non-NULL the pre-activations are saved feature-major (Z[j*ldz + i]) for
+ * ck_mlp_backward. */
+int64_t ck_mlp_param_count(int32_t din, int32_t dout);
+int ck_mlp_forward(int64_t n, int32_t din, int32_t dout, const float* X, int64_t ldx, const float* params, float* G,
+                   int64_t gsr, int64_t gsc, float* Z1, float* Z2, int64_t ldz, void* stream);
+/* grad = grad_scale * dL/dparams given dL/dG at dG[i*dsr + c*dsc] and the saved
+ * pre-activations.  Deterministic: per-64-point partials summed in index order (FP64). */
+size_t ck_mlp_grad_workspace_bytes(int64_t n, int32_t din, int32_t dout);
+int ck_mlp_backward(int64_t n, int32_t din, int32_t dout, const float* X, int64_t ldx, const float* params,
+                    const float* Z1, const float* Z2, int64_t ldz, const float* dG, int64_t dsr, int64_t dsc,
+                    float grad_scale, float* grad, void* work, size_t work_bytes, void* stream);
+/* Adam (Kingma & Ba 2015) in torch.optim.Adam's form, no weight decay; step >= 1. */
+int ck_adam_step(int64_t count, float* params, const float* grad, float* m, float* v, float lr, float beta1,
+                 float beta2, float eps, int32_t step, void* stream);
+/* C (ka x kb, row-major fp64) = X^T Y for X[i*xsr + a*xsc], Y[i*ysr + b*ysc], i < n;
+ * FP64 accumulation, 512-row partials summed in index order. */
+size_t ck_gram_workspace_bytes(int64_t n, int32_t ka, int32_t kb);
+int ck_gram_f64(int64_t n, int32_t ka, int32_t kb, const float* X, int64_t xsr, int64_t xsc, const float* Y,
+                int64_t ysr, int64_t ysc, double* C, void* work, size_t work_bytes, void* stream);
+/* Y[i, c] = alpha sum_j X[i, j] W[j][c] + beta Y[i, c] (beta == 0 ignores Y's old
+ * contents); W (kin x kout) row-major fp32, kin <= 128. */
+int ck_rows_gemm_f32(int64_t n, int32_t kin, int32_t kout, const float* X, int64_t xsr, int64_t xsc, const float* W,
+                     float alpha, float beta, float* Y, int64_t ysr, int64_t ysc, void* stream);
+/* Batched small SPD (k <= 64): A_b + (jitter_abs + jitter_rel * tr(A_b)/k) I = L L^T.
+ * linv (batch x k x k, required) = L^-1; optional inv64/inv32 = A^-1, linvT32 = L^-T.
+ * status[b] = 0, or j+1 when pivot j is not positive (outputs then undefined). */
+int ck_spd_inverse_f64(int32_t batch, int32_t k, const double* A, int64_t lda, int64_t stride, double jitter_rel,
+                       double jitter_abs, double* linv, double* inv64, float* inv32, float* linvT32, int32_t* status,
+                       void* stream);
+/* Captured-energy loss of features G against weighted target vectors U (w_i >= 0):
+ * L = 1 - tr((G^T G)^-1 G^T U W U^T G) / sum w, given Sinv = (G^T G)^-1 (k x k) and
+ * T = U^T G (m x k).  Writes C1 (m x k), C2 (k x k) with dL/dG = U C1 + G C2.  k, m <= 64. */
+int ck_energy_loss_f64(int32_t k, int32_t m, const double* Sinv, const double* T, const double* w, float* C1,
+                       float* C2, double* loss, void* stream);
+
+/* Rank-r pivoted Cholesky of a dense symmetric PSD fp32 matrix (Harbrecht, Peters &
+ * Schneider, Appl. Numer. Math. 62 (2012); the GPyTorch preconditioner of Gardner et al.
+ * 2018): pivot j = argmax of the residual diagonal (ties: smallest index), column
+ * L[j*ldl + i] = (A[p][i] - sum_{q<j} L_q[i] L_q[p]) / sqrt(d_p).  piv[j], dpiv[j] record
+ * the pivots and their residual diagonals; a non-positive dpiv gives a zero column. */
+size_t ck_pivchol_workspace_bytes(int64_t n);
+int ck_pivchol_dense_f32(int64_t n, const float* A, int64_t lda, int32_t rank, float* L, int64_t ldl, int32_t* piv,
+                         float* dpiv, void* work, size_t work_bytes, void* stream);
+/* Dense RBF kernel matrix K_ij = exp(-|p_i - p_j|^2 / (2 l^2)) (the convention of
+ * CK_OP_RBF_F32), exactly symmetric; K (fp32) and/or Kb (bf16, round to nearest even). */
+int ck_rbf_dense_f32(int64_t n, int32_t dim, const float* P, int64_t ldp, float lengthscale, float* K, int64_t ldk,
+                     uint16_t* Kb, int64_t ldkb, void* stream);
+/* Y = scale A X + shift X with A fp32 (row-major) and X, Y fp64 column blocks, FP64
+ * accumulation: refinement residuals of dense systems (Dense._matmat,
+ * cola/ops/operators.py:26-28, at higher precision). */
+int ck_dense_apply_f64acc(int64_t n, const float* A, int64_t lda, double scale, double shift, int32_t nrhs,
+                          const double* X, int64_t ldx, double* Y, int64_t ldy, void* stream);
 
 #ifdef __cplusplus
 }

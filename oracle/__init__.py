@@ -28,6 +28,9 @@ How each part is pinned (what would catch a dropped term, wrong sign or index):
                    vectors and Lanczos runs to full dimension; 10x10 known spectrum.
 * ``hutch``     -- exact trace with Hadamard probes (sum z z^T = n I); closed-form
                    Laplacian trace.
+* ``learned``   -- MLP gradients vs finite differences, energy-loss limits and
+                   invariance, Adam vs torch.optim.Adam, pivoted Cholesky vs LAPACK
+                   dpstrf, Nystrom exactness on low-rank K, Woodbury vs dense inverse.
 * ``tridiag``   -- LAPACK (scipy.linalg.eigh_tridiagonal) in FP64.
 * ``precond``   -- Woodbury identity against a dense FP64 inverse; pivoted Cholesky
                    against full Cholesky at full rank; MLP gradients against
