@@ -329,6 +329,170 @@ def bench_slq(args, dev):
                        "note": "cola's logdet returns |tr log A| (logdet.py:111-117)"}}
 
 
+def dense_rel_residual(K, shift, x, b):
+    """max_j ||b_j - (K + shift I) x_j|| / ||b_j|| in FP64 with K as stored (fp32), on the engine."""
+    from cola_amd._blocks import ld_of
+    n, B = x.shape
+    ld = ld_of(n)
+    X = torch.zeros((B, ld), dtype=torch.float64, device=K.device)
+    X[:, :n] = x.T.double()
+    Y = torch.empty_like(X)
+    N.call("ck_dense_apply_f64acc", n, N.ptr(K), K.stride(0), 1.0, shift, B, N.ptr(X), ld, N.ptr(Y), ld,
+           N.stream_ptr(K.device))
+    bd = b.T.double()
+    return ((Y[:, :n] - bd).norm(dim=1) / bd.norm(dim=1)).cpu().numpy()
+
+
+def bench_learned_cg(args, dev):
+    """Dense RBF systems on 16384 points in [0,1]^4 (lengthscale ~ LogU[0.05,1], noise ~
+    LogU[1e-4,1e-1], 8 N(0,1) rhs each): FP64-refined PCG with the learned (or, when it
+    captures less, pivoted-Cholesky) rank-64 preconditioner vs plain FP32 CG."""
+    from cola_amd.linalg import learned as L
+    from cola_amd.linalg.cg import cg
+    from cola_amd.linalg.mixed import HipMixedCG, MixedSolver
+    n, B = args.n, 8
+    # --- training: top-64 Lanczos eigenvectors of seeded instances, Adam on the engine
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    insts = [L.make_training_instance(s, args.train_n, m=64, ell_range=(0.05, 1.0), device=dev)
+             for s in range(args.train_instances)]
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    net = L.PointMLP(4, 64, device=dev, seed=0)
+    probe = insts[:min(64, len(insts))]
+    loss0 = L.mean_loss(net, probe)
+    t1 = time.perf_counter()
+    losses = L.train(net, insts, epochs=args.train_epochs, lr=2e-3, seed=0)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    loss1 = L.mean_loss(net, probe)
+    del insts, probe
+    # --- held-out test systems
+    systems = []
+    for j in range(args.systems):
+        X, ell, noise = L.sample_instance(100000 + j, n)
+        Xd = X.to(dev)
+        K = L.rbf_matrix(Xd, ell)
+        s32 = float(np.float32(noise))
+        Kop = ca.Dense(K)
+        A = ca.PSD(Kop + noise * ca.I_like(Kop))
+        b = torch.randn(n, B, generator=torch.Generator().manual_seed(200000 + j)).to(dev)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
+        Pl = L.LearnedPreconditioner(K, Xd, net, noise)
+        Pc = L.PivotedCholeskyPreconditioner(K, 64, noise)
+        P, rep = L.select_preconditioner(K, Xd, noise, candidates={"learned": Pl, "pivoted_cholesky": Pc}, A=A)
+        torch.cuda.synchronize()
+        systems.append({"K": K, "A": A, "b": b, "shift": s32, "ell": ell, "noise": noise, "Pl": Pl, "Pc": Pc,
+                        "P": P, "choice": rep["choice"], "setup_s": time.perf_counter() - ts})
+
+    def timed_cg(sysd, P):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        # the FP32 solvers get a 10x tighter recursive-residual target so that their true
+        # (FP64-checked) residual can reach args.tol; 1000 is the reference's step cap
+        x, info = cg(sysd["A"], sysd["b"], P=P, tol=args.tol / 10, max_iters=1000, chunk=16)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        res = dense_rel_residual(sysd["K"], sysd["shift"], x, sysd["b"])
+        return {"steps": int(info["iterations"]) - 1, "s": dt, "max_res": float(res.max()),
+                "solved": int((res <= args.tol).sum())}
+
+    per = []
+    tot = {"plain": [0.0, 0], "pivchol": [0.0, 0], "learned": [0.0, 0]}
+    for sd in systems:
+        row = {"ell": sd["ell"], "noise": sd["noise"], "choice": sd["choice"], "setup_s": sd["setup_s"],
+               "resid_trace_learned": sd["Pl"].residual_trace, "resid_trace_pivchol": sd["Pc"].residual_trace}
+        for name, P in (("plain", None), ("pivchol", sd["Pc"]), ("learned", sd["Pl"])):
+            r = timed_cg(sd, P)
+            row[name] = r
+            tot[name][0] += r["s"]
+            tot[name][1] += r["solved"]
+        per.append(row)
+    # --- headline: learned/selected preconditioner inside FP64-residual refinement
+    solvers = []
+    for sd in systems:
+        alg = HipMixedCG(tol=args.tol, inner_tol=args.inner_tol, inner_max_iters=1000, max_refinements=20, P=sd["P"],
+                         chunk=args.chunk)
+        solvers.append(MixedSolver(sd["A"], B, alg))
+    for _ in range(args.warmup):
+        for sv, sd in zip(solvers, systems):
+            sv.solve(sd["b"])
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    infos = []
+    for _ in range(args.steps):
+        infos = []
+        for sv, sd in zip(solvers, systems):
+            x, info = sv.solve(sd["b"])
+            infos.append((x, info))
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t2) / args.steps
+    solved = 0
+    for (x, info), sd, row in zip(infos, systems, per):
+        res = dense_rel_residual(sd["K"], sd["shift"], x, sd["b"])
+        solved += int((res <= args.tol).sum())
+        row["mixed"] = {"refinements": info["refinements"], "inner_steps": info["inner_iterations"],
+                        "max_res": float(res.max())}
+    total_rhs = B * len(systems)
+    plain_rate = tot["plain"][1] / tot["plain"][0]
+    # --- roofline: the dense matvec (8 rhs) the solves spend their time in
+    from cola_amd._blocks import ld_of
+    ld = ld_of(n)
+    Xb = torch.randn(B, ld, device=dev)
+    Yb = torch.empty_like(Xb)
+    op, keep = systems[0]["A"].hip_op()
+    wb = N.q("ck_op_workspace_bytes", op, B)
+    work = torch.empty(max(wb, 4) // 4, device=dev)
+    t_mv = time_kernel(lambda: N.call("ck_op_apply", op, B, N.ptr(Xb), ld, N.ptr(Yb), ld, N.ptr(work), wb,
+                                      N.stream_ptr(dev)), reps=20)
+    mv_bytes = 4 * n * n + 2 * 4 * n * B
+    roof = {"bound": "hbm", "kernel": "dense_f32_kernel<8,4> (K + noise I, 8 rhs)", "achieved": mv_bytes / t_mv / 1e9,
+            "peak": HBM_PEAK, "unit": "GB/s", "frac": mv_bytes / t_mv / 1e9 / HBM_PEAK, "traffic": None,
+            "launch_us": t_mv * 1e6, "algorithmic_bytes_per_launch": mv_bytes}
+    out = {"metric": METRIC, "value": solved / dt, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32 (matvec, PCG) + f64 (refinement residuals, Woodbury core)",
+           "data": "synthetic (seeded points, lengthscales, noise, N(0,1) rhs)",
+           "config": {"workload": f"learned_cg: {len(systems)} dense RBF systems, N={n} points in [0,1]^4, "
+                                  f"{B} rhs each, to FP64-checked {args.tol:g}",
+                      "global_batch": total_rhs, "seq_len": n, "parallelism": "single GPU",
+                      "rhs_solved": solved, "rhs_total": total_rhs,
+                      "training": {"instances": args.train_instances, "points": args.train_n,
+                                   "epochs": args.train_epochs, "lanczos_data_s": t_data, "train_s": t_train,
+                                   "energy_loss_before": loss0, "energy_loss_after": loss1,
+                                   "last_step_loss": float(losses[-1]) if len(losses) else None},
+                      "plain_fp32_cg_solves_per_s": plain_rate,
+                      "pivchol_pcg_solves_per_s": tot["pivchol"][1] / tot["pivchol"][0],
+                      "learned_pcg_solves_per_s": tot["learned"][1] / tot["learned"][0],
+                      "speedup_vs_plain": (solved / dt) / plain_rate if plain_rate > 0 else None,
+                      "plain_s_per_rhs_attempt": tot["plain"][0] / total_rhs,
+                      "speedup_vs_plain_attempt_time": (tot["plain"][0] / total_rhs) / (dt / max(solved, 1)),
+                      "systems": per},
+           "roofline": roof}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline_dense(systems[0], n, B, per[0]["plain"]["steps"], args.tol)
+    return out
+
+
+def cpu_baseline_dense(sd, n, B, k_steps, tol, iters=10):
+    """The oracle's plain CG (the reference algorithm, torch on the host) on the first system:
+    `iters` steps timed, scaled to the steps the engine's plain CG took."""
+    from oracle import cg as ocg
+    torch.set_num_threads(cpu_threads())
+    K = sd["K"].cpu()
+    b = sd["b"].cpu()
+    s = sd["shift"]
+    mm = lambda V: K @ V + s * V  # noqa: E731
+    t = time.perf_counter()
+    ocg.cg(mm, b, tol=1e-30, max_iters=iters)
+    per_step = (time.perf_counter() - t) / iters
+    total = per_step * max(k_steps, 1)
+    return {"value": B / total, "unit": "solves/s", "cores": cpu_threads(), "kind": "port",
+            "sample": f"{iters} CG steps of the first {n}-point system x {B} rhs ({per_step * 1e3:.1f} ms/step), "
+                      f"scaled to the {k_steps} steps plain CG takes on it"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -339,6 +503,11 @@ def main():
     ap.add_argument("--chunk", type=int, default=8)
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--n", type=int, default=16384, help="points per system (learned_cg)")
+    ap.add_argument("--systems", type=int, default=4, help="held-out systems (learned_cg)")
+    ap.add_argument("--train-instances", type=int, default=2000)
+    ap.add_argument("--train-n", type=int, default=1024)
+    ap.add_argument("--train-epochs", type=int, default=4)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus > 1 is not supported"}))
@@ -346,7 +515,7 @@ def main():
     dev = torch.device("cuda:0")
     N.load()
     fn = {"kron_cg": bench_kron_cg, "copy": bench_copy, "dense_cg": bench_dense_cg, "lanczos": bench_lanczos,
-          "hutch": bench_hutch, "slq": bench_slq}[args.workload]
+          "hutch": bench_hutch, "slq": bench_slq, "learned_cg": bench_learned_cg}[args.workload]
     out = fn(args, dev)
     print(json.dumps(out), flush=True)
 

@@ -6,7 +6,7 @@ hand-written gfx950 HIP kernels behind a C ABI (include/cola_krylov.h).
 from . import annotations, linalg, ops
 from .annotations import PSD, Hermitian, SelfAdjoint, Stiefel, Unitary
 from .fns import add, densify, dot, kron, lazify, mul
-from .ops import (Dense, DenseBF16, Diagonal, I_like, Identity, Kernel, Kronecker, LinearOperator, Product,
+from .ops import (BlockDiag, Dense, DenseBF16, Diagonal, I_like, Identity, Kernel, Kronecker, LinearOperator, Product,
                   RBFKernel, ScalarMul, Sparse, Sum, Tridiagonal, WoodburyInverse)
 
 __version__ = "0.1.0"

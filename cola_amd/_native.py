@@ -54,6 +54,16 @@ class CkOp(ctypes.Structure):
     ]
 
 
+class CkBcgPlan(ctypes.Structure):
+    """Mirror of ck_bcg_plan in include/cola_krylov.h (batched uneven systems)."""
+    _fields_ = [("nsys", ctypes.c_int32), ("rows_per_tile", ctypes.c_int32), ("ntiles_mv", ctypes.c_int32),
+                ("ntiles_vec", ctypes.c_int32), ("max_iters", ctypes.c_int32), ("pad0", ctypes.c_int32)] + \
+               [(name, ctypes.c_void_p) for name in
+                ("tile_sys", "tile_row0", "vtile_sys", "vtile_i0", "n", "lda", "voff", "moff", "mt_begin", "mt_end",
+                 "vt_begin", "vt_end", "shift", "prec", "A32", "A16", "x", "r", "p", "Ap", "part_mv", "part_vec",
+                 "gamma", "alpha", "beta", "thresh", "k", "done")]
+
+
 _i32, _i64, _u64, _f32, _f64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_float, ctypes.c_double
 _p, _sz = ctypes.c_void_p, ctypes.c_size_t
 _op = ctypes.POINTER(CkOp)
@@ -124,6 +134,16 @@ _PROTOS = {
     "ck_pivchol_dense_f32": (ctypes.c_int, [_i64, _p, _i64, _i32, _p, _i64, _p, _p, _p, _sz, _p]),
     "ck_rbf_dense_f32": (ctypes.c_int, [_i64, _i32, _p, _i64, _f32, _p, _i64, _p, _i64, _p]),
     "ck_dense_apply_f64acc": (ctypes.c_int, [_i64, _p, _i64, _f64, _f64, _i32, _p, _i64, _p, _i64, _p]),
+    # batched uneven systems
+    "ck_bcg_vec_tile": (_i32, []),
+    "ck_bcg_start": (ctypes.c_int, [ctypes.POINTER(CkBcgPlan), _p, _f32, _f32, _p]),
+    "ck_bcg_iterate": (ctypes.c_int, [ctypes.POINTER(CkBcgPlan), _i32, _p]),
+    "ck_bcg_matvec": (ctypes.c_int, [ctypes.POINTER(CkBcgPlan), _p]),
+    "ck_bcg_graph_create": (ctypes.c_int, [ctypes.POINTER(CkBcgPlan), _i32, _p, ctypes.POINTER(ctypes.c_void_p)]),
+    "ck_graph_launch": (ctypes.c_int, [_p, _p]),
+    "ck_graph_destroy": (ctypes.c_int, [_p]),
+    "ck_bcg_residual_f64": (ctypes.c_int, [ctypes.POINTER(CkBcgPlan), _p, _p, _p, _p, _p, _p, _p, _p]),
+    "ck_bcg_update_f64": (ctypes.c_int, [ctypes.POINTER(CkBcgPlan), _p, _p, _p]),
 }
 
 _lib = None

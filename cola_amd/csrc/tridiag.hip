@@ -45,59 +45,83 @@ __global__ __launch_bounds__(256) void tridiag_ql_kernel(int m, const float* __r
   if (tid == 0) ctl_fail = 0;
   __syncthreads();
 
+  const int lane = tid & 63;
   for (int l = 0; l < m; ++l) {
     for (int iter = 0;; ++iter) {
-      if (tid == 0) {
-        int mm = l;
-        for (; mm < m - 1; ++mm) {
-          const double dd = fabs(d[mm]) + fabs(d[mm + 1]);
-          if (e[mm] == 0.0 || fabs(e[mm]) <= DBL_EPSILON * dd) break;
+      if (tid < 64) {
+        // first mm >= l with a negligible off-diagonal (m-1 if none): 64 indices per ballot
+        int mm = m - 1;
+        for (int base = l; base < m - 1; base += 64) {
+          const int idx = base + lane;
+          bool small = false;
+          if (idx < m - 1) {
+            const double dd = fabs(d[idx]) + fabs(d[idx + 1]);
+            small = e[idx] == 0.0 || fabs(e[idx]) <= DBL_EPSILON * dd;
+          }
+          const uint64_t bal = __ballot(small);
+          if (bal) {
+            mm = base + __ffsll((unsigned long long)bal) - 1;
+            break;
+          }
         }
-        ctl_mm = mm;
-        ctl_lo = mm;
-        if (mm != l) {
-          if (iter >= kMaxSweeps) {
-            ctl_fail = 1;
-            ctl_mm = l;  // give up on this eigenvalue
-          } else {
-            // Wilkinson shift from the leading 2x2 block
-            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-            double r = hypot(g, 1.0);
-            g = d[mm] - d[l] + e[l] / (g + copysign(r, g));
-            double s = 1.0, c = 1.0, p = 0.0;
-            bool early = false;
-            int i = mm - 1;
-            for (; i >= l; --i) {
-              const double f = s * e[i];
-              const double bb = c * e[i];
-              r = hypot(f, g);
-              e[i + 1] = r;
-              if (r == 0.0) {  // deflation inside the sweep
-                d[i + 1] -= p;
+        if (tid == 0) {
+          ctl_mm = mm;
+          ctl_lo = mm;
+          if (mm != l) {
+            if (iter >= kMaxSweeps) {
+              ctl_fail = 1;
+              ctl_mm = l;  // give up on this eigenvalue
+            } else {
+              // Wilkinson shift from the leading 2x2 block
+              double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+              double r = hypot(g, 1.0);
+              g = d[mm] - d[l] + e[l] / (g + copysign(r, g));
+              double s = 1.0, c = 1.0, p = 0.0;
+              bool early = false;
+              int i = mm - 1;
+              // d[i+1] is read by step i and never written before it; d[i] and e[i] are
+              // fresh loads, fetched one step ahead so only the arithmetic is serial
+              double dip1 = d[mm], di = d[i], ei = e[i];
+              double zcarry = Z ? 0.0 : z0[mm];
+              for (; i >= l; --i) {
+                const double dn = i > l ? d[i - 1] : 0.0;
+                const double en = i > l ? e[i - 1] : 0.0;
+                const double f = s * ei;
+                const double bb = c * ei;
+                r = sqrt(fma(f, f, g * g));
+                e[i + 1] = r;
+                if (r == 0.0) {  // deflation inside the sweep
+                  d[i + 1] = dip1 - p;
+                  e[mm] = 0.0;
+                  early = true;
+                  break;
+                }
+                const double inv = 1.0 / r;
+                s = f * inv;
+                c = g * inv;
+                g = dip1 - p;
+                r = (di - g) * s + 2.0 * c * bb;
+                p = s * r;
+                d[i + 1] = g + p;
+                g = c * r - bb;
+                cs[i] = c;
+                sn[i] = s;
+                if (!Z) {  // track the first row only (column i+1's value is the carry)
+                  const double zi = z0[i];
+                  z0[i + 1] = s * zi + c * zcarry;
+                  zcarry = c * zi - s * zcarry;
+                }
+                dip1 = di;
+                di = dn;
+                ei = en;
+              }
+              ctl_lo = early ? i + 1 : l;
+              if (!Z) z0[early ? i + 1 : l] = zcarry;
+              if (!early) {
+                d[l] -= p;
+                e[l] = g;
                 e[mm] = 0.0;
-                early = true;
-                break;
               }
-              s = f / r;
-              c = g / r;
-              g = d[i + 1] - p;
-              r = (d[i] - g) * s + 2.0 * c * bb;
-              p = s * r;
-              d[i + 1] = g + p;
-              g = c * r - bb;
-              cs[i] = c;
-              sn[i] = s;
-              if (!Z) {  // track the first row only
-                const double zf = z0[i + 1];
-                z0[i + 1] = s * z0[i] + c * zf;
-                z0[i] = c * z0[i] - s * zf;
-              }
-            }
-            ctl_lo = early ? i + 1 : l;
-            if (!early) {
-              d[l] -= p;
-              e[l] = g;
-              e[mm] = 0.0;
             }
           }
         }
@@ -107,15 +131,32 @@ __global__ __launch_bounds__(256) void tridiag_ql_kernel(int m, const float* __r
       __syncthreads();  // every lane has read the control words before lane 0 may rewrite them
       if (mm == l) break;
       if (Z) {
-        // rotations (i, i+1) for i = mm-1 down to lo, applied to every row
+        // rotations (i, i+1) for i = mm-1 down to lo, applied to every row.  The value
+        // rotation i leaves in column i is the one rotation i-1 reads, so it stays in a
+        // register (carry) and each row streams its columns with loads that do not wait
+        // on the stores before them (8 loads in flight per row).
         for (int row = tid; row < m; row += blockDim.x) {
           double* zr = Z + (int64_t)row * m;
-          for (int i = mm - 1; i >= lo; --i) {
-            const double c = cs[i], s = sn[i];
-            const double zf = zr[i + 1];
-            zr[i + 1] = s * zr[i] + c * zf;
-            zr[i] = c * zr[i] - s * zf;
+          double carry = zr[mm];
+          int i = mm - 1;
+          for (; i - 7 >= lo; i -= 8) {
+            double a[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a[u] = zr[i - u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              const double c = cs[i - u], s = sn[i - u];
+              zr[i - u + 1] = s * a[u] + c * carry;
+              carry = c * a[u] - s * carry;
+            }
           }
+          for (; i >= lo; --i) {
+            const double a = zr[i];
+            const double c = cs[i], s = sn[i];
+            zr[i + 1] = s * a + c * carry;
+            carry = c * a - s * carry;
+          }
+          zr[lo] = carry;
         }
       }
       __syncthreads();

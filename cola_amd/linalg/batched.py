@@ -1,0 +1,396 @@
+"""Batched CG over many independent dense systems of uneven size (block-diagonal operators).
+
+cola solves a block-diagonal system block by block: ``inv(BlockDiag) =
+BlockDiag(inv(M_i))`` (cola/linalg/inverse/inv.py:135-136), i.e. one CG loop per
+block (cg.py:94-178).  ``HipBatchedCG`` runs every block in the same launches
+(``ck_bcg_*``): five kernels per iteration whatever the number of blocks, systems
+that have converged skip their work, and runs of iterations are replayed from a
+HIP graph.  ``BatchedMixedSolver`` adds FP64 refinement (residuals with the fp32
+matrices as stored) around inner solves that may read bf16 copies of the
+matrices, with a per-system fall back to fp32 storage when a refinement step does
+not contract.  ``autotune`` picks the matvec tile height by timing the candidates.
+"""
+import ctypes
+import time
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .. import _native as N
+from ..ops import Dense, LinearOperator, Product, Sum
+from .algorithm_base import Algorithm
+
+_ROWS = (4, 8, 16, 32)
+_TUNE_CACHE = {}
+
+
+def _round_up(v, m):
+    return (v + m - 1) // m * m
+
+
+class BatchedSystems:
+    """Concatenated storage for systems (K_s + shift_s I), s = 0..S-1: fp32 matrices (and
+    optional bf16 copies) row-major with a leading dimension that is a multiple of 8."""
+
+    def __init__(self, sizes, shifts, device, bf16=False):
+        self.sizes = np.asarray(sizes, dtype=np.int64)
+        if self.sizes.ndim != 1 or self.sizes.size < 1 or self.sizes.min() < 1:
+            raise ValueError("sizes must be a non-empty list of positive orders")
+        self.nsys = int(self.sizes.size)
+        self.device = torch.device(device)
+        self.lda = _round_up(self.sizes, 8)
+        msz = self.sizes * self.lda
+        self.moff = np.concatenate([[0], np.cumsum(msz)[:-1]]).astype(np.int64)
+        vsz = _round_up(self.sizes, 64)
+        self.voff = np.concatenate([[0], np.cumsum(vsz)[:-1]]).astype(np.int64)
+        self.total_v = int(vsz.sum())
+        self.total_m = int(msz.sum())
+        self.A32 = torch.zeros(self.total_m, dtype=torch.float32, device=self.device)
+        self.A16 = torch.zeros(self.total_m, dtype=torch.bfloat16, device=self.device) if bf16 else None
+        self.shift = torch.tensor(np.asarray(shifts, dtype=np.float32).reshape(-1).repeat(
+            1 if np.size(shifts) == self.nsys else self.nsys), device=self.device)
+
+    def matrix(self, s, dtype=torch.float32):
+        n, ld, o = int(self.sizes[s]), int(self.lda[s]), int(self.moff[s])
+        buf = self.A32 if dtype == torch.float32 else self.A16
+        return buf[o:o + n * ld].view(n, ld)[:, :n]
+
+    def segment(self, vec, s):
+        o, n = int(self.voff[s]), int(self.sizes[s])
+        return vec[o:o + n]
+
+    def concat(self, vectors, dtype=torch.float32):
+        """Concatenate per-system vectors into the padded layout."""
+        out = torch.zeros(self.total_v, dtype=dtype, device=self.device)
+        for s, v in enumerate(vectors):
+            self.segment(out, s).copy_(v.reshape(-1))
+        return out
+
+    def split(self, vec):
+        return [self.segment(vec, s) for s in range(self.nsys)]
+
+    @classmethod
+    def rbf(cls, points, lengthscales, shifts, device, bf16=False):
+        """Dense RBF kernel systems exp(-|p_i - p_j|^2 / (2 l_s^2)) + shift_s I built on the engine."""
+        sizes = [int(P.shape[0]) for P in points]
+        out = cls(sizes, shifts, device, bf16=bf16)
+        st = N.stream_ptr(out.device)
+        for s, P in enumerate(points):
+            Pd = P.to(device=out.device, dtype=torch.float32).contiguous()
+            o = int(out.moff[s])
+            kb = None if out.A16 is None else N.ptr(out.A16[o:])
+            N.call("ck_rbf_dense_f32", sizes[s], Pd.shape[1], N.ptr(Pd), Pd.stride(0), float(lengthscales[s]),
+                   N.ptr(out.A32[o:]), int(out.lda[s]), kb, int(out.lda[s]), st)
+        return out
+
+    @classmethod
+    def dense(cls, mats, shifts, device, bf16=False):
+        out = cls([int(M.shape[0]) for M in mats], shifts, device, bf16=bf16)
+        for s, M in enumerate(mats):
+            out.matrix(s).copy_(M)
+            if bf16:
+                out.matrix(s, torch.bfloat16).copy_(M.to(torch.bfloat16))
+        return out
+
+
+class BatchedCG:
+    """Device state, tile lists and the ck_bcg_plan of one batched CG configuration."""
+
+    def __init__(self, systems, rows_per_tile=16, max_iters=1000):
+        if rows_per_tile not in _ROWS:
+            raise ValueError(f"rows_per_tile must be one of {_ROWS}")
+        sy = self.systems = systems
+        dev = sy.device
+        self.rows = rows_per_tile
+        self.max_iters = int(max_iters)
+        vt = int(N.q("ck_bcg_vec_tile"))
+        tsys, trow, vsys, vi0 = [], [], [], []
+        mtb, mte, vtb, vte = [], [], [], []
+        for s, n in enumerate(sy.sizes.tolist()):
+            mtb.append(len(tsys))
+            rows = list(range(0, n, rows_per_tile))
+            tsys += [s] * len(rows)
+            trow += rows
+            mte.append(len(tsys))
+            vtb.append(len(vsys))
+            idx = list(range(0, n, vt))
+            vsys += [s] * len(idx)
+            vi0 += idx
+            vte.append(len(vsys))
+        i32 = lambda a: torch.tensor(np.asarray(a, dtype=np.int32), device=dev)  # noqa: E731
+        i64 = lambda a: torch.tensor(np.asarray(a, dtype=np.int64), device=dev)  # noqa: E731
+        self.t = {"tile_sys": i32(tsys), "tile_row0": i32(trow), "vtile_sys": i32(vsys), "vtile_i0": i32(vi0),
+                  "n": i32(sy.sizes), "lda": i32(sy.lda), "voff": i64(sy.voff), "moff": i64(sy.moff),
+                  "mt_begin": i32(mtb), "mt_end": i32(mte), "vt_begin": i32(vtb), "vt_end": i32(vte)}
+        S = sy.nsys
+        f32 = lambda m: torch.zeros(m, dtype=torch.float32, device=dev)  # noqa: E731
+        self.x, self.r, self.p, self.Ap = (f32(sy.total_v) for _ in range(4))
+        self.part_mv, self.part_vec = f32(len(tsys)), f32(len(vsys))
+        self.gamma, self.alpha, self.beta, self.thresh = (f32(S) for _ in range(4))
+        self.k = torch.zeros(S, dtype=torch.int32, device=dev)
+        self.done = torch.zeros(S, dtype=torch.int32, device=dev)
+        self.prec = torch.zeros(S, dtype=torch.int32, device=dev)
+        self.ntiles_mv, self.ntiles_vec = len(tsys), len(vsys)
+        pl = N.CkBcgPlan()
+        pl.nsys, pl.rows_per_tile, pl.ntiles_mv, pl.ntiles_vec = S, rows_per_tile, len(tsys), len(vsys)
+        pl.max_iters = self.max_iters
+        for name, tns in self.t.items():
+            setattr(pl, name, tns.data_ptr())
+        pl.shift, pl.prec = sy.shift.data_ptr(), self.prec.data_ptr()
+        pl.A32 = sy.A32.data_ptr()
+        pl.A16 = sy.A16.data_ptr() if sy.A16 is not None else None
+        for name in ("x", "r", "p", "Ap", "part_mv", "part_vec", "gamma", "alpha", "beta", "thresh", "k", "done"):
+            setattr(pl, name, getattr(self, name).data_ptr())
+        self.plan = pl
+        self.stream = torch.cuda.Stream(device=dev)  # default priority; graphs need a non-legacy stream
+        self.graphs = {}
+        self._pinned = torch.empty(S, dtype=torch.int32, pin_memory=True)
+
+    def set_precision(self, bf16_mask):
+        """Per-system matvec storage: True reads the bf16 copy (needs systems built with bf16)."""
+        m = np.asarray(bf16_mask, dtype=np.int32).reshape(-1)
+        if m.any() and self.systems.A16 is None:
+            raise ValueError("these systems have no bf16 copy")
+        self.prec.copy_(torch.from_numpy(np.broadcast_to(m, (self.systems.nsys, )).copy()))
+
+    def _graph(self, iters):
+        g = self.graphs.get(iters)
+        if g is None:
+            h = ctypes.c_void_p()
+            N.call("ck_bcg_graph_create", ctypes.byref(self.plan), int(iters), ctypes.c_void_p(self.stream.cuda_stream),
+                   ctypes.byref(h))
+            g = self.graphs[iters] = h
+        return g
+
+    def __del__(self):
+        try:
+            for h in self.graphs.values():
+                N.q("ck_graph_destroy", h)
+        except Exception:
+            pass
+
+    def matvec(self):
+        N.call("ck_bcg_matvec", ctypes.byref(self.plan), ctypes.c_void_p(self.stream.cuda_stream))
+
+    def solve(self, b, tol_rel=1e-6, tol_abs=0.0, active=None, chunk=8, use_graph=True):
+        """CG on every active system with rhs b (concatenated, fp32) until ||r_s|| <= tol_rel ||b_s|| + tol_abs.
+        Returns (x concatenated view, per-system iteration counts)."""
+        S = self.systems.nsys
+        st = ctypes.c_void_p(self.stream.cuda_stream)
+        self.stream.wait_stream(torch.cuda.current_stream(self.systems.device))
+        with torch.cuda.stream(self.stream):
+            if active is None:
+                self.done.zero_()
+            else:
+                self.done.copy_(torch.from_numpy((~np.asarray(active, dtype=bool)).astype(np.int32)))
+            N.call("ck_bcg_start", ctypes.byref(self.plan), N.ptr(b), float(tol_rel), float(tol_abs), st)
+            steps = 0
+            while steps < self.max_iters:
+                n_it = min(int(chunk), self.max_iters - steps)
+                if use_graph:
+                    N.call("ck_graph_launch", self._graph(n_it), st)
+                else:
+                    N.call("ck_bcg_iterate", ctypes.byref(self.plan), n_it, st)
+                steps += n_it
+                self._pinned.copy_(self.done, non_blocking=True)
+                self.stream.synchronize()
+                if bool(self._pinned.all()):
+                    break
+            ks = self.k.cpu().numpy().copy()
+        torch.cuda.current_stream(self.systems.device).wait_stream(self.stream)
+        return self.x, ks
+
+
+def time_matvec(bcg, reps=10):
+    """Average ck_bcg_matvec duration (HIP events on the solver's stream)."""
+    bcg.done.zero_()
+    bcg.p.normal_()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(bcg.stream):
+        bcg.matvec()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(bcg.stream)
+        for _ in range(reps):
+            bcg.matvec()
+        e1.record(bcg.stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps * 1e-3
+
+
+def autotune(systems, bf16_mask=None, candidates=_ROWS, reps=10, max_iters=1000):
+    """Time the batched matvec for each tile height and return (best rows_per_tile, timings in s).
+    Results are cached per (sizes, precision mask)."""
+    mask = np.zeros(systems.nsys, dtype=bool) if bf16_mask is None else np.asarray(bf16_mask, dtype=bool)
+    key = (systems.sizes.tobytes(), mask.tobytes())
+    if key in _TUNE_CACHE:
+        return _TUNE_CACHE[key]
+    timings = {}
+    for R in candidates:
+        bcg = BatchedCG(systems, R, max_iters)
+        bcg.set_precision(mask)
+        timings[R] = time_matvec(bcg, reps)
+        del bcg
+    best = min(timings, key=timings.get)
+    _TUNE_CACHE[key] = (best, timings)
+    return best, timings
+
+
+@dataclass
+class HipBatchedCG(Algorithm):
+    """Batched CG over the blocks of a block-diagonal operator (one launch set for all blocks).
+
+    Args:
+        tol: relative residual target per block (||r|| <= tol ||b||).
+        max_iters: step cap per block.
+        chunk: iterations per graph replay between reads of the done flags.
+        rows_per_tile: matvec tile height, or None to autotune.
+        use_graph: replay captured iterations from a HIP graph.
+    """
+    tol: float = 1e-6
+    max_iters: int = 1000
+    chunk: int = 8
+    rows_per_tile: object = None
+    use_graph: bool = True
+
+
+class BatchedMixedSolver:
+    """FP64 refinement around batched CG: r = b - A x with the fp32 matrices as stored and FP64
+    arithmetic, inner solves on bf16 (or fp32) storage, per-system fallback to fp32 storage when a
+    refinement step does not cut the residual by ``contraction``."""
+
+    def __init__(self, systems, rows_per_tile=None, inner_max_iters=1000, bf16=True):
+        self.systems = systems
+        mask = np.full(systems.nsys, bool(bf16 and systems.A16 is not None))
+        if rows_per_tile is None:
+            rows_per_tile, self.tune = autotune(systems, mask, max_iters=inner_max_iters)
+        else:
+            self.tune = None
+        self.cg = BatchedCG(systems, rows_per_tile, inner_max_iters)
+        self.bf16 = mask
+        dev = systems.device
+        self.x64 = torch.zeros(systems.total_v, dtype=torch.float64, device=dev)
+        self.b64 = torch.zeros_like(self.x64)
+        self.r32 = torch.zeros(systems.total_v, dtype=torch.float32, device=dev)
+        self.prr = torch.zeros(self.cg.ntiles_mv, dtype=torch.float64, device=dev)
+        self.pbb = torch.zeros_like(self.prr)
+        self.rel = torch.zeros(systems.nsys, dtype=torch.float64, device=dev)
+        self.active = torch.ones(systems.nsys, dtype=torch.int32, device=dev)
+
+    def residual(self, active):
+        cg = self.cg
+        st = ctypes.c_void_p(cg.stream.cuda_stream)
+        with torch.cuda.stream(cg.stream):
+            self.active.copy_(torch.from_numpy(active.astype(np.int32)))
+            N.call("ck_bcg_residual_f64", ctypes.byref(cg.plan), N.ptr(self.active), N.ptr(self.x64), N.ptr(self.b64),
+                   N.ptr(self.r32), N.ptr(self.prr), N.ptr(self.pbb), N.ptr(self.rel), st)
+            rel = self.rel.cpu().numpy().copy()
+        return rel
+
+    def solve(self, b, tol=1e-8, inner_tol=1e-4, max_refinements=20, contraction=0.5, chunk=8, use_graph=True):
+        """b: concatenated rhs (fp32 or fp64).  Returns (x64 concatenated, info)."""
+        t0 = time.time()
+        S = self.systems.nsys
+        cg = self.cg
+        self.stream_sync()
+        self.b64.copy_(b.to(torch.float64))
+        self.x64.zero_()
+        prec = self.bf16.copy()
+        cg.set_precision(prec)
+        everyone = np.ones(S, dtype=bool)
+        rel = self.residual(everyone)
+        rel_all = rel.copy()
+        inner_total = np.zeros(S, dtype=np.int64)
+        fallbacks = 0
+        it = 0
+        history = [float(rel.max())]
+        while it < max_refinements:
+            active = ~(rel_all <= tol)
+            if not active.any():
+                break
+            if not np.all(np.isfinite(rel_all)):
+                raise FloatingPointError("non-finite FP64 residual in batched refinement")
+            _, ks = cg.solve(self.r32, tol_rel=inner_tol, tol_abs=0.0, active=active, chunk=chunk,
+                             use_graph=use_graph)
+            inner_total[active] += ks[active]
+            with torch.cuda.stream(cg.stream):
+                self.active.copy_(torch.from_numpy(active.astype(np.int32)))
+                N.call("ck_bcg_update_f64", ctypes.byref(cg.plan), N.ptr(self.active), N.ptr(self.x64),
+                       ctypes.c_void_p(cg.stream.cuda_stream))
+            new = self.residual(active)
+            stalled = active & prec & ~(new <= contraction * rel_all)
+            if stalled.any():
+                prec = prec & ~stalled
+                fallbacks += int(stalled.sum())
+                cg.set_precision(prec)
+            rel_all = np.where(active, new, rel_all)
+            history.append(float(rel_all.max()))
+            it += 1
+        torch.cuda.synchronize()
+        info = {"refinements": it, "rel_residual_f64": rel_all, "inner_iterations": inner_total,
+                "bf16_systems_final": int(prec.sum()), "precision_fallbacks": fallbacks, "history": history,
+                "converged": bool(np.all(rel_all <= tol)), "time": time.time() - t0}
+        return self.x64, info
+
+    def stream_sync(self):
+        self.cg.stream.wait_stream(torch.cuda.current_stream(self.systems.device))
+
+
+def _dense_block(M):
+    """(matrix, scale, shift) of a block that lowers to scale * Dense + shift * I (fp32, device)."""
+    got = M.hip_op()
+    if got is None or got[0].kind != N.CK_OP_DENSE_F32:
+        raise N.UnsupportedOperator("batched CG", N.CK_ERR_UNSUPPORTED,
+                                    "every block must be a dense fp32 device matrix (plus c * I)")
+    op = got[0]
+
+    def find(X):
+        if isinstance(X, Dense):
+            return X
+        if isinstance(X, (Sum, Product)):
+            for Y in X.Ms:
+                d = find(Y)
+                if d is not None:
+                    return d
+        return getattr(X, "A", None) if isinstance(getattr(X, "A", None), LinearOperator) and find(X.A) else None
+
+    D = find(M)
+    if D is None:
+        raise N.UnsupportedOperator("batched CG", N.CK_ERR_UNSUPPORTED, "no dense block found")
+    return D.A, float(op.scale), float(op.shift)
+
+
+class BatchedInverse(LinearOperator):
+    """inv(BlockDiag(...)) applied by batched CG over all blocks (one rhs column at a time)."""
+
+    def __init__(self, A, alg):
+        self.device = A.device
+        super().__init__(A.dtype, A.shape)
+        self.A, self.alg = A, alg
+        mats, shifts = [], []
+        for M in A.blocks():
+            K, scale, shift = _dense_block(M)
+            mats.append(K if scale == 1.0 else K * scale)
+            shifts.append(shift)
+        self.systems = BatchedSystems.dense(mats, shifts, A.device)
+        rows = alg.rows_per_tile
+        self.tune = None
+        if rows is None:
+            rows, self.tune = autotune(self.systems, max_iters=alg.max_iters)
+        self.cg = BatchedCG(self.systems, rows, alg.max_iters)
+        self.info = {}
+
+    def _matmat(self, X):
+        sy = self.systems
+        cols = []
+        iters = []
+        sizes = sy.sizes.tolist()
+        bounds = np.concatenate([[0], np.cumsum(sizes)])
+        for j in range(X.shape[1]):
+            col = X[:, j].to(torch.float32)
+            b = sy.concat([col[bounds[s]:bounds[s + 1]] for s in range(sy.nsys)])
+            x, ks = self.cg.solve(b, tol_rel=self.alg.tol, chunk=self.alg.chunk, use_graph=self.alg.use_graph)
+            cols.append(torch.cat(sy.split(x)).clone())
+            iters.append(ks)
+        self.info = {"iterations": np.stack(iters), "rows_per_tile": self.cg.rows, "autotune": self.tune}
+        return torch.stack(cols, dim=1).to(X.dtype)

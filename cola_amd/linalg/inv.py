@@ -3,8 +3,9 @@ import numpy as np
 
 from ..annotations import PSD, Unitary
 from ..dispatch import dispatch
-from ..ops import Diagonal, Identity, Kronecker, LinearOperator, Product, ScalarMul
+from ..ops import BlockDiag, Diagonal, Identity, Kronecker, LinearOperator, Product, ScalarMul
 from .algorithm_base import Algorithm, Auto, IterativeOperatorWInfo
+from .batched import BatchedInverse, HipBatchedCG
 from .cg import HipCG
 from .mixed import HipMixedCG
 
@@ -73,6 +74,18 @@ def inv(A: Product, alg: Algorithm):
 def inv(A: Kronecker, alg: Algorithm):
     """inv.py:139-141: factor-wise."""
     return Kronecker(*[inv(M, alg) for M in A.Ms])
+
+
+@dispatch
+def inv(A: BlockDiag, alg: Algorithm):
+    """inv.py:135-136: block by block."""
+    return BlockDiag(*[inv(M, alg) for M in A.Ms], multiplicities=A.multiplicities)
+
+
+@dispatch
+def inv(A: BlockDiag, alg: HipBatchedCG):
+    """Engine algorithm: every block in one batched CG (ck_bcg_*) instead of one loop per block."""
+    return BatchedInverse(A, alg)
 
 
 @dispatch

@@ -333,16 +333,40 @@ class PivotedCholeskyPreconditioner(WoodburyInverse):
         self.build_time = time.time() - t0
 
 
-def select_preconditioner(K, points, noise, net=None, rank=64):
-    """The better of the learned and pivoted-Cholesky preconditioners by Nystrom error trace,
-    or None (plain CG) when the better one's error trace is no smaller than its largest
-    captured diagonal (a lower bound on lambda_max(K)).  Returns (P, report)."""
-    cands = {}
-    if net is not None:
+def select_preconditioner(K, points, noise, net=None, rank=64, candidates=None, A=None, probe_steps=24):
+    """Pick plain CG, the learned or the pivoted-Cholesky preconditioner for A = K + noise I.
+
+    With the system operator ``A``, each choice runs ``probe_steps`` CG steps on one
+    fixed Gaussian probe (the engine's counter stream, seed 0) and the one with the
+    smallest residual wins, so a preconditioner is kept only when it measurably cuts
+    iterations.  Without ``A`` the rule is the bound kappa(P A) <= 1 + tr(E)/noise:
+    the candidate with the smaller Nystrom error trace tr(E) = tr(K - F F^T) is kept
+    unless tr(E) exceeds its largest captured diagonal.  ``candidates`` may pass
+    prebuilt preconditioners by name.  Returns (P, report)."""
+    cands = dict(candidates or {})
+    if net is not None and "learned" not in cands:
         cands["learned"] = LearnedPreconditioner(K, points, net, noise)
-    cands["pivoted_cholesky"] = PivotedCholeskyPreconditioner(K, rank, noise)
+    if "pivoted_cholesky" not in cands:
+        cands["pivoted_cholesky"] = PivotedCholeskyPreconditioner(K, rank, noise)
     report = {name: {"residual_trace": P.residual_trace, "captured_trace": P.captured_trace,
                      "build_s": P.build_time} for name, P in cands.items()}
+    if A is not None:
+        from ._engine import probes
+        from .cg import cg
+        n = A.shape[0]
+        Z, _ = probes("normal", n, 1, 0, K.device)
+        z = Z[0, :n].contiguous()
+        scores = {}
+        for name, P in [("none", None)] + list(cands.items()):
+            _, info = cg(A, z, P=P, tol=0.0, max_iters=int(probe_steps), chunk=int(probe_steps))
+            errs = info["errors"]
+            scores[name] = float(errs[-1]) if len(errs) else float("inf")
+            if name in report:
+                report[name]["probe_residual"] = scores[name]
+        report["none"] = {"probe_residual": scores["none"]}
+        best = min(scores, key=scores.get)
+        report["choice"] = best
+        return (None if best == "none" else cands[best]), report
     best = min(cands, key=lambda c: cands[c].residual_trace)
     P = cands[best]
     if P.residual_trace >= P.top_captured:

@@ -408,6 +408,39 @@ class Sum(LinearOperator):
         return op, keep
 
 
+class BlockDiag(LinearOperator):
+    """operators.py:278-315: [M_0 0 ...; 0 M_1 ...] with optional multiplicities (block i
+    repeated multiplicities[i] times along the diagonal)."""
+
+    def __init__(self, *Ms, multiplicities=None):
+        from .fns import lazify
+        self.Ms = tuple(lazify(M) for M in Ms)
+        self.multiplicities = [1 for _ in Ms] if multiplicities is None else list(multiplicities)
+        shape = (sum(M.shape[-2] * c for M, c in zip(self.Ms, self.multiplicities)),
+                 sum(M.shape[-1] * c for M, c in zip(self.Ms, self.multiplicities)))
+        self.device = self.Ms[0].device
+        dtype = reduce(torch.promote_types, (M.dtype for M in self.Ms))
+        super().__init__(dtype, shape)
+
+    def blocks(self):
+        """The diagonal blocks in order, multiplicities expanded."""
+        return [M for M, c in zip(self.Ms, self.multiplicities) for _ in range(c)]
+
+    def _matmat(self, v):
+        k = v.shape[1]
+        i = 0
+        y = []
+        for M, mult in zip(self.Ms, self.multiplicities):
+            i_end = i + mult * M.shape[-1]
+            elems = M @ v[i:i_end].T.reshape(k * mult, M.shape[-1]).T
+            y.append(elems.T.reshape(k, mult * M.shape[0]).T)
+            i = i_end
+        return torch.cat(y, dim=0)
+
+    def to_dense(self):
+        return torch.block_diag(*[M.to_dense() for M in self.blocks()])
+
+
 class Kronecker(LinearOperator):
     """operators.py:198-230."""
 

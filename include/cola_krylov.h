@@ -385,6 +385,80 @@ int ck_rbf_dense_f32(int64_t n, int32_t dim, const float* P, int64_t ldp, float 
 int ck_dense_apply_f64acc(int64_t n, const float* A, int64_t lda, double scale, double shift, int32_t nrhs,
                           const double* X, int64_t ldx, double* Y, int64_t ldy, void* stream);
 
+/* ---------------------------------------------------------------- batched uneven systems
+ * CG on many independent dense SPD systems (K_s + shift_s I) x_s = b_s of uneven order,
+ * one rhs each: the blocks of a block-diagonal operator.  Replaces the per-block loop
+ * inv(BlockDiag) = BlockDiag(inv(M_i)) (cola/linalg/inverse/inv.py:135-136, each block
+ * then running cg.py:94-178 on its own) with one set of launches for all blocks.
+ *
+ * System s: order n[s]; matrix row-major at element offset moff[s] of A32 (fp32) and,
+ * when prec[s] == 1, of A16 (bf16, same offsets), leading dimension lda[s] (multiple of
+ * 8, padding columns zero); vectors at [voff[s], voff[s] + n[s]) of the concatenated
+ * x, r, p, Ap.  Matvec tiles (tile_sys, tile_row0) cover rows_per_tile (4, 8, 16 or 32)
+ * rows each; vector tiles (vtile_sys, vtile_i0) cover ck_bcg_vec_tile() entries; both
+ * lists are ordered by system, with system s owning tiles [mt_begin[s], mt_end[s]) and
+ * [vt_begin[s], vt_end[s]).  part_mv / part_vec hold one float per tile.  Per-system
+ * state: gamma, alpha, beta, thresh (float), k, done (int32).  No reference counterpart
+ * for the layout. */
+typedef struct ck_bcg_plan {
+  int32_t nsys;
+  int32_t rows_per_tile;
+  int32_t ntiles_mv;
+  int32_t ntiles_vec;
+  int32_t max_iters;
+  int32_t pad0;
+  const int32_t* tile_sys;
+  const int32_t* tile_row0;
+  const int32_t* vtile_sys;
+  const int32_t* vtile_i0;
+  const int32_t* n;
+  const int32_t* lda;
+  const int64_t* voff;
+  const int64_t* moff;
+  const int32_t* mt_begin;
+  const int32_t* mt_end;
+  const int32_t* vt_begin;
+  const int32_t* vt_end;
+  const float* shift;
+  const int32_t* prec;
+  const float* A32;
+  const uint16_t* A16;
+  float* x;
+  float* r;
+  float* p;
+  float* Ap;
+  float* part_mv;
+  float* part_vec;
+  float* gamma;
+  float* alpha;
+  float* beta;
+  float* thresh;
+  int32_t* k;
+  int32_t* done;
+} ck_bcg_plan;
+
+int32_t ck_bcg_vec_tile(void);
+/* x = 0, r = p = b for systems with done[s] == 0 on entry (others are left alone);
+ * thresh = tol_rel * ||b_s|| + tol_abs; systems already below it are marked done. */
+int ck_bcg_start(const ck_bcg_plan* plan, const float* b, float tol_rel, float tol_abs, void* stream);
+/* `iters` CG iterations for every system not done (five launches each, no host sync).
+ * A system stops when ||r|| <= thresh or after max_iters iterations. */
+int ck_bcg_iterate(const ck_bcg_plan* plan, int32_t iters, void* stream);
+/* Ap = (K + shift I) p alone (tile-shape timing for the autotuner). */
+int ck_bcg_matvec(const ck_bcg_plan* plan, void* stream);
+/* Capture `iters` iterations on `stream` (not the legacy default stream) into a HIP
+ * graph; *graph_out is an opaque handle for ck_graph_launch / ck_graph_destroy.  The
+ * graph keeps the plan's pointers: the buffers must outlive it. */
+int ck_bcg_graph_create(const ck_bcg_plan* plan, int32_t iters, void* stream, void** graph_out);
+int ck_graph_launch(void* graph, void* stream);
+int ck_graph_destroy(void* graph);
+/* FP64 refinement on active systems (active[s] != 0): r32 = b - (K + shift I) x with K
+ * as stored in A32, x and b fp64 and FP64 accumulation; rel[s] = ||r|| / ||b||.
+ * part_rr, part_bb: ntiles_mv doubles each.  ck_bcg_update_f64: x64 += plan->x. */
+int ck_bcg_residual_f64(const ck_bcg_plan* plan, const int32_t* active, const double* x64, const double* b64,
+                        float* r32, double* part_rr, double* part_bb, double* rel, void* stream);
+int ck_bcg_update_f64(const ck_bcg_plan* plan, const int32_t* active, double* x64, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

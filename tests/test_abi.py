@@ -88,17 +88,18 @@ def test_bad_arguments_rejected_without_launch(lib):
     assert lib.ck_op_apply(ctypes.byref(op), 1, buf, 4, buf, 4, None, 0, None) == N.CK_ERR_UNSUPPORTED
 
 
-def test_struct_layout_matches_header(tmp_path):
-    """ctypes' ck_op layout equals what the C compiler makes of include/cola_krylov.h."""
-    fields = [f[0] for f in N.CkOp._fields_]
+@pytest.mark.parametrize("cname,pystruct", [("ck_op", N.CkOp), ("ck_bcg_plan", N.CkBcgPlan)])
+def test_struct_layout_matches_header(tmp_path, cname, pystruct):
+    """The ctypes struct layouts equal what the C compiler makes of include/cola_krylov.h."""
+    fields = [f[0] for f in pystruct._fields_]
     src = tmp_path / "layout.c"
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void) {",
-             '  printf("%zu\\n", sizeof(ck_op));']
-    lines += [f'  printf("%zu\\n", offsetof(ck_op, {f}));' for f in fields]
+             f'  printf("%zu\\n", sizeof({cname}));']
+    lines += [f'  printf("%zu\\n", offsetof({cname}, {f}));' for f in fields]
     lines += ["  return 0;", "}"]
     src.write_text("\n".join(lines))
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
     out = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
-    assert out[0] == ctypes.sizeof(N.CkOp)
-    assert out[1:] == [getattr(N.CkOp, f).offset for f in fields]
+    assert out[0] == ctypes.sizeof(pystruct)
+    assert out[1:] == [getattr(pystruct, f).offset for f in fields]

@@ -1,0 +1,118 @@
+"""Batched CG over uneven dense systems: per-system parity with the oracle's CG, HIP-graph
+replays against eager launches, the autotuner, FP64 refinement with bf16 storage, and
+inv(BlockDiag, HipBatchedCG)."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import cola_amd as ca
+from cola_amd.linalg import inv
+from cola_amd.linalg import learned as L
+from cola_amd.linalg.batched import BatchedCG, BatchedMixedSolver, BatchedSystems, HipBatchedCG, autotune
+from oracle import cg as ocg
+
+pytestmark = pytest.mark.gpu
+
+
+def rbf_systems(sizes, seed, ell_range=(0.05, 0.5), dim=3):
+    pts, ells = [], []
+    for i, n in enumerate(sizes):
+        g = torch.Generator().manual_seed(seed + i)
+        pts.append(torch.rand(n, dim, generator=g))
+        u = float(torch.rand(1, generator=g, dtype=torch.float64))
+        ells.append(math.exp(math.log(ell_range[0]) + u * (math.log(ell_range[1]) - math.log(ell_range[0]))))
+    return pts, ells
+
+
+def dense_k(P, ell):
+    return torch.exp(-torch.cdist(P.double(), P.double())**2 / (2 * ell * ell))
+
+
+SIZES = [37, 64, 100, 257, 700, 1023, 1024, 1500, 4500, 9, 333]
+
+
+def test_batched_cg_matches_oracle_per_system(dev):
+    pts, ells = rbf_systems(SIZES, 11)
+    noise = 0.05
+    sy = BatchedSystems.rbf(pts, ells, noise, dev)
+    rhs = [torch.randn(n, generator=torch.Generator().manual_seed(100 + i)) for i, n in enumerate(SIZES)]
+    b = sy.concat([r.to(dev) for r in rhs])
+    bcg = BatchedCG(sy, 16, max_iters=500)
+    x, ks = bcg.solve(b, tol_rel=2e-6, tol_abs=0.0, chunk=4)
+    for s, n in enumerate(SIZES):
+        K = sy.matrix(s).cpu().double()
+        A = K + float(np.float32(noise)) * torch.eye(n, dtype=torch.float64)
+        xs = sy.segment(x, s).cpu().double()
+        res = (A @ xs - rhs[s].double()).norm() / rhs[s].double().norm()
+        # the same threshold in the oracle's terms: tol * ||r0|| + tol on normalised columns
+        _, _, ko, _ = ocg.run_batched_cg(lambda V: A.float() @ V, rhs[s][:, None], torch.zeros(n, 1), 500, 1e-6, None)
+        print(f"n={n}: batched k={ks[s]} oracle k={ko} residual {res:.2e}", flush=True)
+        assert abs(int(ks[s]) - int(ko)) <= 1
+        assert res < 2e-5
+
+
+def test_graph_replays_match_eager(dev):
+    sizes = [300, 1200, 64, 2049]
+    pts, ells = rbf_systems(sizes, 3)
+    sy = BatchedSystems.rbf(pts, ells, 0.02, dev)
+    b = sy.concat([torch.randn(n, generator=torch.Generator().manual_seed(n)).to(dev) for n in sizes])
+    bcg = BatchedCG(sy, 8, max_iters=300)
+    x0, k0 = bcg.solve(b, tol_rel=1e-6, chunk=5, use_graph=False)
+    x0 = x0.clone()
+    for rep in range(3):
+        junk = torch.randn(4096, device=dev) * 2  # eager work between replays
+        torch.cuda.synchronize()
+        x1, k1 = bcg.solve(b, tol_rel=1e-6, chunk=5, use_graph=True)
+        assert np.array_equal(k0, k1), (rep, k0, k1)
+        assert torch.equal(x0, x1), rep
+        del junk
+
+
+def test_autotune(dev):
+    sizes = [512, 2048, 777]
+    pts, ells = rbf_systems(sizes, 5)
+    sy = BatchedSystems.rbf(pts, ells, 0.01, dev, bf16=True)
+    best, timings = autotune(sy, bf16_mask=[True, False, True], reps=3)
+    assert best in timings and set(timings) == {4, 8, 16, 32}
+    assert all(t > 0 for t in timings.values())
+
+
+def test_batched_mixed_refinement(dev):
+    g = torch.Generator().manual_seed(0)
+    sizes = torch.randint(256, 1501, (24, ), generator=g).tolist()
+    pts, ells = rbf_systems(sizes, 1000)
+    sy = BatchedSystems.rbf(pts, ells, 1e-2, dev, bf16=True)
+    rhs = [torch.randn(n, generator=torch.Generator().manual_seed(7 + i)) for i, n in enumerate(sizes)]
+    b = sy.concat([r.to(dev) for r in rhs])
+    solver = BatchedMixedSolver(sy, rows_per_tile=16, inner_max_iters=1000, bf16=True)
+    x, info = solver.solve(b, tol=1e-8, inner_tol=1e-4)
+    print(f"refinements {info['refinements']}, bf16 systems at the end {info['bf16_systems_final']}, "
+          f"fallbacks {info['precision_fallbacks']}, max rel {info['rel_residual_f64'].max():.2e}", flush=True)
+    assert info["converged"]
+    s32 = float(np.float32(1e-2))
+    worst = 0.0
+    for s, n in enumerate(sizes):
+        A = sy.matrix(s).cpu().double() + s32 * torch.eye(n, dtype=torch.float64)
+        xs = sy.segment(x, s).cpu()
+        worst = max(worst, float((A @ xs - rhs[s].double()).norm() / rhs[s].double().norm()))
+    assert worst < 1.2e-8, worst
+
+
+def test_inv_blockdiag_batched(dev):
+    sizes = [90, 300, 45]
+    pts, ells = rbf_systems(sizes, 21)
+    blocks, dense = [], []
+    for P, ell in zip(pts, ells):
+        K = L.rbf_matrix(P.to(dev), ell)
+        D = ca.Dense(K)
+        blocks.append(ca.PSD(D + 0.1 * ca.I_like(D)))
+        dense.append(K.cpu().double() + float(np.float32(0.1)) * torch.eye(K.shape[0], dtype=torch.float64))
+    A = ca.BlockDiag(*blocks)
+    b = torch.randn(sum(sizes), 2, generator=torch.Generator().manual_seed(2))
+    Ainv = inv(A, HipBatchedCG(tol=1e-7, max_iters=500))
+    x = (Ainv @ b.to(dev)).cpu().double()
+    ref = torch.linalg.solve(torch.block_diag(*dense), b.double())
+    assert ((x - ref).norm(dim=0) / ref.norm(dim=0)).max() < 1e-5
+    assert Ainv.info["iterations"].shape == (2, 3)
