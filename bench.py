@@ -475,6 +475,97 @@ def bench_learned_cg(args, dev):
     return out
 
 
+def gp_systems(nsys, dev, bf16=True):
+    """512 (nsys) RBF systems: n_i ~ U{256..4096} (generator seed 0), system i from manual_seed(i):
+    points U[0,1]^3, lengthscale LogU[0.05,0.5], one N(0,1) rhs; noise 1e-2."""
+    from cola_amd.linalg.batched import BatchedSystems
+    sizes = torch.randint(256, 4097, (nsys, ), generator=torch.Generator().manual_seed(0)).tolist()
+    pts, ells, rhs = [], [], []
+    for i, n in enumerate(sizes):
+        g = torch.Generator().manual_seed(i)
+        pts.append(torch.rand(n, 3, generator=g))
+        u = float(torch.rand(1, generator=g, dtype=torch.float64))
+        ells.append(float(np.exp(np.log(0.05) + u * (np.log(0.5) - np.log(0.05)))))
+        rhs.append(torch.randn(n, generator=g))
+    sy = BatchedSystems.rbf(pts, ells, 1e-2, dev, bf16=bf16)
+    b = sy.concat([r.to(dev) for r in rhs])
+    return sy, b, sizes, ells
+
+
+def bench_batched_cg(args, dev):
+    """Config 4: 512 uneven dense GP systems, bf16 storage + FP64 refinement to 1e-8 vs plain FP32 CG."""
+    from cola_amd.linalg.batched import BatchedCG, BatchedMixedSolver, time_matvec
+    nsys = args.systems_batched
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sy, b, sizes, ells = gp_systems(nsys, dev)
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    tol = args.batched_tol
+    t1 = time.perf_counter()
+    solver = BatchedMixedSolver(sy, rows_per_tile=None, inner_max_iters=1000, bf16=True)
+    t_tune = time.perf_counter() - t1
+    for _ in range(args.warmup):
+        solver.solve(b, tol=tol, inner_tol=args.inner_tol, chunk=args.chunk_batched)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        x, info = solver.solve(b, tol=tol, inner_tol=args.inner_tol, chunk=args.chunk_batched)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t2) / args.steps
+    rel = solver.residual(np.ones(nsys, dtype=bool))  # FP64 residuals of the returned x (fp32 K as stored)
+    solved = int((rel <= tol).sum())
+    # independent spot check of a few systems with a dense FP64 product
+    s32 = float(np.float32(1e-2))
+    spot = []
+    for s in [0, nsys // 2, nsys - 1]:
+        A = sy.matrix(s).double() + s32 * torch.eye(sizes[s], dtype=torch.float64, device=dev)
+        xs = sy.segment(x, s)
+        bs = sy.segment(b, s).double()
+        spot.append(float((A @ xs - bs).norm() / bs.norm()))
+    # plain FP32 batched CG on the same systems (fp32 storage, FP32 arithmetic)
+    plain = BatchedCG(sy, solver.cg.rows, 1000)
+    plain.solve(b, tol_rel=tol / 10, chunk=args.chunk_batched)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    xp, kp = plain.solve(b, tol_rel=tol / 10, chunk=args.chunk_batched)
+    torch.cuda.synchronize()
+    dtp = time.perf_counter() - t3
+    solver.x64.copy_(xp.double())
+    relp = solver.residual(np.ones(nsys, dtype=bool))
+    plain_solved = int((relp <= tol).sum())
+    plain_6 = int((relp <= 1e-6).sum())
+    # roofline: the batched matvec with the final per-system storage
+    bcg = solver.cg
+    t_mv = time_matvec(bcg, reps=5)
+    prec = bcg.prec.cpu().numpy().astype(bool)
+    mbytes = float(np.sum(sy.sizes * sy.lda * np.where(prec, 2, 4))) + 8.0 * float(sy.sizes.sum())
+    roof = {"bound": "hbm", "kernel": f"bcg_matvec<{bcg.rows // 4}> (all {nsys} systems, one launch)",
+            "achieved": mbytes / t_mv / 1e9, "peak": HBM_PEAK, "unit": "GB/s", "frac": mbytes / t_mv / 1e9 / HBM_PEAK,
+            "traffic": None, "launch_us": t_mv * 1e6, "algorithmic_bytes_per_launch": mbytes}
+    sum_n2 = int(np.sum(sy.sizes.astype(np.int64)**2))
+    out = {"metric": "batched GP solves/sec to 1e-8 (FP64-checked): bf16 + FP64 refinement vs plain FP32 CG",
+           "value": solved / dt, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "bf16 storage + f32 accumulate (inner), f64 (refinement residuals)",
+           "data": "synthetic (seeded points, lengthscales, rhs)",
+           "config": {"workload": f"batched_cg: {nsys} RBF systems n~U{{256..4096}} in [0,1]^3, noise 1e-2, 1 rhs, "
+                                  f"to FP64-checked {tol:g}",
+                      "sum_n2": sum_n2, "build_s": t_build, "autotune_s": t_tune,
+                      "rows_per_tile": bcg.rows, "autotune_us": {str(k): v * 1e6 for k, v in solver.tune.items()} if solver.tune else None,
+                      "refinements": info["refinements"], "solved": solved,
+                      "bf16_systems_final": info["bf16_systems_final"],
+                      "precision_fallbacks": info["precision_fallbacks"],
+                      "inner_iterations_total": int(info["inner_iterations"].sum()),
+                      "max_rel_residual": float(rel.max()), "spot_check_fp64": spot,
+                      "plain_fp32": {"s": dtp, "solved_1e-8": plain_solved, "solved_1e-6": plain_6,
+                                     "iterations_total": int(kp.sum()), "max_rel_residual": float(relp.max()),
+                                     "solves_per_s_1e-8": plain_solved / dtp},
+                      "speedup_vs_plain_time": dtp / dt},
+           "roofline": roof}
+    return out
+
+
 def cpu_baseline_dense(sd, n, B, k_steps, tol, iters=10):
     """The oracle's plain CG (the reference algorithm, torch on the host) on the first system:
     `iters` steps timed, scaled to the steps the engine's plain CG took."""
@@ -508,6 +599,9 @@ def main():
     ap.add_argument("--train-instances", type=int, default=2000)
     ap.add_argument("--train-n", type=int, default=1024)
     ap.add_argument("--train-epochs", type=int, default=4)
+    ap.add_argument("--systems-batched", type=int, default=512, help="systems (batched_cg)")
+    ap.add_argument("--batched-tol", type=float, default=1e-8)
+    ap.add_argument("--chunk-batched", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus > 1 is not supported"}))
@@ -515,7 +609,8 @@ def main():
     dev = torch.device("cuda:0")
     N.load()
     fn = {"kron_cg": bench_kron_cg, "copy": bench_copy, "dense_cg": bench_dense_cg, "lanczos": bench_lanczos,
-          "hutch": bench_hutch, "slq": bench_slq, "learned_cg": bench_learned_cg}[args.workload]
+          "hutch": bench_hutch, "slq": bench_slq, "learned_cg": bench_learned_cg,
+          "batched_cg": bench_batched_cg}[args.workload]
     out = fn(args, dev)
     print(json.dumps(out), flush=True)
 

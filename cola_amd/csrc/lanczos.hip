@@ -126,7 +126,17 @@ __global__ __launch_bounds__(256) void lz_scale_col(int64_t n, float* __restrict
 // Projection partials h_j = V[:,j] . w over j in [j0, j1] for one tile.
 // FIRST (Lanczos): alpha = sum(dpart), w <- w - (alpha v_i + beta v_{i-1}) first
 // and alpha is written to diag[b][i-1].
-template <bool FIRST>
+// elements idx..idx+3 of p, zero past n (16-byte load when all four are in range)
+__device__ __forceinline__ float4 load4_masked(const float* __restrict__ p, int64_t idx, int64_t n) {
+  if (idx + 3 < n) return *reinterpret_cast<const float4*>(p + idx);
+  float4 r = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (idx < n) r.x = p[idx];
+  if (idx + 1 < n) r.y = p[idx + 1];
+  if (idx + 2 < n) r.z = p[idx + 2];
+  return r;
+}
+
+template <bool FIRST, bool VEC>
 __global__ __launch_bounds__(256) void lz_project(int64_t n, float* __restrict__ V, int64_t ldv, int64_t ldb, int i,
                                                   int j0, int j1, const float* __restrict__ dpart, int P,
                                                   float* __restrict__ hpart, int M2, float* __restrict__ diag,
@@ -168,17 +178,53 @@ __global__ __launch_bounds__(256) void lz_project(int64_t n, float* __restrict__
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int j = j0 + wave; j <= j1; j += 4) {
-    const float* vj = Vb + (int64_t)j * ldv;
-    float acc = 0.f;
-    for (int e = lane; e < kTile; e += 64) {
-      const int64_t idx = i0 + e;
-      if (idx < n) acc = fmaf(vj[idx], wl[e], acc);
+  if (VEC) {
+    // each lane owns 4 consecutive elements of each 256-element quarter; two basis
+    // vectors (j, j+4) per pass keep eight 16-byte loads in flight per lane
+    const float4* wl4 = reinterpret_cast<const float4*>(wl);
+    float4 wv[kTile / 256];
+#pragma unroll
+    for (int q = 0; q < kTile / 256; ++q) wv[q] = wl4[q * 64 + lane];
+    for (int j = j0 + wave; j <= j1; j += 8) {
+      const bool two = j + 4 <= j1;
+      const float* v0 = Vb + (int64_t)j * ldv;
+      const float* v1 = Vb + (int64_t)(j + 4) * ldv;
+      float a0 = 0.f, a1 = 0.f;
+#pragma unroll
+      for (int q = 0; q < kTile / 256; ++q) {
+        const int64_t idx = i0 + 256 * q + 4 * lane;
+        const float4 x0 = load4_masked(v0, idx, n);
+        const float4 x1 = two ? load4_masked(v1, idx, n) : make_float4(0.f, 0.f, 0.f, 0.f);
+        a0 = fmaf(x0.x, wv[q].x, a0);
+        a0 = fmaf(x0.y, wv[q].y, a0);
+        a0 = fmaf(x0.z, wv[q].z, a0);
+        a0 = fmaf(x0.w, wv[q].w, a0);
+        a1 = fmaf(x1.x, wv[q].x, a1);
+        a1 = fmaf(x1.y, wv[q].y, a1);
+        a1 = fmaf(x1.z, wv[q].z, a1);
+        a1 = fmaf(x1.w, wv[q].w, a1);
+      }
+      a0 = wave_sum(a0);
+      a1 = wave_sum(a1);
+      if (lane == 0) {
+        hpart[((int64_t)b * M2 + j) * P + blk] = a0;
+        if (two) hpart[((int64_t)b * M2 + j + 4) * P + blk] = a1;
+      }
     }
-    acc = wave_sum(acc);
-    if (lane == 0) hpart[((int64_t)b * M2 + j) * P + blk] = acc;
+  } else {
+    for (int j = j0 + wave; j <= j1; j += 4) {
+      const float* vj = Vb + (int64_t)j * ldv;
+      float acc = 0.f;
+      for (int e = lane; e < kTile; e += 64) {
+        const int64_t idx = i0 + e;
+        if (idx < n) acc = fmaf(vj[idx], wl[e], acc);
+      }
+      acc = wave_sum(acc);
+      if (lane == 0) hpart[((int64_t)b * M2 + j) * P + blk] = acc;
+    }
   }
 }
+
 
 // h[b][j] = sum_p hpart[b][j][p] for j in [j0, j1]; ACCUM adds into h2 (Arnoldi CGS2 keeps h1+h2)
 __global__ __launch_bounds__(256) void lz_reduce_h(const float* __restrict__ hpart, int P, int M2, int j0, int j1,
@@ -193,7 +239,7 @@ __global__ __launch_bounds__(256) void lz_reduce_h(const float* __restrict__ hpa
 
 // w -= sum_{j0<=j<=j1} V[:,j] h[j]  (the sum is formed first, as lanczos.py:295).
 // NORM: also writes partials of ||w||^2.
-template <bool NORM>
+template <bool NORM, bool VEC>
 __global__ __launch_bounds__(256) void lz_apply(int64_t n, float* __restrict__ V, int64_t ldv, int64_t ldb, int wcol,
                                                 int j0, int j1, const float* __restrict__ h, int M2,
                                                 float* __restrict__ npart, int P, const int* __restrict__ done) {
@@ -206,24 +252,74 @@ __global__ __launch_bounds__(256) void lz_apply(int64_t n, float* __restrict__ V
   float* Vb = V + b * ldb;
   float* w = Vb + (int64_t)wcol * ldv;
   const int64_t i0 = (int64_t)blk * kTile;
-  float acc[4] = {0.f, 0.f, 0.f, 0.f};
-  for (int j = j0; j <= j1; ++j) {
-    const float hj = hs[j];
-    const float* vj = Vb + (int64_t)j * ldv;
+  float ss = 0.f;
+  if (VEC) {
+    // 4 consecutive elements per thread; four basis vectors per pass (sixteen loads in flight)
+    const int64_t idx = i0 + 4 * threadIdx.x;
+    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+    int j = j0;
+    for (; j + 3 <= j1; j += 4) {
+      float4 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = load4_masked(Vb + (int64_t)(j + u) * ldv, idx, n);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float hj = hs[j + u];
+        acc.x = fmaf(v[u].x, hj, acc.x);
+        acc.y = fmaf(v[u].y, hj, acc.y);
+        acc.z = fmaf(v[u].z, hj, acc.z);
+        acc.w = fmaf(v[u].w, hj, acc.w);
+      }
+    }
+    for (; j <= j1; ++j) {
+      const float4 v = load4_masked(Vb + (int64_t)j * ldv, idx, n);
+      const float hj = hs[j];
+      acc.x = fmaf(v.x, hj, acc.x);
+      acc.y = fmaf(v.y, hj, acc.y);
+      acc.z = fmaf(v.z, hj, acc.z);
+      acc.w = fmaf(v.w, hj, acc.w);
+    }
+    if (idx + 3 < n) {
+      float4 wv = *reinterpret_cast<const float4*>(w + idx);
+      wv.x -= acc.x;
+      wv.y -= acc.y;
+      wv.z -= acc.z;
+      wv.w -= acc.w;
+      *reinterpret_cast<float4*>(w + idx) = wv;
+      ss = fmaf(wv.x, wv.x, ss);
+      ss = fmaf(wv.y, wv.y, ss);
+      ss = fmaf(wv.z, wv.z, ss);
+      ss = fmaf(wv.w, wv.w, ss);
+    } else {
+      const float a4[4] = {acc.x, acc.y, acc.z, acc.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (idx + q < n) {
+          const float wv = w[idx + q] - a4[q];
+          w[idx + q] = wv;
+          ss = fmaf(wv, wv, ss);
+        }
+      }
+    }
+  } else {
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    for (int j = j0; j <= j1; ++j) {
+      const float hj = hs[j];
+      const float* vj = Vb + (int64_t)j * ldv;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int64_t idx = i0 + threadIdx.x + 256 * q;
+        if (idx < n) acc[q] = fmaf(vj[idx], hj, acc[q]);
+      }
+    }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int64_t idx = i0 + threadIdx.x + 256 * q;
-      if (idx < n) acc[q] = fmaf(vj[idx], hj, acc[q]);
-    }
-  }
-  float ss = 0.f;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int64_t idx = i0 + threadIdx.x + 256 * q;
-    if (idx < n) {
-      const float wv = w[idx] - acc[q];
-      w[idx] = wv;
-      ss = fmaf(wv, wv, ss);
+      if (idx < n) {
+        const float wv = w[idx] - acc[q];
+        w[idx] = wv;
+        ss = fmaf(wv, wv, ss);
+      }
     }
   }
   if (NORM) {
@@ -363,6 +459,27 @@ __global__ __launch_bounds__(256) void ar_start_ctrl(int nb, int m, float* __res
   }
 }
 
+// 16-byte basis access needs 4-float aligned columns and probe blocks
+inline bool basis_vec4(const float* V, int64_t ldv, int64_t ldb) {
+  return ldv % 4 == 0 && ldb % 4 == 0 && (reinterpret_cast<uintptr_t>(V) & 15) == 0;
+}
+
+template <bool FIRST, typename... Args>
+void launch_project(bool vec, dim3 grid, hipStream_t s, Args... args) {
+  if (vec)
+    hipLaunchKernelGGL((lz_project<FIRST, true>), grid, dim3(256), 0, s, args...);
+  else
+    hipLaunchKernelGGL((lz_project<FIRST, false>), grid, dim3(256), 0, s, args...);
+}
+
+template <bool NORM, typename... Args>
+void launch_apply(bool vec, dim3 grid, hipStream_t s, Args... args) {
+  if (vec)
+    hipLaunchKernelGGL((lz_apply<NORM, true>), grid, dim3(256), 0, s, args...);
+  else
+    hipLaunchKernelGGL((lz_apply<NORM, false>), grid, dim3(256), 0, s, args...);
+}
+
 int check_basis(int64_t n, int nb, const float* V, int64_t ldv, int64_t ldb, int m, int extra) {
   if (n < 1 || nb < 1 || nb > 4096 || !V || ldv < n || m < 1 || ldb < (int64_t)(m + extra) * ldv)
     return CK_ERR_BAD_ARG;
@@ -410,17 +527,18 @@ int lz_post_full(int64_t n, int nb, float* V, int64_t ldv, int64_t ldb, float* d
   const float* w = V + (int64_t)(i + 1) * ldv;
   const float* vi = V + (int64_t)i * ldv;
   dim3 grid(sc.P, nb);
+  const bool vec4 = basis_vec4(V, ldv, ldb);
   hipLaunchKernelGGL(tile_partials<0>, grid, dim3(256), 0, s, n, w, ldb, vi, ldb, sc.dpart, sc.P, done);
-  hipLaunchKernelGGL(lz_project<true>, grid, dim3(256), 0, s, n, V, ldv, ldb, i, 1, i, sc.dpart, sc.P, sc.hpart,
+  launch_project<true>(vec4, grid, s, n, V, ldv, ldb, i, 1, i, sc.dpart, sc.P, sc.hpart,
                      sc.M2, diag, subdiag, m, done);
   const dim3 rgrid((unsigned)cdiv(i, 4), nb);
   hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, 1, i, sc.h, done);
-  hipLaunchKernelGGL(lz_apply<false>, grid, dim3(256), 0, s, n, V, ldv, ldb, i + 1, 1, i, sc.h, sc.M2, sc.dpart,
+  launch_apply<false>(vec4, grid, s, n, V, ldv, ldb, i + 1, 1, i, sc.h, sc.M2, sc.dpart,
                      sc.P, done);
-  hipLaunchKernelGGL(lz_project<false>, grid, dim3(256), 0, s, n, V, ldv, ldb, i, 1, i, sc.dpart, sc.P, sc.hpart,
+  launch_project<false>(vec4, grid, s, n, V, ldv, ldb, i, 1, i, sc.dpart, sc.P, sc.hpart,
                      sc.M2, diag, subdiag, m, done);
   hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, 1, i, sc.h, done);
-  hipLaunchKernelGGL(lz_apply<true>, grid, dim3(256), 0, s, n, V, ldv, ldb, i + 1, 1, i, sc.h, sc.M2, sc.dpart,
+  launch_apply<true>(vec4, grid, s, n, V, ldv, ldb, i + 1, 1, i, sc.h, sc.M2, sc.dpart,
                      sc.P, done);
   hipLaunchKernelGGL(lz_finalize, dim3(1), dim3(256), 0, s, nb, m, sc.dpart, sc.P, subdiag, state, tol, i);
   CK_CHECK_LAUNCH();
@@ -434,6 +552,7 @@ int ar_post_full(int64_t n, int nb, float* Q, int64_t ldv, int64_t ldb, float* H
   const int* done = &static_cast<ck_ar_ctrl*>(state)->done;
   LzScratch sc = scratch_of(scratch, n, nb, m);
   dim3 grid(sc.P, nb);
+  const bool vec4 = basis_vec4(Q, ldv, ldb);
   float* npart;
   if (mgs) {
     float* buf[2] = {sc.dpart, sc.dpart + (size_t)nb * sc.P};
@@ -445,15 +564,15 @@ int ar_post_full(int64_t n, int nb, float* Q, int64_t ldv, int64_t ldb, float* H
     npart = buf[(idx + 1) & 1];
   } else {
     const dim3 rgrid((unsigned)cdiv(idx + 1, 4), nb);
-    hipLaunchKernelGGL(lz_project<false>, grid, dim3(256), 0, s, n, Q, ldv, ldb, idx, 0, idx, sc.dpart, sc.P,
+    launch_project<false>(vec4, grid, s, n, Q, ldv, ldb, idx, 0, idx, sc.dpart, sc.P,
                        sc.hpart, sc.M2, (float*)nullptr, (const float*)nullptr, m, done);
     hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, 0, idx, sc.h, done);
-    hipLaunchKernelGGL(lz_apply<false>, grid, dim3(256), 0, s, n, Q, ldv, ldb, idx + 1, 0, idx, sc.h, sc.M2,
+    launch_apply<false>(vec4, grid, s, n, Q, ldv, ldb, idx + 1, 0, idx, sc.h, sc.M2,
                        sc.dpart, sc.P, done);
-    hipLaunchKernelGGL(lz_project<false>, grid, dim3(256), 0, s, n, Q, ldv, ldb, idx, 0, idx, sc.dpart, sc.P,
+    launch_project<false>(vec4, grid, s, n, Q, ldv, ldb, idx, 0, idx, sc.dpart, sc.P,
                        sc.hpart, sc.M2, (float*)nullptr, (const float*)nullptr, m, done);
     hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, 0, idx, sc.h2, done);
-    hipLaunchKernelGGL(lz_apply<true>, grid, dim3(256), 0, s, n, Q, ldv, ldb, idx + 1, 0, idx, sc.h2, sc.M2,
+    launch_apply<true>(vec4, grid, s, n, Q, ldv, ldb, idx + 1, 0, idx, sc.h2, sc.M2,
                        sc.dpart, sc.P, done);
     hipLaunchKernelGGL(ar_store_h, dim3(nb), dim3(256), 0, s, nb, m, sc.M2, idx, (const float*)sc.h,
                        (const float*)sc.h2, H, done);

@@ -153,6 +153,109 @@ __global__ __launch_bounds__(256) void dense_bf16_kernel(int n, const uint16_t* 
   }
 }
 
+// Multi-rhs dense product with the vector chunk staged in LDS: the four waves of a
+// block share one copy of X[:, c0:c0+1024) instead of each wave re-reading it from
+// L2, so per block and chunk the traffic is 4*ROWS rows of A from HBM against one
+// NB x 1024 slice of X.  Requires n % 4 == 0 (fp32) or n % 8 == 0 (bf16) and 16-byte
+// aligned rows; lanes take 4 (fp32) or 8 (bf16) consecutive columns.
+constexpr int kDenseChunk = 1024;
+
+template <int NB, int ROWS, bool BF>
+__global__ __launch_bounds__(256) void dense_lds_kernel(int n, const void* __restrict__ Av, int64_t lda,
+                                                        const float* __restrict__ X, int64_t ldx,
+                                                        float* __restrict__ Y, int64_t ldy, int nbr, float scale,
+                                                        float shift, const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ float4 xs[NB][kDenseChunk / 4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int row0 = blockIdx.x * (4 * ROWS) + wave * ROWS;
+  float acc[ROWS][NB];
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[r][b] = 0.f;
+  for (int c0 = 0; c0 < n; c0 += kDenseChunk) {
+    const int cn = n - c0 < kDenseChunk ? n - c0 : kDenseChunk;  // multiple of 4
+    __syncthreads();
+    for (int e = threadIdx.x; e < NB * (kDenseChunk / 4); e += 256) {
+      const int b = e / (kDenseChunk / 4), j4 = e % (kDenseChunk / 4);
+      xs[b][j4] = (b < nbr && 4 * j4 < cn) ? *reinterpret_cast<const float4*>(X + (int64_t)b * ldx + c0 + 4 * j4)
+                                            : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    __syncthreads();
+    if (BF) {
+      const uint16_t* A = static_cast<const uint16_t*>(Av);
+#pragma unroll
+      for (int q = 0; q < kDenseChunk / 512; ++q) {
+        const int j = q * 512 + lane * 8;
+        if (j < cn) {
+          uint4 w[ROWS];
+#pragma unroll
+          for (int r = 0; r < ROWS; ++r)
+            w[r] = (row0 + r < n) ? *reinterpret_cast<const uint4*>(A + (int64_t)(row0 + r) * lda + c0 + j)
+                                  : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            const float4 x0 = xs[b][j / 4], x1 = xs[b][j / 4 + 1];
+#pragma unroll
+            for (int r = 0; r < ROWS; ++r) {
+              float t = acc[r][b];
+              t = fmaf(__uint_as_float(w[r].x << 16), x0.x, t);
+              t = fmaf(__uint_as_float(w[r].x & 0xFFFF0000u), x0.y, t);
+              t = fmaf(__uint_as_float(w[r].y << 16), x0.z, t);
+              t = fmaf(__uint_as_float(w[r].y & 0xFFFF0000u), x0.w, t);
+              t = fmaf(__uint_as_float(w[r].z << 16), x1.x, t);
+              t = fmaf(__uint_as_float(w[r].z & 0xFFFF0000u), x1.y, t);
+              t = fmaf(__uint_as_float(w[r].w << 16), x1.z, t);
+              t = fmaf(__uint_as_float(w[r].w & 0xFFFF0000u), x1.w, t);
+              acc[r][b] = t;
+            }
+          }
+        }
+      }
+    } else {
+      const float* A = static_cast<const float*>(Av);
+#pragma unroll
+      for (int q = 0; q < kDenseChunk / 256; ++q) {
+        const int j = q * 256 + lane * 4;
+        if (j < cn) {
+          float4 a[ROWS];
+#pragma unroll
+          for (int r = 0; r < ROWS; ++r)
+            a[r] = (row0 + r < n) ? *reinterpret_cast<const float4*>(A + (int64_t)(row0 + r) * lda + c0 + j)
+                                  : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+          for (int b = 0; b < NB; ++b) {
+            const float4 xv = xs[b][j / 4];
+#pragma unroll
+            for (int r = 0; r < ROWS; ++r) {
+              float t = acc[r][b];
+              t = fmaf(a[r].x, xv.x, t);
+              t = fmaf(a[r].y, xv.y, t);
+              t = fmaf(a[r].z, xv.z, t);
+              t = fmaf(a[r].w, xv.w, t);
+              acc[r][b] = t;
+            }
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const float sum = wave_sum(acc[r][b]);
+      const int row = row0 + r;
+      if (lane == 0 && b < nbr && row < n) {
+        float y = scale * sum;
+        if (shift != 0.f) y = fmaf(shift, X[(int64_t)b * ldx + row], y);
+        Y[(int64_t)b * ldy + row] = y;
+      }
+    }
+  }
+}
+
 int dense_dispatch(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, const int* done,
                    hipStream_t s) {
   const int n = op->n;
@@ -162,10 +265,32 @@ int dense_dispatch(const ck_op* op, int nrhs, const float* X, int64_t ldx, float
   const bool vec = (n % 4 == 0) && (op->lda % 4 == 0) && (ldx % 4 == 0) && aligned16(op->a) && aligned16(X);
   if (bf16 && ((n % 8) || (op->lda % 8) || (ldx % 4) || !aligned16(op->a) || !aligned16(X)))
     return CK_ERR_UNSUPPORTED;
+  // LDS-staged path for 3+ rhs when rows and vectors allow 16-byte loads
+  const bool lds_ok = vec && (!bf16 || n % 8 == 0) && (ldx % 4 == 0) && nrhs >= 3;
   for (int b0 = 0; b0 < nrhs; b0 += 16) {
     const int nb = nrhs - b0 < 16 ? nrhs - b0 : 16;
     const float* Xb = X + b0 * ldx;
     float* Yb = Y + b0 * ldy;
+    if (lds_ok && nb >= 3) {
+#define CK_DENSE_LDS(NB_, ROWS_)                                                                                  \
+  do {                                                                                                            \
+    const int grid = (n + 4 * (ROWS_)-1) / (4 * (ROWS_));                                                         \
+    if (bf16)                                                                                                     \
+      hipLaunchKernelGGL((dense_lds_kernel<NB_, ROWS_, true>), dim3(grid), dim3(256), 0, s, n, op->a, op->lda, Xb, \
+                         ldx, Yb, ldy, nb, op->scale, op->shift, done);                                           \
+    else                                                                                                          \
+      hipLaunchKernelGGL((dense_lds_kernel<NB_, ROWS_, false>), dim3(grid), dim3(256), 0, s, n, op->a, op->lda,   \
+                         Xb, ldx, Yb, ldy, nb, op->scale, op->shift, done);                                       \
+  } while (0)
+      if (nb <= 4)
+        CK_DENSE_LDS(4, 8);
+      else if (nb <= 8)
+        CK_DENSE_LDS(8, 8);
+      else
+        CK_DENSE_LDS(16, 4);
+#undef CK_DENSE_LDS
+      continue;
+    }
     // ROWS rows per wave: more rows reuse each loaded x chunk more often.
     constexpr int WAVES = 4;
 #define CK_DENSE_LAUNCH(NB_, ROWS_)                                                                               \

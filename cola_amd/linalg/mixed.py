@@ -164,7 +164,7 @@ class HipMixedCG(Algorithm):
     P: object = None
     chunk: int = 8
     inner: object = None
-    contraction: float = 0.5
+    contraction: float = 0.05
 
     def __call__(self, A, b):
         is_vector = b.dim() == 1

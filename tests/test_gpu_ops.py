@@ -16,7 +16,8 @@ def rel(a, b):
     return float((a - b).norm() / max(b.norm(), 1e-30))
 
 
-@pytest.mark.parametrize("n,B", [(1, 1), (7, 3), (64, 1), (1024, 1), (1000, 5), (4096, 16), (513, 17)])
+@pytest.mark.parametrize("n,B", [(1, 1), (7, 3), (64, 1), (1024, 1), (1000, 5), (4096, 16), (513, 17), (2052, 8),
+                                 (4100, 4), (1000, 16), (1028, 3), (260, 33)])
 def test_dense_matmat(dev, n, B):
     print(f"dense n={n} B={B}", flush=True)
     g = torch.Generator().manual_seed(n * 31 + B)
@@ -36,10 +37,11 @@ def test_dense_shift_scale(dev):
     assert rel(op @ X.to(dev), ref) < 2e-6
 
 
-def test_dense_bf16(dev):
-    g = torch.Generator().manual_seed(4)
-    A = torch.randn(512, 512, generator=g)
-    X = torch.randn(512, 6, generator=g)
+@pytest.mark.parametrize("n,B", [(512, 6), (1000, 8), (2056, 3), (520, 12), (64, 1), (1032, 2)])
+def test_dense_bf16(dev, n, B):
+    g = torch.Generator().manual_seed(4 + n + B)
+    A = torch.randn(n, n, generator=g)
+    X = torch.randn(n, B, generator=g)
     Ab = A.to(torch.bfloat16).to(torch.float32)
     out = ca.DenseBF16(A.to(dev)) @ X.to(dev)
     assert rel(out, Ab.double() @ X.double()) < 2e-6
