@@ -354,8 +354,7 @@ def bench_learned_cg(args, dev):
     # --- training: top-64 Lanczos eigenvectors of seeded instances, Adam on the engine
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    insts = [L.make_training_instance(s, args.train_n, m=64, ell_range=(0.05, 1.0), device=dev)
-             for s in range(args.train_instances)]
+    insts = L.make_training_set(range(args.train_instances), args.train_n, m=64, ell_range=(0.05, 1.0), device=dev)
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
     net = L.PointMLP(4, 64, device=dev, seed=0)

@@ -353,78 +353,83 @@ __global__ __launch_bounds__(256) void rows_gemm_kernel(int64_t n, int kin, int 
 
 // ------------------------------------------------------ small SPD: chol, L^-1, A^-1
 constexpr int kSpdMax = 64;
+constexpr int kPack = kSpdMax * (kSpdMax + 1) / 2;  // packed lower triangle
 
-// One block per matrix: A + (jitter_abs + jitter_rel * trace(A) / k) I = L L^T.
+__device__ __forceinline__ int pk(int i, int j) { return i * (i + 1) / 2 + j; }  // i >= j
+
+// One block per matrix, everything in LDS (packed lower triangles of L and X = L^-1):
+// A + (jitter_abs + jitter_rel * trace(A) / k) I = L L^T.
 // linv (required, k x k row-major lower) = L^-1; inv64 / inv32 = A^-1; linvT32 = L^-T.
 // status[b] = 0, or j + 1 when the j-th pivot is not positive.
 __global__ __launch_bounds__(256) void spd_inv_kernel(int k, const double* __restrict__ A, int64_t lda, int64_t sa,
                                                       double jitter_rel, double jitter_abs, double* __restrict__ linv,
                                                       double* __restrict__ inv64, float* __restrict__ inv32,
                                                       float* __restrict__ linvT32, int* __restrict__ status) {
-  __shared__ double Ls[kSpdMax][kSpdMax + 1];
+  __shared__ double Ls[kPack];
+  __shared__ double Xs[kPack];
   __shared__ double shift;
   __shared__ int bad;
   const int b = blockIdx.x, t = threadIdx.x;
   const double* Ab = A + (int64_t)b * sa;
-  double* Li = linv + (int64_t)b * k * k;
   for (int e = t; e < k * k; e += 256) {
     const int i = e / k, j = e % k;
-    Ls[i][j] = Ab[(int64_t)i * lda + j];
+    if (j <= i) Ls[pk(i, j)] = Ab[(int64_t)i * lda + j];
   }
   __syncthreads();
   if (t == 0) {
     double tr = 0.0;
-    for (int i = 0; i < k; ++i) tr += Ls[i][i];
+    for (int i = 0; i < k; ++i) tr += Ls[pk(i, i)];
     shift = jitter_abs + jitter_rel * tr / k;
     bad = 0;
   }
   __syncthreads();
-  if (t < k) Ls[t][t] += shift;
+  if (t < k) Ls[pk(t, t)] += shift;
   for (int j = 0; j < k; ++j) {
     __syncthreads();
     if (t == 0) {
-      const double v = Ls[j][j];
+      const double v = Ls[pk(j, j)];
       if (!(v > 0.0)) {
         bad = j + 1;
       } else {
-        Ls[j][j] = sqrt(v);
+        Ls[pk(j, j)] = sqrt(v);
       }
     }
     __syncthreads();
     if (bad) break;
-    const double djj = Ls[j][j];
-    for (int i = j + 1 + t; i < k; i += 256) Ls[i][j] /= djj;
+    const double djj = Ls[pk(j, j)];
+    for (int i = j + 1 + t; i < k; i += 256) Ls[pk(i, j)] /= djj;
     __syncthreads();
     const int m = k - j - 1;
     for (int e = t; e < m * m; e += 256) {
       const int i = j + 1 + e / m, l = j + 1 + e % m;
-      if (l <= i) Ls[i][l] -= Ls[i][j] * Ls[l][j];
+      if (l <= i) Ls[pk(i, l)] -= Ls[pk(i, j)] * Ls[pk(l, j)];
     }
   }
   __syncthreads();
   if (t == 0) status[b] = bad;
   if (bad) return;
-  // column c of L^-1 by forward substitution (each thread owns one column)
+  // column c of X = L^-1 by forward substitution (one thread per column)
   if (t < k) {
     const int c = t;
-    for (int i = 0; i < c; ++i) Li[(int64_t)i * k + c] = 0.0;
-    Li[(int64_t)c * k + c] = 1.0 / Ls[c][c];
+    Xs[pk(c, c)] = 1.0 / Ls[pk(c, c)];
     for (int i = c + 1; i < k; ++i) {
-      double s = 0.0;
-      for (int q = c; q < i; ++q) s = fma(Ls[i][q], Li[(int64_t)q * k + c], s);
-      Li[(int64_t)i * k + c] = -s / Ls[i][i];
+      double acc = 0.0;
+      for (int q = c; q < i; ++q) acc = fma(Ls[pk(i, q)], Xs[pk(q, c)], acc);
+      Xs[pk(i, c)] = -acc / Ls[pk(i, i)];
     }
   }
   __syncthreads();
+  double* Li = linv + (int64_t)b * k * k;
   for (int e = t; e < k * k; e += 256) {
     const int r = e / k, c = e % k;
-    if (linvT32) linvT32[(int64_t)b * k * k + e] = (float)Li[(int64_t)c * k + r];
+    Li[e] = c <= r ? Xs[pk(r, c)] : 0.0;
+    if (linvT32) linvT32[(int64_t)b * k * k + e] = r <= c ? (float)Xs[pk(c, r)] : 0.f;
     if (inv64 || inv32) {
       const int c0 = r > c ? r : c;
-      double s = 0.0;
-      for (int q = c0; q < k; ++q) s = fma(Li[(int64_t)q * k + r], Li[(int64_t)q * k + c], s);
-      if (inv64) inv64[(int64_t)b * k * k + e] = s;
-      if (inv32) inv32[(int64_t)b * k * k + e] = (float)s;
+      double acc = 0.0;
+      for (int q = c0; q < k; ++q) acc = fma(Xs[pk(q, r)], Xs[pk(q, c)], acc);
+      if (inv64) inv64[(int64_t)b * k * k + e] = acc;
+      if (inv32) inv32[(int64_t)b * k * k + e] = (float)acc;
     }
   }
 }
@@ -434,32 +439,58 @@ __global__ __launch_bounds__(256) void spd_inv_kernel(int k, const double* __res
 //   f  = sum_i w_i sum_a P[i][a] T[i][a] / sum w      (= tr(S^-1 G^T U W U^T G) / sum w)
 //   C1 = -(2 / sum w) diag(w) P          (m x k)  so that  dL/dG = U C1 + G C2
 //   C2 =  (2 / sum w) P^T diag(w) P      (k x k)  for the loss L = 1 - f
+// T and the lower triangle of S^-1 live in LDS; P overwrites T once every thread has
+// read its row.
 __global__ __launch_bounds__(256) void energy_kernel(int k, int m, const double* __restrict__ Sinv,
                                                      const double* __restrict__ T, const double* __restrict__ w,
                                                      float* __restrict__ C1, float* __restrict__ C2,
                                                      double* __restrict__ loss) {
-  __shared__ double Ps[kSpdMax][kSpdMax + 1];
+  __shared__ double Ts[kSpdMax * kSpdMax];
+  __shared__ double Sp[kPack];
+  __shared__ double ws[kSpdMax];
   __shared__ double red[4];
   const int t = threadIdx.x;
-  double wsum = 0.0;
-  for (int i = 0; i < m; ++i) wsum += w[i];
-  const double c = 2.0 / wsum;
-  double f = 0.0;
-  for (int e = t; e < m * k; e += 256) {
-    const int i = e / k, a = e % k;
-    double s = 0.0;
-    for (int q = 0; q < k; ++q) s = fma(T[(int64_t)i * k + q], Sinv[(int64_t)q * k + a], s);
-    Ps[i][a] = s;
-    C1[e] = (float)(-c * w[i] * s);
-    f = fma(w[i] * s, T[e], f);
+  for (int e = t; e < m * k; e += 256) Ts[e] = T[e];
+  for (int e = t; e < k * k; e += 256) {
+    const int i = e / k, j = e % k;
+    if (j <= i) Sp[pk(i, j)] = Sinv[e];
   }
-  f = block_sum<256>(f, red);
+  for (int i = t; i < m; i += 256) ws[i] = w[i];
+  __syncthreads();
+  double wsum = 0.0;
+  for (int i = 0; i < m; ++i) wsum += ws[i];
+  const double c = 2.0 / wsum;
+  const int i = t >> 2, g = t & 3;
+  double pv[kSpdMax / 4];
+  double f = 0.0;
+  if (i < m) {
+#pragma unroll
+    for (int u = 0; u < kSpdMax / 4; ++u) {
+      const int a = g + 4 * u;
+      pv[u] = 0.0;
+      if (a < k) {
+        double acc = 0.0;
+        for (int q = 0; q < k; ++q) acc = fma(Ts[i * k + q], Sp[q >= a ? pk(q, a) : pk(a, q)], acc);
+        pv[u] = acc;
+        C1[i * k + a] = (float)(-c * ws[i] * acc);
+        f = fma(ws[i] * acc, Ts[i * k + a], f);
+      }
+    }
+  }
+  f = block_sum<256>(f, red);  // its barriers also order every read of Ts before the writes below
+  if (i < m) {
+#pragma unroll
+    for (int u = 0; u < kSpdMax / 4; ++u) {
+      const int a = g + 4 * u;
+      if (a < k) Ts[i * k + a] = pv[u];
+    }
+  }
   __syncthreads();
   for (int e = t; e < k * k; e += 256) {
-    const int a = e / k, b = e % k;
-    double s = 0.0;
-    for (int i = 0; i < m; ++i) s = fma(w[i] * Ps[i][a], Ps[i][b], s);
-    C2[e] = (float)(c * s);
+    const int a = e / k, bb = e % k;
+    double acc = 0.0;
+    for (int r = 0; r < m; ++r) acc = fma(ws[r] * Ts[r * k + a], Ts[r * k + bb], acc);
+    C2[e] = (float)(c * acc);
   }
   if (t == 0) loss[0] = 1.0 - f / wsum;
 }

@@ -4,9 +4,10 @@
 // small core inverse precomputed (rank x rank, fp64).  All arithmetic after the
 // fp32 loads is fp64: for x in span(F) the subtraction cancels to a relative size
 // s / (s + lambda), so fp32 arithmetic would leave errors of u * lambda / s there
-// (0.06 at lambda / s = 1e6), enough to stall PCG.  F^T x is a tall-skinny
-// reduction (one partial per 1024-row tile, added in index order); the core
-// product and the rank-k update are fused into one pass over F.
+// (0.06 at lambda / s = 1e6), enough to stall PCG.  Both passes stage a tile of F
+// (128 rows at rank 64) in LDS with coalesced loads and serve up to 8 rhs from it:
+// F^T x gives one fp64 partial per tile (added in index order), the core product
+// u = C^-1 t runs once per rhs, and the update re-reads F once more.
 //
 // RBF: S_ij = exp(-|p_i - p_j|^2 / (2 l^2)) is never stored; each block stages
 // 256 points and their x values in LDS and every lane accumulates one row
@@ -18,69 +19,103 @@ using namespace ck;
 
 namespace {
 
-constexpr int kWTile = 1024;
 constexpr int kMaxRank = 128;
+constexpr int kWRhs = 8;        // rhs per pass of the tile kernels
+constexpr int kWFloats = 8192;  // LDS floats for one F tile: rows * (rank + 1)
 
-__global__ __launch_bounds__(256) void wb_ftx_partials(int64_t n, int k, const float* __restrict__ F, int64_t lda,
-                                                       const float* __restrict__ X, int64_t ldx,
-                                                       double* __restrict__ part, int P,
-                                                       const int* __restrict__ done) {
-  if (done && *done) return;
-  const int b = blockIdx.y, blk = blockIdx.x;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int64_t i0 = (int64_t)blk * kWTile;
-  float xv[kWTile / 64];
-#pragma unroll
-  for (int q = 0; q < kWTile / 64; ++q) {
-    const int64_t i = i0 + lane + 64 * q;
-    xv[q] = i < n ? X[b * ldx + i] : 0.f;
-  }
-  for (int c = wave; c < k; c += 4) {
-    double acc = 0.0;
-#pragma unroll
-    for (int q = 0; q < kWTile / 64; ++q) {
-      const int64_t i = i0 + lane + 64 * q;
-      if (i < n) acc = fma((double)F[i * lda + c], (double)xv[q], acc);
-    }
-    acc = wave_sum(acc);
-    if (lane == 0) part[((int64_t)b * k + c) * P + blk] = acc;
+// rows of F per tile: as many as fit 8192 floats of LDS at pitch rank + 1 (128 for rank 63)
+__host__ __device__ inline int wb_tile_rows(int k) {
+  int r = kWFloats / (k + 1);
+  r = r >= 128 ? 128 : (r >= 64 ? 64 : 32);
+  return r;
+}
+
+// stage rows [i0, i0 + tr) of F (row-major, lda) into Fs[row * (k+1) + c]; zero past n
+__device__ __forceinline__ void wb_stage(const float* __restrict__ F, int64_t lda, int64_t n, int k, int64_t i0,
+                                         int tr, float* Fs) {
+  for (int e = threadIdx.x; e < tr * k; e += 256) {
+    const int r = e / k, c = e % k;
+    const int64_t i = i0 + r;
+    Fs[r * (k + 1) + c] = i < n ? F[i * lda + c] : 0.f;
   }
 }
 
-__global__ __launch_bounds__(256) void wb_reduce(int k, const double* __restrict__ part, int P, double* __restrict__ t,
-                                                 const int* __restrict__ done) {
+// part[(b*k + c)*P + tile] = sum over the tile's rows of F[i][c] x_b[i] (fp64), for the
+// kWRhs rhs starting at b0; F is read once per tile for all of them
+__global__ __launch_bounds__(256) void wb_ftx_tiles(int64_t n, int k, const float* __restrict__ F, int64_t lda,
+                                                    const float* __restrict__ X, int64_t ldx, int nrhs, int b0,
+                                                    double* __restrict__ part, int P, const int* __restrict__ done) {
   if (done && *done) return;
-  const int b = blockIdx.y;
-  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (c >= k) return;
-  double acc = 0.0;
-  for (int p = threadIdx.x & 63; p < P; p += 64) acc += part[((int64_t)b * k + c) * P + p];
-  acc = wave_sum(acc);
-  if ((threadIdx.x & 63) == 0) t[b * k + c] = acc;
-}
-
-__global__ __launch_bounds__(256) void wb_apply(int64_t n, int k, const float* __restrict__ F, int64_t lda,
-                                                const double* __restrict__ Cinv, const double* __restrict__ t,
-                                                const float* __restrict__ X, int64_t ldx, float* __restrict__ Y,
-                                                int64_t ldy, double scale, double shift, const int* __restrict__ done) {
-  if (done && *done) return;
-  __shared__ double ts[kMaxRank], us[kMaxRank];
-  const int b = blockIdx.y;
-  for (int c = threadIdx.x; c < k; c += 256) ts[c] = t[b * k + c];
+  __shared__ float Fs[kWFloats];
+  __shared__ float xs[kWRhs][128];
+  const int tr = wb_tile_rows(k);
+  const int tile = blockIdx.x;
+  const int64_t i0 = (int64_t)tile * tr;
+  const int nb = nrhs - b0 < kWRhs ? nrhs - b0 : kWRhs;
+  wb_stage(F, lda, n, k, i0, tr, Fs);
+  for (int e = threadIdx.x; e < kWRhs * tr; e += 256) {
+    const int b = e / tr, r = e % tr;
+    const int64_t i = i0 + r;
+    xs[b][r] = (b < nb && i < n) ? X[(int64_t)(b0 + b) * ldx + i] : 0.f;
+  }
   __syncthreads();
-  for (int c = threadIdx.x; c < k; c += 256) {
+  for (int o = threadIdx.x; o < k * nb; o += 256) {
+    const int c = o % k, b = o / k;
+    double acc = 0.0;
+    for (int r = 0; r < tr; ++r) acc = fma((double)Fs[r * (k + 1) + c], (double)xs[b][r], acc);
+    part[((int64_t)(b0 + b) * k + c) * P + tile] = acc;
+  }
+}
+
+// per rhs b: t = sum of the tile partials (index order), u = Cinv t (both fp64)
+__global__ __launch_bounds__(128) void wb_core(int k, const double* __restrict__ part, int P,
+                                               const double* __restrict__ Cinv, double* __restrict__ u,
+                                               const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ double ts[kMaxRank];
+  const int b = blockIdx.x;
+  for (int c = threadIdx.x; c < k; c += 128) {
+    double acc = 0.0;
+    const double* pc = part + ((int64_t)b * k + c) * P;
+    for (int p = 0; p < P; ++p) acc += pc[p];
+    ts[c] = acc;
+  }
+  __syncthreads();
+  for (int c = threadIdx.x; c < k; c += 128) {
     double acc = 0.0;
     for (int j = 0; j < k; ++j) acc = fma(Cinv[c * k + j], ts[j], acc);
-    us[c] = acc;
+    u[(int64_t)b * k + c] = acc;
+  }
+}
+
+// Y_b = scale (x_b - F u_b) + shift x_b for the tile's rows and kWRhs rhs from b0
+__global__ __launch_bounds__(256) void wb_apply_tiles(int64_t n, int k, const float* __restrict__ F, int64_t lda,
+                                                      const double* __restrict__ u, const float* __restrict__ X,
+                                                      int64_t ldx, float* __restrict__ Y, int64_t ldy, int nrhs,
+                                                      int b0, double scale, double shift,
+                                                      const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ float Fs[kWFloats];
+  __shared__ double us[kWRhs][kMaxRank];
+  const int tr = wb_tile_rows(k);
+  const int64_t i0 = (int64_t)blockIdx.x * tr;
+  const int nb = nrhs - b0 < kWRhs ? nrhs - b0 : kWRhs;
+  wb_stage(F, lda, n, k, i0, tr, Fs);
+  for (int e = threadIdx.x; e < kWRhs * k; e += 256) {
+    const int b = e / k, c = e % k;
+    us[b][c] = b < nb ? u[(int64_t)(b0 + b) * k + c] : 0.0;
   }
   __syncthreads();
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double acc = 0.0;
-  const float* Fi = F + i * lda;
-  for (int c = 0; c < k; ++c) acc = fma((double)Fi[c], us[c], acc);
-  const double x = (double)X[b * ldx + i];
-  Y[b * ldy + i] = (float)(scale * (x - acc) + shift * x);
+  for (int o = threadIdx.x; o < tr * nb; o += 256) {
+    const int r = o % tr, b = o / tr;
+    const int64_t i = i0 + r;
+    if (i >= n) continue;
+    double acc = 0.0;
+    const float* fr = Fs + r * (k + 1);
+    for (int c = 0; c < k; ++c) acc = fma((double)fr[c], us[b][c], acc);
+    const double x = (double)X[(int64_t)(b0 + b) * ldx + i];
+    Y[(int64_t)(b0 + b) * ldy + i] = (float)(scale * (x - acc) + shift * x);
+  }
 }
 
 constexpr int kRbfTile = 256;
@@ -138,8 +173,9 @@ __global__ __launch_bounds__(256) void rbf_kernel(int n, int dim, const float* _
 namespace ck {
 
 size_t woodbury_workspace_bytes(const ck_op* op, int nrhs) {
-  const size_t P = cdiv(op->n, kWTile);
-  return sizeof(double) * ((size_t)nrhs * op->rank * P + (size_t)nrhs * op->rank);
+  const int k = op->rank < 1 ? 1 : op->rank;
+  const size_t P = cdiv(op->n, wb_tile_rows(k));
+  return sizeof(double) * ((size_t)nrhs * k * P + (size_t)nrhs * k);
 }
 
 int woodbury_apply(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,
@@ -148,16 +184,18 @@ int woodbury_apply(const ck_op* op, int nrhs, const float* X, int64_t ldx, float
   if (k < 1 || k > kMaxRank || !op->a || !op->aux || op->lda < k) return CK_ERR_BAD_ARG;
   if (!work || work_bytes < woodbury_workspace_bytes(op, nrhs)) return CK_ERR_WORKSPACE;
   const int64_t n = op->n;
-  const int P = (int)cdiv(n, kWTile);
+  const int tr = wb_tile_rows(k);
+  const int P = (int)cdiv(n, tr);
   double* part = static_cast<double*>(work);
-  double* t = part + (size_t)nrhs * k * P;
+  double* u = part + (size_t)nrhs * k * P;
   const float* F = static_cast<const float*>(op->a);
-  hipLaunchKernelGGL(wb_ftx_partials, dim3(P, nrhs), dim3(256), 0, s, n, k, F, op->lda, X, ldx, part, P, done);
-  hipLaunchKernelGGL(wb_reduce, dim3((unsigned)cdiv(k, 4), nrhs), dim3(256), 0, s, k, (const double*)part, P, t,
-                     done);
-  hipLaunchKernelGGL(wb_apply, dim3((unsigned)cdiv(n, 256), nrhs), dim3(256), 0, s, n, k, F, op->lda,
-                     reinterpret_cast<const double*>(op->aux), (const double*)t, X, ldx, Y, ldy, (double)op->scale,
-                     (double)op->shift, done);
+  for (int b0 = 0; b0 < nrhs; b0 += kWRhs)
+    hipLaunchKernelGGL(wb_ftx_tiles, dim3(P), dim3(256), 0, s, n, k, F, op->lda, X, ldx, nrhs, b0, part, P, done);
+  hipLaunchKernelGGL(wb_core, dim3(nrhs), dim3(128), 0, s, k, (const double*)part, P,
+                     reinterpret_cast<const double*>(op->aux), u, done);
+  for (int b0 = 0; b0 < nrhs; b0 += kWRhs)
+    hipLaunchKernelGGL(wb_apply_tiles, dim3(P), dim3(256), 0, s, n, k, F, op->lda, (const double*)u, X, ldx, Y, ldy,
+                       nrhs, b0, (double)op->scale, (double)op->shift, done);
   CK_CHECK_LAUNCH();
   return CK_OK;
 }

@@ -163,19 +163,18 @@ __global__ __launch_bounds__(256) void tridiag_ql_kernel(int m, const float* __r
     }
   }
 
-  // ascending order (selection sort of the permutation on lane 0)
-  if (tid == 0) {
-    for (int i = 0; i < m; ++i) perm[i] = i;
-    for (int i = 0; i < m; ++i) {
-      int k = i;
-      for (int j = i + 1; j < m; ++j)
-        if (d[perm[j]] < d[perm[k]]) k = j;
-      const int t = perm[i];
-      perm[i] = perm[k];
-      perm[k] = t;
+  // ascending order: each thread ranks its values (ties by index; NaN ranks as +inf so the
+  // ranks always form a permutation), perm[rank] = index
+  for (int i = tid; i < m; i += blockDim.x) {
+    const double di = isnan(d[i]) ? INFINITY : d[i];
+    int rank = 0;
+    for (int j = 0; j < m; ++j) {
+      const double dj = isnan(d[j]) ? INFINITY : d[j];
+      rank += (dj < di || (dj == di && j < i)) ? 1 : 0;
     }
-    if (status) status[b] = ctl_fail ? CK_ERR_NOCONV : CK_OK;
+    perm[rank] = i;
   }
+  if (tid == 0 && status) status[b] = ctl_fail ? CK_ERR_NOCONV : CK_OK;
   __syncthreads();
   for (int c = tid; c < m; c += blockDim.x) {
     evals[(int64_t)b * m + c] = d[perm[c]];

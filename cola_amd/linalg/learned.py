@@ -223,6 +223,51 @@ def top_eigenvectors(K, m, steps, key):
     return U, lam
 
 
+def make_training_set(seeds, n, m=64, din=4, ell_range=(0.05, 1.0), steps=None, device="cuda", batch=256):
+    """Training instances for many seeds: one Lanczos run per instance, then the tridiagonal
+    eigenproblems of up to ``batch`` instances in one ck_tridiag_eigh_f64 launch (one
+    workgroup each) instead of one launch per instance."""
+    steps = steps or (2 * m + 32)
+    out = []
+    seeds = list(seeds)
+    for c0 in range(0, len(seeds), batch):
+        runs = []
+        for seed in seeds[c0:c0 + batch]:
+            X, ell, _ = sample_instance(seed, n, din, ell_range)
+            Xd = X.to(device)
+            K = rbf_matrix(Xd, ell)
+            A = Dense(K)
+            run = run_lanczos(A, default_start(A, seed)[:, 0], min(steps, n), 1e-7)
+            del K, A
+            runs.append((seed, Xd, ell, run))
+        groups = {}
+        for item in runs:
+            groups.setdefault(item[3].iters, []).append(item)
+        for k, grp in groups.items():
+            nb = len(grp)
+            d = torch.stack([it[3].diag[0, :k] for it in grp]).contiguous()
+            e = torch.stack([it[3].subdiag[0, 1:max(k, 2)] for it in grp]).contiguous()
+            ev = torch.empty((nb, k), dtype=torch.float64, device=device)
+            vecs = torch.empty((nb, k, k), dtype=torch.float64, device=device)
+            st = torch.empty(nb, dtype=torch.int32, device=device)
+            N.call("ck_tridiag_eigh_f64", nb, k, N.ptr(d), d.stride(0), N.ptr(e), e.stride(0), N.ptr(ev), N.ptr(vecs),
+                   None, N.ptr(st), N.stream_ptr(device))
+            if int(st.max()) != 0:
+                raise RuntimeError("tridiagonal eigensolve did not converge")
+            mm = min(m, k)
+            for j, (seed, Xd, ell, run) in enumerate(grp):
+                S = vecs[j][:, k - mm:].flip(1).contiguous()
+                lam = ev[j, k - mm:].flip(0).contiguous()
+                U = torch.zeros((mm, run.ld), dtype=torch.float32, device=device)
+                N.call("ck_basis_combine", n, k, mm, N.ptr(run.V[0, 1]), run.ld, N.ptr(S), S.stride(0), N.ptr(U),
+                       run.ld, N.stream_ptr(device))
+                out.append((seed, TrainingInstance(Xd, U, torch.clamp(lam, min=0.0).contiguous(), ell, n)))
+        del runs, groups
+    order = {sd: i for i, sd in enumerate(seeds)}
+    out.sort(key=lambda t: order[t[0]])
+    return [inst for _, inst in out]
+
+
 def make_training_instance(seed, n, m=64, din=4, ell_range=(0.05, 1.0), steps=None, device="cuda"):
     X, ell, _ = sample_instance(seed, n, din, ell_range)
     Xd = X.to(device)

@@ -69,8 +69,8 @@ def test_op_workspace_queries(lib):
     assert N.q("ck_op_workspace_bytes", op, 16) == 16 * 64**3 * 4
     op2 = N.CkOp()
     op2.kind, op2.n, op2.rank = N.CK_OP_WOODBURY_INV, 16384, 64
-    # fp64 partials (16 tiles of 1024 rows) and the reduced F^T x
-    assert N.q("ck_op_workspace_bytes", op2, 8) == 8 * (8 * 64 * 16 + 8 * 64)
+    # fp64 partials (one per 64-row F tile at rank 64: 256 tiles) and u = C^-1 F^T x
+    assert N.q("ck_op_workspace_bytes", op2, 8) == 8 * (8 * 64 * 256 + 8 * 64)
     assert N.q("ck_mlp_param_count", 4, 64) == 4 * 128 + 128 + 128 * 128 + 128 + 128 * 64 + 64
     assert N.q("ck_mlp_param_count", 9, 64) == -1
     assert N.q("ck_mlp_grad_workspace_bytes", 16384, 4, 64) == 4 * 256 * 25408

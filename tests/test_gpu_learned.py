@@ -168,6 +168,17 @@ def test_woodbury_accurate_when_ill_conditioned(dev):
     assert err < 1e-6, err
 
 
+def test_training_set_batched_matches_single(dev):
+    """The batched eigensolve path gives the same targets as one instance at a time."""
+    batch = L.make_training_set([3, 4, 5], 300, m=16, steps=60, device=dev, batch=2)
+    for seed, inst in zip([3, 4, 5], batch):
+        one = L.make_training_instance(seed, 300, m=16, steps=60, device=dev)
+        assert torch.allclose(inst.w, one.w, rtol=1e-12, atol=1e-12)
+        # Ritz vectors up to sign
+        dots = (inst.U[:, :300] * one.U[:, :300]).sum(1).abs()
+        assert torch.allclose(dots, torch.ones_like(dots), atol=1e-4)
+
+
 def _iters(A, b, P=None):
     _, info = cg(A, b, P=P, tol=1e-6, max_iters=600)
     return info["iterations"]
