@@ -488,81 +488,108 @@ def gp_systems(nsys, dev, bf16=True):
         rhs.append(torch.randn(n, generator=g))
     sy = BatchedSystems.rbf(pts, ells, 1e-2, dev, bf16=bf16)
     b = sy.concat([r.to(dev) for r in rhs])
-    return sy, b, sizes, ells
+    return sy, b, sizes, ells, pts
 
 
 def bench_batched_cg(args, dev):
-    """Config 4: 512 uneven dense GP systems, bf16 storage + FP64 refinement to 1e-8 vs plain FP32 CG."""
-    from cola_amd.linalg.batched import BatchedCG, BatchedMixedSolver, time_matvec
+    """Config 4: 512 uneven dense GP systems; FP64 refinement to 1e-8 around batched PCG (bf16 storage
+    where it contracts, per-system learned / pivoted-Cholesky rank-64 preconditioners) vs plain FP32 CG."""
+    from cola_amd.linalg import learned as L
+    from cola_amd.linalg.batched import BatchedCG, BatchedMixedSolver, BatchedPrecond, autotune, time_matvec
     nsys = args.systems_batched
+    tol = args.batched_tol
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    sy, b, sizes, ells = gp_systems(nsys, dev)
+    sy, b, sizes, ells, pts = gp_systems(nsys, dev)
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
-    tol = args.batched_tol
     t1 = time.perf_counter()
-    solver = BatchedMixedSolver(sy, rows_per_tile=None, inner_max_iters=1000, bf16=True)
+    rows, tune = autotune(sy, np.ones(nsys, dtype=bool), max_iters=1000)
     t_tune = time.perf_counter() - t1
-    for _ in range(args.warmup):
-        solver.solve(b, tol=tol, inner_tol=args.inner_tol, chunk=args.chunk_batched)
+    # preconditioners: rank-64 pivoted Cholesky and a 3-input network trained on the engine
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    for _ in range(args.steps):
-        x, info = solver.solve(b, tol=tol, inner_tol=args.inner_tol, chunk=args.chunk_batched)
+    pcc = BatchedPrecond.pivoted_cholesky(sy, 64)
     torch.cuda.synchronize()
-    dt = (time.perf_counter() - t2) / args.steps
-    rel = solver.residual(np.ones(nsys, dtype=bool))  # FP64 residuals of the returned x (fp32 K as stored)
-    solved = int((rel <= tol).sum())
-    # independent spot check of a few systems with a dense FP64 product
+    t_pc = time.perf_counter() - t2
+    t3 = time.perf_counter()
+    insts = L.make_training_set(range(args.batched_train), 1024, m=64, din=3, ell_range=(0.05, 0.5), device=dev)
+    net = L.PointMLP(3, 64, device=dev, seed=0)
+    losses = L.train(net, insts, epochs=args.train_epochs, lr=2e-3, seed=0)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t3
+    del insts
+    t4 = time.perf_counter()
+    pcl = BatchedPrecond.learned(sy, pts, net)
+    torch.cuda.synchronize()
+    t_pl = time.perf_counter() - t4
+    best = BatchedPrecond.best_of(pcc, pcl)
+
+    def run(P, label):
+        solver = BatchedMixedSolver(sy, rows_per_tile=rows, inner_max_iters=1000, bf16=True, precond=P)
+        for _ in range(args.warmup):
+            solver.solve(b, tol=tol, inner_tol=args.inner_tol, chunk=args.chunk_batched)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            x, info = solver.solve(b, tol=tol, inner_tol=args.inner_tol, chunk=args.chunk_batched)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / args.steps
+        rel = solver.residual(np.ones(nsys, dtype=bool))  # FP64 residual of the returned x
+        return solver, x, {"s": dt, "solved": int((rel <= tol).sum()), "solves_per_s": int((rel <= tol).sum()) / dt,
+                           "refinements": info["refinements"],
+                           "inner_iterations_total": int(info["inner_iterations"].sum()),
+                           "bf16_systems_final": info["bf16_systems_final"], "max_rel_residual": float(rel.max())}
+
+    variants = {}
+    solver, x, variants["best"] = run(best, "best")
+    for name, P in (("pivoted_cholesky", pcc), ("learned", pcl), ("none", None)):
+        variants[name] = run(P, name)[2]
+    head = variants["best"]
+    # independent spot check of three systems with a dense FP64 product
     s32 = float(np.float32(1e-2))
     spot = []
-    for s in [0, nsys // 2, nsys - 1]:
-        A = sy.matrix(s).double() + s32 * torch.eye(sizes[s], dtype=torch.float64, device=dev)
-        xs = sy.segment(x, s)
-        bs = sy.segment(b, s).double()
-        spot.append(float((A @ xs - bs).norm() / bs.norm()))
-    # plain FP32 batched CG on the same systems (fp32 storage, FP32 arithmetic)
-    plain = BatchedCG(sy, solver.cg.rows, 1000)
+    for s_ in [0, nsys // 2, nsys - 1]:
+        A = sy.matrix(s_).double() + s32 * torch.eye(sizes[s_], dtype=torch.float64, device=dev)
+        bs = sy.segment(b, s_).double()
+        spot.append(float((A @ sy.segment(x, s_) - bs).norm() / bs.norm()))
+    # plain FP32 batched CG on the same systems (fp32 storage, no preconditioner)
+    plain = BatchedCG(sy, rows, 1000)
     plain.solve(b, tol_rel=tol / 10, chunk=args.chunk_batched)
     torch.cuda.synchronize()
-    t3 = time.perf_counter()
+    t5 = time.perf_counter()
     xp, kp = plain.solve(b, tol_rel=tol / 10, chunk=args.chunk_batched)
     torch.cuda.synchronize()
-    dtp = time.perf_counter() - t3
+    dtp = time.perf_counter() - t5
     solver.x64.copy_(xp.double())
     relp = solver.residual(np.ones(nsys, dtype=bool))
-    plain_solved = int((relp <= tol).sum())
-    plain_6 = int((relp <= 1e-6).sum())
-    # roofline: the batched matvec with the final per-system storage
-    bcg = solver.cg
+    plain_res = {"s": dtp, "solved_1e-8": int((relp <= tol).sum()), "solved_1e-6": int((relp <= 1e-6).sum()),
+                 "iterations_total": int(kp.sum()), "max_rel_residual": float(relp.max()),
+                 "solves_per_s_1e-8": int((relp <= tol).sum()) / dtp}
+    # roofline: the batched matvec over all systems, fp32 storage
+    bcg = BatchedCG(sy, rows, 1000)
     t_mv = time_matvec(bcg, reps=5)
-    prec = bcg.prec.cpu().numpy().astype(bool)
-    mbytes = float(np.sum(sy.sizes * sy.lda * np.where(prec, 2, 4))) + 8.0 * float(sy.sizes.sum())
-    roof = {"bound": "hbm", "kernel": f"bcg_matvec<{bcg.rows // 4}> (all {nsys} systems, one launch)",
+    mbytes = float(np.sum(sy.sizes * sy.lda * 4)) + 8.0 * float(sy.sizes.sum())
+    roof = {"bound": "hbm", "kernel": f"bcg_matvec<{rows // 4}> (all {nsys} systems, fp32, one launch)",
             "achieved": mbytes / t_mv / 1e9, "peak": HBM_PEAK, "unit": "GB/s", "frac": mbytes / t_mv / 1e9 / HBM_PEAK,
             "traffic": None, "launch_us": t_mv * 1e6, "algorithmic_bytes_per_launch": mbytes}
     sum_n2 = int(np.sum(sy.sizes.astype(np.int64)**2))
-    out = {"metric": "batched GP solves/sec to 1e-8 (FP64-checked): bf16 + FP64 refinement vs plain FP32 CG",
-           "value": solved / dt, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "bf16 storage + f32 accumulate (inner), f64 (refinement residuals)",
-           "data": "synthetic (seeded points, lengthscales, rhs)",
-           "config": {"workload": f"batched_cg: {nsys} RBF systems n~U{{256..4096}} in [0,1]^3, noise 1e-2, 1 rhs, "
-                                  f"to FP64-checked {tol:g}",
-                      "sum_n2": sum_n2, "build_s": t_build, "autotune_s": t_tune,
-                      "rows_per_tile": bcg.rows, "autotune_us": {str(k): v * 1e6 for k, v in solver.tune.items()} if solver.tune else None,
-                      "refinements": info["refinements"], "solved": solved,
-                      "bf16_systems_final": info["bf16_systems_final"],
-                      "precision_fallbacks": info["precision_fallbacks"],
-                      "inner_iterations_total": int(info["inner_iterations"].sum()),
-                      "max_rel_residual": float(rel.max()), "spot_check_fp64": spot,
-                      "plain_fp32": {"s": dtp, "solved_1e-8": plain_solved, "solved_1e-6": plain_6,
-                                     "iterations_total": int(kp.sum()), "max_rel_residual": float(relp.max()),
-                                     "solves_per_s_1e-8": plain_solved / dtp},
-                      "speedup_vs_plain_time": dtp / dt},
-           "roofline": roof}
-    return out
+    return {"metric": "batched GP solves/sec to 1e-8 (FP64-checked): learned/pivoted-Cholesky PCG + bf16/FP64 "
+                      "refinement vs plain FP32 CG",
+            "value": head["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": head["s"] * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "bf16/f32 storage + f32 accumulate (inner PCG), f64 (residuals, Woodbury)",
+            "data": "synthetic (seeded points, lengthscales, rhs)",
+            "config": {"workload": f"batched_cg: {nsys} RBF systems n~U{{256..4096}} in [0,1]^3, noise 1e-2, 1 rhs, "
+                                   f"to FP64-checked {tol:g}",
+                       "sum_n2": sum_n2, "build_s": t_build, "autotune_s": t_tune, "rows_per_tile": rows,
+                       "autotune_us": {str(k): v * 1e6 for k, v in tune.items()},
+                       "pivchol_build_s": t_pc, "learned": {"train_instances": args.batched_train, "train_s": t_train,
+                                                            "build_s": t_pl, "last_loss": float(losses[-1]),
+                                                            "systems_choosing_learned": int(best.pick.sum())},
+                       "variants": variants, "spot_check_fp64": spot, "plain_fp32": plain_res,
+                       "speedup_vs_plain_time": dtp / head["s"]},
+            "roofline": roof}
 
 
 def cpu_baseline_dense(sd, n, B, k_steps, tol, iters=10):
@@ -601,6 +628,7 @@ def main():
     ap.add_argument("--systems-batched", type=int, default=512, help="systems (batched_cg)")
     ap.add_argument("--batched-tol", type=float, default=1e-8)
     ap.add_argument("--chunk-batched", type=int, default=16)
+    ap.add_argument("--batched-train", type=int, default=1000, help="training instances of the 3-D network")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; --gpus > 1 is not supported"}))

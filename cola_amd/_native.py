@@ -61,7 +61,9 @@ class CkBcgPlan(ctypes.Structure):
                [(name, ctypes.c_void_p) for name in
                 ("tile_sys", "tile_row0", "vtile_sys", "vtile_i0", "n", "lda", "voff", "moff", "mt_begin", "mt_end",
                  "vt_begin", "vt_end", "shift", "prec", "A32", "A16", "x", "r", "p", "Ap", "part_mv", "part_vec",
-                 "gamma", "alpha", "beta", "thresh", "k", "done")]
+                 "gamma", "alpha", "beta", "thresh", "k", "done")] + \
+               [("rank", ctypes.c_int32), ("pad1", ctypes.c_int32)] + \
+               [(name, ctypes.c_void_p) for name in ("F", "foff", "Cinv", "z", "part_rz", "part_ftr", "u")]
 
 
 _i32, _i64, _u64, _f32, _f64 = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_float, ctypes.c_double

@@ -16,7 +16,9 @@
 //   bcg_alpha    alpha_s = gamma_s / p.Ap                               (one wave per system)
 //   bcg_xr       x += alpha p, r -= alpha Ap, tile r.r                   [HBM]
 //   bcg_beta     beta_s, gamma_s, k_s, done_s                            (one wave per system)
-//   bcg_p        p = r + beta p                                          [HBM]
+//   bcg_p        p = r + beta p  (p = z + beta p when preconditioned)    [HBM]
+// With a per-system low-rank preconditioner (shift_s I + F_s F_s^T)^-1 three more
+// launches per iteration compute z (bcg_pc_ftr, bcg_pc_core, bcg_pc_apply).
 // Every kernel returns at once for systems whose done flag is set, so converged
 // systems stop costing bandwidth.  The launches contain no host synchronisation and
 // no memset, so a run of iterations can be captured into one HIP graph.
@@ -64,6 +66,14 @@ struct BcgArgs {
   float* thresh;
   int32_t* k;
   int32_t* done;
+  int32_t rank;
+  const float* F;
+  const int64_t* foff;
+  const double* Cinv;
+  float* z;
+  float* part_rz;
+  double* part_ftr;
+  double* u;
 };
 
 BcgArgs to_args(const ck_bcg_plan* P) {
@@ -98,6 +108,14 @@ BcgArgs to_args(const ck_bcg_plan* P) {
   a.thresh = P->thresh;
   a.k = P->k;
   a.done = P->done;
+  a.rank = P->rank;
+  a.F = P->F;
+  a.foff = P->foff;
+  a.Cinv = P->Cinv;
+  a.z = P->z;
+  a.part_rz = P->part_rz;
+  a.part_ftr = P->part_ftr;
+  a.u = P->u;
   return a;
 }
 
@@ -241,7 +259,8 @@ __global__ __launch_bounds__(256) void bcg_xr(BcgArgs a) {
 __global__ __launch_bounds__(64) void bcg_beta(BcgArgs a) {
   const int s = blockIdx.x;
   if (a.done[s]) return;
-  const double g1 = range_sum(a.part_vec, a.vt_begin[s], a.vt_end[s]);
+  const double rr = range_sum(a.part_vec, a.vt_begin[s], a.vt_end[s]);
+  const double g1 = a.rank > 0 ? range_sum(a.part_rz, a.vt_begin[s], a.vt_end[s]) : rr;
   if (threadIdx.x == 0) {
     const double g0 = (double)a.gamma[s];
     const double den = fabs(g0) < 1e-40 ? 1e-40 : g0;
@@ -249,7 +268,7 @@ __global__ __launch_bounds__(64) void bcg_beta(BcgArgs a) {
     a.gamma[s] = (float)g1;
     const int k = a.k[s] + 1;
     a.k[s] = k;
-    if (sqrt(g1) <= (double)a.thresh[s] || k >= a.max_iters) a.done[s] = 1;
+    if (sqrt(rr) <= (double)a.thresh[s] || k >= a.max_iters) a.done[s] = 1;
   }
 }
 
@@ -262,10 +281,108 @@ __global__ __launch_bounds__(256) void bcg_p(BcgArgs a) {
   const int i0 = a.vtile_i0[t];
   const int i1 = i0 + kBVec < n ? i0 + kBVec : n;
   const float be = a.beta[s];
+  const float* zr = a.rank > 0 ? a.z : a.r;
   for (int i = i0 + threadIdx.x; i < i1; i += 256) {
     const int64_t g = vo + i;
-    a.p[g] = fmaf(be, a.p[g], a.r[g]);
+    a.p[g] = fmaf(be, a.p[g], zr[g]);
   }
+}
+
+// ------------------------------------------------------------ low-rank preconditioner
+// z = (r - F C^-1 F^T r) / shift = (shift I + F F^T)^-1 r per system, in three launches:
+// tile partials of F^T r (fp64), per-system u = C^-1 (sum of partials in tile order),
+// and the tile update z = (r - F u) / shift with its r.z partial.  F rows are read
+// coalesced (lanes over the rank index).
+__global__ __launch_bounds__(256) void bcg_pc_ftr(BcgArgs a) {
+  __shared__ double red[256];
+  const int t = blockIdx.x;
+  const int s = a.vtile_sys[t];
+  if (a.done[s]) return;
+  const int k = a.rank;
+  const int groups = 256 / k;
+  const int c = threadIdx.x % k, rg = threadIdx.x / k;
+  const int n = a.n[s];
+  const int64_t vo = a.voff[s];
+  const float* Fs = a.F + a.foff[s];
+  const int i0 = a.vtile_i0[t];
+  const int i1 = i0 + kBVec < n ? i0 + kBVec : n;
+  double acc = 0.0;
+  if (rg < groups)
+    for (int i = i0 + rg; i < i1; i += groups) acc = fma((double)Fs[(int64_t)i * k + c], (double)a.r[vo + i], acc);
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < k) {
+    double sum = 0.0;
+    for (int g = 0; g < groups; ++g) sum += red[g * k + threadIdx.x];
+    a.part_ftr[(int64_t)t * k + threadIdx.x] = sum;
+  }
+}
+
+__global__ __launch_bounds__(64) void bcg_pc_core(BcgArgs a) {
+  __shared__ double ts[64];
+  const int s = blockIdx.x;
+  if (a.done[s]) return;
+  const int k = a.rank;
+  const int c = threadIdx.x;
+  if (c < k) {
+    double acc = 0.0;
+    for (int t = a.vt_begin[s]; t < a.vt_end[s]; ++t) acc += a.part_ftr[(int64_t)t * k + c];
+    ts[c] = acc;
+  }
+  __syncthreads();
+  if (c < k) {
+    const double* Ci = a.Cinv + (int64_t)s * k * k + (int64_t)c * k;
+    double acc = 0.0;
+    for (int j = 0; j < k; ++j) acc = fma(Ci[j], ts[j], acc);
+    a.u[(int64_t)s * k + c] = acc;
+  }
+}
+
+__global__ __launch_bounds__(256) void bcg_pc_apply(BcgArgs a) {
+  __shared__ double us[64];
+  __shared__ float red[4];
+  const int t = blockIdx.x;
+  const int s = a.vtile_sys[t];
+  if (a.done[s]) return;
+  const int k = a.rank;
+  for (int c = threadIdx.x; c < k; c += 256) us[c] = a.u[(int64_t)s * k + c];
+  __syncthreads();
+  const int n = a.n[s];
+  const int64_t vo = a.voff[s];
+  const float* Fs = a.F + a.foff[s];
+  const int i0 = a.vtile_i0[t];
+  const int i1 = i0 + kBVec < n ? i0 + kBVec : n;
+  const double inv = 1.0 / (double)a.shift[s];
+  float rz = 0.f;
+  for (int i = i0 + threadIdx.x; i < i1; i += 256) {
+    const float* fr = Fs + (int64_t)i * k;
+    double acc = 0.0;
+    for (int c = 0; c < k; ++c) acc = fma((double)fr[c], us[c], acc);
+    const double rv = (double)a.r[vo + i];
+    const float zv = (float)((rv - acc) * inv);
+    a.z[vo + i] = zv;
+    rz = fmaf((float)rv, zv, rz);
+  }
+  rz = block_sum<256>(rz, red);
+  if (threadIdx.x == 0) a.part_rz[t] = rz;
+}
+
+void enqueue_precond(const ck_bcg_plan* P, const BcgArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(bcg_pc_ftr, dim3(P->ntiles_vec), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(bcg_pc_core, dim3(P->nsys), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(bcg_pc_apply, dim3(P->ntiles_vec), dim3(256), 0, s, a);
+}
+
+// start, preconditioned: p = z for active systems
+__global__ __launch_bounds__(256) void bcg_start_p(BcgArgs a) {
+  const int t = blockIdx.x;
+  const int s = a.vtile_sys[t];
+  if (a.done[s]) return;
+  const int n = a.n[s];
+  const int64_t vo = a.voff[s];
+  const int i0 = a.vtile_i0[t];
+  const int i1 = i0 + kBVec < n ? i0 + kBVec : n;
+  for (int i = i0 + threadIdx.x; i < i1; i += 256) a.p[vo + i] = a.z[vo + i];
 }
 
 // start: x = 0, r = p = b for active systems (done == 0 on entry), tile ||b||^2
@@ -295,8 +412,9 @@ __global__ __launch_bounds__(64) void bcg_start_sys(BcgArgs a, float tol_rel, fl
   const int s = blockIdx.x;
   if (a.done[s]) return;
   const double g = range_sum(a.part_vec, a.vt_begin[s], a.vt_end[s]);
+  const double gz = a.rank > 0 ? range_sum(a.part_rz, a.vt_begin[s], a.vt_end[s]) : g;
   if (threadIdx.x == 0) {
-    a.gamma[s] = (float)g;
+    a.gamma[s] = (float)gz;
     const float th = tol_rel * (float)sqrt(g) + tol_abs;
     a.thresh[s] = th;
     a.k[s] = 0;
@@ -394,6 +512,9 @@ int check_plan(const ck_bcg_plan* P) {
     return CK_ERR_BAD_ARG;
   const int R = P->rows_per_tile;
   if (R != 4 && R != 8 && R != 16 && R != 32) return CK_ERR_UNSUPPORTED;
+  if (P->rank < 0 || P->rank > 64 || (P->rank > 0 && 256 % P->rank)) return CK_ERR_UNSUPPORTED;
+  if (P->rank > 0 && (!P->F || !P->foff || !P->Cinv || !P->z || !P->part_rz || !P->part_ftr || !P->u))
+    return CK_ERR_BAD_ARG;
   return CK_OK;
 }
 
@@ -417,6 +538,7 @@ void enqueue_iterations(const ck_bcg_plan* P, int iters, hipStream_t s) {
     matvec_dispatch(P, a, s);
     hipLaunchKernelGGL(bcg_alpha, dim3(P->nsys), dim3(64), 0, s, a);
     hipLaunchKernelGGL(bcg_xr, dim3(P->ntiles_vec), dim3(256), 0, s, a);
+    if (P->rank > 0) enqueue_precond(P, a, s);
     hipLaunchKernelGGL(bcg_beta, dim3(P->nsys), dim3(64), 0, s, a);
     hipLaunchKernelGGL(bcg_p, dim3(P->ntiles_vec), dim3(256), 0, s, a);
   }
@@ -438,6 +560,9 @@ int ck_bcg_start(const ck_bcg_plan* plan, const float* b, float tol_rel, float t
   hipStream_t s = (hipStream_t)stream;
   const BcgArgs a = to_args(plan);
   hipLaunchKernelGGL(bcg_start_vec, dim3(plan->ntiles_vec), dim3(256), 0, s, a, b);
+  if (plan->rank > 0) enqueue_precond(plan, a, s);
+  // the threshold test in bcg_start_sys may mark systems done: p = z runs first
+  if (plan->rank > 0) hipLaunchKernelGGL(bcg_start_p, dim3(plan->ntiles_vec), dim3(256), 0, s, a);
   hipLaunchKernelGGL(bcg_start_sys, dim3(plan->nsys), dim3(64), 0, s, a, tol_rel, tol_abs);
   CK_CHECK_LAUNCH();
   return CK_OK;

@@ -4,7 +4,7 @@
 code written against cola's names runs on the HIP kernels unchanged."""
 from .algorithm_base import Algorithm, Auto, IterativeOperatorWInfo
 from .arnoldi import HipArnoldi, arnoldi, arnoldi_eigs
-from .batched import BatchedMixedSolver, BatchedSystems, HipBatchedCG, autotune
+from .batched import BatchedMixedSolver, BatchedPrecond, BatchedSystems, HipBatchedCG, autotune
 from .cg import HipCG, cg, run_batched_cg
 from .eig import eig, eigmax, eigmin, get_slice
 from .inv import inv, solve
@@ -30,4 +30,4 @@ __all__ = ["Algorithm", "Auto", "IterativeOperatorWInfo", "HipCG", "CG", "cg", "
            "HipMixedCG", "KroneckerEigInverse", "mixed_cg", "LanczosUnary", "apply_unary", "log", "exp", "sqrt", "isqrt", "pow", "PRNGKey", "next_key",
            "PointMLP", "LearnedPreconditioner", "PivotedCholeskyPreconditioner", "select_preconditioner",
            "make_training_instance", "train", "rbf_matrix",
-           "HipBatchedCG", "BatchedSystems", "BatchedMixedSolver", "autotune"]
+           "HipBatchedCG", "BatchedSystems", "BatchedMixedSolver", "BatchedPrecond", "autotune"]

@@ -97,10 +97,11 @@ class BatchedSystems:
 class BatchedCG:
     """Device state, tile lists and the ck_bcg_plan of one batched CG configuration."""
 
-    def __init__(self, systems, rows_per_tile=16, max_iters=1000):
+    def __init__(self, systems, rows_per_tile=16, max_iters=1000, precond=None):
         if rows_per_tile not in _ROWS:
             raise ValueError(f"rows_per_tile must be one of {_ROWS}")
         sy = self.systems = systems
+        self.precond = precond
         dev = sy.device
         self.rows = rows_per_tile
         self.max_iters = int(max_iters)
@@ -142,6 +143,16 @@ class BatchedCG:
         pl.A16 = sy.A16.data_ptr() if sy.A16 is not None else None
         for name in ("x", "r", "p", "Ap", "part_mv", "part_vec", "gamma", "alpha", "beta", "thresh", "k", "done"):
             setattr(pl, name, getattr(self, name).data_ptr())
+        if precond is not None:
+            k = precond.rank
+            self.z = f32(sy.total_v)
+            self.part_rz = f32(len(vsys))
+            self.part_ftr = torch.zeros(len(vsys) * k, dtype=torch.float64, device=dev)
+            self.u = torch.zeros(S * k, dtype=torch.float64, device=dev)
+            pl.rank = k
+            pl.F, pl.foff, pl.Cinv = precond.F.data_ptr(), precond.foff_t.data_ptr(), precond.Cinv.data_ptr()
+            for name in ("z", "part_rz", "part_ftr", "u"):
+                setattr(pl, name, getattr(self, name).data_ptr())
         self.plan = pl
         self.stream = torch.cuda.Stream(device=dev)  # default priority; graphs need a non-legacy stream
         self.graphs = {}
@@ -261,14 +272,14 @@ class BatchedMixedSolver:
     by about the inner tolerance, so a bf16 system that contracts by less than 0.05 per step costs
     more refinements than its halved matrix traffic saves."""
 
-    def __init__(self, systems, rows_per_tile=None, inner_max_iters=1000, bf16=True):
+    def __init__(self, systems, rows_per_tile=None, inner_max_iters=1000, bf16=True, precond=None):
         self.systems = systems
         mask = np.full(systems.nsys, bool(bf16 and systems.A16 is not None))
         if rows_per_tile is None:
             rows_per_tile, self.tune = autotune(systems, mask, max_iters=inner_max_iters)
         else:
             self.tune = None
-        self.cg = BatchedCG(systems, rows_per_tile, inner_max_iters)
+        self.cg = BatchedCG(systems, rows_per_tile, inner_max_iters, precond=precond)
         self.bf16 = mask
         dev = systems.device
         self.x64 = torch.zeros(systems.total_v, dtype=torch.float64, device=dev)
@@ -316,10 +327,29 @@ class BatchedMixedSolver:
                              use_graph=use_graph)
             inner_total[active] += ks[active]
             with torch.cuda.stream(cg.stream):
+                keep = self.x64.clone()
                 self.active.copy_(torch.from_numpy(active.astype(np.int32)))
                 N.call("ck_bcg_update_f64", ctypes.byref(cg.plan), N.ptr(self.active), N.ptr(self.x64),
                        ctypes.c_void_p(cg.stream.cuda_stream))
             new = self.residual(active)
+            broken = active & ~np.isfinite(new)
+            if broken.any():
+                # a correction solve broke down (bf16 storage can make K_s + shift I indefinite):
+                # drop its correction, recompute that residual and continue on fp32 storage
+                with torch.cuda.stream(cg.stream):
+                    for s_ in np.flatnonzero(broken):
+                        o, n_ = int(self.systems.voff[s_]), int(self.systems.sizes[s_])
+                        self.x64[o:o + n_].copy_(keep[o:o + n_])
+                redo = self.residual(broken)
+                new = np.where(broken, redo, new)
+                if not np.all(np.isfinite(new[broken])):
+                    raise FloatingPointError("non-finite FP64 residual in batched refinement")
+                stuck = broken & ~prec
+                if stuck.any():
+                    raise FloatingPointError("an fp32 correction solve broke down")
+                prec = prec & ~broken
+                fallbacks += int(broken.sum())
+                cg.set_precision(prec)
             stalled = active & prec & ~(new <= contraction * rel_all)
             if stalled.any():
                 prec = prec & ~stalled
@@ -396,3 +426,117 @@ class BatchedInverse(LinearOperator):
             iters.append(ks)
         self.info = {"iterations": np.stack(iters), "rows_per_tile": self.cg.rows, "autotune": self.tune}
         return torch.stack(cols, dim=1).to(X.dtype)
+
+
+class BatchedPrecond:
+    """Per-system low-rank preconditioners (shift_s I + F_s F_s^T)^-1 for batched CG.
+
+    F_s (n_s x rank, row-major) sits at element offset foff[s] of one fp32 buffer, and
+    Cinv_s = (shift_s I + F_s^T F_s)^-1 (FP64) at s * rank^2.  ``captured[s]`` is
+    tr(F_s F_s^T), so tr(K_s) - captured[s] is the trace of the Nystrom error."""
+
+    def __init__(self, systems, rank, F, Cinv, captured, kind):
+        self.systems, self.rank, self.F, self.Cinv, self.kind = systems, int(rank), F, Cinv, kind
+        self.foff = np.concatenate([[0], np.cumsum(systems.sizes * rank)[:-1]]).astype(np.int64)
+        self.foff_t = torch.tensor(self.foff, device=systems.device)
+        self.captured = captured
+
+    @staticmethod
+    def _alloc(systems, rank):
+        if rank < 1 or rank > 64 or 256 % rank:
+            raise ValueError("rank must divide 256 and be at most 64")
+        if int(systems.sizes.min()) < rank:
+            raise ValueError("every system needs order >= rank")
+        return torch.zeros(int((systems.sizes * rank).sum()), dtype=torch.float32, device=systems.device)
+
+    @staticmethod
+    def _cores(systems, F, rank):
+        """Cinv_s = (shift_s I + F_s^T F_s)^-1 for every system (one batched small-SPD launch)."""
+        from .learned import gram
+        S = systems.nsys
+        foff = np.concatenate([[0], np.cumsum(systems.sizes * rank)[:-1]]).astype(np.int64)
+        G = torch.empty((S, rank, rank), dtype=torch.float64, device=systems.device)
+        for s in range(S):
+            n = int(systems.sizes[s])
+            Fs = F[int(foff[s]):int(foff[s]) + n * rank]
+            G[s].copy_(gram(Fs, rank, 1, Fs, rank, 1, n, rank, rank))
+        captured = torch.diagonal(G, dim1=1, dim2=2).sum(1).cpu().numpy()
+        G.diagonal(dim1=1, dim2=2).add_(systems.shift.double()[:, None])
+        linv = torch.empty_like(G)
+        inv = torch.empty_like(G)
+        st = torch.zeros(S, dtype=torch.int32, device=systems.device)
+        N.call("ck_spd_inverse_f64", S, rank, N.ptr(G), rank, rank * rank, 0.0, 0.0, N.ptr(linv), N.ptr(inv), None,
+               None, N.ptr(st), N.stream_ptr(systems.device))
+        if int(st.max()) != 0:
+            raise np.linalg.LinAlgError("a Woodbury core is not positive definite")
+        return inv.reshape(-1).contiguous(), captured
+
+    @classmethod
+    def pivoted_cholesky(cls, systems, rank=32):
+        """Rank-r pivoted Cholesky of every K_s (ck_pivchol_dense_f32 on the fp32 matrices)."""
+        F = cls._alloc(systems, rank)
+        dev = systems.device
+        st = N.stream_ptr(dev)
+        nmax = int(systems.sizes.max())
+        ldm = _round_up(nmax, 64)
+        L = torch.zeros((rank, ldm), dtype=torch.float32, device=dev)
+        piv = torch.empty(rank, dtype=torch.int32, device=dev)
+        dpiv = torch.empty(rank, dtype=torch.float32, device=dev)
+        wb = N.q("ck_pivchol_workspace_bytes", nmax)
+        work = torch.empty((wb + 3) // 4, dtype=torch.float32, device=dev)
+        off = 0
+        for s in range(systems.nsys):
+            n = int(systems.sizes[s])
+            A = systems.A32[int(systems.moff[s]):]
+            N.call("ck_pivchol_dense_f32", n, N.ptr(A), int(systems.lda[s]), rank, N.ptr(L), ldm, N.ptr(piv),
+                   N.ptr(dpiv), N.ptr(work), wb, st)
+            N.call("ck_transpose_f32", n, rank, N.ptr(L), ldm, N.ptr(F[off:]), rank, 0, st)
+            off += n * rank
+        Cinv, captured = cls._cores(systems, F, rank)
+        return cls(systems, rank, F, Cinv, captured, "pivoted_cholesky")
+
+    @classmethod
+    def learned(cls, systems, points, net, jitter_rel=1e-6):
+        """Nystrom completion F_s = K_s G_s (G_s^T K_s G_s)^-1/2 of the network's sketch G_s per system."""
+        from .learned import gram_blocks, rows_gemm, spd_inverse
+        from ..ops import _op
+        rank = net.dout
+        F = cls._alloc(systems, rank)
+        dev = systems.device
+        st = N.stream_ptr(dev)
+        off = 0
+        for s in range(systems.nsys):
+            n = int(systems.sizes[s])
+            G = net.forward(points[s].to(dev))
+            ld = G.stride(0)
+            op = _op(N.CK_OP_DENSE_F32, n)
+            op.a = systems.A32[int(systems.moff[s]):].data_ptr()
+            op.lda = int(systems.lda[s])
+            Y = torch.zeros_like(G)
+            wb = N.q("ck_op_workspace_bytes", op, rank)
+            work = torch.empty(max(wb, 4) // 4, dtype=torch.float32, device=dev)
+            N.call("ck_op_apply", op, rank, N.ptr(G), ld, N.ptr(Y), ld, N.ptr(work), wb, st)
+            C = gram_blocks(G, Y, n)
+            fac = spd_inverse(C, jitter_rel=jitter_rel, linvT32=True)
+            rows_gemm(Y, 1, ld, fac["linvT32"], F[off:], rank, 1, n)
+            off += n * rank
+        Cinv, captured = cls._cores(systems, F, rank)
+        return cls(systems, rank, F, Cinv, captured, "learned")
+
+    @classmethod
+    def best_of(cls, a, b):
+        """Per system, the factor with the larger captured trace (smaller Nystrom error trace)."""
+        if a.rank != b.rank or a.systems is not b.systems:
+            raise ValueError("candidates must share systems and rank")
+        sy, k = a.systems, a.rank
+        F = torch.empty_like(a.F)
+        Cinv = torch.empty_like(a.Cinv)
+        pick = np.where(b.captured > a.captured, 1, 0)
+        for s in range(sy.nsys):
+            src = b if pick[s] else a
+            o, m = int(a.foff[s]), int(sy.sizes[s]) * k
+            F[o:o + m].copy_(src.F[o:o + m])
+            Cinv[s * k * k:(s + 1) * k * k].copy_(src.Cinv[s * k * k:(s + 1) * k * k])
+        out = cls(sy, k, F, Cinv, np.maximum(a.captured, b.captured), f"best({a.kind},{b.kind})")
+        out.pick = pick
+        return out

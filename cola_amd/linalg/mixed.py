@@ -231,14 +231,21 @@ class MixedSolver:
             inner_steps.append(int(k))
             # the next correction solve usually needs as many steps: enqueue that many before the first check
             chunk = max(1, min(alg.chunk, int(k)))
+            keep = self.x64.clone() if inner is not self.A else None
             N.call("ck_ir_update_f64", n, B, N.ptr(self.x64), ld, N.ptr(self.st.x), ld, s)
             prev = rel.max()
             rel = self.residual(with_ax=True)
-            history.append(float(rel.max()))
             it += 1
-            if inner is not self.A and not (rel.max() <= alg.contraction * prev):
+            if inner is not self.A and not np.all(np.isfinite(rel)):
+                # the lower-precision correction solve broke down: drop it, go on with A itself
+                self.x64.copy_(keep)
+                rel = self.residual(with_ax=True)
                 inner = self.A
                 fallback_at = it
+            elif inner is not self.A and not (rel.max() <= alg.contraction * prev):
+                inner = self.A
+                fallback_at = it
+            history.append(float(rel.max()))
         # the refined iterate is returned in FP64: rounding it to FP32 would cost ~kappa * 6e-8 of residual
         x = self.x64[:, :n].T
         info = {"refinements": it, "inner_iterations": inner_steps, "rel_residual_f64": rel,

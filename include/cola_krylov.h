@@ -435,6 +435,20 @@ typedef struct ck_bcg_plan {
   float* thresh;
   int32_t* k;
   int32_t* done;
+  /* optional low-rank preconditioner P = (shift_s I + F_s F_s^T)^-1 per system (rank 0: none):
+   * F_s row-major (n_s x rank) at element offset foff[s] of F; Cinv_s = (shift_s I + F_s^T F_s)^-1
+   * (rank x rank, fp64) at Cinv + s*rank*rank; z, part_rz (ntiles_vec floats), part_ftr
+   * (ntiles_vec * rank doubles) and u (nsys * rank doubles) are its scratch.  rank <= 64,
+   * a divisor of 256. */
+  int32_t rank;
+  int32_t pad1;
+  const float* F;
+  const int64_t* foff;
+  const double* Cinv;
+  float* z;
+  float* part_rz;
+  double* part_ftr;
+  double* u;
 } ck_bcg_plan;
 
 int32_t ck_bcg_vec_tile(void);
