@@ -13,7 +13,7 @@ _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "lib
 
 CK_OK, CK_ERR_BAD_ARG, CK_ERR_HIP, CK_ERR_UNSUPPORTED, CK_ERR_WORKSPACE, CK_ERR_NOCONV = range(6)
 (CK_OP_IDENTITY, CK_OP_DENSE_F32, CK_OP_DENSE_BF16, CK_OP_CSR_F32, CK_OP_KRON_F32, CK_OP_DIAG_F32,
- CK_OP_WOODBURY_INV, CK_OP_RBF_F32, CK_OP_KRON_EIG_F32) = range(9)
+ CK_OP_WOODBURY_INV, CK_OP_RBF_F32, CK_OP_KRON_EIG_F32, CK_OP_DENSE_BATCHED_F32) = range(10)
 CK_CG_MULT, CK_CG_TOL, CK_CG_GAMMA, CK_CG_ALPHA, CK_CG_BETA, CK_CG_RNORM, CK_CG_ERRORS = range(7)
 CTRL_BYTES = 64
 

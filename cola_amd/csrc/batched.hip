@@ -119,6 +119,9 @@ BcgArgs to_args(const ck_bcg_plan* P) {
   return a;
 }
 
+typedef float f32x4_nt __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4_nt __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
 __device__ __forceinline__ float bf_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
 
@@ -133,7 +136,9 @@ __device__ __forceinline__ void gemv_chunk_f32(const float* __restrict__ A, int6
     for (int r = 0; r < ROWS; ++r) {
       const int row = row0 + r;
       if (row < n) {
-        const float4 a = *reinterpret_cast<const float4*>(A + (int64_t)row * lda + c0 + j);
+        // streamed once per product: non-temporal, so x and the CG vectors keep the caches
+        const f32x4_nt av = __builtin_nontemporal_load(reinterpret_cast<const f32x4_nt*>(A + (int64_t)row * lda + c0 + j));
+        const float4 a = make_float4(av.x, av.y, av.z, av.w);
         VecT s = acc[r];
         s += (VecT)a.x * x0;
         s += (VecT)a.y * x1;
@@ -156,7 +161,8 @@ __device__ __forceinline__ void gemv_chunk_bf16(const uint16_t* __restrict__ A, 
     for (int r = 0; r < ROWS; ++r) {
       const int row = row0 + r;
       if (row < n) {
-        const uint4 w = *reinterpret_cast<const uint4*>(A + (int64_t)row * lda + c0 + j);
+        const u32x4_nt wv = __builtin_nontemporal_load(reinterpret_cast<const u32x4_nt*>(A + (int64_t)row * lda + c0 + j));
+        const uint4 w = make_uint4(wv.x, wv.y, wv.z, wv.w);
         VecT s = acc[r];
         s += (VecT)bf_lo(w.x) * xv[0];
         s += (VecT)bf_hi(w.x) * xv[1];

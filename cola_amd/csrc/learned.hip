@@ -399,10 +399,14 @@ __global__ __launch_bounds__(256) void spd_inv_kernel(int k, const double* __res
     const double djj = Ls[pk(j, j)];
     for (int i = j + 1 + t; i < k; i += 256) Ls[pk(i, j)] /= djj;
     __syncthreads();
-    const int m = k - j - 1;
-    for (int e = t; e < m * m; e += 256) {
-      const int i = j + 1 + e / m, l = j + 1 + e % m;
-      if (l <= i) Ls[pk(i, l)] -= Ls[pk(i, j)] * Ls[pk(l, j)];
+    // trailing update of the lower triangle: row i = j+1 + t/4, every 4th column from j+1+t%4
+    {
+      const int i = j + 1 + (t >> 2);
+      if (i < k) {
+        const double lij = Ls[pk(i, j)];
+        const int base = i * (i + 1) / 2;
+        for (int l = j + 1 + (t & 3); l <= i; l += 4) Ls[base + l] -= lij * Ls[pk(l, j)];
+      }
     }
   }
   __syncthreads();

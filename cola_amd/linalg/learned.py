@@ -27,7 +27,7 @@ import torch
 
 from .. import _native as N
 from .._blocks import ld_of
-from ..ops import Dense, WoodburyInverse
+from ..ops import Dense, LinearOperator, WoodburyInverse
 from ._engine import apply_blocks, lower, workspace
 from .lanczos import default_start, run_lanczos, tridiag_eigh
 
@@ -223,30 +223,80 @@ def top_eigenvectors(K, m, steps, key):
     return U, lam
 
 
+class _InstanceStack(LinearOperator):
+    """Engine-only operator for a batched Lanczos run: probe b is multiplied by its own
+    matrix K[b] (CK_OP_DENSE_BATCHED_F32), so many training instances share every launch."""
+
+    def __init__(self, K):
+        self.K = K  # (batch, n, n) fp32
+        self.device = K.device
+        super().__init__(torch.float32, K.shape[1:])
+
+    def hip_op(self):
+        from ..ops import _op
+        n = self.K.shape[1]
+        op = _op(N.CK_OP_DENSE_BATCHED_F32, n)
+        op.a, op.lda, op.nnz = self.K.data_ptr(), self.K.stride(1), self.K.stride(0)
+        return op, [self.K]
+
+    def _matmat(self, X):
+        raise NotImplementedError("per-probe matrices exist only on the engine")
+
+
+def _lanczos_instances(seeds, n, din, ell_range, steps, device):
+    """One batched Lanczos run over the instances of `seeds` (all of order n): every probe
+    starts from its instance's default start vector and multiplies by its own kernel matrix."""
+    Xs, ells = [], []
+    K = torch.empty((len(seeds), n, n), dtype=torch.float32, device=device)
+    starts = torch.empty((n, len(seeds)), dtype=torch.float32, device=device)
+    for j, seed in enumerate(seeds):
+        X, ell, _ = sample_instance(seed, n, din, ell_range)
+        Xd = X.to(device)
+        N.call("ck_rbf_dense_f32", n, din, N.ptr(Xd), Xd.stride(0), float(ell), N.ptr(K[j]), n, None, n,
+               N.stream_ptr(device))
+        starts[:, j] = default_start(Dense(K[j]), seed)[:, 0]
+        Xs.append(Xd)
+        ells.append(ell)
+    run = run_lanczos(_InstanceStack(K), starts, min(steps, n), 1e-7)
+    return Xs, ells, run
+
+
 def make_training_set(seeds, n, m=64, din=4, ell_range=(0.05, 1.0), steps=None, device="cuda", batch=256):
-    """Training instances for many seeds: one Lanczos run per instance, then the tridiagonal
-    eigenproblems of up to ``batch`` instances in one ck_tridiag_eigh_f64 launch (one
-    workgroup each) instead of one launch per instance."""
+    """Training instances for many seeds: batches of up to ``batch`` instances run one
+    Lanczos together (CK_OP_DENSE_BATCHED_F32, every launch serves all of them), and their
+    tridiagonal eigenproblems are solved in one ck_tridiag_eigh_f64 launch."""
     steps = steps or (2 * m + 32)
     out = []
     seeds = list(seeds)
+    batched_ok = n % 4 == 0
     for c0 in range(0, len(seeds), batch):
+        chunk = seeds[c0:c0 + batch]
         runs = []
-        for seed in seeds[c0:c0 + batch]:
+        if batched_ok:
+            Xs, ells, run = _lanczos_instances(chunk, n, din, ell_range, steps, device)
+            finite = torch.isfinite(run.diag[:, :run.iters]).all(1) & torch.isfinite(run.subdiag[:, 1:run.iters]).all(1)
+            finite = finite.cpu().numpy()
+            for j, seed in enumerate(chunk):
+                if finite[j]:
+                    runs.append((seed, Xs[j], ells[j], run, j))
+            redo = [sd for j, sd in enumerate(chunk) if not finite[j]]
+        else:
+            redo = chunk
+        for seed in redo:  # a probe that broke down in the shared run is redone on its own
             X, ell, _ = sample_instance(seed, n, din, ell_range)
             Xd = X.to(device)
             K = rbf_matrix(Xd, ell)
             A = Dense(K)
-            run = run_lanczos(A, default_start(A, seed)[:, 0], min(steps, n), 1e-7)
+            run1 = run_lanczos(A, default_start(A, seed)[:, 0], min(steps, n), 1e-7)
             del K, A
-            runs.append((seed, Xd, ell, run))
+            runs.append((seed, Xd, ell, run1, 0))
         groups = {}
         for item in runs:
             groups.setdefault(item[3].iters, []).append(item)
         for k, grp in groups.items():
             nb = len(grp)
-            d = torch.stack([it[3].diag[0, :k] for it in grp]).contiguous()
-            e = torch.stack([it[3].subdiag[0, 1:max(k, 2)] for it in grp]).contiguous()
+            d = torch.stack([it[3].diag[it[4], :k] for it in grp]).contiguous()
+            e = torch.stack([it[3].subdiag[it[4], 1:max(k, 2)] for it in grp]).contiguous()
             ev = torch.empty((nb, k), dtype=torch.float64, device=device)
             vecs = torch.empty((nb, k, k), dtype=torch.float64, device=device)
             st = torch.empty(nb, dtype=torch.int32, device=device)
@@ -255,11 +305,11 @@ def make_training_set(seeds, n, m=64, din=4, ell_range=(0.05, 1.0), steps=None, 
             if int(st.max()) != 0:
                 raise RuntimeError("tridiagonal eigensolve did not converge")
             mm = min(m, k)
-            for j, (seed, Xd, ell, run) in enumerate(grp):
+            for j, (seed, Xd, ell, run, b) in enumerate(grp):
                 S = vecs[j][:, k - mm:].flip(1).contiguous()
                 lam = ev[j, k - mm:].flip(0).contiguous()
                 U = torch.zeros((mm, run.ld), dtype=torch.float32, device=device)
-                N.call("ck_basis_combine", n, k, mm, N.ptr(run.V[0, 1]), run.ld, N.ptr(S), S.stride(0), N.ptr(U),
+                N.call("ck_basis_combine", n, k, mm, N.ptr(run.V[b, 1]), run.ld, N.ptr(S), S.stride(0), N.ptr(U),
                        run.ld, N.stream_ptr(device))
                 out.append((seed, TrainingInstance(Xd, U, torch.clamp(lam, min=0.0).contiguous(), ell, n)))
         del runs, groups

@@ -64,6 +64,11 @@ const char* ck_status_string(int status);
  *                    (K1 (x) K2 (x) K3 + s I)^-1 with aux = 1/(lambda + s); no engine
  *                    counterpart in the reference (it inverts bare Kronecker products
  *                    factor-wise, cola/linalg/inverse/inv.py:139-141); shift must be 0
+ *   CK_OP_DENSE_BATCHED_F32  vector b is multiplied by its own row-major fp32 matrix
+ *                    a + b * nnz (element stride nnz, leading dimension lda): many
+ *                    independent operators of one order run as one launch (e.g. the
+ *                    probes of a batched Lanczos run, each on its own instance); n,
+ *                    lda, nnz, ldx, ldy multiples of 4, 16-byte aligned
  * The shift term is how Sum(S, ScalarMul(c) @ I) (cola/ops/operators.py:182-183,
  * 97-98) reaches one kernel epilogue.
  */
@@ -76,7 +81,8 @@ enum {
   CK_OP_DIAG_F32 = 5,
   CK_OP_WOODBURY_INV = 6,
   CK_OP_RBF_F32 = 7,
-  CK_OP_KRON_EIG_F32 = 8
+  CK_OP_KRON_EIG_F32 = 8,
+  CK_OP_DENSE_BATCHED_F32 = 9
 };
 
 typedef struct ck_op {

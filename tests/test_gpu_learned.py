@@ -169,14 +169,31 @@ def test_woodbury_accurate_when_ill_conditioned(dev):
 
 
 def test_training_set_batched_matches_single(dev):
-    """The batched eigensolve path gives the same targets as one instance at a time."""
+    """The batched path (one Lanczos over several instances, one eigensolve launch) gives the
+    targets of one instance at a time, up to the fp32 summation order of the two matvec kernels."""
     batch = L.make_training_set([3, 4, 5], 300, m=16, steps=60, device=dev, batch=2)
     for seed, inst in zip([3, 4, 5], batch):
         one = L.make_training_instance(seed, 300, m=16, steps=60, device=dev)
-        assert torch.allclose(inst.w, one.w, rtol=1e-12, atol=1e-12)
-        # Ritz vectors up to sign
-        dots = (inst.U[:, :300] * one.U[:, :300]).sum(1).abs()
-        assert torch.allclose(dots, torch.ones_like(dots), atol=1e-4)
+        assert torch.allclose(inst.w, one.w, rtol=1e-4, atol=1e-6 * float(one.w.max()))
+        # leading Ritz vectors up to sign
+        dots = (inst.U[:8, :300] * one.U[:8, :300]).sum(1).abs()
+        assert torch.allclose(dots, torch.ones_like(dots), atol=1e-3)
+
+
+def test_dense_batched_operator(dev):
+    """CK_OP_DENSE_BATCHED_F32: vector b multiplied by its own matrix."""
+    from cola_amd.ops import _op
+    g = torch.Generator().manual_seed(12)
+    nb, n = 5, 260
+    K = torch.randn(nb, n, n, generator=g)
+    X = torch.randn(nb, n, generator=g)
+    Kd, Xd = K.to(dev), X.to(dev)
+    Y = torch.empty_like(Xd)
+    op = _op(N.CK_OP_DENSE_BATCHED_F32, n, scale=0.5, shift=0.25)
+    op.a, op.lda, op.nnz = Kd.data_ptr(), n, n * n
+    N.call("ck_op_apply", op, nb, N.ptr(Xd), n, N.ptr(Y), n, None, 0, N.stream_ptr(dev))
+    ref = 0.5 * torch.einsum("bij,bj->bi", K.double(), X.double()) + 0.25 * X.double()
+    assert torch.allclose(Y.cpu().double(), ref, rtol=1e-5, atol=1e-4)
 
 
 def _iters(A, b, P=None):
