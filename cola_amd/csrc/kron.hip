@@ -75,35 +75,84 @@ __device__ __forceinline__ typename MF<T>::acc tile_product(const T* a, const T*
   return c;
 }
 
+// the four 16-column tiles of a 16x64 strip at once: four independent accumulation
+// chains interleaved per k step keep the matrix pipe busy (one chain alone waits out
+// each MFMA's latency before the next can issue)
+template <typename T>
+__device__ __forceinline__ void tile_product4(const T* a, const T* __restrict__ brow0, int64_t bstride,
+                                              typename MF<T>::acc (&c)[4]) {
+  T bb[4][16];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) load16(brow0 + t * bstride, bb[t]);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) c[t] = typename MF<T>::acc{0, 0, 0, 0};
+#pragma unroll
+  for (int ks = 0; ks < 16; ++ks)
+#pragma unroll
+    for (int t = 0; t < 4; ++t) c[t] = MF<T>::step(a[ks], bb[t][ks], c[t]);
+}
+
+// Pass A walks kSlabsPerBlock slabs per block.  The factor operands every slab reuses
+// (a lane's four K3 row slices and its K2 row slice) stay in registers, the next slab's
+// rows are prefetched while the current one is on the MFMA pipe, and the intermediate
+// alternates between two LDS buffers so each slab needs one barrier.
+constexpr int kSlabsPerBlock = 2;
+
 template <typename T>
 __global__ __launch_bounds__(256) void kron64_pass_a(const T* __restrict__ K2, const T* __restrict__ K3,
                                                      const T* __restrict__ X, int64_t ldx, T* __restrict__ U,
-                                                     int64_t ldu, const int* __restrict__ done) {
+                                                     int64_t ldu, int nslabs, const int* __restrict__ done) {
   if (done && *done) return;
-  __shared__ __align__(16) T Tt[64 * TP];
-  const int i1 = blockIdx.x, b = blockIdx.y;
+  // two intermediates in flight for fp32 (2 x 17 KB); fp64 keeps one (34 KB) and a second barrier
+  constexpr int NBUF = sizeof(T) == 4 ? 2 : 1;
+  __shared__ __align__(16) T Tt[NBUF][64 * TP];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, li = lane & 15;
-  const T* S = X + (int64_t)b * ldx + (int64_t)i1 * 4096;
+  T k3[4][16];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) load16(K3 + (16 * t + li) * 64 + 16 * g, k3[t]);
+  T k2[16];
+  load16(K2 + (16 * w + li) * 64 + 16 * g, k2);
+  const int s0 = blockIdx.x * kSlabsPerBlock;
+  const int64_t row_off = (16 * w + li) * 64 + 16 * g;
+  auto slab = [&](int sidx) { return X + (int64_t)(sidx >> 6) * ldx + (int64_t)(sidx & 63) * 4096; };
   T a[16];
-  // T = S K3^T: T[r][c] = sum_k S[r][k] K3[c][k]
-  load16(S + (16 * w + li) * 64 + 16 * g, a);
+  if (s0 < nslabs) load16(slab(s0) + row_off, a);
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const auto c = tile_product<T>(a, K3 + (16 * t + li) * 64 + 16 * g);
-    // c[it] = T[16w + row(g, it)][16t + li] -> Tt[16t + li][16w + row(g, it)]
-    T* dst = Tt + (16 * t + li) * TP + 16 * w;
+  for (int it = 0; it < kSlabsPerBlock; ++it) {
+    const int sidx = s0 + it;
+    if (sidx >= nslabs) break;  // uniform over the block
+    const bool more = it + 1 < kSlabsPerBlock && sidx + 1 < nslabs;
+    T an[16];
+    if (more) load16(slab(sidx + 1) + row_off, an);
+    // T = S K3^T: T[r][c] = sum_k S[r][k] K3[c][k]
+    typename MF<T>::acc c[4];
 #pragma unroll
-    for (int it = 0; it < 4; ++it) dst[MF<T>::row(g, it)] = c[it];
-  }
-  __syncthreads();
-  // U = K2 T: U[r][c] = sum_k K2[r][k] Tt[c][k]
-  load16(K2 + (16 * w + li) * 64 + 16 * g, a);
-  T* Ub = U + (int64_t)b * ldu + (int64_t)i1 * 4096;
+    for (int t = 0; t < 4; ++t) c[t] = typename MF<T>::acc{0, 0, 0, 0};
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const auto c = tile_product<T>(a, Tt + (16 * t + li) * TP + 16 * g);
+    for (int ks = 0; ks < 16; ++ks)
 #pragma unroll
-    for (int it = 0; it < 4; ++it) Ub[(16 * w + MF<T>::row(g, it)) * 64 + 16 * t + li] = c[it];
+      for (int t = 0; t < 4; ++t) c[t] = MF<T>::step(a[ks], k3[t][ks], c[t]);
+    T* tb = Tt[it % NBUF];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      // c[t][it] = T[16w + row(g, it)][16t + li] -> Tt[16t + li][16w + row(g, it)]
+      T* dst = tb + (16 * t + li) * TP + 16 * w;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) dst[MF<T>::row(g, q)] = c[t][q];
+    }
+    __syncthreads();
+    // U = K2 T: U[r][c] = sum_k K2[r][k] Tt[c][k]
+    tile_product4<T>(k2, tb + li * TP + 16 * g, 16 * TP, c);
+    T* Ub = U + (int64_t)(sidx >> 6) * ldu + (int64_t)(sidx & 63) * 4096;
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) Ub[(16 * w + MF<T>::row(g, q)) * 64 + 16 * t + li] = c[t][q];
+    if (more) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) a[q] = an[q];
+    }
+    if (NBUF == 1) __syncthreads();  // every wave is done reading tb before it is rewritten
   }
 }
 
@@ -145,9 +194,11 @@ __global__ __launch_bounds__(256) void kron64_pass_b(const T* __restrict__ K1, c
   load16(K1 + (16 * w + li) * 64 + 16 * g, a);
   float dot = 0.f;
   const int64_t xb = (int64_t)b * ldx, yb = (int64_t)b * ldy;
+  typename MF<T>::acc ct4[4];
+  tile_product4<T>(a, Ut + li * TP + 16 * g, 16 * TP, ct4);
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    const auto c = tile_product<T>(a, Ut + (16 * t + li) * TP + 16 * g);
+    const auto& c = ct4[t];
 #pragma unroll
     for (int it = 0; it < 4; ++it) {
       const int64_t off = (int64_t)(16 * w + MF<T>::row(g, it)) * 4096 + c0 + 16 * t + li;
@@ -169,7 +220,9 @@ template <typename T>
 int launch_kron64(const T* const* f, int nrhs, const T* X, int64_t ldx, T* Y, int64_t ldy, T* U, T scale, T shift,
                   const T* dvec, float* partials, int nparts, const int* done, hipStream_t s) {
   const int64_t n = 64 * 64 * 64;
-  hipLaunchKernelGGL(kron64_pass_a<T>, dim3(64, nrhs), dim3(256), 0, s, f[1], f[2], X, ldx, U, n, done);
+  const int nslabs = 64 * nrhs;
+  const unsigned grid_a = (unsigned)((nslabs + kSlabsPerBlock - 1) / kSlabsPerBlock);
+  hipLaunchKernelGGL(kron64_pass_a<T>, dim3(grid_a), dim3(256), 0, s, f[1], f[2], X, ldx, U, n, nslabs, done);
   hipLaunchKernelGGL(kron64_pass_b<T>, dim3(64, nrhs), dim3(256), 0, s, f[0], (const T*)U, n, X, ldx, Y, ldy, scale,
                      shift, dvec, partials, nparts, done);
   CK_CHECK_LAUNCH();
@@ -226,8 +279,9 @@ extern "C" int ck_kron_apply_last2_f32(const float* K2, const float* K3, int32_t
   if (!K2 || !K3 || !X || !Y || nrhs < 1 || ldx < n || ldy < n) return CK_ERR_BAD_ARG;
   if (!aligned16(K2) || !aligned16(K3) || !aligned16(X) || !aligned16(Y) || (ldx % 4) || (ldy % 4))
     return CK_ERR_UNSUPPORTED;
-  hipLaunchKernelGGL(kron64_pass_a<float>, dim3(64, nrhs), dim3(256), 0, (hipStream_t)stream, K2, K3, X, ldx, Y, ldy,
-                     (const int*)nullptr);
+  const int nslabs = 64 * nrhs;
+  hipLaunchKernelGGL(kron64_pass_a<float>, dim3((unsigned)((nslabs + kSlabsPerBlock - 1) / kSlabsPerBlock)), dim3(256),
+                     0, (hipStream_t)stream, K2, K3, X, ldx, Y, ldy, nslabs, (const int*)nullptr);
   CK_CHECK_LAUNCH();
   return CK_OK;
 }

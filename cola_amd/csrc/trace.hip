@@ -94,11 +94,18 @@ __global__ __launch_bounds__(256) void trace_bits_partials(int n, const int32_t*
   double acc = 0.0;
   if (row < n) {
     const int k0 = rowptr[row], k1 = rowptr[row + 1];
-    for (int k = k0; k < k1; ++k) {
-      const int c = colind[k];
-      int agree = 0;
-      for (int g = 0; g < ngroups; ++g) agree += 64 - 2 * __popcll(W[(int64_t)g * n + row] ^ W[(int64_t)g * n + c]);
-      acc += (double)val[k] * (double)agree;
+    if (ngroups == 1) {  // 64 probes: the row's own word is loaded once
+      const uint64_t wr = W[row];
+      // __popcll returns unsigned: convert before the subtraction can wrap
+      for (int k = k0; k < k1; ++k) acc += (double)val[k] * (double)(64 - 2 * (int)__popcll(wr ^ W[colind[k]]));
+    } else {
+      for (int k = k0; k < k1; ++k) {
+        const int c = colind[k];
+        int agree = 0;
+        for (int g = 0; g < ngroups; ++g)
+          agree += 64 - 2 * (int)__popcll(W[(int64_t)g * n + row] ^ W[(int64_t)g * n + c]);
+        acc += (double)val[k] * (double)agree;
+      }
     }
   }
   acc = block_sum<256>(acc, red);

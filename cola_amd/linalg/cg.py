@@ -121,7 +121,9 @@ def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, l
            int(max_iters), N.ptr(st.partials), s)
     fused = aop is not None and (P is None or pop is not None)
     steps = 0
-    k, done = read_ints(st.state, 2, dev)
+    # the loop flag is first read after one chunk: a column block that starts converged costs
+    # one chunk of early-exiting launches instead of an extra host round trip on every call
+    k, done = 0, False
     while not done and steps < max_iters:
         nst = min(int(chunk), max_iters - steps)
         if fused:
@@ -137,6 +139,8 @@ def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, l
                 N.call("ck_cg_step_p", n, B, N.ptr(st.r), N.ptr(st.z), N.ptr(st.p), ld, N.ptr(st.state),
                        N.ptr(st.partials), s)
         steps += nst
+        k, done = read_ints(st.state, 2, dev)
+    if steps == 0:
         k, done = read_ints(st.state, 2, dev)
     N.call("ck_cg_finish", n, B, N.ptr(st.x), N.ptr(st.r), ld, N.ptr(st.state), s)
     if not want_info:

@@ -197,7 +197,7 @@ class MixedSolver:
         wbytes = N.q("ck_ir_workspace_bytes", nrhs)
         self.irwork = torch.empty(wbytes // 8, dtype=torch.float64, device=dev)
 
-    def residual(self, with_ax=True):
+    def residual(self, with_ax=True, read=True):
         n, B, ld = self.n, self.B, self.ld
         s = N.stream_ptr(self.b64.device)
         if with_ax:
@@ -205,6 +205,8 @@ class MixedSolver:
         N.call("ck_ir_residual_f64", n, B, N.ptr(self.b64), ld, N.ptr(self.Ax64) if with_ax else None, ld,
                N.ptr(self.r64), ld, N.ptr(self.r32), ld, N.ptr(self.norms), N.ptr(self.irwork),
                self.irwork.numel() * 8, s)
+        if not read:
+            return None
         nr = self.norms.view(B, 2).cpu()
         return (nr[:, 0] / torch.clamp(nr[:, 1], min=1e-300)).numpy()
 
@@ -215,12 +217,17 @@ class MixedSolver:
         self.b64.zero_()
         self.b64[:, :n].copy_(b.T)
         self.x64.zero_()
-        rel = self.residual(with_ax=False)
-        history = [float(rel.max())]
+        # x = 0: r = b, relative residual 1 for every column (0 for a zero column, which the
+        # first correction solve leaves at x = 0); no host read is needed before the first solve
+        self.residual(with_ax=False, read=False)
+        rel = np.ones(B)
+        history = [1.0]
         inner_steps = []
         s = N.stream_ptr(b.device)
         it = 0
-        chunk = alg.chunk
+        # the first correction solve usually takes as many steps as in the previous call on this
+        # solver: enqueue that many before the first read (steps past convergence still cost launches)
+        chunk = max(1, min(alg.chunk, getattr(self, "_first_steps", alg.chunk)))
         inner = self.inner
         fallback_at = None
         while not (np.all(np.isfinite(rel)) and rel.max() <= alg.tol) and it < alg.max_refinements:
@@ -229,6 +236,8 @@ class MixedSolver:
             k, _, _ = cg_blocks(inner, self.r32, self.st, None, alg.inner_max_iters, alg.inner_tol, alg.P,
                                 chunk, self.lowered_for[id(inner)], want_info=False)
             inner_steps.append(int(k))
+            if it == 0:
+                self._first_steps = max(1, int(k))
             # the next correction solve usually needs as many steps: enqueue that many before the first check
             chunk = max(1, min(alg.chunk, int(k)))
             keep = self.x64.clone() if inner is not self.A else None

@@ -76,6 +76,20 @@ def test_hutch_csr_packed_vs_oracle(dev):
     assert torch.allclose(s2.cpu().double(), dq, rtol=1e-5, atol=1e-3)
 
 
+@pytest.mark.parametrize("probes", [64, 192])
+def test_trace_csr_bits_group_counts(dev, probes):
+    """One and several 64-probe words per row (the single-word path has its own loop)."""
+    m = 40
+    ip, ix, v = oop.laplacian_2d_csr(m, 0.0)
+    n = m * m
+    A = ca.Sparse.from_csr(torch.from_numpy(ip).to(dev), torch.from_numpy(ix).to(dev), torch.from_numpy(v).to(dev),
+                           (n, n))
+    tr = float(hutch_trace_csr(A, probes, seed=11))
+    Z = orng.rademacher(11, n, probes, torch.float64).T
+    ref = float(ohu.trace_from_probes(lambda X: oop.csr_matmat(ip, ix, torch.from_numpy(v).double(), X), Z))
+    assert abs(tr - ref) <= 1e-9 * abs(ref)
+
+
 def test_hipHutch_matches_oracle_with_same_probes(dev):
     n = 300
     A = spd(np.linspace(0.5, 4, n), 11)
