@@ -312,10 +312,19 @@ __global__ __launch_bounds__(256) void bcg_pc_ftr(BcgArgs a) {
   const float* Fs = a.F + a.foff[s];
   const int i0 = a.vtile_i0[t];
   const int i1 = i0 + kBVec < n ? i0 + kBVec : n;
-  double acc = 0.0;
-  if (rg < groups)
-    for (int i = i0 + rg; i < i1; i += groups) acc = fma((double)Fs[(int64_t)i * k + c], (double)a.r[vo + i], acc);
-  red[threadIdx.x] = acc;
+  // four independent partial sums per thread keep several row loads in flight
+  double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
+  if (rg < groups) {
+    int i = i0 + rg;
+    for (; i + 3 * groups < i1; i += 4 * groups) {
+      acc0 = fma((double)Fs[(int64_t)i * k + c], (double)a.r[vo + i], acc0);
+      acc1 = fma((double)Fs[(int64_t)(i + groups) * k + c], (double)a.r[vo + i + groups], acc1);
+      acc2 = fma((double)Fs[(int64_t)(i + 2 * groups) * k + c], (double)a.r[vo + i + 2 * groups], acc2);
+      acc3 = fma((double)Fs[(int64_t)(i + 3 * groups) * k + c], (double)a.r[vo + i + 3 * groups], acc3);
+    }
+    for (; i < i1; i += groups) acc0 = fma((double)Fs[(int64_t)i * k + c], (double)a.r[vo + i], acc0);
+  }
+  red[threadIdx.x] = (acc0 + acc1) + (acc2 + acc3);
   __syncthreads();
   if (threadIdx.x < k) {
     double sum = 0.0;

@@ -92,11 +92,14 @@ def run_batched_cg(A, b, x0=None, max_iters=1000, tol=1e-6, P=None, chunk=16, re
     return x, r, k, info
 
 
-def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, lowered=None, want_info=True):
+def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, lowered=None, want_info=True,
+              read=True):
     """Batched CG on column blocks: solves A X = Bb[:, :n] into st.x (st.r the residual).
 
     ``st`` (a CGState) is reused across calls; ``lowered`` may carry precomputed
-    (aop, pop, work, wb). Returns (k, done, info)."""
+    (aop, pop, work, wb). Returns (k, done, info).  With ``read=False`` exactly
+    min(chunk, max_iters) steps are enqueued (each stops on the device flag) and
+    nothing is read back: returns (None, None, {})."""
     n, B, dev, ld = st.n, st.B, st.device, st.ld
     if lowered is None:
         aop = lower(A)
@@ -139,6 +142,9 @@ def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, l
                 N.call("ck_cg_step_p", n, B, N.ptr(st.r), N.ptr(st.z), N.ptr(st.p), ld, N.ptr(st.state),
                        N.ptr(st.partials), s)
         steps += nst
+        if not read:
+            N.call("ck_cg_finish", n, B, N.ptr(st.x), N.ptr(st.r), ld, N.ptr(st.state), s)
+            return None, None, {}
         k, done = read_ints(st.state, 2, dev)
     if steps == 0:
         k, done = read_ints(st.state, 2, dev)
