@@ -2,10 +2,12 @@
 // fp64: v_mfma_f64_16x16x4f64).
 //
 // Layout: vector b of a column block is a 64x64x64 tensor X[i1][i2][i3].
-//   pass A (one block per (b, i1) slab S = X[b][i1] as a 64x64 matrix):
+//   pass A (per (b, i1) slab S = X[b][i1] as a 64x64 matrix; a block walks
+//   several consecutive slabs):
 //       U = K2 S K3^T          (modes 2 and 3 fused; the 64x64 intermediate
 //                               stays in LDS, transposed)
-//   pass B (one block per (b, 64-column tile of the i2*i3 plane)):
+//   pass B (per (b, 64-column tile of the i2*i3 plane); an fp32 block walks
+//   four consecutive tiles):
 //       Y = scale * K1 U + shift * X   (optionally times an n-vector d, and
 //                                        emitting per-block partials of X.Y)
 //
@@ -92,38 +94,45 @@ __device__ __forceinline__ void tile_product4(const T* a, const T* __restrict__ 
     for (int t = 0; t < 4; ++t) c[t] = MF<T>::step(a[ks], bb[t][ks], c[t]);
 }
 
-// Pass A walks kSlabsPerBlock slabs per block.  The factor operands every slab reuses
-// (a lane's four K3 row slices and its K2 row slice) stay in registers, the next slab's
-// rows are prefetched while the current one is on the MFMA pipe, and the intermediate
-// alternates between two LDS buffers so each slab needs one barrier.
-constexpr int kSlabsPerBlock = 2;
+// Pass A walks SLABS consecutive slabs per block (SLABS divides 64, so a block never
+// straddles two right-hand sides and every block has a full set).  The factor operands
+// every slab reuses (a lane's four K3 row slices and its K2 row slice) stay in registers;
+// the next slab's rows are loaded while the current one is on the MFMA pipe.  The
+// scheduling barriers pin the prefetch ahead of the MFMAs and the stores behind them:
+// left to itself the compiler sinks the prefetch into the MFMA stream and then waits for
+// it together with the current slab's rows.  fp32 alternates two LDS intermediates (one
+// barrier per slab); fp64 (34 KB per intermediate) keeps one and a second barrier.
+template <typename T>
+constexpr int slabs_per_block() {
+  return sizeof(T) == 4 ? 4 : 2;
+}
 
 template <typename T>
 __global__ __launch_bounds__(256) void kron64_pass_a(const T* __restrict__ K2, const T* __restrict__ K3,
                                                      const T* __restrict__ X, int64_t ldx, T* __restrict__ U,
-                                                     int64_t ldu, int nslabs, const int* __restrict__ done) {
+                                                     int64_t ldu, const int* __restrict__ done) {
   if (done && *done) return;
-  // two intermediates in flight for fp32 (2 x 17 KB); fp64 keeps one (34 KB) and a second barrier
+  constexpr int SLABS = slabs_per_block<T>();
   constexpr int NBUF = sizeof(T) == 4 ? 2 : 1;
   __shared__ __align__(16) T Tt[NBUF][64 * TP];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, li = lane & 15;
+  const int s0 = blockIdx.x * SLABS;
+  const int64_t row_off = (16 * w + li) * 64 + 16 * g;
+  auto slab = [&](int sidx) { return X + (int64_t)(sidx >> 6) * ldx + (int64_t)(sidx & 63) * 4096; };
+  T a[16];
+  load16(slab(s0) + row_off, a);
   T k3[4][16];
 #pragma unroll
   for (int t = 0; t < 4; ++t) load16(K3 + (16 * t + li) * 64 + 16 * g, k3[t]);
   T k2[16];
   load16(K2 + (16 * w + li) * 64 + 16 * g, k2);
-  const int s0 = blockIdx.x * kSlabsPerBlock;
-  const int64_t row_off = (16 * w + li) * 64 + 16 * g;
-  auto slab = [&](int sidx) { return X + (int64_t)(sidx >> 6) * ldx + (int64_t)(sidx & 63) * 4096; };
-  T a[16];
-  if (s0 < nslabs) load16(slab(s0) + row_off, a);
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-  for (int it = 0; it < kSlabsPerBlock; ++it) {
+  for (int it = 0; it < SLABS; ++it) {
     const int sidx = s0 + it;
-    if (sidx >= nslabs) break;  // uniform over the block
-    const bool more = it + 1 < kSlabsPerBlock && sidx + 1 < nslabs;
     T an[16];
-    if (more) load16(slab(sidx + 1) + row_off, an);
+    if (it + 1 < SLABS) load16(slab(sidx + 1) + row_off, an);
+    __builtin_amdgcn_sched_barrier(0);
     // T = S K3^T: T[r][c] = sum_k S[r][k] K3[c][k]
     typename MF<T>::acc c[4];
 #pragma unroll
@@ -135,7 +144,7 @@ __global__ __launch_bounds__(256) void kron64_pass_a(const T* __restrict__ K2, c
     T* tb = Tt[it % NBUF];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      // c[t][it] = T[16w + row(g, it)][16t + li] -> Tt[16t + li][16w + row(g, it)]
+      // c[t][q] = T[16w + row(g, q)][16t + li] -> Tt[16t + li][16w + row(g, q)]
       T* dst = tb + (16 * t + li) * TP + 16 * w;
 #pragma unroll
       for (int q = 0; q < 4; ++q) dst[MF<T>::row(g, q)] = c[t][q];
@@ -148,11 +157,93 @@ __global__ __launch_bounds__(256) void kron64_pass_a(const T* __restrict__ K2, c
     for (int t = 0; t < 4; ++t)
 #pragma unroll
       for (int q = 0; q < 4; ++q) Ub[(16 * w + MF<T>::row(g, q)) * 64 + 16 * t + li] = c[t][q];
-    if (more) {
+    __builtin_amdgcn_sched_barrier(0);
+    if (it + 1 < SLABS) {
 #pragma unroll
       for (int q = 0; q < 16; ++q) a[q] = an[q];
     }
     if (NBUF == 1) __syncthreads();  // every wave is done reading tb before it is rewritten
+  }
+}
+
+// Pass B for fp32: four consecutive 64-column tiles of one right-hand side per block.
+// Staging map: thread (wave w, lane g*16 + li) loads U[16r + li][cs .. cs+3] for r = 0..3,
+// cs = 4 * (4w + g); a wave's transposed LDS writes Ut[cs + e][16r + li] then fall in
+// distinct banks.  The next tile's U is loaded into registers while the current tile is
+// on the MFMA pipe, and the staging buffer alternates (one barrier per tile besides the
+// partial sum's).  Per tile the arithmetic, its order and the partial are those of the
+// one-tile kernel below.
+constexpr int kTilesPerBlockB = 4;
+
+__global__ __launch_bounds__(256) void kron64_pass_b4(const float* __restrict__ K1, const float* __restrict__ U,
+                                                      int64_t ldu, const float* __restrict__ X, int64_t ldx,
+                                                      float* __restrict__ Y, int64_t ldy, float scale, float shift,
+                                                      const float* __restrict__ dvec, float* __restrict__ partials,
+                                                      int nparts, const int* __restrict__ done) {
+  if (done && *done) return;
+  constexpr int TILES = kTilesPerBlockB;
+  __shared__ __align__(16) float Ut[2][64 * TP];
+  __shared__ float red[4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, li = lane & 15;
+  const int b = blockIdx.x / (64 / TILES), ct0 = (blockIdx.x % (64 / TILES)) * TILES;
+  const float* Ubase = U + (int64_t)b * ldu;
+  const int64_t xb = (int64_t)b * ldx, yb = (int64_t)b * ldy;
+  const bool need_x = shift != 0.f || partials != nullptr;
+  const int cs = 4 * (4 * w + g);
+  float4 uq[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    uq[r] = *reinterpret_cast<const float4*>(Ubase + (int64_t)(16 * r + li) * 4096 + 64 * ct0 + cs);
+  float a[16];
+  load16(K1 + (16 * w + li) * 64 + 16 * g, a);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int it = 0; it < TILES; ++it) {
+    const int ct = ct0 + it, c0 = 64 * ct;
+    float* ut = Ut[it % 2];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int k = 16 * r + li;
+      ut[(cs + 0) * TP + k] = uq[r].x;
+      ut[(cs + 1) * TP + k] = uq[r].y;
+      ut[(cs + 2) * TP + k] = uq[r].z;
+      ut[(cs + 3) * TP + k] = uq[r].w;
+    }
+    if (it + 1 < TILES) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        uq[r] = *reinterpret_cast<const float4*>(Ubase + (int64_t)(16 * r + li) * 4096 + c0 + 64 + cs);
+    }
+    float xv[4][4];
+    if (need_x) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) xv[t][q] = X[xb + (int64_t)(16 * w + 4 * g + q) * 4096 + c0 + 16 * t + li];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    __syncthreads();
+    floatx4 c[4];
+    tile_product4<float>(a, ut + li * TP + 16 * g, 16 * TP, c);
+    float dot = 0.f;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int64_t off = (int64_t)(16 * w + 4 * g + q) * 4096 + c0 + 16 * t + li;
+        const float x = need_x ? xv[t][q] : 0.f;
+        float y = scale * c[t][q];
+        if (dvec) y *= dvec[off];
+        y += shift * x;
+        Y[yb + off] = y;
+        dot += x * y;
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (partials) {
+      const float sum = ck::block_sum<256>(dot, red);
+      if (threadIdx.x == 0) partials[(int64_t)b * nparts + ct] = sum;
+    }
   }
 }
 
@@ -220,11 +311,16 @@ template <typename T>
 int launch_kron64(const T* const* f, int nrhs, const T* X, int64_t ldx, T* Y, int64_t ldy, T* U, T scale, T shift,
                   const T* dvec, float* partials, int nparts, const int* done, hipStream_t s) {
   const int64_t n = 64 * 64 * 64;
-  const int nslabs = 64 * nrhs;
-  const unsigned grid_a = (unsigned)((nslabs + kSlabsPerBlock - 1) / kSlabsPerBlock);
-  hipLaunchKernelGGL(kron64_pass_a<T>, dim3(grid_a), dim3(256), 0, s, f[1], f[2], X, ldx, U, n, nslabs, done);
-  hipLaunchKernelGGL(kron64_pass_b<T>, dim3(64, nrhs), dim3(256), 0, s, f[0], (const T*)U, n, X, ldx, Y, ldy, scale,
-                     shift, dvec, partials, nparts, done);
+  const unsigned grid_a = (unsigned)(64 / slabs_per_block<T>() * nrhs);
+  hipLaunchKernelGGL(kron64_pass_a<T>, dim3(grid_a), dim3(256), 0, s, f[1], f[2], X, ldx, U, n, done);
+  if constexpr (sizeof(T) == 4) {
+    if (nparts != 0 && nparts != 64) return CK_ERR_BAD_ARG;
+    hipLaunchKernelGGL(kron64_pass_b4, dim3(64 / kTilesPerBlockB * nrhs), dim3(256), 0, s, f[0], (const T*)U, n, X,
+                       ldx, Y, ldy, scale, shift, dvec, partials, nparts, done);
+  } else {
+    hipLaunchKernelGGL(kron64_pass_b<T>, dim3(64, nrhs), dim3(256), 0, s, f[0], (const T*)U, n, X, ldx, Y, ldy,
+                       scale, shift, dvec, partials, nparts, done);
+  }
   CK_CHECK_LAUNCH();
   return CK_OK;
 }
@@ -279,9 +375,8 @@ extern "C" int ck_kron_apply_last2_f32(const float* K2, const float* K3, int32_t
   if (!K2 || !K3 || !X || !Y || nrhs < 1 || ldx < n || ldy < n) return CK_ERR_BAD_ARG;
   if (!aligned16(K2) || !aligned16(K3) || !aligned16(X) || !aligned16(Y) || (ldx % 4) || (ldy % 4))
     return CK_ERR_UNSUPPORTED;
-  const int nslabs = 64 * nrhs;
-  hipLaunchKernelGGL(kron64_pass_a<float>, dim3((unsigned)((nslabs + kSlabsPerBlock - 1) / kSlabsPerBlock)), dim3(256),
-                     0, (hipStream_t)stream, K2, K3, X, ldx, Y, ldy, nslabs, (const int*)nullptr);
+  hipLaunchKernelGGL(kron64_pass_a<float>, dim3((unsigned)(64 / slabs_per_block<float>() * nrhs)), dim3(256), 0,
+                     (hipStream_t)stream, K2, K3, X, ldx, Y, ldy, (const int*)nullptr);
   CK_CHECK_LAUNCH();
   return CK_OK;
 }
