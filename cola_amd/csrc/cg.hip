@@ -371,11 +371,12 @@ int step_xr(int64_t n, int nrhs, const float* p, const float* Ap, float* x, floa
   return CK_OK;
 }
 
+// rz_ready: the preconditioner's last pass already wrote the r.z partials (q.p2)
 int step_p(int64_t n, int nrhs, const float* r, const float* z, float* p, int64_t ld, void* state, const Parts& q,
-           hipStream_t s) {
+           hipStream_t s, bool rz_ready = false) {
   const int precond = (z != r) ? 1 : 0;
   const int* done = &static_cast<ck_cg_ctrl*>(state)->done;
-  if (precond) {
+  if (precond && !rz_ready) {
     int st = launch_partials(0, n, nrhs, r, z, ld, q.p2, q, done, s);
     if (st) return st;
   }
@@ -466,6 +467,10 @@ int ck_cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float*
   const int* done = &static_cast<ck_cg_ctrl*>(state)->done;
   // the fused epilogue writes nrhs x kron_nparts partials into the p.Ap region (nrhs x P)
   const bool fuse_kron = A->kind == CK_OP_KRON_F32 && A->nfactors == 3 && kron_nparts(A) <= q.P;
+  // the 64^3 eigenbasis preconditioner's last pass emits the r.z partials, one per 4096-element
+  // slab: the same chunking as the generic partial sums when n = 64^3 (P = 64)
+  const bool fuse_eig = P && P->kind == CK_OP_KRON_EIG_F32 && q.P == 64 && q.per == 4096 &&
+                        kron64_eligible(P, r, ld, z, ld);
   for (int it = 0; it < nsteps; ++it) {
     int st;
     if (fuse_kron) {
@@ -482,11 +487,14 @@ int ck_cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float*
       st = step_xr(n, nrhs, p, Ap, x, r, ld, state, q, q.p0, q.P, s);
     }
     if (st) return st;
-    if (P) {
+    if (fuse_eig) {
+      st = kron64_eig_apply_f32(P, nrhs, r, ld, z, ld, work, work_bytes, done, s, q.p2);
+      if (st) return st;
+    } else if (P) {
       st = op_apply_internal(P, nrhs, r, ld, z, ld, work, work_bytes, done, s);
       if (st) return st;
     }
-    st = step_p(n, nrhs, r, z, p, ld, state, q, s);
+    st = step_p(n, nrhs, r, z, p, ld, state, q, s, fuse_eig);
     if (st) return st;
   }
   return CK_OK;

@@ -107,14 +107,20 @@ constexpr int slabs_per_block() {
   return sizeof(T) == 4 ? 4 : 2;
 }
 
+//
+// With R given (fp32 only), each slab also emits partials[b * 64 + i1] = sum over the slab of
+// R * U: the r.z partial sums of a preconditioned CG step when this pass writes z = P r.
 template <typename T>
 __global__ __launch_bounds__(256) void kron64_pass_a(const T* __restrict__ K2, const T* __restrict__ K3,
                                                      const T* __restrict__ X, int64_t ldx, T* __restrict__ U,
-                                                     int64_t ldu, const int* __restrict__ done) {
+                                                     int64_t ldu, const int* __restrict__ done,
+                                                     const T* __restrict__ R, int64_t ldr,
+                                                     float* __restrict__ partials) {
   if (done && *done) return;
   constexpr int SLABS = slabs_per_block<T>();
   constexpr int NBUF = sizeof(T) == 4 ? 2 : 1;
   __shared__ __align__(16) T Tt[NBUF][64 * TP];
+  __shared__ float red[4];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, li = lane & 15;
   const int s0 = blockIdx.x * SLABS;
   const int64_t row_off = (16 * w + li) * 64 + 16 * g;
@@ -158,6 +164,16 @@ __global__ __launch_bounds__(256) void kron64_pass_a(const T* __restrict__ K2, c
 #pragma unroll
       for (int q = 0; q < 4; ++q) Ub[(16 * w + MF<T>::row(g, q)) * 64 + 16 * t + li] = c[t][q];
     __builtin_amdgcn_sched_barrier(0);
+    if (R) {
+      const T* Rb = R + (int64_t)(sidx >> 6) * ldr + (int64_t)(sidx & 63) * 4096;
+      float dot = 0.f;
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) dot += (float)(Rb[(16 * w + MF<T>::row(g, q)) * 64 + 16 * t + li] * c[t][q]);
+      const float sum = ck::block_sum<256>(dot, red);
+      if (threadIdx.x == 0) partials[sidx] = sum;
+    }
     if (it + 1 < SLABS) {
 #pragma unroll
       for (int q = 0; q < 16; ++q) a[q] = an[q];
@@ -247,6 +263,75 @@ __global__ __launch_bounds__(256) void kron64_pass_b4(const float* __restrict__ 
   }
 }
 
+// The two mode-1 products of the eigenbasis preconditioner Q diag(d) Q^T, fused:
+//   V[b][:, c] = scale * Q1 (d[:, c] * (Q1^T U[b][:, c]))   for each column c of a tile.
+// A 64-column tile holds every i1 of its columns, so both products and the diagonal
+// scaling run on one LDS-resident tile; the tile walk, staging map and prefetch are those
+// of kron64_pass_b4.  Q1T is Q1^T stored explicitly (row-major).
+__global__ __launch_bounds__(256) void kron64_pass_bb(const float* __restrict__ Q1T, const float* __restrict__ Q1,
+                                                      const float* __restrict__ U, int64_t ldu,
+                                                      float* __restrict__ V, int64_t ldv, float scale,
+                                                      const float* __restrict__ dvec, const int* __restrict__ done) {
+  if (done && *done) return;
+  constexpr int TILES = kTilesPerBlockB;
+  __shared__ __align__(16) float Ut[2][64 * TP];
+  __shared__ __align__(16) float Wt[64 * TP];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, li = lane & 15;
+  const int b = blockIdx.x / (64 / TILES), ct0 = (blockIdx.x % (64 / TILES)) * TILES;
+  const float* Ubase = U + (int64_t)b * ldu;
+  const int64_t vb = (int64_t)b * ldv;
+  const int cs = 4 * (4 * w + g);
+  float4 uq[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    uq[r] = *reinterpret_cast<const float4*>(Ubase + (int64_t)(16 * r + li) * 4096 + 64 * ct0 + cs);
+  float a1[16], a2[16];
+  load16(Q1T + (16 * w + li) * 64 + 16 * g, a1);
+  load16(Q1 + (16 * w + li) * 64 + 16 * g, a2);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int it = 0; it < TILES; ++it) {
+    const int c0 = 64 * (ct0 + it);
+    float* ut = Ut[it % 2];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int k = 16 * r + li;
+      ut[(cs + 0) * TP + k] = uq[r].x;
+      ut[(cs + 1) * TP + k] = uq[r].y;
+      ut[(cs + 2) * TP + k] = uq[r].z;
+      ut[(cs + 3) * TP + k] = uq[r].w;
+    }
+    if (it + 1 < TILES) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        uq[r] = *reinterpret_cast<const float4*>(Ubase + (int64_t)(16 * r + li) * 4096 + c0 + 64 + cs);
+    }
+    float dv[4][4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) dv[t][q] = dvec[(int64_t)(16 * w + 4 * g + q) * 4096 + c0 + 16 * t + li];
+    __builtin_amdgcn_sched_barrier(0);
+    __syncthreads();
+    floatx4 c[4];
+    // c[t][q] = (Q1^T U)[16w + 4g + q][c0 + 16t + li]
+    tile_product4<float>(a1, ut + li * TP + 16 * g, 16 * TP, c);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      float* dst = Wt + (16 * t + li) * TP + 16 * w + 4 * g;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) dst[q] = c[t][q] * dv[t][q];
+    }
+    __syncthreads();
+    tile_product4<float>(a2, Wt + li * TP + 16 * g, 16 * TP, c);
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) V[vb + (int64_t)(16 * w + 4 * g + q) * 4096 + c0 + 16 * t + li] = scale * c[t][q];
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 // Y[b][j1][c] = d * scale * sum_k K1[j1][k] U[b][k][c] + shift * X[b][j1][c], c in tile
 template <typename T>
 __global__ __launch_bounds__(256) void kron64_pass_b(const T* __restrict__ K1, const T* __restrict__ U, int64_t ldu,
@@ -312,7 +397,8 @@ int launch_kron64(const T* const* f, int nrhs, const T* X, int64_t ldx, T* Y, in
                   const T* dvec, float* partials, int nparts, const int* done, hipStream_t s) {
   const int64_t n = 64 * 64 * 64;
   const unsigned grid_a = (unsigned)(64 / slabs_per_block<T>() * nrhs);
-  hipLaunchKernelGGL(kron64_pass_a<T>, dim3(grid_a), dim3(256), 0, s, f[1], f[2], X, ldx, U, n, done);
+  hipLaunchKernelGGL(kron64_pass_a<T>, dim3(grid_a), dim3(256), 0, s, f[1], f[2], X, ldx, U, n, done,
+                     (const T*)nullptr, (int64_t)0, (float*)nullptr);
   if constexpr (sizeof(T) == 4) {
     if (nparts != 0 && nparts != 64) return CK_ERR_BAD_ARG;
     hipLaunchKernelGGL(kron64_pass_b4, dim3(64 / kTilesPerBlockB * nrhs), dim3(256), 0, s, f[0], (const T*)U, n, X,
@@ -346,25 +432,29 @@ int kron64_apply_f32(const ck_op* op, int nrhs, const float* X, int64_t ldx, flo
                               partials, nparts, done, s);
 }
 
-// y = scale * Q (d * (Q^T x)) + shift * x, Q = Q0 x Q1 x Q2 (factor), Q^T factors in factor2, d = aux
+// y = scale * Q (d * (Q^T x)), Q = Q0 x Q1 x Q2 (factor), Q^T factors in factor2, d = aux.
+// Three launches: modes 2+3 of Q^T (pass A), both mode-1 products with the diagonal in
+// between (pass BB), modes 2+3 of Q (pass A, writing Y).  With rz_partials, the last pass
+// also emits the 64 per-slab partial sums of X . Y per column (r.z of a PCG step).
 int kron64_eig_apply_f32(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,
-                         size_t work_bytes, const int* done, hipStream_t s) {
+                         size_t work_bytes, const int* done, hipStream_t s, float* rz_partials) {
   const int64_t n = op->n;
   if (!work || work_bytes < 2 * (size_t)nrhs * n * sizeof(float)) return CK_ERR_WORKSPACE;
   for (int i = 0; i < 3; ++i)
     if (!op->factor2[i] || !aligned16(op->factor2[i])) return CK_ERR_BAD_ARG;
   if (!op->aux) return CK_ERR_BAD_ARG;
+  if (op->shift != 0.f) return CK_ERR_UNSUPPORTED;
   float* U = static_cast<float*>(work);
   float* V = U + (size_t)nrhs * n;
-  const float* ft[3] = {op->factor2[0], op->factor2[1], op->factor2[2]};
-  const float* f[3] = {op->factor[0], op->factor[1], op->factor[2]};
-  // V = d * (Q^T x)   (ldx of V = n)
-  int st = launch_kron64<float>(ft, nrhs, X, ldx, V, n, U, 1.f, 0.f, op->aux, nullptr, 0, done, s);
-  if (st) return st;
-  // Y = scale * Q V + shift * X: pass B's shift term reads X at Y's layout, so apply it separately when needed
-  if (op->shift == 0.f)
-    return launch_kron64<float>(f, nrhs, V, n, Y, ldy, U, op->scale, 0.f, nullptr, nullptr, 0, done, s);
-  return CK_ERR_UNSUPPORTED;
+  const unsigned grid_a = (unsigned)(64 / slabs_per_block<float>() * nrhs);
+  hipLaunchKernelGGL(kron64_pass_a<float>, dim3(grid_a), dim3(256), 0, s, op->factor2[1], op->factor2[2], X, ldx, U,
+                     n, done, (const float*)nullptr, (int64_t)0, (float*)nullptr);
+  hipLaunchKernelGGL(kron64_pass_bb, dim3(64 / kTilesPerBlockB * nrhs), dim3(256), 0, s, op->factor2[0],
+                     op->factor[0], (const float*)U, n, V, n, op->scale, op->aux, done);
+  hipLaunchKernelGGL(kron64_pass_a<float>, dim3(grid_a), dim3(256), 0, s, op->factor[1], op->factor[2],
+                     (const float*)V, n, Y, ldy, done, rz_partials ? X : nullptr, ldx, rz_partials);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
 }
 
 }  // namespace ck
@@ -376,7 +466,8 @@ extern "C" int ck_kron_apply_last2_f32(const float* K2, const float* K3, int32_t
   if (!aligned16(K2) || !aligned16(K3) || !aligned16(X) || !aligned16(Y) || (ldx % 4) || (ldy % 4))
     return CK_ERR_UNSUPPORTED;
   hipLaunchKernelGGL(kron64_pass_a<float>, dim3((unsigned)(64 / slabs_per_block<float>() * nrhs)), dim3(256), 0,
-                     (hipStream_t)stream, K2, K3, X, ldx, Y, ldy, (const int*)nullptr);
+                     (hipStream_t)stream, K2, K3, X, ldx, Y, ldy, (const int*)nullptr, (const float*)nullptr,
+                     (int64_t)0, (float*)nullptr);
   CK_CHECK_LAUNCH();
   return CK_OK;
 }

@@ -22,8 +22,9 @@ size_t kron_workspace_bytes(const ck_op* op, int nrhs);
 bool kron64_eligible(const ck_op* op, const void* X, int64_t ldx, const void* Y, int64_t ldy);
 int kron64_apply_f32(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,
                      size_t work_bytes, const int* done, float* partials, int nparts, hipStream_t s);
+// rz_partials (optional): nrhs x 64 per-slab partial sums of X . Y (64^3 operators only)
 int kron64_eig_apply_f32(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,
-                         size_t work_bytes, const int* done, hipStream_t s);
+                         size_t work_bytes, const int* done, hipStream_t s, float* rz_partials = nullptr);
 
 int woodbury_apply(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,
                    size_t work_bytes, const int* done, hipStream_t s);
