@@ -81,6 +81,8 @@ _PROTOS = {
     "ck_ir_workspace_bytes": (_sz, [_i32]),
     "ck_ir_residual_f64": (ctypes.c_int, [_i64, _i32, _p, _i64, _p, _i64, _p, _i64, _p, _i64, _p, _p, _sz, _p]),
     "ck_ir_update_f64": (ctypes.c_int, [_i64, _i32, _p, _i64, _p, _i64, _p]),
+    "ck_ir_inner_start": (ctypes.c_int, [_i64, _i32, _p, _i64, _p, _p, _p, _i64, _p, _p]),
+    "ck_ir_update_scaled_f64": (ctypes.c_int, [_i64, _i32, _p, _i64, _p, _i64, _p, _p]),
     "ck_f32_to_bf16": (ctypes.c_int, [_i64, _p, _p, _p]),
     "ck_stream_copy_f32": (ctypes.c_int, [_i64, _p, _p, _p]),
     "ck_dot_f32": (ctypes.c_int, [_i64, _i32, _p, _i64, _p, _i64, _p, _p, _sz, _p]),

@@ -4,6 +4,8 @@
 //   ck_ir_residual_f64: r = b - Ax in fp64, its fp32 copy for the inner solve,
 //                       and per-column ||r||^2, ||b||^2 partials -> fp64 norms
 //   ck_ir_update_f64:   x += d (fp32 correction accumulated in fp64)
+//   ck_ir_inner_start / ck_ir_update_scaled_f64: the inner CG's column normalisation
+//                       and un-normalisation folded into the refinement passes
 #include "common.hpp"
 
 using namespace ck;
@@ -52,6 +54,33 @@ __global__ __launch_bounds__(256) void ir_update_kernel(int64_t n, double* __res
   const int col = blockIdx.y;
   if (i >= n) return;
   x[col * ldx + i] += (double)d[col * ldd + i];
+}
+
+// Start of an inner correction solve on the refinement residual: r = r32 / safe(||r||),
+// x = 0, mult = ||r|| (fp32) -- the column normalisation of the inner CG done with the
+// FP64 norm the residual pass already produced.
+__global__ __launch_bounds__(256) void ir_inner_start_kernel(int64_t n, const float* __restrict__ r32, int64_t ld32,
+                                                             const double* __restrict__ norms, float* __restrict__ r,
+                                                             float* __restrict__ x, int64_t ld,
+                                                             float* __restrict__ mult) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int col = blockIdx.y;
+  const float m = (float)norms[2 * col];
+  if (i == 0) mult[col] = m;
+  if (i >= n) return;
+  const float ms = fabsf(m) < 1e-40f ? 1e-40f : m;
+  r[col * ld + i] = r32[col * ld32 + i] / ms;
+  x[col * ld + i] = 0.f;
+}
+
+// x += mult * d: the inner solve's un-normalisation (fp32 product) folded into the FP64 update
+__global__ __launch_bounds__(256) void ir_update_scaled_kernel(int64_t n, double* __restrict__ x, int64_t ldx,
+                                                               const float* __restrict__ d, int64_t ldd,
+                                                               const float* __restrict__ mult) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int col = blockIdx.y;
+  if (i >= n) return;
+  x[col * ldx + i] += (double)(d[col * ldd + i] * mult[col]);
 }
 
 __global__ __launch_bounds__(256) void to_bf16_kernel(int64_t count, const float* __restrict__ src,
@@ -121,6 +150,24 @@ int ck_ir_update_f64(int64_t n, int32_t nrhs, double* x, int64_t ldx, const floa
   if (n < 1 || nrhs < 1 || !x || !d || ldx < n || ldd < n) return CK_ERR_BAD_ARG;
   hipLaunchKernelGGL(ir_update_kernel, dim3((unsigned)cdiv(n, 256), nrhs), dim3(256), 0, (hipStream_t)stream, n, x,
                      ldx, d, ldd);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+int ck_ir_inner_start(int64_t n, int32_t nrhs, const float* r32, int64_t ld32, const double* norms, float* r,
+                      float* x, int64_t ld, float* mult, void* stream) {
+  if (n < 1 || nrhs < 1 || !r32 || !norms || !r || !x || !mult || ld32 < n || ld < n) return CK_ERR_BAD_ARG;
+  hipLaunchKernelGGL(ir_inner_start_kernel, dim3((unsigned)cdiv(n, 256), nrhs), dim3(256), 0, (hipStream_t)stream, n,
+                     r32, ld32, norms, r, x, ld, mult);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+int ck_ir_update_scaled_f64(int64_t n, int32_t nrhs, double* x, int64_t ldx, const float* d, int64_t ldd,
+                            const float* mult, void* stream) {
+  if (n < 1 || nrhs < 1 || !x || !d || !mult || ldx < n || ldd < n) return CK_ERR_BAD_ARG;
+  hipLaunchKernelGGL(ir_update_scaled_kernel, dim3((unsigned)cdiv(n, 256), nrhs), dim3(256), 0, (hipStream_t)stream,
+                     n, x, ldx, d, ldd, mult);
   CK_CHECK_LAUNCH();
   return CK_OK;
 }

@@ -93,13 +93,15 @@ def run_batched_cg(A, b, x0=None, max_iters=1000, tol=1e-6, P=None, chunk=16, re
 
 
 def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, lowered=None, want_info=True,
-              read=True):
+              read=True, prepared=False, finish=True):
     """Batched CG on column blocks: solves A X = Bb[:, :n] into st.x (st.r the residual).
 
     ``st`` (a CGState) is reused across calls; ``lowered`` may carry precomputed
     (aop, pop, work, wb). Returns (k, done, info).  With ``read=False`` exactly
     min(chunk, max_iters) steps are enqueued (each stops on the device flag) and
-    nothing is read back: returns (None, None, {})."""
+    nothing is read back: returns (None, None, {}).  ``prepared``: st.r already holds
+    the normalised rhs, st.x zeros and the state its column norms (``Bb`` unused).
+    ``finish=False`` leaves x and r normalised (the caller un-normalises)."""
     n, B, dev, ld = st.n, st.B, st.device, st.ld
     if lowered is None:
         aop = lower(A)
@@ -109,8 +111,15 @@ def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, l
         aop, pop, work, wb = lowered
     s = N.stream_ptr(dev)
     t0 = time.time()
-    N.call("ck_cg_normalize", n, B, N.ptr(Bb), Bb.stride(0), N.ptr(st.bn), ld, N.ptr(st.state), N.ptr(st.partials), s)
-    if x0b is None:
+    if prepared:
+        if x0b is not None:
+            raise ValueError("a prepared start has x = 0")
+    else:
+        N.call("ck_cg_normalize", n, B, N.ptr(Bb), Bb.stride(0), N.ptr(st.bn), ld, N.ptr(st.state),
+               N.ptr(st.partials), s)
+    if prepared:
+        pass
+    elif x0b is None:
         st.x.zero_()
         st.r.copy_(st.bn)
     else:
@@ -143,12 +152,14 @@ def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, l
                        N.ptr(st.partials), s)
         steps += nst
         if not read:
-            N.call("ck_cg_finish", n, B, N.ptr(st.x), N.ptr(st.r), ld, N.ptr(st.state), s)
+            if finish:
+                N.call("ck_cg_finish", n, B, N.ptr(st.x), N.ptr(st.r), ld, N.ptr(st.state), s)
             return None, None, {}
         k, done = read_ints(st.state, 2, dev)
     if steps == 0:
         k, done = read_ints(st.state, 2, dev)
-    N.call("ck_cg_finish", n, B, N.ptr(st.x), N.ptr(st.r), ld, N.ptr(st.state), s)
+    if finish:
+        N.call("ck_cg_finish", n, B, N.ptr(st.x), N.ptr(st.r), ld, N.ptr(st.state), s)
     if not want_info:
         return k, done, {"iterations": k + 1}
     errs = st.floats(N.CK_CG_ERRORS, k + 1).cpu().numpy().astype(np.float64)

@@ -191,7 +191,9 @@ class MixedSolver:
             work, wb = workspace([aop[0] if aop else None, pop[0] if pop else None], nrhs, dev)
             self.lowered_for[id(op)] = (aop, pop, work, wb)
         z64 = lambda: torch.zeros((nrhs, self.ld), dtype=torch.float64, device=dev)  # noqa: E731
-        self.b64, self.x64, self.Ax64, self.r64 = z64(), z64(), z64(), z64()
+        self.b64, self.x64, self.Ax64 = z64(), z64(), z64()
+        # the inner CG's column norms: written by ck_ir_inner_start, read by the scaled update
+        self.mult = self.st.floats(N.CK_CG_MULT, nrhs)
         self.r32 = torch.zeros((nrhs, self.ld), dtype=torch.float32, device=dev)
         self.norms = torch.zeros(2 * nrhs, dtype=torch.float64, device=dev)
         wbytes = N.q("ck_ir_workspace_bytes", nrhs)
@@ -203,12 +205,24 @@ class MixedSolver:
         if with_ax:
             self.f64(self.x64, self.Ax64, B)
         N.call("ck_ir_residual_f64", n, B, N.ptr(self.b64), ld, N.ptr(self.Ax64) if with_ax else None, ld,
-               N.ptr(self.r64), ld, N.ptr(self.r32), ld, N.ptr(self.norms), N.ptr(self.irwork),
-               self.irwork.numel() * 8, s)
+               None, ld, N.ptr(self.r32), ld, N.ptr(self.norms), N.ptr(self.irwork), self.irwork.numel() * 8, s)
         if not read:
             return None
         nr = self.norms.view(B, 2).cpu()
         return (nr[:, 0] / torch.clamp(nr[:, 1], min=1e-300)).numpy()
+
+    def correction(self, inner, max_steps, chunk, read=True):
+        """One inner correction solve on the current residual and its FP64 update of x:
+        the inner CG starts from the residual pass's own norms (ck_ir_inner_start) and
+        its un-normalisation is folded into the update (ck_ir_update_scaled_f64)."""
+        n, B, ld, alg = self.n, self.B, self.ld, self.alg
+        s = N.stream_ptr(self.b64.device)
+        N.call("ck_ir_inner_start", n, B, N.ptr(self.r32), ld, N.ptr(self.norms), N.ptr(self.st.r), N.ptr(self.st.x),
+               ld, N.ptr(self.mult), s)
+        k, _, _ = cg_blocks(inner, None, self.st, None, max_steps, alg.inner_tol, alg.P, chunk,
+                            self.lowered_for[id(inner)], want_info=False, read=read, prepared=True, finish=False)
+        N.call("ck_ir_update_scaled_f64", n, B, N.ptr(self.x64), ld, N.ptr(self.st.x), ld, N.ptr(self.mult), s)
+        return k
 
     def solve(self, b):
         alg = self.alg
@@ -223,7 +237,6 @@ class MixedSolver:
         rel = np.ones(B)
         history = [1.0]
         inner_steps = []
-        s = N.stream_ptr(b.device)
         it = 0
         # the first correction solve usually takes as many steps as in the previous call on this
         # solver: enqueue that many before the first read (steps past convergence still cost launches)
@@ -236,9 +249,7 @@ class MixedSolver:
             # between, then read the FP64 residual once; if that is not yet enough, the loop below
             # continues from the refined iterate
             for k_steps in plan:
-                cg_blocks(inner, self.r32, self.st, None, k_steps, alg.inner_tol, alg.P, k_steps,
-                          self.lowered_for[id(inner)], want_info=False, read=False)
-                N.call("ck_ir_update_f64", n, B, N.ptr(self.x64), ld, N.ptr(self.st.x), ld, s)
+                self.correction(inner, k_steps, k_steps, read=False)
                 self.residual(with_ax=True, read=False)
             nr = self.norms.view(B, 2).cpu()
             rel = (nr[:, 0] / torch.clamp(nr[:, 1], min=1e-300)).numpy()
@@ -248,15 +259,13 @@ class MixedSolver:
         while not (np.all(np.isfinite(rel)) and rel.max() <= alg.tol) and it < alg.max_refinements:
             if not np.all(np.isfinite(rel)):
                 raise FloatingPointError("non-finite FP64 residual in iterative refinement")
-            k, _, _ = cg_blocks(inner, self.r32, self.st, None, alg.inner_max_iters, alg.inner_tol, alg.P,
-                                chunk, self.lowered_for[id(inner)], want_info=False)
+            keep = self.x64.clone() if inner is not self.A else None
+            k = self.correction(inner, alg.inner_max_iters, chunk)
             inner_steps.append(int(k))
             if it == 0:
                 self._first_steps = max(1, int(k))
             # the next correction solve usually needs as many steps: enqueue that many before the first check
             chunk = max(1, min(alg.chunk, int(k)))
-            keep = self.x64.clone() if inner is not self.A else None
-            N.call("ck_ir_update_f64", n, B, N.ptr(self.x64), ld, N.ptr(self.st.x), ld, s)
             prev = rel.max()
             rel = self.residual(with_ax=True)
             it += 1

@@ -133,6 +133,16 @@ int ck_ir_residual_f64(int64_t n, int32_t nrhs, const double* b, int64_t ldb, co
                        double* r, int64_t ldr, float* r32, int64_t ldr32, double* norms, void* work,
                        size_t work_bytes, void* stream);
 int ck_ir_update_f64(int64_t n, int32_t nrhs, double* x, int64_t ldx, const float* d, int64_t ldd, void* stream);
+/* Inner-solve normalisation folded into the refinement (replaces ck_cg_normalize +
+ * zeroing x + copying r, and ck_cg_finish + ck_ir_update_f64, inside a refinement step):
+ * ck_ir_inner_start: r = r32 / max(||r||, 1e-40), x = 0, mult[col] = (float)||r|| with
+ *   ||r|| = norms[2*col] from ck_ir_residual_f64; mult points at the CG state's
+ *   CK_CG_MULT array, so the inner CG then runs from ck_cg_start as if normalised.
+ * ck_ir_update_scaled_f64: x += (double)(d * mult[col]). */
+int ck_ir_inner_start(int64_t n, int32_t nrhs, const float* r32, int64_t ld32, const double* norms, float* r,
+                      float* x, int64_t ld, float* mult, void* stream);
+int ck_ir_update_scaled_f64(int64_t n, int32_t nrhs, double* x, int64_t ldx, const float* d, int64_t ldd,
+                            const float* mult, void* stream);
 int ck_f32_to_bf16(int64_t count, const float* src, uint16_t* dst, void* stream);
 /* dst = src for n floats (n % 4 == 0, 16-byte aligned): the bandwidth probe that sets the
  * measured HBM peak and calibrates byte counters; no reference counterpart. */

@@ -77,6 +77,36 @@ def test_ir_residual_kernel(dev):
     assert torch.allclose(norms.view(B, 2)[:, 1], b[:, :n].norm(dim=1))
 
 
+def test_ir_inner_start_and_scaled_update(dev):
+    """The inner solve's normalisation folded into the refinement: r = r32 / ||r|| (fp32
+    division, a zero column stays zero), x = 0, mult = float(||r||); then x64 += d * mult
+    with the product rounded in fp32 and the sum in fp64."""
+    n, B, ld = 1000, 3, 1024
+    g = torch.Generator().manual_seed(5)
+    r32 = torch.randn(B, ld, generator=g)
+    r32[2] = 0.0
+    norms = torch.zeros(2 * B, dtype=torch.float64)
+    norms[0::2] = r32[:, :n].double().norm(dim=1)
+    r32d, normsd = r32.to(dev), norms.to(dev)
+    r = torch.full((B, ld), 7.0, device=dev)
+    x = torch.full((B, ld), 7.0, device=dev)
+    mult = torch.full((B, ), -1.0, device=dev)
+    N.call("ck_ir_inner_start", n, B, N.ptr(r32d), ld, N.ptr(normsd), N.ptr(r), N.ptr(x), ld, N.ptr(mult),
+           N.stream_ptr(dev))
+    m32 = norms[0::2].float()
+    assert torch.equal(mult.cpu(), m32)
+    safe = torch.where(m32.abs() < 1e-40, torch.full_like(m32, 1e-40), m32)
+    assert torch.equal(r[:, :n].cpu(), r32[:, :n] / safe[:, None])
+    assert torch.all(x[:, :n] == 0) and torch.all(r[:, n:] == 7.0)
+    x64 = torch.randn(B, ld, generator=g, dtype=torch.float64)
+    d = torch.randn(B, ld, generator=g)
+    xd = x64.to(dev)
+    N.call("ck_ir_update_scaled_f64", n, B, N.ptr(xd), ld, N.ptr(d.to(dev)), ld, N.ptr(mult), N.stream_ptr(dev))
+    ref = x64[:, :n] + (d[:, :n] * m32[:, None]).double()
+    assert torch.equal(xd[:, :n].cpu(), ref)
+    assert torch.equal(xd[:, n:].cpu(), x64[:, n:])
+
+
 def test_mixed_cg_kron_reaches_fp64_tolerance(dev):
     A, facs, b = kron_system(dev, 4, seed=0)
     P = KroneckerEigInverse([f.to(dev) for f in facs], 1.0, 0.01)
