@@ -32,7 +32,7 @@ METRIC = "CG solves/sec to 1e-6 rel. residual: learned+mixed-prec vs plain; HBM 
 HBM_PEAK = 8000.0  # GB/s
 # HBM bytes per kron64_pass_a<float> launch at 64^3 x 16 rhs, from rocprofv3 --pmc FETCH_SIZE (x2.0) plus
 # WRITE_SIZE (x1.0), the factors a 1 GiB stream copy reads on the same counters (profiles/kron64_cg_64cubed_hbm_traffic.md)
-PASS_A_TRAFFIC_BYTES = (16574.4 + 16245.9) * 1024
+PASS_A_TRAFFIC_BYTES = (16682.4 + 16246.0) * 1024
 FP32_MFMA_PEAK = 157.3  # TFLOP/s
 
 
@@ -92,7 +92,7 @@ def bench_kron_cg(args, dev):
     torch.cuda.synchronize()
     t_setup = time.perf_counter()
     P = KroneckerEigInverse([f.to(dev) for f in facs], 1.0, 0.01)
-    alg = HipMixedCG(tol=1e-6, inner_tol=args.inner_tol, P=P, chunk=args.chunk)
+    alg = HipMixedCG(tol=1e-6, inner_tol=args.inner_tol, P=P, chunk=args.chunk, graph=not args.no_graph)
     solver = MixedSolver(A, B, alg)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t_setup
@@ -147,6 +147,7 @@ def bench_kron_cg(args, dev):
            "data": "synthetic (seeded RBF Kronecker factors, N(0,1) rhs)",
            "config": {"workload": "kron_cg: K1xK2xK3+0.01I, N=64^3=262144, 16 rhs, to FP64-checked 1e-6",
                       "method": "FP64-residual iterative refinement around FP32 PCG, Kronecker eigenbasis preconditioner",
+                      "hip_graph": alg.graph,
                       "global_batch": B, "seq_len": n, "parallelism": "single GPU",
                       "refinements": info["refinements"], "inner_cg_steps": info["inner_iterations"],
                       "max_fp64_rel_residual_solver": res_x64, "max_fp64_rel_residual_bench_check": float(res.max()),
@@ -634,6 +635,7 @@ def main():
     ap.add_argument("--chunk", type=int, default=8)
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch the refinement schedule eagerly (no HIP graph)")
     ap.add_argument("--n", type=int, default=16384, help="points per system (learned_cg)")
     ap.add_argument("--systems", type=int, default=4, help="held-out systems (learned_cg)")
     ap.add_argument("--train-instances", type=int, default=2000)
