@@ -156,6 +156,10 @@ class HipMixedCG(Algorithm):
             the FP64 residual by this factor switches the remaining corrections to A
             itself (the perturbation A - inner is then too large for refinement to
             converge on this system).
+        graph: replay a repeated solve's schedule (refinements x inner steps, taken from
+            the previous call on the same solver) as one captured HIP graph instead of
+            launch by launch.  Every captured launch is a kernel (no memset or host
+            read), and each inner step still stops on the device flag.
     """
     tol: float = 1e-6
     inner_tol: float = 1e-4
@@ -165,6 +169,7 @@ class HipMixedCG(Algorithm):
     chunk: int = 8
     inner: object = None
     contraction: float = 0.05
+    graph: bool = False
 
     def __call__(self, A, b):
         is_vector = b.dim() == 1
@@ -224,16 +229,28 @@ class MixedSolver:
         N.call("ck_ir_update_scaled_f64", n, B, N.ptr(self.x64), ld, N.ptr(self.st.x), ld, N.ptr(self.mult), s)
         return k
 
+    def _schedule(self, plan):
+        """x = 0, then one correction solve of k steps and one FP64 residual per plan entry:
+        launches only (no host reads), so it can be captured as a graph."""
+        self.x64.zero_()
+        self.residual(with_ax=False, read=False)
+        for k_steps in plan:
+            self.correction(self.A, k_steps, k_steps, read=False)
+            self.residual(with_ax=True, read=False)
+
     def solve(self, b):
         alg = self.alg
         n, B, ld = self.n, self.B, self.ld
         t0 = time.time()
-        self.b64.zero_()
+        # (the residual and update passes touch columns [:n] only: b64's padding is never read)
         self.b64[:, :n].copy_(b.T)
-        self.x64.zero_()
-        # x = 0: r = b, relative residual 1 for every column (0 for a zero column, which the
-        # first correction solve leaves at x = 0); no host read is needed before the first solve
-        self.residual(with_ax=False, read=False)
+        plan = getattr(self, "_plan", None)
+        replay = bool(plan) and self.inner is self.A and len(plan) <= alg.max_refinements
+        if not replay:
+            self.x64.zero_()
+            # x = 0: r = b, relative residual 1 for every column (0 for a zero column, which the
+            # first correction solve leaves at x = 0); no host read is needed before the first solve
+            self.residual(with_ax=False, read=False)
         rel = np.ones(B)
         history = [1.0]
         inner_steps = []
@@ -243,14 +260,21 @@ class MixedSolver:
         chunk = max(1, min(alg.chunk, getattr(self, "_first_steps", alg.chunk)))
         inner = self.inner
         fallback_at = None
-        plan = getattr(self, "_plan", None)
-        if plan and inner is self.A and len(plan) <= alg.max_refinements:
+        if replay:
             # replay the previous call's schedule (refinements x inner steps) with no host read in
             # between, then read the FP64 residual once; if that is not yet enough, the loop below
             # continues from the refined iterate
-            for k_steps in plan:
-                self.correction(inner, k_steps, k_steps, read=False)
-                self.residual(with_ax=True, read=False)
+            if alg.graph:
+                if getattr(self, "_graph_plan", None) != plan:
+                    self._graph = None
+                    torch.cuda.synchronize(b.device)
+                    g = torch.cuda.CUDAGraph()
+                    with torch.cuda.graph(g):
+                        self._schedule(plan)
+                    self._graph, self._graph_plan = g, list(plan)
+                self._graph.replay()
+            else:
+                self._schedule(plan)
             nr = self.norms.view(B, 2).cpu()
             rel = (nr[:, 0] / torch.clamp(nr[:, 1], min=1e-300)).numpy()
             it = len(plan)

@@ -124,3 +124,31 @@ def test_mixed_cg_kron_reaches_fp64_tolerance(dev):
     print("returned x residual", res, "x err", float((x.cpu() - ref).norm() / ref.norm()), flush=True)
     assert float(res.max()) <= 1e-6
     assert float((x.cpu() - ref).norm() / ref.norm()) < 1e-4
+
+
+def test_mixed_cg_graph_replays_match_eager(dev):
+    """A repeated solve replays the previous schedule; captured as a HIP graph it must give
+    the eager replay's bits on every replay, with eager work and syncs in between, and on a
+    new right-hand side (the graph reads the solver's own rhs buffer)."""
+    from cola_amd.linalg.mixed import MixedSolver
+    A, facs, b = kron_system(dev, 4, seed=2)
+    P = KroneckerEigInverse([f.to(dev) for f in facs], 1.0, 0.01)
+    bd = b.to(dev)
+    eager = MixedSolver(A, 4, HipMixedCG(tol=1e-6, inner_tol=1e-4, P=P, graph=False))
+    graphed = MixedSolver(A, 4, HipMixedCG(tol=1e-6, inner_tol=1e-4, P=P, graph=True))
+    for s in (eager, graphed):
+        s.solve(bd)  # records the schedule
+    x_ref, info_ref = eager.solve(bd)
+    x_ref = x_ref.clone()
+    for rep in range(3):
+        junk = torch.randn(1 << 20, device=dev).sum()  # eager work between replays
+        torch.cuda.synchronize()
+        x, info = graphed.solve(bd)
+        assert info["converged"] and info["refinements"] == info_ref["refinements"], (rep, info)
+        assert torch.equal(x, x_ref), rep
+        del junk
+    b2 = torch.randn(64**3, 4, generator=torch.Generator().manual_seed(9)).to(dev)
+    x2_ref, _ = eager.solve(b2)
+    x2, info2 = graphed.solve(b2)
+    assert info2["converged"]
+    assert torch.equal(x2, x2_ref)
