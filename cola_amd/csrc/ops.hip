@@ -2,7 +2,8 @@
 //
 //   dense  : each wavefront owns ROWS rows; lanes stride the row in float4 chunks,
 //            one DPP wave reduction per (row, rhs).            [HBM / L2 bound]
-//   csr    : one lane per row (short rows) or one wavefront per row (long rows);
+//   csr    : one lane per row, short rows batched into registers (short rows) or
+//            one wavefront per row (long rows);
 //            int32 column indices; coalesced rowptr/value streams. [HBM bound]
 //   kron   : factors of order <= 64; the last two modes are fused per slab
 //            (U = K2 S K3^T with S the 64x64 slab) and the first mode is a
@@ -353,25 +354,6 @@ int dense_dispatch(const ck_op* op, int nrhs, const float* X, int64_t ldx, float
 }
 
 // ============================================================== CSR
-__global__ __launch_bounds__(256) void csr_lane_per_row(int n, const int32_t* __restrict__ rowptr,
-                                                        const int32_t* __restrict__ colind,
-                                                        const float* __restrict__ val, const float* __restrict__ X,
-                                                        int64_t ldx, float* __restrict__ Y, int64_t ldy, int nrhs,
-                                                        float scale, float shift, const int* __restrict__ done) {
-  if (done && *done) return;
-  const int row = blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= n) return;
-  const int k0 = rowptr[row], k1 = rowptr[row + 1];
-  for (int b = 0; b < nrhs; ++b) {
-    const float* xb = X + b * ldx;
-    float acc = 0.f;
-    for (int k = k0; k < k1; ++k) acc = fmaf(val[k], xb[colind[k]], acc);
-    float y = scale * acc;
-    if (shift != 0.f) y = fmaf(shift, xb[row], y);
-    Y[b * ldy + row] = y;
-  }
-}
-
 __global__ __launch_bounds__(256) void csr_wave_per_row(int n, const int32_t* __restrict__ rowptr,
                                                         const int32_t* __restrict__ colind,
                                                         const float* __restrict__ val, const float* __restrict__ X,
@@ -395,6 +377,65 @@ __global__ __launch_bounds__(256) void csr_wave_per_row(int n, const int32_t* __
   }
 }
 
+// Short rows: one lane per row.  A row of at most kCsrShort nonzeros is loaded into
+// registers in one batch (column indices and values, then the x gathers of NB vectors),
+// so each lane waits for memory twice per NB vectors instead of once per nonzero; the
+// matrix is read once for all vectors.  Longer rows take a plain loop.  Every row is
+// summed k ascending with one fma per nonzero, in both paths, so results do not depend
+// on the path or on the vector count.
+constexpr int kCsrShort = 8;
+
+template <int NB>
+__global__ __launch_bounds__(256) void csr_lane_per_row(int n, const int32_t* __restrict__ rowptr,
+                                                        const int32_t* __restrict__ colind,
+                                                        const float* __restrict__ val, const float* __restrict__ X,
+                                                        int64_t ldx, float* __restrict__ Y, int64_t ldy, int nrhs,
+                                                        float scale, float shift, const int* __restrict__ done) {
+  if (done && *done) return;
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  const int k0 = rowptr[row], k1 = rowptr[row + 1];
+  const int len = k1 - k0;
+  if (len <= kCsrShort) {
+    int c[kCsrShort];
+    float v[kCsrShort];
+#pragma unroll
+    for (int i = 0; i < kCsrShort; ++i) {
+      c[i] = i < len ? colind[k0 + i] : row;
+      v[i] = i < len ? val[k0 + i] : 0.f;
+    }
+    for (int b0 = 0; b0 < nrhs; b0 += NB) {
+      float xv[NB][kCsrShort];
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+#pragma unroll
+        for (int i = 0; i < kCsrShort; ++i)
+          xv[j][i] = (i < len && b0 + j < nrhs) ? X[(int64_t)(b0 + j) * ldx + c[i]] : 0.f;
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const int b = b0 + j;
+        if (b >= nrhs) break;
+        float acc = 0.f;
+#pragma unroll
+        for (int i = 0; i < kCsrShort; ++i)
+          if (i < len) acc = fmaf(v[i], xv[j][i], acc);
+        float y = scale * acc;
+        if (shift != 0.f) y = fmaf(shift, X[(int64_t)b * ldx + row], y);
+        Y[(int64_t)b * ldy + row] = y;
+      }
+    }
+    return;
+  }
+  for (int b = 0; b < nrhs; ++b) {
+    const float* xb = X + (int64_t)b * ldx;
+    float acc = 0.f;
+    for (int k = k0; k < k1; ++k) acc = fmaf(val[k], xb[colind[k]], acc);
+    float y = scale * acc;
+    if (shift != 0.f) y = fmaf(shift, xb[row], y);
+    Y[(int64_t)b * ldy + row] = y;
+  }
+}
+
 int csr_dispatch(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, const int* done,
                  hipStream_t s) {
   const int n = op->n;
@@ -405,9 +446,16 @@ int csr_dispatch(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* 
     hipLaunchKernelGGL(csr_wave_per_row, dim3(grid), dim3(256), 0, s, n, op->rowptr, op->colind, op->values, X, ldx,
                        Y, ldy, nrhs, op->scale, op->shift, done);
   } else {
-    const int grid = (n + 255) / 256;
-    hipLaunchKernelGGL(csr_lane_per_row, dim3(grid), dim3(256), 0, s, n, op->rowptr, op->colind, op->values, X, ldx,
-                       Y, ldy, nrhs, op->scale, op->shift, done);
+    const dim3 grid((unsigned)((n + 255) / 256)), blk(256);
+    if (nrhs == 1)
+      hipLaunchKernelGGL(csr_lane_per_row<1>, grid, blk, 0, s, n, op->rowptr, op->colind, op->values, X, ldx, Y, ldy,
+                         nrhs, op->scale, op->shift, done);
+    else if (nrhs == 2)
+      hipLaunchKernelGGL(csr_lane_per_row<2>, grid, blk, 0, s, n, op->rowptr, op->colind, op->values, X, ldx, Y, ldy,
+                         nrhs, op->scale, op->shift, done);
+    else
+      hipLaunchKernelGGL(csr_lane_per_row<4>, grid, blk, 0, s, n, op->rowptr, op->colind, op->values, X, ldx, Y, ldy,
+                         nrhs, op->scale, op->shift, done);
   }
   CK_CHECK_LAUNCH();
   return CK_OK;

@@ -58,7 +58,7 @@ def test_csr_known_answer(dev):
     assert torch.allclose(out[:3].cpu(), torch.tensor([2., 3., -2.]))
 
 
-@pytest.mark.parametrize("m,B", [(5, 1), (33, 3), (128, 2)])
+@pytest.mark.parametrize("m,B", [(5, 1), (33, 3), (128, 2), (64, 5), (64, 8), (40, 16), (40, 17)])
 def test_csr_laplacian(dev, m, B):
     ip, ix, v = oop.laplacian_2d_csr(m, 0.0)
     g = torch.Generator().manual_seed(m)
@@ -69,6 +69,21 @@ def test_csr_laplacian(dev, m, B):
     op = A + 1e-3 * ca.I_like(A)
     out = op @ X.to(dev)
     assert rel(out, ref + 1e-3 * X.double()) < 2e-6
+
+
+@pytest.mark.parametrize("n,density,B", [(1000, 0.024, 3), (777, 0.01, 16), (257, 0.05, 1)])
+def test_csr_short_rows_irregular(dev, n, density, B):
+    """Short rows of uneven length, empty rows and a partial last block; at 24 nonzeros per
+    row a 256-row block holds more nonzeros than the LDS stage and reads them from memory."""
+    g = torch.Generator().manual_seed(n)
+    D = torch.randn(n, n, generator=g) * (torch.rand(n, n, generator=g) < density)
+    D[::7] = 0.0  # empty rows
+    r, c = D.nonzero(as_tuple=True)
+    A = ca.Sparse(D[r, c].to(dev), r.to(dev), c.to(dev), (n, n))
+    X = torch.randn(n, B, generator=g)
+    ip = torch.cat([torch.zeros(1, dtype=torch.int64), torch.bincount(r, minlength=n).cumsum(0)])
+    ref = oop.csr_matmat(ip.numpy(), c.numpy(), D[r, c].double(), X.double())
+    assert rel(A @ X.to(dev), ref) < 2e-6
 
 
 def test_csr_long_rows(dev):
