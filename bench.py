@@ -360,26 +360,24 @@ def bench_hutch(args, dev):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     exact = 4.001 * n
-    byts = 4 * 2 * 5238784 + 4 * (n + 1) + 8 * n * 2  # values, columns, rowptr, probe words (write + read)
-    # roofline: the two device passes alone (probe words, then the popcount trace), HIP events
+    byts = 4 * 2 * 5238784 + 4 * (n + 1)  # values, columns, rowptr (probe words are recomputed in the pass)
+    # roofline: the device pass alone (popcount trace with in-pass probe words + final sum), HIP events
     import ctypes
     hop, keep = op.hip_op()
-    W = torch.empty(n, dtype=torch.int64, device=dev)
     wb = N.q("ck_trace_workspace_bytes", n)
     work = torch.empty(wb // 8 + 1, dtype=torch.float64, device=dev)
     out = torch.empty(1, dtype=torch.float64, device=dev)
     s = N.stream_ptr(dev)
 
     def passes():
-        N.call("ck_rng_rademacher_bits", n, 1, ctypes.c_uint64(7), N.ptr(W), s)
-        N.call("ck_trace_csr_rademacher_bits", hop, 1, N.ptr(W), N.ptr(out), N.ptr(work), wb, s)
+        N.call("ck_trace_csr_rademacher_seeded", hop, 1, ctypes.c_uint64(7), N.ptr(out), N.ptr(work), wb, s)
     t_k = time_kernel(passes, reps=50)
     return {"metric": "Hutchinson trace estimates/sec (config 3b: 64 Rademacher probes, CSR 1024^2 Laplacian)",
             "value": 1 / dt, "unit": "estimates/s", "ms_per_step": dt * 1e3, "dtype": "f32 values, popcount probes",
             "config": {"workload": "hutch", "estimate": float(tr), "exact": exact,
                        "rel_err": abs(float(tr) - exact) / exact, "expected_std_rel": 361.9 / exact,
                        "algorithmic_bytes": byts, "device_us": t_k * 1e6},
-            "roofline": {"bound": "hbm", "kernel": "rademacher bits + trace_bits_partials/final",
+            "roofline": {"bound": "hbm", "kernel": "trace_seeded_partials + trace_bits_final",
                          "achieved": byts / t_k / 1e9, "peak": HBM_PEAK, "unit": "GB/s",
                          "frac": byts / t_k / 1e9 / HBM_PEAK, "traffic": None}}
 

@@ -121,6 +121,7 @@ _PROTOS = {
     "ck_hutch_accumulate": (ctypes.c_int, [_i64, _i32, _p, _i64, _p, _i64, _p, _p, _p]),
     "ck_hutch_csr_bits": (ctypes.c_int, [_op, _i32, _p, _p, _p, _p]),
     "ck_trace_csr_rademacher_bits": (ctypes.c_int, [_op, _i32, _p, _p, _p, _sz, _p]),
+    "ck_trace_csr_rademacher_seeded": (ctypes.c_int, [_op, _i32, ctypes.c_uint64, _p, _p, _sz, _p]),
     "ck_trace_workspace_bytes": (_sz, [_i64]),
     # learned preconditioner, pivoted Cholesky, dense helpers
     "ck_mlp_param_count": (_i64, [_i32, _i32]),

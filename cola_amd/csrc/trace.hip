@@ -85,6 +85,8 @@ __global__ __launch_bounds__(256) void hutch_csr_bits_kernel(int n, const int32_
   }
 }
 
+constexpr int kBitsShort = 8;
+
 __global__ __launch_bounds__(256) void trace_bits_partials(int n, const int32_t* __restrict__ rowptr,
                                                            const int32_t* __restrict__ colind,
                                                            const float* __restrict__ val, int ngroups,
@@ -94,7 +96,25 @@ __global__ __launch_bounds__(256) void trace_bits_partials(int n, const int32_t*
   double acc = 0.0;
   if (row < n) {
     const int k0 = rowptr[row], k1 = rowptr[row + 1];
-    if (ngroups == 1) {  // 64 probes: the row's own word is loaded once
+    const int len = k1 - k0;
+    if (ngroups == 1 && len <= kBitsShort) {
+      // 64 probes, short row: column indices and values in one batch, then the word gathers,
+      // so the lane waits for memory twice per row; same summation order as the loop below
+      const uint64_t wr = W[row];
+      int c[kBitsShort];
+      float v[kBitsShort];
+#pragma unroll
+      for (int i = 0; i < kBitsShort; ++i) {
+        c[i] = i < len ? colind[k0 + i] : row;
+        v[i] = i < len ? val[k0 + i] : 0.f;
+      }
+      uint64_t wc[kBitsShort];
+#pragma unroll
+      for (int i = 0; i < kBitsShort; ++i) wc[i] = i < len ? W[c[i]] : wr;
+#pragma unroll
+      for (int i = 0; i < kBitsShort; ++i)
+        if (i < len) acc += (double)v[i] * (double)(64 - 2 * (int)__popcll(wr ^ wc[i]));
+    } else if (ngroups == 1) {  // 64 probes: the row's own word is loaded once
       const uint64_t wr = W[row];
       // __popcll returns unsigned: convert before the subtraction can wrap
       for (int k = k0; k < k1; ++k) acc += (double)val[k] * (double)(64 - 2 * (int)__popcll(wr ^ W[colind[k]]));
@@ -104,6 +124,48 @@ __global__ __launch_bounds__(256) void trace_bits_partials(int n, const int32_t*
         int agree = 0;
         for (int g = 0; g < ngroups; ++g)
           agree += 64 - 2 * (int)__popcll(W[(int64_t)g * n + row] ^ W[(int64_t)g * n + c]);
+        acc += (double)val[k] * (double)agree;
+      }
+    }
+  }
+  acc = block_sum<256>(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// The same estimate with the probe words recomputed where they are used: word (g, i) is
+// stream_word(stream_key(seed, g), i), exactly what ck_rng_rademacher_bits stores, so the
+// result equals the stored-word path bit for bit while the pass reads only the matrix.
+__global__ __launch_bounds__(256) void trace_seeded_partials(int n, const int32_t* __restrict__ rowptr,
+                                                             const int32_t* __restrict__ colind,
+                                                             const float* __restrict__ val, int ngroups,
+                                                             uint64_t seed, double* __restrict__ part) {
+  __shared__ double red[4];
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (row < n) {
+    const int k0 = rowptr[row], k1 = rowptr[row + 1];
+    const int len = k1 - k0;
+    if (ngroups == 1 && len <= kBitsShort) {
+      const uint64_t key = stream_key(seed, 0);
+      const uint64_t wr = stream_word(key, (uint64_t)row);
+      int c[kBitsShort];
+      float v[kBitsShort];
+#pragma unroll
+      for (int i = 0; i < kBitsShort; ++i) {
+        c[i] = i < len ? colind[k0 + i] : row;
+        v[i] = i < len ? val[k0 + i] : 0.f;
+      }
+#pragma unroll
+      for (int i = 0; i < kBitsShort; ++i)
+        if (i < len) acc += (double)v[i] * (double)(64 - 2 * (int)__popcll(wr ^ stream_word(key, (uint64_t)c[i])));
+    } else {
+      for (int k = k0; k < k1; ++k) {
+        const int c = colind[k];
+        int agree = 0;
+        for (int g = 0; g < ngroups; ++g) {
+          const uint64_t key = stream_key(seed, (uint64_t)g);
+          agree += 64 - 2 * (int)__popcll(stream_word(key, (uint64_t)row) ^ stream_word(key, (uint64_t)c));
+        }
         acc += (double)val[k] * (double)agree;
       }
     }
@@ -268,6 +330,22 @@ int ck_trace_csr_rademacher_bits(const ck_op* A, int32_t ngroups, const uint64_t
   double* part = static_cast<double*>(work);
   hipLaunchKernelGGL(trace_bits_partials, dim3(nparts), dim3(256), 0, s, A->n, A->rowptr, A->colind, A->values,
                      ngroups, W, part);
+  hipLaunchKernelGGL(trace_bits_final, dim3(1), dim3(256), 0, s, nparts, (const double*)part, (double)A->scale,
+                     (double)A->shift * (double)A->n, 1.0 / (64.0 * ngroups), out);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+int ck_trace_csr_rademacher_seeded(const ck_op* A, int32_t ngroups, uint64_t seed, double* out, void* work,
+                                   size_t work_bytes, void* stream) {
+  if (!A || A->kind != CK_OP_CSR_F32 || !A->rowptr || !A->colind || !A->values || ngroups < 1 || !out)
+    return CK_ERR_BAD_ARG;
+  if (!work || work_bytes < ck_trace_workspace_bytes(A->n)) return CK_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  const int nparts = (int)cdiv(A->n, 256);
+  double* part = static_cast<double*>(work);
+  hipLaunchKernelGGL(trace_seeded_partials, dim3(nparts), dim3(256), 0, s, A->n, A->rowptr, A->colind, A->values,
+                     ngroups, seed, part);
   hipLaunchKernelGGL(trace_bits_final, dim3(1), dim3(256), 0, s, nparts, (const double*)part, (double)A->scale,
                      (double)A->shift * (double)A->n, 1.0 / (64.0 * ngroups), out);
   CK_CHECK_LAUNCH();

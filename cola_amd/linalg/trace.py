@@ -117,14 +117,21 @@ def hutch_trace_csr(A, num_probes=64, seed=0):
     if groups * 64 != num_probes:
         raise ValueError("num_probes must be a multiple of 64 for packed probes")
     dev = A.device
-    W = torch.empty((groups, n), dtype=torch.int64, device=dev)
     s = N.stream_ptr(dev)
-    N.call("ck_rng_rademacher_bits", n, groups, ctypes.c_uint64(seed & (2**64 - 1)), N.ptr(W), s)
     wb = N.q("ck_trace_workspace_bytes", n)
-    work = torch.empty((wb + 7) // 8, dtype=torch.float64, device=dev)
+    key = (str(dev), wb, s.value)
+    work = _TRACE_WORK.get(key)
+    if work is None:
+        # per-stream scratch of the block partials (reused: launches on one stream are ordered)
+        work = _TRACE_WORK[key] = torch.empty((wb + 7) // 8, dtype=torch.float64, device=dev)
     out = torch.empty(1, dtype=torch.float64, device=dev)
-    N.call("ck_trace_csr_rademacher_bits", op, groups, N.ptr(W), N.ptr(out), N.ptr(work), wb, s)
+    # the probe words (those of ck_rng_rademacher_bits(seed)) are recomputed inside the pass
+    N.call("ck_trace_csr_rademacher_seeded", op, groups, ctypes.c_uint64(seed & (2**64 - 1)), N.ptr(out), N.ptr(work),
+           wb, s)
     return out[0]
+
+
+_TRACE_WORK = {}
 
 
 @dispatch.abstract

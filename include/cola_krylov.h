@@ -333,10 +333,14 @@ int ck_hutch_accumulate(int64_t n, int32_t nprobes, const float* Z, int64_t ldz,
  * including the op's scale and shift. */
 int ck_hutch_csr_bits(const ck_op* A, int32_t ngroups, const uint64_t* W, float* sum, float* sumsq, void* stream);
 /* Trace-only CSR estimate from packed Rademacher words W (ck_rng_rademacher_bits):
- * out = (1/(64*ngroups)) sum_i sum_k val_k * sum_g (64 - 2 popcount(W[g,i] ^ W[g,col_k]))
+ * out = scale * (1/(64*ngroups)) sum_i sum_k val_k * sum_g (64 - 2 popcount(W[g,i] ^ W[g,col_k]))
  *       + shift * n, fp64.  work >= ck_trace_workspace_bytes(n). */
 int ck_trace_csr_rademacher_bits(const ck_op* A, int32_t ngroups, const uint64_t* W, double* out, void* work,
                                  size_t work_bytes, void* stream);
+/* The same estimate with the words of ck_rng_rademacher_bits(seed) recomputed inside the
+ * pass instead of read from memory: identical result, no probe buffer. */
+int ck_trace_csr_rademacher_seeded(const ck_op* A, int32_t ngroups, uint64_t seed, double* out, void* work,
+                                   size_t work_bytes, void* stream);
 size_t ck_trace_workspace_bytes(int64_t n);
 
 /* ---------------------------------------------------------------- learned preconditioner

@@ -90,6 +90,34 @@ def test_trace_csr_bits_group_counts(dev, probes):
     assert abs(tr - ref) <= 1e-9 * abs(ref)
 
 
+@pytest.mark.parametrize("groups", [1, 3])
+def test_trace_seeded_equals_stored_words(dev, groups):
+    """Recomputing the probe words inside the pass gives the stored-word estimate bit for bit,
+    on short rows (the batched path) and rows longer than 8 (the loop)."""
+    import ctypes
+    g = torch.Generator().manual_seed(groups)
+    n = 700
+    D = torch.randn(n, n, generator=g) * (torch.rand(n, n, generator=g) < 0.006)
+    D[5, :40] = 1.0  # one long row
+    D += torch.eye(n)
+    r, c = D.nonzero(as_tuple=True)
+    A = ca.Sparse(D[r, c].to(dev), r.to(dev), c.to(dev), (n, n))
+    op, keep = (A + 0.5 * ca.I_like(A)).hip_op()
+    W = torch.empty((groups, n), dtype=torch.int64, device=dev)
+    s = N.stream_ptr(dev)
+    N.call("ck_rng_rademacher_bits", n, groups, ctypes.c_uint64(123), N.ptr(W), s)
+    wb = N.q("ck_trace_workspace_bytes", n)
+    work = torch.empty(wb // 8 + 1, dtype=torch.float64, device=dev)
+    out1 = torch.empty(1, dtype=torch.float64, device=dev)
+    out2 = torch.empty(1, dtype=torch.float64, device=dev)
+    N.call("ck_trace_csr_rademacher_bits", op, groups, N.ptr(W), N.ptr(out1), N.ptr(work), wb, s)
+    N.call("ck_trace_csr_rademacher_seeded", op, groups, ctypes.c_uint64(123), N.ptr(out2), N.ptr(work), wb, s)
+    assert float(out1) == float(out2)
+    Z = orng.rademacher(123, n, 64 * groups, torch.float64).T
+    ref = float(ohu.trace_from_probes(lambda X: D.double() @ X + 0.5 * X, Z))
+    assert abs(float(out2) - ref) <= 1e-9 * abs(ref)
+
+
 def test_hipHutch_matches_oracle_with_same_probes(dev):
     n = 300
     A = spd(np.linspace(0.5, 4, n), 11)
