@@ -453,9 +453,14 @@ int ck_cg_step_p(int64_t n, int32_t nrhs, const float* r, const float* z, float*
   return step_p(n, nrhs, r, z, p, ld, state, q, (hipStream_t)stream);
 }
 
-int ck_cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r, float* p, float* Ap, float* z,
-                  int64_t ld, void* state, float* partials, void* work, size_t work_bytes, int32_t nsteps,
-                  void* stream) {
+}  // extern "C"
+
+namespace {
+
+// nsteps steps; with last_partial the final one stops after the x/r update
+int cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r, float* p, float* Ap, float* z,
+               int64_t ld, void* state, float* partials, void* work, size_t work_bytes, int32_t nsteps,
+               bool last_partial, void* stream) {
   if (!A || nrhs < 1 || nrhs > 1024 || !x || !r || !p || !Ap || !state || !partials || nsteps < 0)
     return CK_ERR_BAD_ARG;
   const int64_t n = A->n;
@@ -487,6 +492,7 @@ int ck_cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float*
       st = step_xr(n, nrhs, p, Ap, x, r, ld, state, q, q.p0, q.P, s);
     }
     if (st) return st;
+    if (last_partial && it + 1 == nsteps) break;
     if (fuse_eig) {
       st = kron64_eig_apply_f32(P, nrhs, r, ld, z, ld, work, work_bytes, done, s, q.p2);
       if (st) return st;
@@ -498,6 +504,23 @@ int ck_cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float*
     if (st) return st;
   }
   return CK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ck_cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r, float* p, float* Ap, float* z,
+                  int64_t ld, void* state, float* partials, void* work, size_t work_bytes, int32_t nsteps,
+                  void* stream) {
+  return cg_iterate(A, P, nrhs, x, r, p, Ap, z, ld, state, partials, work, work_bytes, nsteps, false, stream);
+}
+
+int ck_cg_iterate_final(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r, float* p, float* Ap,
+                        float* z, int64_t ld, void* state, float* partials, void* work, size_t work_bytes,
+                        int32_t nsteps, void* stream) {
+  if (nsteps < 1) return CK_ERR_BAD_ARG;
+  return cg_iterate(A, P, nrhs, x, r, p, Ap, z, ld, state, partials, work, work_bytes, nsteps, true, stream);
 }
 
 int ck_cg_finish(int64_t n, int32_t nrhs, float* x, float* r, int64_t ld, const void* state, void* stream) {

@@ -93,7 +93,7 @@ def run_batched_cg(A, b, x0=None, max_iters=1000, tol=1e-6, P=None, chunk=16, re
 
 
 def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, lowered=None, want_info=True,
-              read=True, prepared=False, finish=True):
+              read=True, prepared=False, finish=True, final=False):
     """Batched CG on column blocks: solves A X = Bb[:, :n] into st.x (st.r the residual).
 
     ``st`` (a CGState) is reused across calls; ``lowered`` may carry precomputed
@@ -101,7 +101,9 @@ def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, l
     min(chunk, max_iters) steps are enqueued (each stops on the device flag) and
     nothing is read back: returns (None, None, {}).  ``prepared``: st.r already holds
     the normalised rhs, st.x zeros and the state its column norms (``Bb`` unused).
-    ``finish=False`` leaves x and r normalised (the caller un-normalises)."""
+    ``finish=False`` leaves x and r normalised (the caller un-normalises).  ``final`` (with
+    ``read=False``): the last enqueued step stops after the x/r update, for a caller that
+    takes only x after exactly that many steps (ck_cg_iterate_final)."""
     n, B, dev, ld = st.n, st.B, st.device, st.ld
     if lowered is None:
         aop = lower(A)
@@ -138,14 +140,18 @@ def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, l
     k, done = 0, False
     while not done and steps < max_iters:
         nst = min(int(chunk), max_iters - steps)
+        last_partial = final and not read
         if fused:
-            N.call("ck_cg_iterate", aop[0], pop[0] if pop else None, B, N.ptr(st.x), N.ptr(st.r), N.ptr(st.p),
-                   N.ptr(st.Ap), N.ptr(st.z), ld, N.ptr(st.state), N.ptr(st.partials), N.ptr(work), wb, nst, s)
+            N.call("ck_cg_iterate_final" if last_partial else "ck_cg_iterate", aop[0], pop[0] if pop else None, B,
+                   N.ptr(st.x), N.ptr(st.r), N.ptr(st.p), N.ptr(st.Ap), N.ptr(st.z), ld, N.ptr(st.state),
+                   N.ptr(st.partials), N.ptr(work), wb, nst, s)
         else:
-            for _ in range(nst):
+            for i in range(nst):
                 apply_blocks(A, aop, st.p, st.Ap, n, B, work, wb)
                 N.call("ck_cg_step_xr", n, B, N.ptr(st.p), N.ptr(st.Ap), N.ptr(st.x), N.ptr(st.r), ld,
                        N.ptr(st.state), N.ptr(st.partials), s)
+                if last_partial and i + 1 == nst:
+                    break
                 if P is not None:
                     apply_blocks(P, pop, st.r, st.z, n, B, work, wb)
                 N.call("ck_cg_step_p", n, B, N.ptr(st.r), N.ptr(st.z), N.ptr(st.p), ld, N.ptr(st.state),

@@ -226,6 +226,14 @@ int ck_cg_step_p(int64_t n, int32_t nrhs, const float* r, const float* z, float*
 int ck_cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r, float* p, float* Ap, float* z,
                   int64_t ld, void* state, float* partials, void* work, size_t work_bytes, int32_t nsteps,
                   void* stream);
+/* The same nsteps steps, except that the last one stops after the x/r update: no
+ * preconditioner application, p update or step scalars (k, done and errors keep the
+ * previous step's values).  For callers that take x after exactly nsteps steps and
+ * discard the Krylov state, e.g. the correction solves of iterative refinement
+ * (no reference counterpart). */
+int ck_cg_iterate_final(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r, float* p, float* Ap,
+                        float* z, int64_t ld, void* state, float* partials, void* work, size_t work_bytes,
+                        int32_t nsteps, void* stream);
 /* x *= mult; r *= mult  (cg.py:119). */
 int ck_cg_finish(int64_t n, int32_t nrhs, float* x, float* r, int64_t ld, const void* state, void* stream);
 
