@@ -101,6 +101,8 @@ _PROTOS = {
     "ck_cg_step_p": (ctypes.c_int, [_i64, _i32, _p, _p, _p, _i64, _p, _p, _p]),
     "ck_cg_iterate": (ctypes.c_int, [_op, _op, _i32, _p, _p, _p, _p, _p, _i64, _p, _p, _p, _sz, _i32, _p]),
     "ck_cg_iterate_final": (ctypes.c_int, [_op, _op, _i32, _p, _p, _p, _p, _p, _i64, _p, _p, _p, _sz, _i32, _p]),
+    "ck_cg_iterate_final_f64": (ctypes.c_int, [_op, _op, _i32, _p, _p, _p, _p, _p, _i64, _p, _p, _p, _sz, _i32, _p,
+                                               _i64, _p]),
     "ck_cg_finish": (ctypes.c_int, [_i64, _i32, _p, _p, _i64, _p, _p]),
     "ck_lanczos_state_bytes": (_sz, [_i32, _i32]),
     "ck_lanczos_scratch_bytes": (_sz, [_i64, _i32, _i32]),

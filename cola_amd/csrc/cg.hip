@@ -248,6 +248,58 @@ __global__ __launch_bounds__(256) void update_xr_kernel(int64_t n, const float* 
   if (threadIdx.x == 0) part1[(int64_t)b * P + blk] = rr;
 }
 
+// The last step of a refinement correction with the FP64 update folded in:
+// x64 += (double)((x + alpha p) * mult), alpha as in update_xr_kernel (x + 0 when the loop
+// had already stopped).  The fp32 x and r are left as they were: nothing reads them after.
+template <bool V4>
+__global__ __launch_bounds__(256) void update_x64_kernel(int64_t n, const float* __restrict__ p,
+                                                         const float* __restrict__ x, int64_t ld,
+                                                         double* __restrict__ x64, int64_t ldx64,
+                                                         const float* __restrict__ part0, int np0, int64_t per,
+                                                         void* state, int nrhs) {
+  CgView v = view(state, nrhs);
+  const bool stopped = v.ctrl->done != 0;
+  __shared__ float alpha_s;
+  const int b = blockIdx.y, blk = blockIdx.x;
+  if (!stopped && threadIdx.x < 64) {
+    const double pap = column_total(part0, np0, b);
+    if (threadIdx.x == 0) {
+      double a = safe_div((double)v.gamma[b], pap);
+      if ((double)v.rnorm[b] < kSmall) a = 0.0;
+      alpha_s = (float)a;
+    }
+  }
+  __syncthreads();
+  const float alpha = stopped ? 0.f : alpha_s;
+  const float m = v.mult[b];
+  const int64_t i0 = blk * per;
+  const int64_t i1 = i0 + per < n ? i0 + per : n;
+  const float* xb = x + b * ld;
+  const float* pb = p + b * ld;
+  double* yb = x64 + b * ldx64;
+  if (V4) {
+    for (int64_t i = i0 + 4 * threadIdx.x; i < i1; i += 1024) {
+      float4 xv = *reinterpret_cast<const float4*>(xb + i);
+      if (!stopped) {
+        const float4 pv = *reinterpret_cast<const float4*>(pb + i);
+        xv.x = fmaf(alpha, pv.x, xv.x);
+        xv.y = fmaf(alpha, pv.y, xv.y);
+        xv.z = fmaf(alpha, pv.z, xv.z);
+        xv.w = fmaf(alpha, pv.w, xv.w);
+      }
+      yb[i] += (double)(xv.x * m);
+      yb[i + 1] += (double)(xv.y * m);
+      yb[i + 2] += (double)(xv.z * m);
+      yb[i + 3] += (double)(xv.w * m);
+    }
+  } else {
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+      const float xv = stopped ? xb[i] : fmaf(alpha, pb[i], xb[i]);
+      yb[i] += (double)(xv * m);
+    }
+  }
+}
+
 // End of step: rnorm, gamma1, beta, k, errors, loop condition.
 __global__ __launch_bounds__(256) void step_final_kernel(int nrhs, const float* __restrict__ part1,
                                                          const float* __restrict__ part2, int P, int precond,
@@ -460,7 +512,7 @@ namespace {
 // nsteps steps; with last_partial the final one stops after the x/r update
 int cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r, float* p, float* Ap, float* z,
                int64_t ld, void* state, float* partials, void* work, size_t work_bytes, int32_t nsteps,
-               bool last_partial, void* stream) {
+               bool last_partial, double* x64, int64_t ldx64, void* stream) {
   if (!A || nrhs < 1 || nrhs > 1024 || !x || !r || !p || !Ap || !state || !partials || nsteps < 0)
     return CK_ERR_BAD_ARG;
   const int64_t n = A->n;
@@ -478,21 +530,33 @@ int cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r,
                         kron64_eligible(P, r, ld, z, ld);
   for (int it = 0; it < nsteps; ++it) {
     int st;
+    const bool last = last_partial && it + 1 == nsteps;
+    int np0 = q.P;
     if (fuse_kron) {
       // p.Ap partials come out of the Kronecker epilogue
-      const int kp = kron_nparts(A);
-      st = kron_dispatch(A, nrhs, p, ld, Ap, ld, work, work_bytes, done, q.p0, kp, s);
+      np0 = kron_nparts(A);
+      st = kron_dispatch(A, nrhs, p, ld, Ap, ld, work, work_bytes, done, q.p0, np0, s);
       if (st) return st;
-      st = step_xr(n, nrhs, p, Ap, x, r, ld, state, q, q.p0, kp, s);
     } else {
       st = op_apply_internal(A, nrhs, p, ld, Ap, ld, work, work_bytes, done, s);
       if (st) return st;
       st = launch_partials(0, n, nrhs, p, Ap, ld, q.p0, q, done, s);
       if (st) return st;
-      st = step_xr(n, nrhs, p, Ap, x, r, ld, state, q, q.p0, q.P, s);
     }
+    if (last && x64) {
+      dim3 grid(q.P, nrhs);
+      if (use_v4(n, ld, p, x, nullptr, nullptr) && ldx64 % 4 == 0 && (reinterpret_cast<uintptr_t>(x64) & 31) == 0)
+        hipLaunchKernelGGL(update_x64_kernel<true>, grid, dim3(256), 0, s, n, p, x, ld, x64, ldx64, q.p0, np0, q.per,
+                           state, nrhs);
+      else
+        hipLaunchKernelGGL(update_x64_kernel<false>, grid, dim3(256), 0, s, n, p, x, ld, x64, ldx64, q.p0, np0, q.per,
+                           state, nrhs);
+      CK_CHECK_LAUNCH();
+      break;
+    }
+    st = step_xr(n, nrhs, p, Ap, x, r, ld, state, q, q.p0, np0, s);
     if (st) return st;
-    if (last_partial && it + 1 == nsteps) break;
+    if (last) break;
     if (fuse_eig) {
       st = kron64_eig_apply_f32(P, nrhs, r, ld, z, ld, work, work_bytes, done, s, q.p2);
       if (st) return st;
@@ -513,14 +577,24 @@ extern "C" {
 int ck_cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r, float* p, float* Ap, float* z,
                   int64_t ld, void* state, float* partials, void* work, size_t work_bytes, int32_t nsteps,
                   void* stream) {
-  return cg_iterate(A, P, nrhs, x, r, p, Ap, z, ld, state, partials, work, work_bytes, nsteps, false, stream);
+  return cg_iterate(A, P, nrhs, x, r, p, Ap, z, ld, state, partials, work, work_bytes, nsteps, false, nullptr, 0,
+                    stream);
 }
 
 int ck_cg_iterate_final(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r, float* p, float* Ap,
                         float* z, int64_t ld, void* state, float* partials, void* work, size_t work_bytes,
                         int32_t nsteps, void* stream) {
   if (nsteps < 1) return CK_ERR_BAD_ARG;
-  return cg_iterate(A, P, nrhs, x, r, p, Ap, z, ld, state, partials, work, work_bytes, nsteps, true, stream);
+  return cg_iterate(A, P, nrhs, x, r, p, Ap, z, ld, state, partials, work, work_bytes, nsteps, true, nullptr, 0,
+                    stream);
+}
+
+int ck_cg_iterate_final_f64(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r, float* p, float* Ap,
+                            float* z, int64_t ld, void* state, float* partials, void* work, size_t work_bytes,
+                            int32_t nsteps, double* x64, int64_t ldx64, void* stream) {
+  if (nsteps < 1 || !x64 || !A || ldx64 < A->n) return CK_ERR_BAD_ARG;
+  return cg_iterate(A, P, nrhs, x, r, p, Ap, z, ld, state, partials, work, work_bytes, nsteps, true, x64, ldx64,
+                    stream);
 }
 
 int ck_cg_finish(int64_t n, int32_t nrhs, float* x, float* r, int64_t ld, const void* state, void* stream) {

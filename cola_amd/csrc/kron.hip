@@ -104,7 +104,7 @@ __device__ __forceinline__ void tile_product4(const T* a, const T* __restrict__ 
 // barrier per slab); fp64 (34 KB per intermediate) keeps one and a second barrier.
 template <typename T>
 constexpr int slabs_per_block() {
-  return sizeof(T) == 4 ? 4 : 2;
+  return 4;
 }
 
 //

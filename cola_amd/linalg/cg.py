@@ -93,7 +93,7 @@ def run_batched_cg(A, b, x0=None, max_iters=1000, tol=1e-6, P=None, chunk=16, re
 
 
 def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, lowered=None, want_info=True,
-              read=True, prepared=False, finish=True, final=False):
+              read=True, prepared=False, finish=True, final=False, x64=None):
     """Batched CG on column blocks: solves A X = Bb[:, :n] into st.x (st.r the residual).
 
     ``st`` (a CGState) is reused across calls; ``lowered`` may carry precomputed
@@ -103,7 +103,9 @@ def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, l
     the normalised rhs, st.x zeros and the state its column norms (``Bb`` unused).
     ``finish=False`` leaves x and r normalised (the caller un-normalises).  ``final`` (with
     ``read=False``): the last enqueued step stops after the x/r update, for a caller that
-    takes only x after exactly that many steps (ck_cg_iterate_final)."""
+    takes only x after exactly that many steps (ck_cg_iterate_final); with ``x64`` as well
+    (an FP64 (B, ld) block) that step adds mult * x to x64 instead of updating x and r
+    (ck_cg_iterate_final_f64), and the returned info holds ``x64_updated``."""
     n, B, dev, ld = st.n, st.B, st.device, st.ld
     if lowered is None:
         aop = lower(A)
@@ -141,7 +143,13 @@ def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, l
     while not done and steps < max_iters:
         nst = min(int(chunk), max_iters - steps)
         last_partial = final and not read
-        if fused:
+        x64_updated = False
+        if fused and last_partial and x64 is not None:
+            N.call("ck_cg_iterate_final_f64", aop[0], pop[0] if pop else None, B, N.ptr(st.x), N.ptr(st.r),
+                   N.ptr(st.p), N.ptr(st.Ap), N.ptr(st.z), ld, N.ptr(st.state), N.ptr(st.partials), N.ptr(work), wb,
+                   nst, N.ptr(x64), x64.stride(0), s)
+            x64_updated = True
+        elif fused:
             N.call("ck_cg_iterate_final" if last_partial else "ck_cg_iterate", aop[0], pop[0] if pop else None, B,
                    N.ptr(st.x), N.ptr(st.r), N.ptr(st.p), N.ptr(st.Ap), N.ptr(st.z), ld, N.ptr(st.state),
                    N.ptr(st.partials), N.ptr(work), wb, nst, s)
@@ -158,9 +166,9 @@ def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, l
                        N.ptr(st.partials), s)
         steps += nst
         if not read:
-            if finish:
+            if finish and not x64_updated:
                 N.call("ck_cg_finish", n, B, N.ptr(st.x), N.ptr(st.r), ld, N.ptr(st.state), s)
-            return None, None, {}
+            return None, None, {"x64_updated": x64_updated}
         k, done = read_ints(st.state, 2, dev)
     if steps == 0:
         k, done = read_ints(st.state, 2, dev)

@@ -226,10 +226,11 @@ class MixedSolver:
                ld, N.ptr(self.mult), s)
         # a replayed correction runs exactly max_steps steps and only its x is used: its last step
         # skips the preconditioner and the next search direction
-        k, _, _ = cg_blocks(inner, None, self.st, None, max_steps, alg.inner_tol, alg.P, chunk,
-                            self.lowered_for[id(inner)], want_info=False, read=read, prepared=True, finish=False,
-                            final=not read)
-        N.call("ck_ir_update_scaled_f64", n, B, N.ptr(self.x64), ld, N.ptr(self.st.x), ld, N.ptr(self.mult), s)
+        k, _, info = cg_blocks(inner, None, self.st, None, max_steps, alg.inner_tol, alg.P, chunk,
+                               self.lowered_for[id(inner)], want_info=False, read=read, prepared=True, finish=False,
+                               final=not read, x64=self.x64)
+        if not info.get("x64_updated"):
+            N.call("ck_ir_update_scaled_f64", n, B, N.ptr(self.x64), ld, N.ptr(self.st.x), ld, N.ptr(self.mult), s)
         return k
 
     def _schedule(self, plan):

@@ -234,6 +234,13 @@ int ck_cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float*
 int ck_cg_iterate_final(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r, float* p, float* Ap,
                         float* z, int64_t ld, void* state, float* partials, void* work, size_t work_bytes,
                         int32_t nsteps, void* stream);
+/* ck_cg_iterate_final whose last step writes x nowhere but into an FP64 accumulator:
+ * x64 += (double)((x + alpha p) * mult) with mult the state's CK_CG_MULT (the column
+ * norms of the normalised start); x and r keep their previous values.  The refinement
+ * update of a correction solve folded into its last CG step (no reference counterpart). */
+int ck_cg_iterate_final_f64(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r, float* p, float* Ap,
+                            float* z, int64_t ld, void* state, float* partials, void* work, size_t work_bytes,
+                            int32_t nsteps, double* x64, int64_t ldx64, void* stream);
 /* x *= mult; r *= mult  (cg.py:119). */
 int ck_cg_finish(int64_t n, int32_t nrhs, float* x, float* r, int64_t ld, const void* state, void* stream);
 

@@ -136,10 +136,14 @@ def test_mixed_cg_graph_replays_match_eager(dev):
     bd = b.to(dev)
     eager = MixedSolver(A, 4, HipMixedCG(tol=1e-6, inner_tol=1e-4, P=P, graph=False))
     graphed = MixedSolver(A, 4, HipMixedCG(tol=1e-6, inner_tol=1e-4, P=P, graph=True))
-    for s in (eager, graphed):
-        s.solve(bd)  # records the schedule
+    x_first, _ = eager.solve(bd)  # records the schedule (host-checked loop, full CG steps)
+    x_first = x_first.clone()
+    graphed.solve(bd)
     x_ref, info_ref = eager.solve(bd)
     x_ref = x_ref.clone()
+    # the replay's last step adds mult * (x + alpha p) to x64 directly: the same arithmetic
+    # as the full step followed by the FP64 update, so the bits agree
+    assert torch.equal(x_ref, x_first)
     for rep in range(3):
         junk = torch.randn(1 << 20, device=dev).sum()  # eager work between replays
         torch.cuda.synchronize()
