@@ -381,7 +381,7 @@ __global__ __launch_bounds__(256) void kron64_pass_b(const T* __restrict__ K1, c
       const T x = X[xb + off];
       T y = scale * c[it];
       if (dvec) y *= dvec[off];
-      y += shift * x;
+      y = fma(shift, x, y);  // explicit: kron64_resid_b repeats this arithmetic bit for bit
       Y[yb + off] = y;
       dot += (float)(x * y);
     }
@@ -390,6 +390,63 @@ __global__ __launch_bounds__(256) void kron64_pass_b(const T* __restrict__ K1, c
     const float s = ck::block_sum<256>(dot, red);
     if (threadIdx.x == 0) partials[(int64_t)b * nparts + ct] = s;
   }
+}
+
+// FP64 refinement residual in pass B's epilogue: per element ax = scale * (K1 U) + shift * x
+// (the arithmetic of kron64_pass_b<double>), r = b - ax; writes r as fp32 and per-tile sums
+// of r^2 and b^2 (rpart[(b * 64 + tile) * 2 + {0, 1}]) instead of storing A x.
+__global__ __launch_bounds__(256) void kron64_resid_b(const double* __restrict__ K1, const double* __restrict__ U,
+                                                      int64_t ldu, const double* __restrict__ X, int64_t ldx,
+                                                      const double* __restrict__ Bv, int64_t ldb,
+                                                      float* __restrict__ R32, int64_t ld32, double scale,
+                                                      double shift, double* __restrict__ rpart) {
+  __shared__ __align__(16) double Ut[64 * TP];
+  __shared__ double red[4];
+  const int ct = blockIdx.x, b = blockIdx.y;
+  const int c0 = 64 * ct;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, li = lane & 15;
+  const double* Ub = U + (int64_t)b * ldu + c0;
+  for (int idx = threadIdx.x; idx < 64 * 32; idx += 256) {
+    const int k = idx / 32, cc = (idx % 32) * 2;
+    const double2 q = *reinterpret_cast<const double2*>(Ub + (int64_t)k * 4096 + cc);
+    Ut[cc * TP + k] = q.x;
+    Ut[(cc + 1) * TP + k] = q.y;
+  }
+  __syncthreads();
+  double a[16];
+  load16(K1 + (16 * w + li) * 64 + 16 * g, a);
+  double rr = 0.0, bb = 0.0;
+  const int64_t xb = (int64_t)b * ldx, bbase = (int64_t)b * ldb, rb = (int64_t)b * ld32;
+  MF<double>::acc ct4[4];
+  tile_product4<double>(a, Ut + li * TP + 16 * g, 16 * TP, ct4);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+#pragma unroll
+    for (int it = 0; it < 4; ++it) {
+      const int64_t off = (int64_t)(16 * w + MF<double>::row(g, it)) * 4096 + c0 + 16 * t + li;
+      double y = scale * ct4[t][it];
+      y = fma(shift, X[xb + off], y);
+      const double bv = Bv[bbase + off];
+      const double rv = bv - y;
+      R32[rb + off] = (float)rv;
+      rr += rv * rv;
+      bb += bv * bv;
+    }
+  }
+  rr = ck::block_sum<256>(rr, red);
+  bb = ck::block_sum<256>(bb, red);
+  if (threadIdx.x == 0) {
+    rpart[((int64_t)b * 64 + ct) * 2] = rr;
+    rpart[((int64_t)b * 64 + ct) * 2 + 1] = bb;
+  }
+}
+
+// norms[2c] = sqrt(sum of the 64 tile sums of r^2), norms[2c+1] likewise for b^2 (fixed order)
+__global__ __launch_bounds__(64) void kron64_resid_norms(const double* __restrict__ rpart, double* __restrict__ norms) {
+  const int col = blockIdx.x, which = blockIdx.y;
+  double v = rpart[((int64_t)col * 64 + threadIdx.x) * 2 + which];
+  v = ck::wave_sum(v);
+  if (threadIdx.x == 0) norms[2 * col + which] = sqrt(v);
 }
 
 template <typename T>
@@ -468,6 +525,32 @@ extern "C" int ck_kron_apply_last2_f32(const float* K2, const float* K3, int32_t
   hipLaunchKernelGGL(kron64_pass_a<float>, dim3((unsigned)(64 / slabs_per_block<float>() * nrhs)), dim3(256), 0,
                      (hipStream_t)stream, K2, K3, X, ldx, Y, ldy, (const int*)nullptr, (const float*)nullptr,
                      (int64_t)0, (float*)nullptr);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+extern "C" size_t ck_kron_residual_workspace_bytes(int32_t nrhs) {
+  return sizeof(double) * ((size_t)nrhs * 64 * 64 * 64 + (size_t)nrhs * 64 * 2);
+}
+
+extern "C" int ck_kron_residual_f64(const double* f0, const double* f1, const double* f2, double scale, double shift,
+                                    int32_t nrhs, const double* X, int64_t ldx, const double* Bv, int64_t ldb,
+                                    float* R32, int64_t ld32, double* norms, void* work, size_t work_bytes,
+                                    void* stream) {
+  const int64_t n = 64 * 64 * 64;
+  if (!f0 || !f1 || !f2 || nrhs < 1 || !X || !Bv || !R32 || !norms || ldx < n || ldb < n || ld32 < n)
+    return CK_ERR_BAD_ARG;
+  if (!aligned16(f0) || !aligned16(f1) || !aligned16(f2) || !aligned16(X) || !aligned16(Bv) || (ldx % 2) || (ldb % 2))
+    return CK_ERR_UNSUPPORTED;
+  if (!work || work_bytes < ck_kron_residual_workspace_bytes(nrhs)) return CK_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  double* U = static_cast<double*>(work);
+  double* rpart = U + (size_t)nrhs * n;
+  hipLaunchKernelGGL(kron64_pass_a<double>, dim3((unsigned)(64 / slabs_per_block<double>() * nrhs)), dim3(256), 0, s,
+                     f1, f2, X, ldx, U, n, (const int*)nullptr, (const double*)nullptr, (int64_t)0, (float*)nullptr);
+  hipLaunchKernelGGL(kron64_resid_b, dim3(64, nrhs), dim3(256), 0, s, f0, (const double*)U, n, X, ldx, Bv, ldb, R32,
+                     ld32, scale, shift, rpart);
+  hipLaunchKernelGGL(kron64_resid_norms, dim3(nrhs, 2), dim3(64), 0, s, (const double*)rpart, norms);
   CK_CHECK_LAUNCH();
   return CK_OK;
 }

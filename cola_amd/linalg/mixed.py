@@ -109,8 +109,8 @@ class _F64Apply:
             self.f64 = [f.double().contiguous() for f in ks[0]]
             self.scale, self.shift = ks[1], ks[2]
             self.dims = torch.tensor([64, 64, 64], dtype=torch.int32)
-            n = 64**3
-            self.work = torch.empty(nrhs * n, dtype=torch.float64, device=A.device)
+            wbytes = max(N.q("ck_kron_residual_workspace_bytes", nrhs), nrhs * 64**3 * 8)
+            self.work = torch.empty(wbytes // 8, dtype=torch.float64, device=A.device)
         elif got is not None and got[0].kind == N.CK_OP_DENSE_F32:
             self.kind = "dense"
             self.op, self.keep = got
@@ -118,6 +118,16 @@ class _F64Apply:
             raise N.UnsupportedOperator("mixed-precision refinement", N.CK_ERR_UNSUPPORTED,
                                         "FP64 residuals need a dense fp32 matrix or a 64^3 Kronecker operator "
                                         "(plus shift)")
+
+    def residual(self, X, Bv, R32, norms, nrhs):
+        """R32 = float(Bv - A X), norms[2c] = ||Bv_c - A X_c||, norms[2c+1] = ||Bv_c|| in one
+        FP64 pass pair (64^3 Kronecker operators); returns False for other operators."""
+        if self.kind != "kron64":
+            return False
+        N.call("ck_kron_residual_f64", N.ptr(self.f64[0]), N.ptr(self.f64[1]), N.ptr(self.f64[2]), self.scale,
+               self.shift, nrhs, N.ptr(X), X.stride(0), N.ptr(Bv), Bv.stride(0), N.ptr(R32), R32.stride(0),
+               N.ptr(norms), N.ptr(self.work), self.work.numel() * 8, N.stream_ptr(X.device))
+        return True
 
     def __call__(self, X, Y, nrhs):
         ld = X.stride(0)
@@ -207,10 +217,12 @@ class MixedSolver:
     def residual(self, with_ax=True, read=True):
         n, B, ld = self.n, self.B, self.ld
         s = N.stream_ptr(self.b64.device)
-        if with_ax:
-            self.f64(self.x64, self.Ax64, B)
-        N.call("ck_ir_residual_f64", n, B, N.ptr(self.b64), ld, N.ptr(self.Ax64) if with_ax else None, ld,
-               None, ld, N.ptr(self.r32), ld, N.ptr(self.norms), N.ptr(self.irwork), self.irwork.numel() * 8, s)
+        # 64^3 Kronecker operators: one fused FP64 pass pair; otherwise A x, then the residual pass
+        if not (with_ax and self.f64.residual(self.x64, self.b64, self.r32, self.norms, B)):
+            if with_ax:
+                self.f64(self.x64, self.Ax64, B)
+            N.call("ck_ir_residual_f64", n, B, N.ptr(self.b64), ld, N.ptr(self.Ax64) if with_ax else None, ld,
+                   None, ld, N.ptr(self.r32), ld, N.ptr(self.norms), N.ptr(self.irwork), self.irwork.numel() * 8, s)
         if not read:
             return None
         nr = self.norms.view(B, 2).cpu()

@@ -121,6 +121,14 @@ int ck_op_apply(const ck_op* op, int32_t nrhs, const float* X, int64_t ldx, floa
 int ck_kron_apply_f64(int32_t nfactors, const int32_t* fdim, const double* f0, const double* f1, const double* f2,
                       double scale, double shift, int32_t nrhs, const double* X, int64_t ldx, double* Y, int64_t ldy,
                       void* work, size_t work_bytes, void* stream);
+/* FP64 refinement residual of the 64^3 Kronecker operator in two passes: r = b - (scale
+ * (f0 x f1 x f2) + shift I) x, written as fp32 R32, with norms[2c] = ||r_c||, norms[2c+1] =
+ * ||b_c|| (fp64) -- ck_kron_apply_f64 followed by ck_ir_residual_f64 without storing A x.
+ * work >= ck_kron_residual_workspace_bytes(nrhs). */
+size_t ck_kron_residual_workspace_bytes(int32_t nrhs);
+int ck_kron_residual_f64(const double* f0, const double* f1, const double* f2, double scale, double shift,
+                         int32_t nrhs, const double* X, int64_t ldx, const double* Bv, int64_t ldb, float* R32,
+                         int64_t ld32, double* norms, void* work, size_t work_bytes, void* stream);
 
 /* ---------------------------------------------------------------- mixed precision
  * Iterative refinement with FP64 residuals around fp32 inner solves (no

@@ -77,6 +77,40 @@ def test_ir_residual_kernel(dev):
     assert torch.allclose(norms.view(B, 2)[:, 1], b[:, :n].norm(dim=1))
 
 
+def test_kron_residual_matches_apply_then_residual(dev):
+    """The fused FP64 residual (A x never stored) against ck_kron_apply_f64 + ck_ir_residual_f64
+    and against the oracle's FP64 Kronecker product."""
+    g = torch.Generator().manual_seed(4)
+    facs = [torch.randn(64, 64, generator=g, dtype=torch.float64) for _ in range(3)]
+    B, n = 3, 64**3
+    X = torch.randn(n, B, generator=g, dtype=torch.float64)
+    Bv = oop.kron_matmat(facs, X) * 0.7 + 0.25 * X + 1e-3 * torch.randn(n, B, generator=g, dtype=torch.float64)
+    f = [t.contiguous().to(dev) for t in facs]
+    Xb, Bb = X.T.contiguous().to(dev), Bv.T.contiguous().to(dev)
+    s = N.stream_ptr(dev)
+    wb = N.q("ck_kron_residual_workspace_bytes", B)
+    work = torch.empty(wb // 8, dtype=torch.float64, device=dev)
+    r32 = torch.empty(B, n, device=dev)
+    norms = torch.empty(2 * B, dtype=torch.float64, device=dev)
+    N.call("ck_kron_residual_f64", N.ptr(f[0]), N.ptr(f[1]), N.ptr(f[2]), 0.7, 0.25, B, N.ptr(Xb), n, N.ptr(Bb), n,
+           N.ptr(r32), n, N.ptr(norms), N.ptr(work), wb, s)
+    # two-pass path
+    Ax = torch.empty_like(Xb)
+    dims = (ctypes.c_int32 * 3)(64, 64, 64)
+    N.call("ck_kron_apply_f64", 3, dims, N.ptr(f[0]), N.ptr(f[1]), N.ptr(f[2]), 0.7, 0.25, B, N.ptr(Xb), n, N.ptr(Ax), n,
+           N.ptr(work), wb, s)
+    r32b = torch.empty_like(r32)
+    norms2 = torch.empty_like(norms)
+    w2 = torch.empty(N.q("ck_ir_workspace_bytes", B) // 8, dtype=torch.float64, device=dev)
+    N.call("ck_ir_residual_f64", n, B, N.ptr(Bb), n, N.ptr(Ax), n, None, n, N.ptr(r32b), n, N.ptr(norms2), N.ptr(w2),
+           w2.numel() * 8, s)
+    assert torch.equal(r32, r32b)
+    assert torch.allclose(norms, norms2, rtol=1e-12, atol=0)
+    ref = Bv - (0.7 * oop.kron_matmat(facs, X) + 0.25 * X)
+    assert torch.allclose(norms.view(B, 2)[:, 0].cpu(), ref.norm(dim=0), rtol=1e-9)
+    assert torch.allclose(norms.view(B, 2)[:, 1].cpu(), Bv.norm(dim=0), rtol=1e-12)
+
+
 def test_ir_inner_start_and_scaled_update(dev):
     """The inner solve's normalisation folded into the refinement: r = r32 / ||r|| (fp32
     division, a zero column stays zero), x = 0, mult = float(||r||); then x64 += d * mult
