@@ -84,6 +84,7 @@ _PROTOS = {
     "ck_kron_residual_f64": (ctypes.c_int, [_p, _p, _p, ctypes.c_double, ctypes.c_double, _i32, _p, _i64, _p, _i64, _p,
                                             _i64, _p, _p, _sz, _p]),
     "ck_ir_update_f64": (ctypes.c_int, [_i64, _i32, _p, _i64, _p, _i64, _p]),
+    "ck_ir_load_rhs": (ctypes.c_int, [_i64, _i32, _p, _i64, _i64, _p, _p, _p, _i64, _p, _p, _sz, _p]),
     "ck_ir_inner_start": (ctypes.c_int, [_i64, _i32, _p, _i64, _p, _p, _p, _i64, _p, _p]),
     "ck_ir_update_scaled_f64": (ctypes.c_int, [_i64, _i32, _p, _i64, _p, _i64, _p, _p]),
     "ck_f32_to_bf16": (ctypes.c_int, [_i64, _p, _p, _p]),

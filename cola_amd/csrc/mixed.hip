@@ -56,6 +56,51 @@ __global__ __launch_bounds__(256) void ir_update_kernel(int64_t n, double* __res
   x[col * ldx + i] += (double)d[col * ldd + i];
 }
 
+// Start of a refinement solve from the caller's right-hand sides b (element (i, c) at
+// b[i * rs + c * cs], fp32): b64 = b, r32 = b, x64 = 0 in the solver's column blocks, and
+// the ||b||^2 partials of ir_residual_kernel for both norms (r = b at x = 0).  One pass
+// over b instead of a transposing copy, a fill and a residual pass.  Up to kLoadCols
+// columns per pass; a thread owns one row.
+constexpr int kLoadCols = 16;
+
+__global__ __launch_bounds__(256) void ir_load_rhs_kernel(int64_t n, int nrhs, const float* __restrict__ b,
+                                                          int64_t rs, int64_t cs, double* __restrict__ b64,
+                                                          float* __restrict__ r32, double* __restrict__ x64,
+                                                          int64_t ld, int c0, double* __restrict__ part) {
+  __shared__ double red[4];
+  const int blk = blockIdx.x;
+  const int64_t per = cdiv(n, kIrParts);
+  const int64_t i0 = blk * per, i1 = i0 + per < n ? i0 + per : n;
+  const int nc = nrhs - c0 < kLoadCols ? nrhs - c0 : kLoadCols;
+  double acc[kLoadCols];
+#pragma unroll
+  for (int j = 0; j < kLoadCols; ++j) acc[j] = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+#pragma unroll
+    for (int j = 0; j < kLoadCols; ++j) {
+      if (j < nc) {
+        const int c = c0 + j;
+        const float v = b[i * rs + (int64_t)c * cs];
+        const double dv = (double)v;
+        b64[c * ld + i] = dv;
+        r32[c * ld + i] = v;
+        x64[c * ld + i] = 0.0;
+        acc[j] += dv * dv;
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < kLoadCols; ++j) {
+    if (j < nc) {
+      const double t = block_sum<256>(acc[j], red);
+      if (threadIdx.x == 0) {
+        part[(2 * (c0 + j)) * kIrParts + blk] = t;
+        part[(2 * (c0 + j) + 1) * kIrParts + blk] = t;
+      }
+    }
+  }
+}
+
 // Start of an inner correction solve on the refinement residual: r = r32 / safe(||r||),
 // x = 0, mult = ||r|| (fp32) -- the column normalisation of the inner CG done with the
 // FP64 norm the residual pass already produced.
@@ -150,6 +195,20 @@ int ck_ir_update_f64(int64_t n, int32_t nrhs, double* x, int64_t ldx, const floa
   if (n < 1 || nrhs < 1 || !x || !d || ldx < n || ldd < n) return CK_ERR_BAD_ARG;
   hipLaunchKernelGGL(ir_update_kernel, dim3((unsigned)cdiv(n, 256), nrhs), dim3(256), 0, (hipStream_t)stream, n, x,
                      ldx, d, ldd);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+int ck_ir_load_rhs(int64_t n, int32_t nrhs, const float* b, int64_t rs, int64_t cs, double* b64, float* r32,
+                   double* x64, int64_t ld, double* norms, void* work, size_t work_bytes, void* stream) {
+  if (n < 1 || nrhs < 1 || !b || !b64 || !r32 || !x64 || !norms || ld < n) return CK_ERR_BAD_ARG;
+  if (!work || work_bytes < ck_ir_workspace_bytes(nrhs)) return CK_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  double* part = static_cast<double*>(work);
+  for (int c0 = 0; c0 < nrhs; c0 += kLoadCols)
+    hipLaunchKernelGGL(ir_load_rhs_kernel, dim3(kIrParts), dim3(256), 0, s, n, nrhs, b, rs, cs, b64, r32, x64, ld, c0,
+                       part);
+  hipLaunchKernelGGL(ir_norms_kernel, dim3(nrhs, 2), dim3(64), 0, s, (const double*)part, norms);
   CK_CHECK_LAUNCH();
   return CK_OK;
 }

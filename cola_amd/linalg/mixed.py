@@ -246,10 +246,8 @@ class MixedSolver:
         return k
 
     def _schedule(self, plan):
-        """x = 0, then one correction solve of k steps and one FP64 residual per plan entry:
-        launches only (no host reads), so it can be captured as a graph."""
-        self.x64.zero_()
-        self.residual(with_ax=False, read=False)
+        """One correction solve of k steps and one FP64 residual per plan entry (from x = 0, set
+        by the rhs load): launches only (no host reads), so it can be captured as a graph."""
         for k_steps in plan:
             self.correction(self.A, k_steps, k_steps, read=False)
             self.residual(with_ax=True, read=False)
@@ -258,15 +256,16 @@ class MixedSolver:
         alg = self.alg
         n, B, ld = self.n, self.B, self.ld
         t0 = time.time()
-        # (the residual and update passes touch columns [:n] only: b64's padding is never read)
-        self.b64[:, :n].copy_(b.T)
+        # b64 = b, r = b, x = 0 and the norms of b in one pass over the caller's b (any strides; the
+        # residual and update passes touch columns [:n] only, so the blocks' padding is never read).
+        # Relative residual 1 for every column (0 for a zero column, which the first correction
+        # solve leaves at x = 0): no host read is needed before the first solve.
+        bf = b if b.dtype == torch.float32 else b.float()
+        N.call("ck_ir_load_rhs", n, B, N.ptr(bf), bf.stride(0), bf.stride(1), N.ptr(self.b64), N.ptr(self.r32),
+               N.ptr(self.x64), ld, N.ptr(self.norms), N.ptr(self.irwork), self.irwork.numel() * 8,
+               N.stream_ptr(b.device))
         plan = getattr(self, "_plan", None)
         replay = bool(plan) and self.inner is self.A and len(plan) <= alg.max_refinements
-        if not replay:
-            self.x64.zero_()
-            # x = 0: r = b, relative residual 1 for every column (0 for a zero column, which the
-            # first correction solve leaves at x = 0); no host read is needed before the first solve
-            self.residual(with_ax=False, read=False)
         rel = np.ones(B)
         history = [1.0]
         inner_steps = []

@@ -141,6 +141,11 @@ int ck_ir_residual_f64(int64_t n, int32_t nrhs, const double* b, int64_t ldb, co
                        double* r, int64_t ldr, float* r32, int64_t ldr32, double* norms, void* work,
                        size_t work_bytes, void* stream);
 int ck_ir_update_f64(int64_t n, int32_t nrhs, double* x, int64_t ldx, const float* d, int64_t ldd, void* stream);
+/* Start of a refinement solve: b64 = b, r32 = b, x64 = 0 (column blocks with leading
+ * dimension ld) from b's element (i, c) at b[i*rs + c*cs], and norms as ck_ir_residual_f64
+ * gives them for x = 0 (both ||b_c||).  work >= ck_ir_workspace_bytes(nrhs). */
+int ck_ir_load_rhs(int64_t n, int32_t nrhs, const float* b, int64_t rs, int64_t cs, double* b64, float* r32,
+                   double* x64, int64_t ld, double* norms, void* work, size_t work_bytes, void* stream);
 /* Inner-solve normalisation folded into the refinement (replaces ck_cg_normalize +
  * zeroing x + copying r, and ck_cg_finish + ck_ir_update_f64, inside a refinement step):
  * ck_ir_inner_start: r = r32 / max(||r||, 1e-40), x = 0, mult[col] = (float)||r|| with

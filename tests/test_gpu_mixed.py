@@ -111,6 +111,29 @@ def test_kron_residual_matches_apply_then_residual(dev):
     assert torch.allclose(norms.view(B, 2)[:, 1].cpu(), Bv.norm(dim=0), rtol=1e-12)
 
 
+@pytest.mark.parametrize("B,transposed", [(3, False), (20, False), (5, True)])
+def test_ir_load_rhs(dev, B, transposed):
+    """b64 = b, r32 = b, x64 = 0 and both norms = ||b|| from an (n, B) view of any strides;
+    more than 16 columns take two passes."""
+    n, ld = 1000, 1024
+    g = torch.Generator().manual_seed(B)
+    base = torch.randn(B, n, generator=g) if transposed else torch.randn(n, B, generator=g)
+    b = (base.T if transposed else base).to(dev)
+    b64 = torch.full((B, ld), 3.0, dtype=torch.float64, device=dev)
+    x64 = torch.full((B, ld), 3.0, dtype=torch.float64, device=dev)
+    r32 = torch.full((B, ld), 3.0, device=dev)
+    norms = torch.empty(2 * B, dtype=torch.float64, device=dev)
+    w = torch.empty(N.q("ck_ir_workspace_bytes", B) // 8, dtype=torch.float64, device=dev)
+    N.call("ck_ir_load_rhs", n, B, N.ptr(b), b.stride(0), b.stride(1), N.ptr(b64), N.ptr(r32), N.ptr(x64), ld,
+           N.ptr(norms), N.ptr(w), w.numel() * 8, N.stream_ptr(dev))
+    bt = b.T.cpu()
+    assert torch.equal(b64[:, :n].cpu(), bt.double()) and torch.equal(r32[:, :n].cpu(), bt)
+    assert torch.all(x64[:, :n] == 0) and torch.all(x64[:, n:] == 3.0)
+    nb = bt.double().norm(dim=1)
+    assert torch.allclose(norms.view(B, 2)[:, 0].cpu(), nb, rtol=1e-13)
+    assert torch.allclose(norms.view(B, 2)[:, 1].cpu(), nb, rtol=1e-13)
+
+
 def test_ir_inner_start_and_scaled_update(dev):
     """The inner solve's normalisation folded into the refinement: r = r32 / ||r|| (fp32
     division, a zero column stays zero), x = 0, mult = float(||r||); then x64 += d * mult
