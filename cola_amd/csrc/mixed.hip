@@ -63,6 +63,7 @@ __global__ __launch_bounds__(256) void ir_update_kernel(int64_t n, double* __res
 // columns per pass; a thread owns one row.
 constexpr int kLoadCols = 16;
 
+template <bool V4>
 __global__ __launch_bounds__(256) void ir_load_rhs_kernel(int64_t n, int nrhs, const float* __restrict__ b,
                                                           int64_t rs, int64_t cs, double* __restrict__ b64,
                                                           float* __restrict__ r32, double* __restrict__ x64,
@@ -76,11 +77,24 @@ __global__ __launch_bounds__(256) void ir_load_rhs_kernel(int64_t n, int nrhs, c
 #pragma unroll
   for (int j = 0; j < kLoadCols; ++j) acc[j] = 0.0;
   for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+    // V4: a row's columns are contiguous and 16-byte aligned (cs == 1, rs and c0 multiples of 4)
+    float row[kLoadCols];
+    if (V4) {
+#pragma unroll
+      for (int j = 0; j < kLoadCols; j += 4)
+        if (j < nc) {
+          const float4 q = *reinterpret_cast<const float4*>(b + i * rs + c0 + j);
+          row[j] = q.x;
+          row[j + 1] = q.y;
+          row[j + 2] = q.z;
+          row[j + 3] = q.w;
+        }
+    }
 #pragma unroll
     for (int j = 0; j < kLoadCols; ++j) {
       if (j < nc) {
         const int c = c0 + j;
-        const float v = b[i * rs + (int64_t)c * cs];
+        const float v = V4 ? row[j] : b[i * rs + (int64_t)c * cs];
         const double dv = (double)v;
         b64[c * ld + i] = dv;
         r32[c * ld + i] = v;
@@ -205,9 +219,15 @@ int ck_ir_load_rhs(int64_t n, int32_t nrhs, const float* b, int64_t rs, int64_t 
   if (!work || work_bytes < ck_ir_workspace_bytes(nrhs)) return CK_ERR_WORKSPACE;
   hipStream_t s = (hipStream_t)stream;
   double* part = static_cast<double*>(work);
-  for (int c0 = 0; c0 < nrhs; c0 += kLoadCols)
-    hipLaunchKernelGGL(ir_load_rhs_kernel, dim3(kIrParts), dim3(256), 0, s, n, nrhs, b, rs, cs, b64, r32, x64, ld, c0,
-                       part);
+  const bool v4 = cs == 1 && rs % 4 == 0 && nrhs % 4 == 0 && (reinterpret_cast<uintptr_t>(b) & 15) == 0;
+  for (int c0 = 0; c0 < nrhs; c0 += kLoadCols) {
+    if (v4)
+      hipLaunchKernelGGL(ir_load_rhs_kernel<true>, dim3(kIrParts), dim3(256), 0, s, n, nrhs, b, rs, cs, b64, r32, x64,
+                         ld, c0, part);
+    else
+      hipLaunchKernelGGL(ir_load_rhs_kernel<false>, dim3(kIrParts), dim3(256), 0, s, n, nrhs, b, rs, cs, b64, r32,
+                         x64, ld, c0, part);
+  }
   hipLaunchKernelGGL(ir_norms_kernel, dim3(nrhs, 2), dim3(64), 0, s, (const double*)part, norms);
   CK_CHECK_LAUNCH();
   return CK_OK;
