@@ -32,7 +32,7 @@ METRIC = "CG solves/sec to 1e-6 rel. residual: learned+mixed-prec vs plain; HBM 
 HBM_PEAK = 8000.0  # GB/s
 # HBM bytes per kron64_pass_a<float> launch at 64^3 x 16 rhs, from rocprofv3 --pmc FETCH_SIZE (x2.0) plus
 # WRITE_SIZE (x1.0), the factors a 1 GiB stream copy reads on the same counters (profiles/kron64_cg_64cubed_hbm_traffic.md)
-PASS_A_TRAFFIC_BYTES = (16682.4 + 16246.0) * 1024
+PASS_A_TRAFFIC_BYTES = (16651.2 + 16222.7) * 1024
 FP32_MFMA_PEAK = 157.3  # TFLOP/s
 
 
