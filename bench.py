@@ -619,10 +619,16 @@ def bench_batched_cg(args, dev):
         return solver, x, {"s": dt, "solved": int((rel <= tol).sum()), "solves_per_s": int((rel <= tol).sum()) / dt,
                            "refinements": info["refinements"],
                            "inner_iterations_total": int(info["inner_iterations"].sum()),
-                           "bf16_systems_final": info["bf16_systems_final"], "max_rel_residual": float(rel.max())}
+                           "bf16_systems_final": info["bf16_systems_final"], "max_rel_residual": float(rel.max()),
+                           "history_max_rel": info["history"],
+                           "per_refinement_active_steps_bf16": info["per_refinement"]}
 
     variants = {}
     solver, x, variants["best"] = run(best, "best")
+    # a-priori bf16 admissibility: 2 ||K_bf16 - K||_2 (power-iteration estimate) against 5% of the shift
+    ratio = 2.0 * solver.bf16_norm_estimate / sy.shift.cpu().numpy().astype(np.float64)
+    screen = {"admitted": int(solver.admissible.sum()), "threshold": 0.05,
+              "ratio_2norm_over_shift_min_median_max": [float(ratio.min()), float(np.median(ratio)), float(ratio.max())]}
     for name, P in (("pivoted_cholesky", pcc), ("learned", pcl), ("none", None)):
         variants[name] = run(P, name)[2]
     head = variants["best"]
@@ -667,7 +673,7 @@ def bench_batched_cg(args, dev):
                        "pivchol_build_s": t_pc, "learned": {"train_instances": args.batched_train, "train_s": t_train,
                                                             "build_s": t_pl, "last_loss": float(losses[-1]),
                                                             "systems_choosing_learned": int(best.pick.sum())},
-                       "variants": variants, "spot_check_fp64": spot, "plain_fp32": plain_res,
+                       "variants": variants, "bf16_screen": screen, "spot_check_fp64": spot, "plain_fp32": plain_res,
                        "speedup_vs_plain_time": dtp / head["s"]},
             "roofline": roof}
 

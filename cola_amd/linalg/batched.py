@@ -289,6 +289,57 @@ class BatchedMixedSolver:
         self.pbb = torch.zeros_like(self.prr)
         self.rel = torch.zeros(systems.nsys, dtype=torch.float64, device=dev)
         self.active = torch.ones(systems.nsys, dtype=torch.int32, device=dev)
+        self.admissible = None
+
+    def screen_bf16(self, iters=8, margin=0.05, seed=0):
+        """A-priori bf16 admissibility per system.  Refinement with corrections solved on the
+        bf16 copy converges at rate ||A^-1 E|| <= ||E|| / lambda_min(A), E = K_bf16 - K, and
+        lambda_min(A) >= shift for PSD K.  ||E||_2 is estimated by `iters` power iterations on E
+        (two batched matvecs each, the shift cancels); a system keeps bf16 storage only if twice
+        the estimate (power iteration approaches ||E|| from below) is at most margin * shift.
+        Returns the per-system estimates; the mask is applied to later solves."""
+        sy, cg = self.systems, self.cg
+        S = sy.nsys
+        if sy.A16 is None or not self.bf16.any():
+            self.admissible = np.zeros(S, dtype=bool)
+            return np.zeros(S)
+        dev = sy.device
+        seg = getattr(self, "_seg", None)
+        if seg is None:
+            ids = np.full(sy.total_v, -1, dtype=np.int64)
+            for k in range(S):
+                o, n = int(sy.voff[k]), int(sy.sizes[k])
+                ids[o:o + n] = k
+            valid = torch.from_numpy(ids >= 0).to(dev)
+            seg = self._seg = (torch.from_numpy(np.maximum(ids, 0)).to(dev), valid)
+        ids_t, valid = seg
+        g = torch.Generator(device=dev).manual_seed(seed)
+        est = torch.zeros(S, dtype=torch.float64, device=dev)
+        self.stream_sync()
+        with torch.cuda.stream(cg.stream):
+            v = torch.randn(sy.total_v, generator=g, device=dev) * valid
+
+            def seg_norm(x):
+                return torch.zeros(S, dtype=torch.float64, device=dev).index_add_(0, ids_t, x.double()**2).sqrt()
+
+            v = v / seg_norm(v).clamp_min(1e-300).float()[ids_t]
+            cg.done.zero_()
+            for _ in range(iters):
+                cg.p.copy_(v)
+                cg.set_precision(np.ones(S, dtype=bool))
+                cg.matvec()
+                y16 = cg.Ap.clone()
+                cg.set_precision(np.zeros(S, dtype=bool))
+                cg.matvec()
+                ev = (y16 - cg.Ap) * valid
+                est = seg_norm(ev)  # ||E v|| with ||v|| = 1
+                v = ev / est.clamp_min(1e-300).float()[ids_t]
+            est_h = est.cpu().numpy()
+        torch.cuda.current_stream(dev).wait_stream(cg.stream)
+        shifts = sy.shift.cpu().numpy().astype(np.float64)
+        self.admissible = 2.0 * est_h <= margin * shifts
+        self.bf16_norm_estimate = est_h
+        return est_h
 
     def residual(self, active):
         cg = self.cg
@@ -308,7 +359,10 @@ class BatchedMixedSolver:
         self.stream_sync()
         self.b64.copy_(b.to(torch.float64))
         self.x64.zero_()
-        prec = self.bf16.copy()
+        if self.bf16.any() and self.admissible is None:
+            self.screen_bf16()
+        prec = self.bf16 & self.admissible if self.admissible is not None else self.bf16.copy()
+        bf16_start = int(prec.sum())
         cg.set_precision(prec)
         everyone = np.ones(S, dtype=bool)
         rel = self.residual(everyone)
@@ -317,6 +371,7 @@ class BatchedMixedSolver:
         fallbacks = 0
         it = 0
         history = [float(rel.max())]
+        per_refinement = []  # (active systems, inner steps summed over them, bf16 systems among them)
         while it < max_refinements:
             active = ~(rel_all <= tol)
             if not active.any():
@@ -326,6 +381,7 @@ class BatchedMixedSolver:
             _, ks = cg.solve(self.r32, tol_rel=inner_tol, tol_abs=0.0, active=active, chunk=chunk,
                              use_graph=use_graph)
             inner_total[active] += ks[active]
+            per_refinement.append((int(active.sum()), int(ks[active].sum()), int((active & prec).sum())))
             with torch.cuda.stream(cg.stream):
                 keep = self.x64.clone()
                 self.active.copy_(torch.from_numpy(active.astype(np.int32)))
@@ -361,6 +417,7 @@ class BatchedMixedSolver:
         torch.cuda.synchronize()
         info = {"refinements": it, "rel_residual_f64": rel_all, "inner_iterations": inner_total,
                 "bf16_systems_final": int(prec.sum()), "precision_fallbacks": fallbacks, "history": history,
+                "per_refinement": per_refinement, "bf16_systems_start": bf16_start,
                 "converged": bool(np.all(rel_all <= tol)), "time": time.time() - t0}
         return self.x64, info
 

@@ -157,3 +157,26 @@ def test_batched_pcg_matches_single_system_pcg(dev):
     for rep in range(2):
         x2, k2 = bp.solve(b, tol_rel=2e-5, use_graph=True)
         assert np.array_equal(k1, k2) and torch.equal(x1, x2), rep
+
+
+def test_bf16_screen_estimates_rounding_norm(dev):
+    """The power-iteration estimate of ||K_bf16 - K||_2 lies within a factor 2 below the dense
+    value, and a system keeps bf16 storage only when twice the estimate is under 5% of its
+    shift; the screened solve still reaches the tolerance."""
+    sizes = [300, 700, 1200]
+    pts, ells = rbf_systems(sizes, 41, ell_range=(0.1, 0.3))
+    shifts = [1.0, 1e-3, 1e-2]
+    sy = BatchedSystems.rbf(pts, ells, shifts, dev, bf16=True)
+    solver = BatchedMixedSolver(sy, rows_per_tile=16, inner_max_iters=1000, bf16=True)
+    est = solver.screen_bf16(iters=12)
+    for s, n in enumerate(sizes):
+        E = (sy.matrix(s, torch.bfloat16).float() - sy.matrix(s)).double().cpu()
+        true = float(torch.linalg.eigvalsh(E).abs().max())
+        print(f"n={n}: estimate {est[s]:.3e} dense {true:.3e} shift {shifts[s]} admissible {solver.admissible[s]}",
+              flush=True)
+        assert 0.5 * true <= est[s] <= true * (1 + 1e-6)
+        assert solver.admissible[s] == (2 * est[s] <= 0.05 * float(np.float32(shifts[s])))
+    b = sy.concat([torch.randn(n, generator=torch.Generator().manual_seed(60 + i)).to(dev)
+                   for i, n in enumerate(sizes)])
+    x, info = solver.solve(b, tol=1e-8, inner_tol=1e-4)
+    assert info["converged"] and info["bf16_systems_start"] == int(solver.admissible.sum())
