@@ -86,89 +86,71 @@ __global__ __launch_bounds__(256) void hutch_csr_bits_kernel(int n, const int32_
 }
 
 constexpr int kBitsShort = 8;
+constexpr int kTraceRowsPerThread = 4;  // rows b*1024 + t + 256 j, j < 4: four rows' loads in flight
 
-__global__ __launch_bounds__(256) void trace_bits_partials(int n, const int32_t* __restrict__ rowptr,
-                                                           const int32_t* __restrict__ colind,
-                                                           const float* __restrict__ val, int ngroups,
-                                                           const uint64_t* __restrict__ W, double* __restrict__ part) {
-  __shared__ double red[4];
-  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+// Probe words of the trace pass: stored (W from ck_rng_rademacher_bits) or recomputed from
+// the seed.  Both give stream_word(stream_key(seed, g), i) for word (g, i).
+struct StoredWords {
+  const uint64_t* W;
+  int64_t n;
+  __device__ uint64_t operator()(int g, int64_t i) const { return W[(int64_t)g * n + i]; }
+};
+struct SeededWords {
+  uint64_t seed;
+  __device__ uint64_t operator()(int g, int64_t i) const {
+    return stream_word(stream_key(seed, (uint64_t)g), (uint64_t)i);
+  }
+};
+
+// sum_k val_k * sum_g (64 - 2 popcount(w_g(row) ^ w_g(col_k))) for one row, k ascending.
+// Short rows with one word group load their column indices and values in one batch.
+template <typename Words>
+__device__ __forceinline__ double trace_row(int row, const int32_t* __restrict__ rowptr,
+                                            const int32_t* __restrict__ colind, const float* __restrict__ val,
+                                            int ngroups, const Words& words) {
+  const int k0 = rowptr[row], k1 = rowptr[row + 1];
+  const int len = k1 - k0;
   double acc = 0.0;
-  if (row < n) {
-    const int k0 = rowptr[row], k1 = rowptr[row + 1];
-    const int len = k1 - k0;
-    if (ngroups == 1 && len <= kBitsShort) {
-      // 64 probes, short row: column indices and values in one batch, then the word gathers,
-      // so the lane waits for memory twice per row; same summation order as the loop below
-      const uint64_t wr = W[row];
-      int c[kBitsShort];
-      float v[kBitsShort];
+  if (ngroups == 1 && len <= kBitsShort) {
+    const uint64_t wr = words(0, row);
+    int c[kBitsShort];
+    float v[kBitsShort];
 #pragma unroll
-      for (int i = 0; i < kBitsShort; ++i) {
-        c[i] = i < len ? colind[k0 + i] : row;
-        v[i] = i < len ? val[k0 + i] : 0.f;
-      }
-      uint64_t wc[kBitsShort];
+    for (int i = 0; i < kBitsShort; ++i) {
+      c[i] = i < len ? colind[k0 + i] : row;
+      v[i] = i < len ? val[k0 + i] : 0.f;
+    }
+    uint64_t wc[kBitsShort];
 #pragma unroll
-      for (int i = 0; i < kBitsShort; ++i) wc[i] = i < len ? W[c[i]] : wr;
+    for (int i = 0; i < kBitsShort; ++i) wc[i] = i < len ? words(0, c[i]) : wr;
+    // __popcll returns unsigned: convert before the subtraction can wrap
 #pragma unroll
-      for (int i = 0; i < kBitsShort; ++i)
-        if (i < len) acc += (double)v[i] * (double)(64 - 2 * (int)__popcll(wr ^ wc[i]));
-    } else if (ngroups == 1) {  // 64 probes: the row's own word is loaded once
-      const uint64_t wr = W[row];
-      // __popcll returns unsigned: convert before the subtraction can wrap
-      for (int k = k0; k < k1; ++k) acc += (double)val[k] * (double)(64 - 2 * (int)__popcll(wr ^ W[colind[k]]));
-    } else {
-      for (int k = k0; k < k1; ++k) {
-        const int c = colind[k];
-        int agree = 0;
-        for (int g = 0; g < ngroups; ++g)
-          agree += 64 - 2 * (int)__popcll(W[(int64_t)g * n + row] ^ W[(int64_t)g * n + c]);
-        acc += (double)val[k] * (double)agree;
-      }
+    for (int i = 0; i < kBitsShort; ++i)
+      if (i < len) acc += (double)v[i] * (double)(64 - 2 * (int)__popcll(wr ^ wc[i]));
+  } else {
+    for (int k = k0; k < k1; ++k) {
+      const int c = colind[k];
+      int agree = 0;
+      for (int g = 0; g < ngroups; ++g) agree += 64 - 2 * (int)__popcll(words(g, row) ^ words(g, c));
+      acc += (double)val[k] * (double)agree;
     }
   }
-  acc = block_sum<256>(acc, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+  return acc;
 }
 
-// The same estimate with the probe words recomputed where they are used: word (g, i) is
-// stream_word(stream_key(seed, g), i), exactly what ck_rng_rademacher_bits stores, so the
-// result equals the stored-word path bit for bit while the pass reads only the matrix.
-__global__ __launch_bounds__(256) void trace_seeded_partials(int n, const int32_t* __restrict__ rowptr,
-                                                             const int32_t* __restrict__ colind,
-                                                             const float* __restrict__ val, int ngroups,
-                                                             uint64_t seed, double* __restrict__ part) {
+// One partial per 1024 rows: a thread adds its four rows' sums (j ascending), then the block
+// sum (fixed order), so the stored-word and seeded passes give the same bits.
+template <typename Words>
+__global__ __launch_bounds__(256) void trace_partials(int n, const int32_t* __restrict__ rowptr,
+                                                      const int32_t* __restrict__ colind,
+                                                      const float* __restrict__ val, int ngroups, Words words,
+                                                      double* __restrict__ part) {
   __shared__ double red[4];
-  const int row = blockIdx.x * blockDim.x + threadIdx.x;
   double acc = 0.0;
-  if (row < n) {
-    const int k0 = rowptr[row], k1 = rowptr[row + 1];
-    const int len = k1 - k0;
-    if (ngroups == 1 && len <= kBitsShort) {
-      const uint64_t key = stream_key(seed, 0);
-      const uint64_t wr = stream_word(key, (uint64_t)row);
-      int c[kBitsShort];
-      float v[kBitsShort];
 #pragma unroll
-      for (int i = 0; i < kBitsShort; ++i) {
-        c[i] = i < len ? colind[k0 + i] : row;
-        v[i] = i < len ? val[k0 + i] : 0.f;
-      }
-#pragma unroll
-      for (int i = 0; i < kBitsShort; ++i)
-        if (i < len) acc += (double)v[i] * (double)(64 - 2 * (int)__popcll(wr ^ stream_word(key, (uint64_t)c[i])));
-    } else {
-      for (int k = k0; k < k1; ++k) {
-        const int c = colind[k];
-        int agree = 0;
-        for (int g = 0; g < ngroups; ++g) {
-          const uint64_t key = stream_key(seed, (uint64_t)g);
-          agree += 64 - 2 * (int)__popcll(stream_word(key, (uint64_t)row) ^ stream_word(key, (uint64_t)c));
-        }
-        acc += (double)val[k] * (double)agree;
-      }
-    }
+  for (int j = 0; j < kTraceRowsPerThread; ++j) {
+    const int row = blockIdx.x * (256 * kTraceRowsPerThread) + j * 256 + threadIdx.x;
+    if (row < n) acc += trace_row(row, rowptr, colind, val, ngroups, words);
   }
   acc = block_sum<256>(acc, red);
   if (threadIdx.x == 0) part[blockIdx.x] = acc;
@@ -326,10 +308,10 @@ int ck_trace_csr_rademacher_bits(const ck_op* A, int32_t ngroups, const uint64_t
     return CK_ERR_BAD_ARG;
   if (!work || work_bytes < ck_trace_workspace_bytes(A->n)) return CK_ERR_WORKSPACE;
   hipStream_t s = (hipStream_t)stream;
-  const int nparts = (int)cdiv(A->n, 256);
+  const int nparts = (int)cdiv(A->n, 256 * kTraceRowsPerThread);
   double* part = static_cast<double*>(work);
-  hipLaunchKernelGGL(trace_bits_partials, dim3(nparts), dim3(256), 0, s, A->n, A->rowptr, A->colind, A->values,
-                     ngroups, W, part);
+  hipLaunchKernelGGL(trace_partials<StoredWords>, dim3(nparts), dim3(256), 0, s, A->n, A->rowptr, A->colind,
+                     A->values, ngroups, StoredWords{W, (int64_t)A->n}, part);
   hipLaunchKernelGGL(trace_bits_final, dim3(1), dim3(256), 0, s, nparts, (const double*)part, (double)A->scale,
                      (double)A->shift * (double)A->n, 1.0 / (64.0 * ngroups), out);
   CK_CHECK_LAUNCH();
@@ -342,10 +324,10 @@ int ck_trace_csr_rademacher_seeded(const ck_op* A, int32_t ngroups, uint64_t see
     return CK_ERR_BAD_ARG;
   if (!work || work_bytes < ck_trace_workspace_bytes(A->n)) return CK_ERR_WORKSPACE;
   hipStream_t s = (hipStream_t)stream;
-  const int nparts = (int)cdiv(A->n, 256);
+  const int nparts = (int)cdiv(A->n, 256 * kTraceRowsPerThread);
   double* part = static_cast<double*>(work);
-  hipLaunchKernelGGL(trace_seeded_partials, dim3(nparts), dim3(256), 0, s, A->n, A->rowptr, A->colind, A->values,
-                     ngroups, seed, part);
+  hipLaunchKernelGGL(trace_partials<SeededWords>, dim3(nparts), dim3(256), 0, s, A->n, A->rowptr, A->colind,
+                     A->values, ngroups, SeededWords{seed}, part);
   hipLaunchKernelGGL(trace_bits_final, dim3(1), dim3(256), 0, s, nparts, (const double*)part, (double)A->scale,
                      (double)A->shift * (double)A->n, 1.0 / (64.0 * ngroups), out);
   CK_CHECK_LAUNCH();
