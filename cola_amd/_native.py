@@ -105,6 +105,7 @@ _PROTOS = {
     "ck_cg_step_p": (ctypes.c_int, [_i64, _i32, _p, _p, _p, _i64, _p, _p, _p]),
     "ck_cg_iterate": (ctypes.c_int, [_op, _op, _i32, _p, _p, _p, _p, _p, _i64, _p, _p, _p, _sz, _i32, _p]),
     "ck_cg_iterate_final": (ctypes.c_int, [_op, _op, _i32, _p, _p, _p, _p, _p, _i64, _p, _p, _p, _sz, _i32, _p]),
+    "ck_pcg_step1_f64": (ctypes.c_int, [_op, _op, _i32, _p, _p, _p, _i64, _p, _i64, _p, _p, _sz, _p]),
     "ck_cg_iterate_final_f64": (ctypes.c_int, [_op, _op, _i32, _p, _p, _p, _p, _p, _i64, _p, _p, _p, _sz, _i32, _p,
                                                _i64, _p]),
     "ck_cg_finish": (ctypes.c_int, [_i64, _i32, _p, _p, _i64, _p, _p]),

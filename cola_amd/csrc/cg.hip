@@ -300,6 +300,40 @@ __global__ __launch_bounds__(256) void update_x64_kernel(int64_t n, const float*
   }
 }
 
+// One PCG step from x = 0 added to an FP64 accumulator: alpha = (r.z) / (z.Az) per column from
+// the two partial-sum rows (the generic step's safe division, alpha = 0 for a zero column),
+// x64 += (double)(alpha * z).
+template <bool V4>
+__global__ __launch_bounds__(256) void step1_x64_kernel(int64_t n, const float* __restrict__ z, int64_t ld,
+                                                        double* __restrict__ x64, int64_t ldx64,
+                                                        const float* __restrict__ prz, const float* __restrict__ pzaz,
+                                                        int np, int64_t per) {
+  __shared__ float alpha_s;
+  const int b = blockIdx.y, blk = blockIdx.x;
+  if (threadIdx.x < 64) {
+    const double rz = column_total(prz, np, b);
+    const double zaz = column_total(pzaz, np, b);
+    if (threadIdx.x == 0) alpha_s = (float)safe_div((double)(float)rz, zaz);
+  }
+  __syncthreads();
+  const float alpha = alpha_s;
+  const int64_t i0 = blk * per;
+  const int64_t i1 = i0 + per < n ? i0 + per : n;
+  const float* zb = z + b * ld;
+  double* yb = x64 + b * ldx64;
+  if (V4) {
+    for (int64_t i = i0 + 4 * threadIdx.x; i < i1; i += 1024) {
+      const float4 zv = *reinterpret_cast<const float4*>(zb + i);
+      yb[i] += (double)(alpha * zv.x);
+      yb[i + 1] += (double)(alpha * zv.y);
+      yb[i + 2] += (double)(alpha * zv.z);
+      yb[i + 3] += (double)(alpha * zv.w);
+    }
+  } else {
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) yb[i] += (double)(alpha * zb[i]);
+  }
+}
+
 // End of step: rnorm, gamma1, beta, k, errors, loop condition.
 __global__ __launch_bounds__(256) void step_final_kernel(int nrhs, const float* __restrict__ part1,
                                                          const float* __restrict__ part2, int P, int precond,
@@ -587,6 +621,33 @@ int ck_cg_iterate_final(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, 
   if (nsteps < 1) return CK_ERR_BAD_ARG;
   return cg_iterate(A, P, nrhs, x, r, p, Ap, z, ld, state, partials, work, work_bytes, nsteps, true, nullptr, 0,
                     stream);
+}
+
+int ck_pcg_step1_f64(const ck_op* A, const ck_op* P, int32_t nrhs, const float* r, float* z, float* Az, int64_t ld,
+                     double* x64, int64_t ldx64, float* partials, void* work, size_t work_bytes, void* stream) {
+  if (!A || !P || nrhs < 1 || nrhs > 1024 || !r || !z || !Az || !x64 || !partials || ld < A->n || ldx64 < A->n)
+    return CK_ERR_BAD_ARG;
+  const int64_t n = A->n;
+  if (P->n != n) return CK_ERR_BAD_ARG;
+  Parts q = parts_of(partials, n, nrhs);
+  // the fused Kronecker partials (one per 64-column tile of pass B, one per slab of the
+  // preconditioner's last pass) line up with the generic 4096-element chunks only at n = 64^3
+  const bool ok = A->kind == CK_OP_KRON_F32 && A->nfactors == 3 && kron64_eligible(A, z, ld, Az, ld) &&
+                  P->kind == CK_OP_KRON_EIG_F32 && kron64_eligible(P, r, ld, z, ld) && q.P == 64 && q.per == 4096 &&
+                  kron_nparts(A) == 64;
+  if (!ok) return CK_ERR_UNSUPPORTED;
+  hipStream_t s = (hipStream_t)stream;
+  int st = kron64_eig_apply_f32(P, nrhs, r, ld, z, ld, work, work_bytes, nullptr, s, q.p2);
+  if (st) return st;
+  st = kron_dispatch(A, nrhs, z, ld, Az, ld, work, work_bytes, nullptr, q.p0, 64, s);
+  if (st) return st;
+  dim3 grid(q.P, nrhs);
+  if (use_v4(n, ld, z, nullptr, nullptr, nullptr))
+    hipLaunchKernelGGL(step1_x64_kernel<true>, grid, dim3(256), 0, s, n, z, ld, x64, ldx64, q.p2, q.p0, q.P, q.per);
+  else
+    hipLaunchKernelGGL(step1_x64_kernel<false>, grid, dim3(256), 0, s, n, z, ld, x64, ldx64, q.p2, q.p0, q.P, q.per);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
 }
 
 int ck_cg_iterate_final_f64(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r, float* p, float* Ap,

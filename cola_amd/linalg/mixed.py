@@ -234,6 +234,12 @@ class MixedSolver:
         its un-normalisation is folded into the update (ck_ir_update_scaled_f64)."""
         n, B, ld, alg = self.n, self.B, self.ld, self.alg
         s = N.stream_ptr(self.b64.device)
+        if not read and max_steps == 1 and self._step1_ok(inner):
+            # one PCG step from x = 0 is scale invariant: no normalisation, no CG state (ck_pcg_step1_f64)
+            aop, pop, work, wb = self.lowered_for[id(inner)]
+            N.call("ck_pcg_step1_f64", aop[0], pop[0], B, N.ptr(self.r32), N.ptr(self.st.z), N.ptr(self.st.Ap), ld,
+                   N.ptr(self.x64), ld, N.ptr(self.st.partials), N.ptr(work), wb, s)
+            return 1
         N.call("ck_ir_inner_start", n, B, N.ptr(self.r32), ld, N.ptr(self.norms), N.ptr(self.st.r), N.ptr(self.st.x),
                ld, N.ptr(self.mult), s)
         # a replayed correction runs exactly max_steps steps and only its x is used: its last step
@@ -244,6 +250,16 @@ class MixedSolver:
         if not info.get("x64_updated"):
             N.call("ck_ir_update_scaled_f64", n, B, N.ptr(self.x64), ld, N.ptr(self.st.x), ld, N.ptr(self.mult), s)
         return k
+
+    def _step1_ok(self, inner):
+        """Whether ck_pcg_step1_f64 covers (inner, P): a lowered 64^3 Kronecker operator with the
+        Kronecker eigenbasis preconditioner (checked once per operator)."""
+        cache = self.__dict__.setdefault("_step1", {})
+        if id(inner) not in cache:
+            aop, pop, work, wb = self.lowered_for[id(inner)]
+            cache[id(inner)] = (aop is not None and pop is not None and aop[0].kind == N.CK_OP_KRON_F32
+                                and pop[0].kind == N.CK_OP_KRON_EIG_F32 and self.n == 64**3)
+        return cache[id(inner)]
 
     def _schedule(self, plan):
         """One correction solve of k steps and one FP64 residual per plan entry (from x = 0, set

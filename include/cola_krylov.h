@@ -254,6 +254,15 @@ int ck_cg_iterate_final(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, 
 int ck_cg_iterate_final_f64(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r, float* p, float* Ap,
                             float* z, int64_t ld, void* state, float* partials, void* work, size_t work_bytes,
                             int32_t nsteps, double* x64, int64_t ldx64, void* stream);
+/* One preconditioned CG step from x = 0 on the residual block r, added to an FP64
+ * accumulator: z = P r, alpha = (r.z)/(z.A z) per column (0 for a zero column),
+ * x64 += (double)(alpha z).  The result of ck_cg_normalize + ck_cg_start +
+ * ck_cg_iterate_final_f64(nsteps = 1) without the normalisation (one step is scale
+ * invariant) in five launches: the preconditioner's passes emit r.z, A's emit z.A z.
+ * Supported for a 64^3 Kronecker A with the Kronecker eigenbasis P; CK_ERR_UNSUPPORTED
+ * otherwise.  partials: ck_cg_partials_bytes(n, nrhs); work: A's and P's workspace. */
+int ck_pcg_step1_f64(const ck_op* A, const ck_op* P, int32_t nrhs, const float* r, float* z, float* Az, int64_t ld,
+                     double* x64, int64_t ldx64, float* partials, void* work, size_t work_bytes, void* stream);
 /* x *= mult; r *= mult  (cg.py:119). */
 int ck_cg_finish(int64_t n, int32_t nrhs, float* x, float* r, int64_t ld, const void* state, void* stream);
 

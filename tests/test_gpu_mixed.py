@@ -198,9 +198,9 @@ def test_mixed_cg_graph_replays_match_eager(dev):
     graphed.solve(bd)
     x_ref, info_ref = eager.solve(bd)
     x_ref = x_ref.clone()
-    # the replay's last step adds mult * (x + alpha p) to x64 directly: the same arithmetic
-    # as the full step followed by the FP64 update, so the bits agree
-    assert torch.equal(x_ref, x_first)
+    # one-step corrections replay as ck_pcg_step1_f64 (no normalisation): equal to fp32 rounding
+    assert info_ref["converged"]
+    assert float((x_ref - x_first).norm() / x_first.norm()) < 1e-5
     for rep in range(3):
         junk = torch.randn(1 << 20, device=dev).sum()  # eager work between replays
         torch.cuda.synchronize()
@@ -213,3 +213,34 @@ def test_mixed_cg_graph_replays_match_eager(dev):
     x2, info2 = graphed.solve(b2)
     assert info2["converged"]
     assert torch.equal(x2, x2_ref)
+
+
+def test_pcg_step1_matches_one_cg_step(dev):
+    """ck_pcg_step1_f64 against the generic path (normalise, start, one final CG step into
+    the FP64 accumulator) on the 64^3 Kronecker system with its eigenbasis preconditioner;
+    a zero column stays zero."""
+    from cola_amd.linalg.cg import CGState, cg_blocks
+    from cola_amd.linalg._engine import lower, workspace
+    A, facs, b = kron_system(dev, 3, seed=5)
+    P = KroneckerEigInverse([f.to(dev) for f in facs], 1.0, 0.01)
+    n, B = 64**3, 3
+    r = b.T.contiguous().to(dev) * torch.tensor([1.0, 1e-3, 0.0], device=dev)[:, None]
+    aop, pop = lower(A), lower(P)
+    work, wb = workspace([aop[0], pop[0]], B, dev)
+    st = CGState(n, B, 10, dev, True)
+    x1 = torch.zeros(B, n, dtype=torch.float64, device=dev)
+    N.call("ck_pcg_step1_f64", aop[0], pop[0], B, N.ptr(r), N.ptr(st.z), N.ptr(st.Ap), n, N.ptr(x1), n,
+           N.ptr(st.partials), N.ptr(work), wb, N.stream_ptr(dev))
+    x2 = torch.zeros(B, n, dtype=torch.float64, device=dev)
+    st2 = CGState(n, B, 10, dev, True)
+    cg_blocks(A, r, st2, None, 1, 1e-3, P, 1, (aop, pop, work, wb), want_info=False, read=False, finish=False,
+              final=True, x64=x2)
+    # (the generic final step already adds mult * x to x2: both are un-normalised)
+    for c in range(2):
+        assert float((x1[c] - x2[c]).norm() / x2[c].norm()) < 1e-5, c
+    assert torch.all(x1[2] == 0)
+    # x1 = alpha z with alpha = r.z / z.Az (the one-step minimiser), from the kernel's own z and A z
+    z, az, rd = st.z[:, :n].double(), st.Ap[:, :n].double(), r.double()
+    for c in range(2):
+        alpha = float((rd[c] * z[c]).sum() / (z[c] * az[c]).sum())
+        assert float((x1[c] - alpha * z[c]).norm() / x1[c].norm()) < 1e-6, c
