@@ -80,6 +80,8 @@ _PROTOS = {
     "ck_kron_apply_last2_f32": (ctypes.c_int, [_p, _p, _i32, _p, _i64, _p, _i64, _p]),
     "ck_ir_workspace_bytes": (_sz, [_i32]),
     "ck_ir_residual_f64": (ctypes.c_int, [_i64, _i32, _p, _i64, _p, _i64, _p, _i64, _p, _i64, _p, _p, _sz, _p]),
+    "ck_ir_kron_step1_f64": (ctypes.c_int, [_op, _op, _p, _p, _p, ctypes.c_double, ctypes.c_double, _i32, _p, _p, _p,
+                                            _i64, _p, _p, _i32, _p, _p, _p, _sz, _p, _sz, _p]),
     "ck_kron_residual_workspace_bytes": (ctypes.c_size_t, [_i32]),
     "ck_kron_residual_f64": (ctypes.c_int, [_p, _p, _p, ctypes.c_double, ctypes.c_double, _i32, _p, _i64, _p, _i64, _p,
                                             _i64, _p, _p, _sz, _p]),

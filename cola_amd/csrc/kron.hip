@@ -110,23 +110,78 @@ constexpr int slabs_per_block() {
 //
 // With R given (fp32 only), each slab also emits partials[b * 64 + i1] = sum over the slab of
 // R * U: the r.z partial sums of a preconditioned CG step when this pass writes z = P r.
-template <typename T>
+//
+// With upd.z given (fp64 refinement residuals), the input is first updated in place:
+// X = (upd.accumulate ? X : 0) + (double)(alpha_b * z), alpha_b = (r.z)/(z.A z) of column b from
+// the two partial-sum rows (a one-step PCG correction), and the pass runs on the new X.
+struct XUpdate {
+  const float* z;
+  int64_t ldz;
+  const float* prz;
+  const float* pzaz;
+  int np;
+  int accumulate;
+};
+
+__device__ __forceinline__ double part_total(const float* __restrict__ part, int np, int b) {
+  double acc = 0.0;
+  for (int i = threadIdx.x & 63; i < np; i += 64) acc += (double)part[(int64_t)b * np + i];
+  return ck::wave_sum(acc);
+}
+
+template <typename T, bool UPD = false>
 __global__ __launch_bounds__(256) void kron64_pass_a(const T* __restrict__ K2, const T* __restrict__ K3,
                                                      const T* __restrict__ X, int64_t ldx, T* __restrict__ U,
                                                      int64_t ldu, const int* __restrict__ done,
                                                      const T* __restrict__ R, int64_t ldr,
-                                                     float* __restrict__ partials) {
+                                                     float* __restrict__ partials, XUpdate upd) {
   if (done && *done) return;
   constexpr int SLABS = slabs_per_block<T>();
   constexpr int NBUF = sizeof(T) == 4 ? 2 : 1;
   __shared__ __align__(16) T Tt[NBUF][64 * TP];
   __shared__ float red[4];
+  __shared__ float alpha_s;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, li = lane & 15;
   const int s0 = blockIdx.x * SLABS;
   const int64_t row_off = (16 * w + li) * 64 + 16 * g;
   auto slab = [&](int sidx) { return X + (int64_t)(sidx >> 6) * ldx + (int64_t)(sidx & 63) * 4096; };
+  float alpha = 0.f;
+  if constexpr (UPD) {
+    // all SLABS slabs of a block belong to one column (SLABS divides 64)
+    if (threadIdx.x < 64) {
+      const double rz = part_total(upd.prz, upd.np, s0 >> 6);
+      const double zaz = part_total(upd.pzaz, upd.np, s0 >> 6);
+      if (threadIdx.x == 0) {
+        double den = zaz;
+        if (fabs(den) < (double)(float)1e-40) den = (double)(float)1e-40;
+        alpha_s = (float)((double)(float)rz / den);
+      }
+    }
+    __syncthreads();
+    alpha = alpha_s;
+  }
+  // the slab rows of this lane, after the optional update (written back to X)
+  auto fetch = [&](int sidx, T* dst) {
+    T* xp = const_cast<T*>(slab(sidx)) + row_off;
+    if constexpr (UPD) {
+      const float* zp = upd.z + (int64_t)(sidx >> 6) * upd.ldz + (int64_t)(sidx & 63) * 4096 + row_off;
+      T zv[16];
+      float zf[16];
+      load16(zp, zf);
+      if (upd.accumulate) load16(xp, dst);
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        zv[q] = (T)(alpha * zf[q]);
+        dst[q] = upd.accumulate ? dst[q] + zv[q] : zv[q];
+      }
+#pragma unroll
+      for (int q = 0; q < 16; ++q) xp[q] = dst[q];
+    } else {
+      load16(xp, dst);
+    }
+  };
   T a[16];
-  load16(slab(s0) + row_off, a);
+  fetch(s0, a);
   T k3[4][16];
 #pragma unroll
   for (int t = 0; t < 4; ++t) load16(K3 + (16 * t + li) * 64 + 16 * g, k3[t]);
@@ -137,7 +192,7 @@ __global__ __launch_bounds__(256) void kron64_pass_a(const T* __restrict__ K2, c
   for (int it = 0; it < SLABS; ++it) {
     const int sidx = s0 + it;
     T an[16];
-    if (it + 1 < SLABS) load16(slab(sidx + 1) + row_off, an);
+    if (it + 1 < SLABS) fetch(sidx + 1, an);
     __builtin_amdgcn_sched_barrier(0);
     // T = S K3^T: T[r][c] = sum_k S[r][k] K3[c][k]
     typename MF<T>::acc c[4];
@@ -455,7 +510,7 @@ int launch_kron64(const T* const* f, int nrhs, const T* X, int64_t ldx, T* Y, in
   const int64_t n = 64 * 64 * 64;
   const unsigned grid_a = (unsigned)(64 / slabs_per_block<T>() * nrhs);
   hipLaunchKernelGGL(kron64_pass_a<T>, dim3(grid_a), dim3(256), 0, s, f[1], f[2], X, ldx, U, n, done,
-                     (const T*)nullptr, (int64_t)0, (float*)nullptr);
+                     (const T*)nullptr, (int64_t)0, (float*)nullptr, XUpdate{});
   if constexpr (sizeof(T) == 4) {
     if (nparts != 0 && nparts != 64) return CK_ERR_BAD_ARG;
     hipLaunchKernelGGL(kron64_pass_b4, dim3(64 / kTilesPerBlockB * nrhs), dim3(256), 0, s, f[0], (const T*)U, n, X,
@@ -505,11 +560,11 @@ int kron64_eig_apply_f32(const ck_op* op, int nrhs, const float* X, int64_t ldx,
   float* V = U + (size_t)nrhs * n;
   const unsigned grid_a = (unsigned)(64 / slabs_per_block<float>() * nrhs);
   hipLaunchKernelGGL(kron64_pass_a<float>, dim3(grid_a), dim3(256), 0, s, op->factor2[1], op->factor2[2], X, ldx, U,
-                     n, done, (const float*)nullptr, (int64_t)0, (float*)nullptr);
+                     n, done, (const float*)nullptr, (int64_t)0, (float*)nullptr, XUpdate{});
   hipLaunchKernelGGL(kron64_pass_bb, dim3(64 / kTilesPerBlockB * nrhs), dim3(256), 0, s, op->factor2[0],
                      op->factor[0], (const float*)U, n, V, n, op->scale, op->aux, done);
   hipLaunchKernelGGL(kron64_pass_a<float>, dim3(grid_a), dim3(256), 0, s, op->factor[1], op->factor[2],
-                     (const float*)V, n, Y, ldy, done, rz_partials ? X : nullptr, ldx, rz_partials);
+                     (const float*)V, n, Y, ldy, done, rz_partials ? X : nullptr, ldx, rz_partials, XUpdate{});
   CK_CHECK_LAUNCH();
   return CK_OK;
 }
@@ -524,7 +579,7 @@ extern "C" int ck_kron_apply_last2_f32(const float* K2, const float* K3, int32_t
     return CK_ERR_UNSUPPORTED;
   hipLaunchKernelGGL(kron64_pass_a<float>, dim3((unsigned)(64 / slabs_per_block<float>() * nrhs)), dim3(256), 0,
                      (hipStream_t)stream, K2, K3, X, ldx, Y, ldy, (const int*)nullptr, (const float*)nullptr,
-                     (int64_t)0, (float*)nullptr);
+                     (int64_t)0, (float*)nullptr, XUpdate{});
   CK_CHECK_LAUNCH();
   return CK_OK;
 }
@@ -547,10 +602,47 @@ extern "C" int ck_kron_residual_f64(const double* f0, const double* f1, const do
   double* U = static_cast<double*>(work);
   double* rpart = U + (size_t)nrhs * n;
   hipLaunchKernelGGL(kron64_pass_a<double>, dim3((unsigned)(64 / slabs_per_block<double>() * nrhs)), dim3(256), 0, s,
-                     f1, f2, X, ldx, U, n, (const int*)nullptr, (const double*)nullptr, (int64_t)0, (float*)nullptr);
+                     f1, f2, X, ldx, U, n, (const int*)nullptr, (const double*)nullptr, (int64_t)0, (float*)nullptr,
+                     XUpdate{});
   hipLaunchKernelGGL(kron64_resid_b, dim3(64, nrhs), dim3(256), 0, s, f0, (const double*)U, n, X, ldx, Bv, ldb, R32,
                      ld32, scale, shift, rpart);
   hipLaunchKernelGGL(kron64_resid_norms, dim3(nrhs, 2), dim3(64), 0, s, (const double*)rpart, norms);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+extern "C" int ck_ir_kron_step1_f64(const ck_op* A, const ck_op* P, const double* f0, const double* f1,
+                                    const double* f2, double scale, double shift, int32_t nrhs, float* r32, float* z,
+                                    float* Az, int64_t ld, double* x64, const double* b64, int32_t accumulate,
+                                    double* norms, float* partials, void* work, size_t work_bytes, void* work64,
+                                    size_t work64_bytes, void* stream) {
+  const int64_t n = 64 * 64 * 64;
+  if (!A || !P || !f0 || !f1 || !f2 || nrhs < 1 || nrhs > 1024 || !r32 || !z || !Az || !x64 || !b64 || !partials ||
+      ld < n)
+    return CK_ERR_BAD_ARG;
+  if (A->kind != CK_OP_KRON_F32 || P->kind != CK_OP_KRON_EIG_F32 || !ck::kron64_eligible(A, z, ld, Az, ld) ||
+      !ck::kron64_eligible(P, r32, ld, z, ld))
+    return CK_ERR_UNSUPPORTED;
+  if (!aligned16(f0) || !aligned16(f1) || !aligned16(f2) || !aligned16(x64) || !aligned16(b64) || (ld % 2))
+    return CK_ERR_UNSUPPORTED;
+  if (!work64 || work64_bytes < ck_kron_residual_workspace_bytes(nrhs)) return CK_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  // partial-sum rows in the CG layout (64 per column at n = 64^3): z.Az in row 0, r.z in row 2
+  float* pzaz = partials;
+  float* prz = partials + 2 * (size_t)nrhs * 64;
+  int st = ck::kron64_eig_apply_f32(P, nrhs, r32, ld, z, ld, work, work_bytes, nullptr, s, prz);
+  if (st) return st;
+  st = ck::kron_dispatch(A, nrhs, z, ld, Az, ld, work, work_bytes, nullptr, pzaz, 64, s);
+  if (st) return st;
+  double* U = static_cast<double*>(work64);
+  double* rpart = U + (size_t)nrhs * n;
+  XUpdate upd{z, ld, prz, pzaz, 64, accumulate ? 1 : 0};
+  hipLaunchKernelGGL((kron64_pass_a<double, true>), dim3((unsigned)(64 / slabs_per_block<double>() * nrhs)), dim3(256),
+                     0, s, f1, f2, (const double*)x64, ld, U, n, (const int*)nullptr, (const double*)nullptr,
+                     (int64_t)0, (float*)nullptr, upd);
+  hipLaunchKernelGGL(kron64_resid_b, dim3(64, nrhs), dim3(256), 0, s, f0, (const double*)U, n, (const double*)x64, ld,
+                     b64, ld, r32, ld, scale, shift, rpart);
+  if (norms) hipLaunchKernelGGL(kron64_resid_norms, dim3(nrhs, 2), dim3(64), 0, s, (const double*)rpart, norms);
   CK_CHECK_LAUNCH();
   return CK_OK;
 }

@@ -263,10 +263,26 @@ class MixedSolver:
 
     def _schedule(self, plan):
         """One correction solve of k steps and one FP64 residual per plan entry (from x = 0, set
-        by the rhs load): launches only (no host reads), so it can be captured as a graph."""
-        for k_steps in plan:
-            self.correction(self.A, k_steps, k_steps, read=False)
-            self.residual(with_ax=True, read=False)
+        by the rhs load): launches only (no host reads), so it can be captured as a graph.
+        One-step entries on a 64^3 Kronecker system with its eigenbasis preconditioner run as one
+        ck_ir_kron_step1_f64 call (the x update inside the residual's first pass); only the last
+        residual's norms are formed (the host reads them once)."""
+        for i, k_steps in enumerate(plan):
+            last = i + 1 == len(plan)
+            if k_steps == 1 and self.f64.kind == "kron64" and self._step1_ok(self.A):
+                self._kron_step1(accumulate=i > 0, norms=last)
+            else:
+                self.correction(self.A, k_steps, k_steps, read=False)
+                self.residual(with_ax=True, read=False)
+
+    def _kron_step1(self, accumulate, norms):
+        n, B, ld = self.n, self.B, self.ld
+        aop, pop, work, wb = self.lowered_for[id(self.A)]
+        f = self.f64
+        N.call("ck_ir_kron_step1_f64", aop[0], pop[0], N.ptr(f.f64[0]), N.ptr(f.f64[1]), N.ptr(f.f64[2]), f.scale,
+               f.shift, B, N.ptr(self.r32), N.ptr(self.st.z), N.ptr(self.st.Ap), ld, N.ptr(self.x64), N.ptr(self.b64),
+               1 if accumulate else 0, N.ptr(self.norms) if norms else None, N.ptr(self.st.partials), N.ptr(work), wb,
+               N.ptr(f.work), f.work.numel() * 8, N.stream_ptr(self.b64.device))
 
     def solve(self, b):
         alg = self.alg

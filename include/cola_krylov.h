@@ -129,6 +129,18 @@ size_t ck_kron_residual_workspace_bytes(int32_t nrhs);
 int ck_kron_residual_f64(const double* f0, const double* f1, const double* f2, double scale, double shift,
                          int32_t nrhs, const double* X, int64_t ldx, const double* Bv, int64_t ldb, float* R32,
                          int64_t ld32, double* norms, void* work, size_t work_bytes, void* stream);
+/* One whole refinement step for a 64^3 Kronecker A (fp32 op, FP64 factor copies f0..f2 with
+ * scale and shift) and its eigenbasis preconditioner P: z = P r32 and A z with their r.z and
+ * z.A z partial sums, then, inside the FP64 residual's first pass, x64 = (accumulate ? x64 :
+ * 0) + alpha z with alpha = (r.z)/(z.A z) per column (one PCG step from zero, as
+ * ck_pcg_step1_f64), then r32 = b - A x64 and, when norms is not NULL, the norms of
+ * ck_kron_residual_f64.  r32, z, Az: (nrhs, ld) fp32 blocks; x64, b64: fp64 with the same ld.
+ * partials: ck_cg_partials_bytes(64^3, nrhs); work: A's and P's fp32 workspace; work64 >=
+ * ck_kron_residual_workspace_bytes(nrhs).  No reference counterpart. */
+int ck_ir_kron_step1_f64(const ck_op* A, const ck_op* P, const double* f0, const double* f1, const double* f2,
+                         double scale, double shift, int32_t nrhs, float* r32, float* z, float* Az, int64_t ld,
+                         double* x64, const double* b64, int32_t accumulate, double* norms, float* partials,
+                         void* work, size_t work_bytes, void* work64, size_t work64_bytes, void* stream);
 
 /* ---------------------------------------------------------------- mixed precision
  * Iterative refinement with FP64 residuals around fp32 inner solves (no

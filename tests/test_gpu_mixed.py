@@ -244,3 +244,45 @@ def test_pcg_step1_matches_one_cg_step(dev):
     for c in range(2):
         alpha = float((rd[c] * z[c]).sum() / (z[c] * az[c]).sum())
         assert float((x1[c] - alpha * z[c]).norm() / x1[c].norm()) < 1e-6, c
+
+
+def test_ir_kron_step1_matches_two_calls(dev):
+    """ck_ir_kron_step1_f64 (x update inside the FP64 residual's first pass) against
+    ck_pcg_step1_f64 followed by ck_kron_residual_f64: the same bits for x64, r32 and the
+    norms, for a first (x64 = alpha z) and an accumulating step."""
+    from cola_amd.linalg.cg import CGState
+    from cola_amd.linalg._engine import lower, workspace
+    A, facs, b = kron_system(dev, 2, seed=7)
+    P = KroneckerEigInverse([f.to(dev) for f in facs], 1.0, 0.01)
+    n, B = 64**3, 2
+    f64 = [f.double().contiguous().to(dev) for f in facs]
+    s32 = float(np.float32(0.01))
+    aop, pop = lower(A), lower(P)
+    work, wb = workspace([aop[0], pop[0]], B, dev)
+    w64b = N.q("ck_kron_residual_workspace_bytes", B)
+    s = N.stream_ptr(dev)
+    b64 = b.T.contiguous().double().to(dev)
+    outs = []
+    for fused in (True, False):
+        st = CGState(n, B, 10, dev, True)
+        w64 = torch.empty(w64b // 8, dtype=torch.float64, device=dev)
+        r32 = b.T.contiguous().to(dev)
+        x64 = torch.full((B, n), 5.0, dtype=torch.float64, device=dev)
+        norms = torch.empty(2 * B, dtype=torch.float64, device=dev)
+        for step in range(2):
+            if fused:
+                N.call("ck_ir_kron_step1_f64", aop[0], pop[0], N.ptr(f64[0]), N.ptr(f64[1]), N.ptr(f64[2]), 1.0, s32,
+                       B, N.ptr(r32), N.ptr(st.z), N.ptr(st.Ap), n, N.ptr(x64), N.ptr(b64), step, N.ptr(norms),
+                       N.ptr(st.partials), N.ptr(work), wb, N.ptr(w64), w64b, s)
+            else:
+                if step == 0:
+                    x64.zero_()
+                N.call("ck_pcg_step1_f64", aop[0], pop[0], B, N.ptr(r32), N.ptr(st.z), N.ptr(st.Ap), n, N.ptr(x64), n,
+                       N.ptr(st.partials), N.ptr(work), wb, s)
+                N.call("ck_kron_residual_f64", N.ptr(f64[0]), N.ptr(f64[1]), N.ptr(f64[2]), 1.0, s32, B, N.ptr(x64),
+                       n, N.ptr(b64), n, N.ptr(r32), n, N.ptr(norms), N.ptr(w64), w64b, s)
+        outs.append((x64.clone(), r32.clone(), norms.clone()))
+    for u, v in zip(outs[0], outs[1]):
+        assert torch.equal(u, v)
+    rel = outs[0][2].view(B, 2)
+    assert float((rel[:, 0] / rel[:, 1]).max()) < 1e-6
