@@ -98,11 +98,12 @@ __global__ __launch_bounds__(256) void ir_load_rhs_kernel(int64_t n, int nrhs, c
         const double dv = (double)v;
         b64[c * ld + i] = dv;
         r32[c * ld + i] = v;
-        x64[c * ld + i] = 0.0;
+        if (x64) x64[c * ld + i] = 0.0;
         acc[j] += dv * dv;
       }
     }
   }
+  if (!part) return;  // uniform: no norms requested
 #pragma unroll
   for (int j = 0; j < kLoadCols; ++j) {
     if (j < nc) {
@@ -215,10 +216,10 @@ int ck_ir_update_f64(int64_t n, int32_t nrhs, double* x, int64_t ldx, const floa
 
 int ck_ir_load_rhs(int64_t n, int32_t nrhs, const float* b, int64_t rs, int64_t cs, double* b64, float* r32,
                    double* x64, int64_t ld, double* norms, void* work, size_t work_bytes, void* stream) {
-  if (n < 1 || nrhs < 1 || !b || !b64 || !r32 || !x64 || !norms || ld < n) return CK_ERR_BAD_ARG;
-  if (!work || work_bytes < ck_ir_workspace_bytes(nrhs)) return CK_ERR_WORKSPACE;
+  if (n < 1 || nrhs < 1 || !b || !b64 || !r32 || ld < n) return CK_ERR_BAD_ARG;
+  if (norms && (!work || work_bytes < ck_ir_workspace_bytes(nrhs))) return CK_ERR_WORKSPACE;
   hipStream_t s = (hipStream_t)stream;
-  double* part = static_cast<double*>(work);
+  double* part = norms ? static_cast<double*>(work) : nullptr;
   const bool v4 = cs == 1 && rs % 4 == 0 && nrhs % 4 == 0 && (reinterpret_cast<uintptr_t>(b) & 15) == 0;
   for (int c0 = 0; c0 < nrhs; c0 += kLoadCols) {
     if (v4)
@@ -228,7 +229,7 @@ int ck_ir_load_rhs(int64_t n, int32_t nrhs, const float* b, int64_t rs, int64_t 
       hipLaunchKernelGGL(ir_load_rhs_kernel<false>, dim3(kIrParts), dim3(256), 0, s, n, nrhs, b, rs, cs, b64, r32,
                          x64, ld, c0, part);
   }
-  hipLaunchKernelGGL(ir_norms_kernel, dim3(nrhs, 2), dim3(64), 0, s, (const double*)part, norms);
+  if (norms) hipLaunchKernelGGL(ir_norms_kernel, dim3(nrhs, 2), dim3(64), 0, s, (const double*)part, norms);
   CK_CHECK_LAUNCH();
   return CK_OK;
 }

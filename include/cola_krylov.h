@@ -155,7 +155,8 @@ int ck_ir_residual_f64(int64_t n, int32_t nrhs, const double* b, int64_t ldb, co
 int ck_ir_update_f64(int64_t n, int32_t nrhs, double* x, int64_t ldx, const float* d, int64_t ldd, void* stream);
 /* Start of a refinement solve: b64 = b, r32 = b, x64 = 0 (column blocks with leading
  * dimension ld) from b's element (i, c) at b[i*rs + c*cs], and norms as ck_ir_residual_f64
- * gives them for x = 0 (both ||b_c||).  work >= ck_ir_workspace_bytes(nrhs). */
+ * gives them for x = 0 (both ||b_c||).  x64 and norms may be NULL (not written; work is then
+ * unused), else work >= ck_ir_workspace_bytes(nrhs). */
 int ck_ir_load_rhs(int64_t n, int32_t nrhs, const float* b, int64_t rs, int64_t cs, double* b64, float* r32,
                    double* x64, int64_t ld, double* norms, void* work, size_t work_bytes, void* stream);
 /* Inner-solve normalisation folded into the refinement (replaces ck_cg_normalize +

@@ -132,6 +132,12 @@ def test_ir_load_rhs(dev, B, transposed):
     nb = bt.double().norm(dim=1)
     assert torch.allclose(norms.view(B, 2)[:, 0].cpu(), nb, rtol=1e-13)
     assert torch.allclose(norms.view(B, 2)[:, 1].cpu(), nb, rtol=1e-13)
+    # lean form: no x64 and no norms (nothing else written)
+    b64.fill_(3.0), x64.fill_(3.0), r32.fill_(3.0), norms.fill_(-1.0)
+    N.call("ck_ir_load_rhs", n, B, N.ptr(b), b.stride(0), b.stride(1), N.ptr(b64), N.ptr(r32), None, ld, None, None, 0,
+           N.stream_ptr(dev))
+    assert torch.equal(b64[:, :n].cpu(), bt.double()) and torch.equal(r32[:, :n].cpu(), bt)
+    assert torch.all(x64 == 3.0) and torch.all(norms == -1.0)
 
 
 def test_ir_inner_start_and_scaled_update(dev):
