@@ -161,7 +161,7 @@ def bench_kron_cg(args, dev):
     return out
 
 
-def cpu_baseline_kron(facs, b, k_steps, tol, iters=20):
+def cpu_baseline_kron(facs, b, k_steps, tol, iters=640):
     """The CPU torch restatement of cola's run_batched_cg with Kronecker._matmat (what
     cola.linalg.inv(PSD(A)) @ B runs), timed for `iters` CG steps on the full 16-rhs
     system and scaled to the k_steps (the 1000-step default cap) of one call."""
@@ -678,9 +678,10 @@ def bench_batched_cg(args, dev):
             "roofline": roof}
 
 
-def cpu_baseline_dense(sd, n, B, k_steps, tol, iters=10):
+def cpu_baseline_dense(sd, n, B, k_steps, tol, iters=130):
     """The oracle's plain CG (the reference algorithm, torch on the host) on the first system:
-    `iters` steps timed, scaled to the steps the engine's plain CG took."""
+    `iters` steps timed (about 10 s on 16 host threads), scaled to the steps the engine's plain
+    CG took."""
     from oracle import cg as ocg
     torch.set_num_threads(cpu_threads())
     K = sd["K"].cpu()
