@@ -136,7 +136,7 @@ def bench_kron_cg(args, dev):
     wb = N.q("ck_op_workspace_bytes", op, B)
     work = torch.empty(wb // 4, device=dev)
     t_mv = time_kernel(lambda: N.call("ck_op_apply", op, B, N.ptr(Xb), ld, N.ptr(Yb), ld, N.ptr(work), wb, s))
-    roof = {"bound": "mfma", "kernel": "kron64_pass_a<float> (fused mode-2/3 Kronecker pass)",
+    roof = {"bound": "mfma", "kernel": "kron64_pass_a<float, false> (fused mode-2/3 Kronecker pass)",
             "achieved": flops_a / t_a / 1e12, "peak": FP32_MFMA_PEAK, "unit": "TFLOP/s",
             "frac": flops_a / t_a / 1e12 / FP32_MFMA_PEAK, "traffic": PASS_A_TRAFFIC_BYTES,
             "launch_us": t_a * 1e6, "algorithmic_flops_per_launch": flops_a,
