@@ -142,3 +142,24 @@ def test_arnoldi_eigs_extremes_vs_oracle(dev):
     # values after n-2 FP32 MGS steps are not converged and differ with evaluation order
     assert abs(got[0] - ref[0]) < 2e-3 and abs(got[-1] - ref[-1]) < 2e-3
     assert abs(got[-1] - 3.0) < 1e-3 and abs(got[0] - 1.0) < 1e-3
+
+
+def test_lanczos_full_config_matches_golden(dev):
+    """Config 3a at full size (1024^2-grid Laplacian + 1e-3 I, 120 steps, seed-42 start): the
+    engine's top-32 Ritz values against the oracle's FP64 Lanczos on the same start vector
+    (tests/golden/full_config.json, made by tests/golden/make_full_config.py), 1e-5 relative."""
+    import json
+    import os
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "full_config.json")))
+    ref = np.array(gold["lanczos_laplacian_1024sq"]["top32_ritz_fp64"])
+    m = 1024
+    ip, ix, v = oop.laplacian_2d_csr(m, 0.0)
+    A = ca.Sparse.from_csr(torch.from_numpy(ip).to(dev), torch.from_numpy(ix).to(dev), torch.from_numpy(v).to(dev),
+                           (m * m, m * m))
+    op = ca.SelfAdjoint(A + 1e-3 * ca.I_like(A))
+    s0 = torch.randn(m * m, generator=torch.Generator().manual_seed(42))
+    vals, _ = eig(op, 32, "LM", HipLanczos(start_vector=s0.to(dev), max_iters=120, tol=1e-7))
+    got = np.sort(vals.cpu().double().numpy())
+    err = float(np.max(np.abs(got - ref) / np.abs(ref)))
+    print(f"top-32 Ritz values vs FP64 oracle: max rel err {err:.2e}", flush=True)
+    assert err < 1e-5

@@ -193,3 +193,24 @@ def test_diag_structural(dev):
     d = torch.arange(1., 5.)
     K = ca.Kronecker(ca.Diagonal(d.to(dev)), ca.Dense(torch.diag(d).to(dev)))
     assert torch.allclose(diag(K, 0).cpu(), torch.kron(d, d))
+
+
+def test_slq_kron64_full_config_matches_golden(dev):
+    """Config 2b at full size (64^3 Kronecker + 0.01 I, 32 Rademacher probes of key 5, 30 steps):
+    the engine's SLQ log-determinant against the oracle's FP64 value on the same probes
+    (tests/golden/full_config.json, made by tests/golden/make_full_config.py), 1e-4 relative."""
+    import json
+    import os
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "full_config.json")))["slq_kron64"]
+    g = torch.Generator().manual_seed(0)
+    facs = []
+    for _ in range(3):
+        t = torch.sort(torch.rand(64, generator=g)).values
+        facs.append(torch.exp(-(t[:, None] - t[None, :])**2 / (2 * 0.1**2)))
+    K = ca.Kronecker(*[ca.Dense(f.to(dev)) for f in facs])
+    op = ca.PSD(K + 0.01 * ca.I_like(K))
+    est = float(logdet(op, HipSLQ(num_probes=32, max_iters=30, key=5)))
+    ref = gold["logdet_fp64"]
+    err = abs(est - abs(ref)) / abs(ref)  # cola's logdet returns |tr log A| (logdet.py:111-117)
+    print(f"SLQ logdet {est:.6f} vs FP64 oracle {ref:.6f}: rel err {err:.2e}", flush=True)
+    assert err < 1e-4
