@@ -66,7 +66,9 @@ __global__ __launch_bounds__(256) void mlp_fwd_kernel(int n, int din, int dout, 
   __shared__ float h1[kRows][kPitch];
   __shared__ float h2[kRows][kPitch];
   const MlpLayout L = mlp_layout(din, dout);
-  const int r = threadIdx.x & 63, g = threadIdx.x >> 6;
+  // g is the wave index: made wave-uniform (an SGPR), so the weight loads below, whose
+  // addresses depend only on g and the loop counters, become scalar loads
+  const int r = threadIdx.x & 63, g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t row = (int64_t)blockIdx.x * kRows + r;
   const bool valid = row < n;
   for (int e = threadIdx.x; e < kRows * kMaxIn; e += 256) {
@@ -146,7 +148,7 @@ __global__ __launch_bounds__(256) void mlp_bwd_kernel(int n, int din, int dout, 
   const MlpLayout L = mlp_layout(din, dout);
   float* out = part + (int64_t)blockIdx.x * L.total;
   const int t = threadIdx.x;
-  const int r = t & 63, g = t >> 6;
+  const int r = t & 63, g = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform: scalar weight loads
   const int64_t row = (int64_t)blockIdx.x * kRows + r;
   const bool valid = row < n;
   for (int e = t; e < kRows * kMaxIn; e += 256) {
