@@ -56,7 +56,27 @@ __device__ __forceinline__ float gelu_grad(float z) {
   return fmaf(z, pdf, cdf);
 }
 
+// xs[r][d] = X[(64 b + r) ldx + d] for the block's points, zero past n and din; the
+// loads are unconditional (indexes clamped) so they issue together
+__device__ __forceinline__ void load_points(float (*xs)[kMaxIn], const float* __restrict__ X, int64_t ldx, int n,
+                                            int din, int t) {
+#pragma unroll
+  for (int k = 0; k < kRows * kMaxIn / 256; ++k) {
+    const int e = t + 256 * k, rr = e / kMaxIn, d = e % kMaxIn;
+    const int64_t i = (int64_t)blockIdx.x * kRows + rr;
+    const float v = X[min(i, (int64_t)n - 1) * ldx + min(d, din - 1)];
+    xs[rr][d] = (i < n && d < din) ? v : 0.f;
+  }
+}
+
 // ------------------------------------------------------------------ MLP forward
+// Weights of layers 2 and 3 are staged in LDS once per block and read back as
+// wave-wide broadcasts; layer 1 (din <= 8 rows) reads W1 through scalar loads.
+// Summation order per output is the input index order.
+// One launch runs a few blocks once each, so its instruction cache starts cold: the
+// activation epilogues are rolled loops over LDS and layer 3 runs in 8-column chunks,
+// which keeps the code small (its size, not the arithmetic, set the time of an
+// unrolled form).
 template <bool SAVE>
 __global__ __launch_bounds__(256) void mlp_fwd_kernel(int n, int din, int dout, const float* __restrict__ X,
                                                       int64_t ldx, const float* __restrict__ prm,
@@ -65,17 +85,23 @@ __global__ __launch_bounds__(256) void mlp_fwd_kernel(int n, int din, int dout, 
   __shared__ float xs[kRows][kMaxIn];
   __shared__ float h1[kRows][kPitch];
   __shared__ float h2[kRows][kPitch];
+  __shared__ __align__(16) float ws[kH * kMaxOut];  // W2 (kH x kH), then W3 (kH x dout), row-major
   const MlpLayout L = mlp_layout(din, dout);
-  // g is the wave index: made wave-uniform (an SGPR), so the weight loads below, whose
+  const int t = threadIdx.x;
+  // g is the wave index: made wave-uniform (an SGPR), so the W1 loads below, whose
   // addresses depend only on g and the loop counters, become scalar loads
-  const int r = threadIdx.x & 63, g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int r = t & 63, g = __builtin_amdgcn_readfirstlane(t >> 6);
   const int64_t row = (int64_t)blockIdx.x * kRows + r;
   const bool valid = row < n;
-  for (int e = threadIdx.x; e < kRows * kMaxIn; e += 256) {
-    const int rr = e / kMaxIn, d = e % kMaxIn;
-    const int64_t i = (int64_t)blockIdx.x * kRows + rr;
-    xs[rr][d] = (i < n && d < din) ? X[i * ldx + d] : 0.f;
-  }
+  // W2 and W3 start at multiples of kH floats (16-byte aligned); kH x dout is a
+  // multiple of 4 floats
+  constexpr int kQ = kH * kH / 4 / 256;  // float4 per thread for W2
+  const float4* w2g = reinterpret_cast<const float4*>(prm + L.w2);
+  const float4* w3g = reinterpret_cast<const float4*>(prm + L.w3);
+  float4* ws4 = reinterpret_cast<float4*>(ws);
+#pragma unroll
+  for (int q = 0; q < kQ; ++q) ws4[t + 256 * q] = w2g[t + 256 * q];
+  load_points(xs, X, ldx, n, din, t);
   __syncthreads();
   const int c0 = g * 32;  // each wave owns 32 hidden units
   {
@@ -89,10 +115,14 @@ __global__ __launch_bounds__(256) void mlp_fwd_kernel(int n, int din, int dout, 
       for (int cc = 0; cc < 32; ++cc) acc[cc] = fmaf(xv, w[cc], acc[cc]);
     }
 #pragma unroll
-    for (int cc = 0; cc < 32; ++cc) {
-      if (SAVE && valid) Z1[(int64_t)(c0 + cc) * ldz + row] = acc[cc];
-      h1[r][c0 + cc] = gelu(acc[cc]);
-    }
+    for (int cc = 0; cc < 32; ++cc) h1[r][c0 + cc] = acc[cc];
+  }
+  // each lane activates its own row's 32 entries: no barrier needed
+#pragma unroll 1
+  for (int cc = 0; cc < 32; ++cc) {
+    const float z = h1[r][c0 + cc];
+    if (SAVE && valid) Z1[(int64_t)(c0 + cc) * ldz + row] = z;
+    h1[r][c0 + cc] = gelu(z);
   }
   __syncthreads();
   {
@@ -101,34 +131,52 @@ __global__ __launch_bounds__(256) void mlp_fwd_kernel(int n, int din, int dout, 
     for (int cc = 0; cc < 32; ++cc) acc[cc] = prm[L.b2 + c0 + cc];
     for (int j = 0; j < kH; ++j) {
       const float hv = h1[r][j];
-      const float* w = prm + L.w2 + (int64_t)j * kH + c0;
+      const float4* w = reinterpret_cast<const float4*>(ws + j * kH + c0);
 #pragma unroll
-      for (int cc = 0; cc < 32; ++cc) acc[cc] = fmaf(hv, w[cc], acc[cc]);
+      for (int c4 = 0; c4 < 8; ++c4) {
+        const float4 wv = w[c4];
+        acc[4 * c4 + 0] = fmaf(hv, wv.x, acc[4 * c4 + 0]);
+        acc[4 * c4 + 1] = fmaf(hv, wv.y, acc[4 * c4 + 1]);
+        acc[4 * c4 + 2] = fmaf(hv, wv.z, acc[4 * c4 + 2]);
+        acc[4 * c4 + 3] = fmaf(hv, wv.w, acc[4 * c4 + 3]);
+      }
     }
 #pragma unroll
-    for (int cc = 0; cc < 32; ++cc) {
-      if (SAVE && valid) Z2[(int64_t)(c0 + cc) * ldz + row] = acc[cc];
-      h2[r][c0 + cc] = gelu(acc[cc]);
-    }
+    for (int cc = 0; cc < 32; ++cc) h2[r][c0 + cc] = acc[cc];
   }
-  __syncthreads();
-  {
-    const int per = (dout + 3) / 4;  // <= 32 outputs per wave
-    const int o0 = g * per;
-    float acc[32];
+#pragma unroll 1
+  for (int cc = 0; cc < 32; ++cc) {
+    const float z = h2[r][c0 + cc];
+    if (SAVE && valid) Z2[(int64_t)(c0 + cc) * ldz + row] = z;
+    h2[r][c0 + cc] = gelu(z);
+  }
+  __syncthreads();  // W2 reads done, h2 complete
+  const int n3 = kH * dout / 4;  // float4 of W3, <= 256 kQ
 #pragma unroll
-    for (int cc = 0; cc < 32; ++cc) acc[cc] = (cc < per && o0 + cc < dout) ? prm[L.b3 + o0 + cc] : 0.f;
+  for (int q = 0; q < kQ; ++q)
+    if (t + 256 * q < n3) ws4[t + 256 * q] = w3g[t + 256 * q];
+  __syncthreads();
+  // layer 3: wave g owns outputs o0 .. o0 + per - 1, in chunks of 8. A chunk's tail
+  // past its owned range reads other LDS words (within ws: o0 + c8 + 7 <= 4 per + 6
+  // and row 127 ends at 128 dout) into accumulators that are never stored.
+  const int per = (dout + 3) / 4;  // <= 32 outputs per wave
+  const int o0 = g * per;
+#pragma unroll 1
+  for (int c8 = 0; c8 < per; c8 += 8) {
+    float acc[8];
+#pragma unroll
+    for (int cc = 0; cc < 8; ++cc)
+      acc[cc] = (c8 + cc < per && o0 + c8 + cc < dout) ? prm[L.b3 + o0 + c8 + cc] : 0.f;
     for (int j = 0; j < kH; ++j) {
       const float hv = h2[r][j];
-      const float* w = prm + L.w3 + (int64_t)j * dout + o0;
+      const float* w = ws + j * dout + o0 + c8;
 #pragma unroll
-      for (int cc = 0; cc < 32; ++cc)
-        if (cc < per && o0 + cc < dout) acc[cc] = fmaf(hv, w[cc], acc[cc]);
+      for (int cc = 0; cc < 8; ++cc) acc[cc] = fmaf(hv, w[cc], acc[cc]);
     }
     if (valid) {
 #pragma unroll
-      for (int cc = 0; cc < 32; ++cc)
-        if (cc < per && o0 + cc < dout) G[row * gsr + (int64_t)(o0 + cc) * gsc] = acc[cc];
+      for (int cc = 0; cc < 8; ++cc)
+        if (c8 + cc < per && o0 + c8 + cc < dout) G[row * gsr + (int64_t)(o0 + c8 + cc) * gsc] = acc[cc];
     }
   }
 }
@@ -136,45 +184,98 @@ __global__ __launch_bounds__(256) void mlp_fwd_kernel(int n, int din, int dout, 
 // ----------------------------------------------------------------- MLP backward
 // Given dL/dG for a block of 64 points, writes that block's contribution to every
 // parameter gradient (flat layout) into part[blockIdx.x * total + ...].
+// W3 (row pitch dout rounded up to 4, zero-padded) and then W2 are staged in LDS;
+// dZ2 and dZ1 are accumulated in registers over 4 columns at a time, with the weight
+// rows read as wave-wide float4 broadcasts. dG and dZ2 share one LDS array (dG is
+// dead once dZ2 is in registers). The saved pre-activations are loaded into
+// registers up front (one latency); GELU and its derivative run in rolled loops over
+// LDS so the code stays small. Summation order per output is the index order.
 __global__ __launch_bounds__(256) void mlp_bwd_kernel(int n, int din, int dout, const float* __restrict__ X,
                                                       int64_t ldx, const float* __restrict__ prm,
                                                       const float* __restrict__ Z1, const float* __restrict__ Z2,
                                                       int64_t ldz, const float* __restrict__ dG, int64_t dsr,
                                                       int64_t dsc, float* __restrict__ part) {
   __shared__ float xs[kRows][kMaxIn];
-  __shared__ float a[kRows][kPitch];   // h2, then h1, then dZ1
-  __shared__ float dz[kRows][kPitch];  // dZ2
-  __shared__ float dg[kRows][kPitch];  // dG (dout <= 128)
+  __shared__ float a[kRows][kPitch];   // Z2 -> h2; Z2; Z1 -> h1; dZ1
+  __shared__ float dd[kRows][kPitch];  // dG (columns up to dp, zero beyond dout); dZ2; Z1
+  __shared__ __align__(16) float ws[kH * kMaxOut];  // W3 (kH x dp), then W2 (kH x kH)
   const MlpLayout L = mlp_layout(din, dout);
   float* out = part + (int64_t)blockIdx.x * L.total;
   const int t = threadIdx.x;
-  const int r = t & 63, g = __builtin_amdgcn_readfirstlane(t >> 6);  // wave-uniform: scalar weight loads
+  const int r = t & 63, g = __builtin_amdgcn_readfirstlane(t >> 6);
   const int64_t row = (int64_t)blockIdx.x * kRows + r;
   const bool valid = row < n;
-  for (int e = t; e < kRows * kMaxIn; e += 256) {
-    const int rr = e / kMaxIn, d = e % kMaxIn;
-    const int64_t i = (int64_t)blockIdx.x * kRows + rr;
-    xs[rr][d] = (i < n && d < din) ? X[i * ldx + d] : 0.f;
+  const int dp = (dout + 3) & ~3;
+  const int c0 = g * 32;  // this wave's 32 hidden units
+  constexpr int kQ = kH * kH / 4 / 256;  // float4 of W2 (and at most of W3) per thread
+  // every global load below is unconditional (row and column indexes clamped into
+  // range, the value selected afterwards), so each group is issued back to back and
+  // waited for once
+  const int64_t rowc = valid ? row : (int64_t)n - 1;
+  float z2[32], z1[32], gv[32];
+#pragma unroll
+  for (int jj = 0; jj < 32; ++jj) {
+    const float v2 = Z2[(int64_t)(c0 + jj) * ldz + rowc], v1 = Z1[(int64_t)(c0 + jj) * ldz + rowc];
+    z2[jj] = valid ? v2 : 0.f;
+    z1[jj] = valid ? v1 : 0.f;
   }
-  for (int c = g; c < dout; c += 4) dg[r][c] = valid ? dG[row * dsr + (int64_t)c * dsc] : 0.f;
-  for (int j = g; j < kH; j += 4) a[r][j] = valid ? gelu(Z2[(int64_t)j * ldz + row]) : 0.f;
+#pragma unroll
+  for (int i = 0; i < 32; ++i) {  // dG columns c = g + 4 i
+    const int c = g + 4 * i;
+    const float v = dG[rowc * dsr + (int64_t)min(c, dout - 1) * dsc];
+    gv[i] = (valid && c < dout) ? v : 0.f;
+  }
+  load_points(xs, X, ldx, n, din, t);
+  if (dp == dout) {  // W3 rows are 16-byte aligned in both layouts: straight float4 copy
+    const float4* w3g = reinterpret_cast<const float4*>(prm + L.w3);
+    float4* ws4 = reinterpret_cast<float4*>(ws);
+    const int n3 = kH * dout / 4;
+#pragma unroll
+    for (int q = 0; q < kQ; ++q)
+      if (t + 256 * q < n3) ws4[t + 256 * q] = w3g[t + 256 * q];
+  } else {
+    // lane r copies columns r and r + 64 of rows g, g + 4, ...; zero padding to dp
+#pragma unroll 1
+    for (int j0 = 0; j0 < kH; j0 += 32) {
+      float v0[8], v1[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        const float* w = prm + L.w3 + (int64_t)(j0 + 4 * i + g) * dout;
+        const float u0 = w[min(r, dout - 1)], u1 = w[min(r + 64, dout - 1)];
+        v0[i] = r < dout ? u0 : 0.f;
+        v1[i] = r + 64 < dout ? u1 : 0.f;
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        float* w = ws + (j0 + 4 * i + g) * dp;
+        if (r < dp) w[r] = v0[i];
+        if (r + 64 < dp) w[r + 64] = v1[i];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 32; ++i)
+    if (g + 4 * i < dp) dd[r][g + 4 * i] = gv[i];
+#pragma unroll
+  for (int jj = 0; jj < 32; ++jj) a[r][c0 + jj] = z2[jj];
+#pragma unroll 1
+  for (int jj = 0; jj < 32; ++jj) a[r][c0 + jj] = gelu(a[r][c0 + jj]);  // own row: no barrier
   __syncthreads();
 
   // dW3[j][c] = sum_r h2[r][j] dG[r][c]; db3[c] = sum_r dG[r][c]
   {
     const int j = t & 127, par = t >> 7;
     const int nq = (dout - par + 1) / 2;  // columns c = par + 2 q
+#pragma unroll 1
     for (int q0 = 0; q0 < nq; q0 += 16) {
       float acc[16];
 #pragma unroll
       for (int qq = 0; qq < 16; ++qq) acc[qq] = 0.f;
+      // columns past dout are clamped to the pad column kH; those sums are not stored
       for (int rr = 0; rr < kRows; ++rr) {
         const float av = a[rr][j];
 #pragma unroll
-        for (int qq = 0; qq < 16; ++qq) {
-          const int c = par + 2 * (q0 + qq);
-          if (q0 + qq < nq) acc[qq] = fmaf(av, dg[rr][c], acc[qq]);
-        }
+        for (int qq = 0; qq < 16; ++qq) acc[qq] = fmaf(av, dd[rr][min(par + 2 * (q0 + qq), kH)], acc[qq]);
       }
 #pragma unroll
       for (int qq = 0; qq < 16; ++qq)
@@ -182,62 +283,102 @@ __global__ __launch_bounds__(256) void mlp_bwd_kernel(int n, int din, int dout, 
     }
     for (int c = t; c < dout; c += 256) {
       float s = 0.f;
-      for (int rr = 0; rr < kRows; ++rr) s += dg[rr][c];
+      for (int rr = 0; rr < kRows; ++rr) s += dd[rr][c];
       out[L.b3 + c] = s;
     }
   }
-  // dZ2[r][j] = (sum_c dG[r][c] W3[j][c]) * gelu'(Z2[r][j])
-  for (int jj = 0; jj < 32; ++jj) {
-    const int j = g * 32 + jj;
-    const float* w = prm + L.w3 + (int64_t)j * dout;
-    float acc = 0.f;
-    for (int c = 0; c < dout; ++c) acc = fmaf(dg[r][c], w[c], acc);
-    dz[r][j] = valid ? acc * gelu_grad(Z2[(int64_t)j * ldz + row]) : 0.f;
+  // dZ2[r][j] = (sum_c dG[r][c] W3[j][c]) * gelu'(Z2[r][j]), j = c0 + jj
+  float acc[32];
+#pragma unroll
+  for (int jj = 0; jj < 32; ++jj) acc[jj] = 0.f;
+  for (int c = 0; c < dp; c += 4) {
+    const float x0 = dd[r][c], x1 = dd[r][c + 1], x2 = dd[r][c + 2], x3 = dd[r][c + 3];
+#pragma unroll
+    for (int jj = 0; jj < 32; ++jj) {
+      const float4 w = *reinterpret_cast<const float4*>(ws + (c0 + jj) * dp + c);
+      acc[jj] = fmaf(x0, w.x, acc[jj]);
+      acc[jj] = fmaf(x1, w.y, acc[jj]);
+      acc[jj] = fmaf(x2, w.z, acc[jj]);
+      acc[jj] = fmaf(x3, w.w, acc[jj]);
+    }
   }
-  __syncthreads();
-  for (int j = g; j < kH; j += 4) a[r][j] = valid ? gelu(Z1[(int64_t)j * ldz + row]) : 0.f;
+  __syncthreads();  // dG, h2 and W3 reads done
+  {  // W2 starts 16-byte aligned
+    const float4* w2g = reinterpret_cast<const float4*>(prm + L.w2);
+    float4* ws4 = reinterpret_cast<float4*>(ws);
+#pragma unroll
+    for (int q = 0; q < kQ; ++q) ws4[t + 256 * q] = w2g[t + 256 * q];
+  }
+#pragma unroll
+  for (int jj = 0; jj < 32; ++jj) {
+    dd[r][c0 + jj] = acc[jj];
+    a[r][c0 + jj] = z2[jj];
+  }
+#pragma unroll 1
+  for (int jj = 0; jj < 32; ++jj) {
+    const float v = dd[r][c0 + jj];
+    dd[r][c0 + jj] = valid ? v * gelu_grad(a[r][c0 + jj]) : 0.f;
+  }
+#pragma unroll
+  for (int jj = 0; jj < 32; ++jj) a[r][c0 + jj] = z1[jj];
+#pragma unroll 1
+  for (int jj = 0; jj < 32; ++jj) a[r][c0 + jj] = gelu(a[r][c0 + jj]);
   __syncthreads();
   // dW2[j][k] = sum_r h1[r][j] dZ2[r][k]; db2[k] = sum_r dZ2[r][k]
   {
     const int j = t & 127, par = t >> 7;
+#pragma unroll 1
     for (int q0 = 0; q0 < kH / 2; q0 += 16) {
-      float acc[16];
+      float acc2[16];
 #pragma unroll
-      for (int qq = 0; qq < 16; ++qq) acc[qq] = 0.f;
+      for (int qq = 0; qq < 16; ++qq) acc2[qq] = 0.f;
       for (int rr = 0; rr < kRows; ++rr) {
         const float av = a[rr][j];
 #pragma unroll
-        for (int qq = 0; qq < 16; ++qq) acc[qq] = fmaf(av, dz[rr][par + 2 * (q0 + qq)], acc[qq]);
+        for (int qq = 0; qq < 16; ++qq) acc2[qq] = fmaf(av, dd[rr][par + 2 * (q0 + qq)], acc2[qq]);
       }
 #pragma unroll
-      for (int qq = 0; qq < 16; ++qq) out[L.w2 + (int64_t)j * kH + par + 2 * (q0 + qq)] = acc[qq];
+      for (int qq = 0; qq < 16; ++qq) out[L.w2 + (int64_t)j * kH + par + 2 * (q0 + qq)] = acc2[qq];
     }
     for (int k = t; k < kH; k += 256) {
       float s = 0.f;
-      for (int rr = 0; rr < kRows; ++rr) s += dz[rr][k];
+      for (int rr = 0; rr < kRows; ++rr) s += dd[rr][k];
       out[L.b2 + k] = s;
     }
   }
-  // dZ1[r][j] = (sum_k dZ2[r][k] W2[j][k]) * gelu'(Z1[r][j]), kept in registers until a is free
-  float d1[32];
-  for (int jj = 0; jj < 32; ++jj) {
-    const int j = g * 32 + jj;
-    const float* w = prm + L.w2 + (int64_t)j * kH;
-    float acc = 0.f;
-    for (int k = 0; k < kH; ++k) acc = fmaf(dz[r][k], w[k], acc);
-    d1[jj] = valid ? acc * gelu_grad(Z1[(int64_t)j * ldz + row]) : 0.f;
-  }
-  __syncthreads();
+  // dZ1[r][j] = (sum_k dZ2[r][k] W2[j][k]) * gelu'(Z1[r][j]), j = c0 + jj
 #pragma unroll
-  for (int jj = 0; jj < 32; ++jj) a[r][g * 32 + jj] = d1[jj];
+  for (int jj = 0; jj < 32; ++jj) acc[jj] = 0.f;
+  for (int k = 0; k < kH; k += 4) {
+    const float x0 = dd[r][k], x1 = dd[r][k + 1], x2 = dd[r][k + 2], x3 = dd[r][k + 3];
+#pragma unroll
+    for (int jj = 0; jj < 32; ++jj) {
+      const float4 w = *reinterpret_cast<const float4*>(ws + (c0 + jj) * kH + k);
+      acc[jj] = fmaf(x0, w.x, acc[jj]);
+      acc[jj] = fmaf(x1, w.y, acc[jj]);
+      acc[jj] = fmaf(x2, w.z, acc[jj]);
+      acc[jj] = fmaf(x3, w.w, acc[jj]);
+    }
+  }
+  __syncthreads();  // h1 and dZ2 reads done
+#pragma unroll
+  for (int jj = 0; jj < 32; ++jj) {
+    a[r][c0 + jj] = acc[jj];
+    dd[r][c0 + jj] = z1[jj];
+  }
+#pragma unroll 1
+  for (int jj = 0; jj < 32; ++jj) {
+    const float v = a[r][c0 + jj];
+    a[r][c0 + jj] = valid ? v * gelu_grad(dd[r][c0 + jj]) : 0.f;
+  }
   __syncthreads();
   // dW1[d][j] = sum_r x[r][d] dZ1[r][j]; db1[j] = sum_r dZ1[r][j]
   {
     const int j = t & 127, par = t >> 7;
     for (int d = par; d < din; d += 2) {
-      float acc = 0.f;
-      for (int rr = 0; rr < kRows; ++rr) acc = fmaf(xs[rr][d], a[rr][j], acc);
-      out[L.w1 + (int64_t)d * kH + j] = acc;
+      float s = 0.f;
+      for (int rr = 0; rr < kRows; ++rr) s = fmaf(xs[rr][d], a[rr][j], s);
+      out[L.w1 + (int64_t)d * kH + j] = s;
     }
     if (t < kH) {
       float s = 0.f;
@@ -273,26 +414,39 @@ __global__ __launch_bounds__(256) void adam_kernel(int64_t count, float* __restr
 }
 
 // ------------------------------------------------------------------ gram C = X^T Y
-constexpr int kGramRows = 512;
+constexpr int kGramChunk = 64;        // rows staged per pass
+constexpr int64_t kGramMaxParts = 1024;  // partials per output entry, at most
+// rows per partial: one 64-row chunk per block until n needs more than kGramMaxParts
+inline int64_t gram_rows(int64_t n) { return kGramChunk * cdiv(cdiv(n, kGramChunk), kGramMaxParts); }
 
-__global__ __launch_bounds__(256) void gram_partials(int64_t n, int ka, int kb, const float* __restrict__ X,
-                                                     int64_t xsr, int64_t xsc, const float* __restrict__ Y,
-                                                     int64_t ysr, int64_t ysc, double* __restrict__ part) {
+__global__ __launch_bounds__(256) void gram_partials(int64_t n, int64_t rows, int ka, int kb,
+                                                     const float* __restrict__ X, int64_t xsr, int64_t xsc,
+                                                     const float* __restrict__ Y, int64_t ysr, int64_t ysc,
+                                                     double* __restrict__ part) {
   __shared__ float xs[64][33];
   __shared__ float ys[64][33];
   const int ta = blockIdx.y, tb = blockIdx.z;
   const int al = threadIdx.x >> 3, bq = (threadIdx.x & 7) * 4;
+  const int64_t r0 = (int64_t)blockIdx.x * rows;
+  const int64_t r1 = r0 + rows < n ? r0 + rows : n;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  const int64_t r0 = (int64_t)blockIdx.x * kGramRows;
-  const int64_t r1 = r0 + kGramRows < n ? r0 + kGramRows : n;
-  for (int64_t rb = r0; rb < r1; rb += 64) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < 64 * 32; e += 256) {
-      const int rr = e & 63, cc = e >> 6;
-      const int64_t i = rb + rr;
-      const int ai = ta * 32 + cc, bi = tb * 32 + cc;
-      xs[rr][cc] = (i < r1 && ai < ka) ? X[i * xsr + (int64_t)ai * xsc] : 0.f;
-      ys[rr][cc] = (i < r1 && bi < kb) ? Y[i * ysr + (int64_t)bi * ysc] : 0.f;
+  for (int64_t rb = r0; rb < r1; rb += kGramChunk) {
+    // loads unconditional (indexes clamped) so a chunk's loads issue together
+    float vx[8], vy[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = threadIdx.x + 256 * q, rr = e & 63, cc = e >> 6;
+      const int64_t i = rb + rr < r1 ? rb + rr : r1 - 1;
+      vx[q] = X[i * xsr + (int64_t)min(ta * 32 + cc, ka - 1) * xsc];
+      vy[q] = Y[i * ysr + (int64_t)min(tb * 32 + cc, kb - 1) * ysc];
+    }
+    __syncthreads();  // previous chunk's reads done
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = threadIdx.x + 256 * q, rr = e & 63, cc = e >> 6;
+      const bool in = rb + rr < r1;
+      xs[rr][cc] = (in && ta * 32 + cc < ka) ? vx[q] : 0.f;
+      ys[rr][cc] = (in && tb * 32 + cc < kb) ? vy[q] : 0.f;
     }
     __syncthreads();
     for (int rr = 0; rr < 64; ++rr) {
@@ -330,16 +484,38 @@ __global__ __launch_bounds__(256) void rows_gemm_kernel(int64_t n, int kin, int 
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int c0 = blockIdx.y * 32;
   const int nc = kout - c0 < 32 ? kout - c0 : 32;
-  for (int e = threadIdx.x; e < kin * 32; e += 256) {
-    const int j = e / 32, cc = e % 32;
-    ws[e] = cc < nc ? W[(int64_t)j * kout + c0 + cc] : 0.f;
+  // W's column slab in groups of eight loads per thread (indexes clamped, so a group
+  // issues together)
+  for (int e0 = 0; e0 < kin * 32; e0 += 2048) {
+    float v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = min(e0 + (int)threadIdx.x + 256 * q, kin * 32 - 1), j = e / 32, cc = e % 32;
+      v[q] = W[(int64_t)j * kout + c0 + min(cc, nc - 1)];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = e0 + threadIdx.x + 256 * q;
+      if (e < kin * 32) ws[e] = e % 32 < nc ? v[q] : 0.f;
+    }
   }
   __syncthreads();
   if (i >= n) return;
   float acc[32];
 #pragma unroll
   for (int cc = 0; cc < 32; ++cc) acc[cc] = 0.f;
-  for (int j = 0; j < kin; ++j) {
+  // X row entries eight at a time so their loads are in flight together
+  int j = 0;
+  for (; j + 8 <= kin; j += 8) {
+    float xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) xv[u] = X[i * xsr + (int64_t)(j + u) * xsc];
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+#pragma unroll
+      for (int cc = 0; cc < 32; ++cc) acc[cc] = fmaf(xv[u], ws[(j + u) * 32 + cc], acc[cc]);
+  }
+  for (; j < kin; ++j) {
     const float xv = X[i * xsr + (int64_t)j * xsc];
 #pragma unroll
     for (int cc = 0; cc < 32; ++cc) acc[cc] = fmaf(xv, ws[j * 32 + cc], acc[cc]);
@@ -369,73 +545,147 @@ __global__ __launch_bounds__(256) void spd_inv_kernel(int k, const double* __res
                                                       float* __restrict__ linvT32, int* __restrict__ status) {
   __shared__ double Ls[kPack];
   __shared__ double Xs[kPack];
+  __shared__ double dgl[kSpdMax];  // Cholesky diagonal, kept apart until the factorization ends
   __shared__ double shift;
-  __shared__ int bad;
   const int b = blockIdx.x, t = threadIdx.x;
+  const int kk = k * k;
   const double* Ab = A + (int64_t)b * sa;
-  for (int e = t; e < k * k; e += 256) {
-    const int i = e / k, j = e % k;
-    if (j <= i) Ls[pk(i, j)] = Ab[(int64_t)i * lda + j];
+  // lower triangle in; loads are unconditional (index clamped) so each group of eight
+  // issues together
+  for (int e0 = 0; e0 < kk; e0 += 2048) {
+    double v[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = min(e0 + t + 256 * q, kk - 1), i = e / k, j = e % k;
+      v[q] = Ab[(int64_t)i * lda + j];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = e0 + t + 256 * q, i = e / k, j = e % k;
+      if (e < kk && j <= i) Ls[pk(i, j)] = v[q];
+    }
   }
+  for (int e = t; e < k * (k + 1) / 2; e += 256) Xs[e] = 0.0;
   __syncthreads();
   if (t == 0) {
     double tr = 0.0;
     for (int i = 0; i < k; ++i) tr += Ls[pk(i, i)];
     shift = jitter_abs + jitter_rel * tr / k;
-    bad = 0;
   }
   __syncthreads();
   if (t < k) Ls[pk(t, t)] += shift;
+  // this thread's entries of the packed lower triangle, e = t + 256 m, as (row, column);
+  // the per-step updates below touch only these, unrolled so their LDS accesses issue
+  // back to back
+  constexpr int kM = (kPack + 255) / 256;
+  const int np = k * (k + 1) / 2;
+  int er[kM], ec[kM], rb[kM], cb[kM];  // row, column, and the packed offsets of their rows
+#pragma unroll
+  for (int m = 0; m < kM; ++m) {
+    const int e = t + 256 * m;
+    int i = (int)((sqrtf(8.f * (float)e + 1.f) - 1.f) * 0.5f);
+    while (i * (i + 1) / 2 > e) --i;
+    while ((i + 1) * (i + 2) / 2 <= e) ++i;
+    er[m] = e < np ? i : -1;
+    ec[m] = e < np ? e - i * (i + 1) / 2 : kSpdMax;
+    rb[m] = e < np ? i * (i + 1) / 2 : 0;
+    cb[m] = e < np ? ec[m] * (ec[m] + 1) / 2 : 0;
+  }
+  // right-looking Cholesky, two barriers per column: every thread reads the pivot
+  // and takes its square root itself (the same value everywhere, so the break on a
+  // non-positive pivot is block-uniform); thread 0 records it in dgl
+  int bad = 0;
   for (int j = 0; j < k; ++j) {
     __syncthreads();
-    if (t == 0) {
-      const double v = Ls[pk(j, j)];
-      if (!(v > 0.0)) {
-        bad = j + 1;
-      } else {
-        Ls[pk(j, j)] = sqrt(v);
-      }
+    const double v = Ls[pk(j, j)];
+    if (!(v > 0.0)) {
+      bad = j + 1;
+      break;
     }
-    __syncthreads();
-    if (bad) break;
-    const double djj = Ls[pk(j, j)];
+    const double djj = sqrt(v);
+    if (t == 0) dgl[j] = djj;
     for (int i = j + 1 + t; i < k; i += 256) Ls[pk(i, j)] /= djj;
     __syncthreads();
-    // trailing update of the lower triangle: row i = j+1 + t/4, every 4th column from j+1+t%4
-    {
-      const int i = j + 1 + (t >> 2);
-      if (i < k) {
-        const double lij = Ls[pk(i, j)];
-        const int base = i * (i + 1) / 2;
-        for (int l = j + 1 + (t & 3); l <= i; l += 4) Ls[base + l] -= lij * Ls[pk(l, j)];
-      }
+    // trailing update of the lower triangle: entries (i, l) with j < l <= i. It reads
+    // column j and writes only columns > j, so all reads go first (unconditional: the
+    // offsets stay inside the triangle) and the writes after
+    double lv[kM], rv[kM], ov[kM];
+#pragma unroll
+    for (int m = 0; m < kM; ++m) {
+      lv[m] = Ls[rb[m] + min(j, er[m] < 0 ? 0 : er[m])];
+      rv[m] = Ls[cb[m] + (ec[m] < kSpdMax ? min(j, ec[m]) : 0)];
+      ov[m] = Ls[min(t + 256 * m, kPack - 1)];
     }
+#pragma unroll
+    for (int m = 0; m < kM; ++m)
+      if (ec[m] > j && ec[m] < kSpdMax) Ls[t + 256 * m] = ov[m] - lv[m] * rv[m];
   }
-  __syncthreads();
   if (t == 0) status[b] = bad;
   if (bad) return;
-  // column c of X = L^-1 by forward substitution (one thread per column)
-  if (t < k) {
-    const int c = t;
-    Xs[pk(c, c)] = 1.0 / Ls[pk(c, c)];
-    for (int i = c + 1; i < k; ++i) {
-      double acc = 0.0;
-      for (int q = c; q < i; ++q) acc = fma(Ls[pk(i, q)], Xs[pk(q, c)], acc);
-      Xs[pk(i, c)] = -acc / Ls[pk(i, i)];
+  __syncthreads();
+  if (t < k) Ls[pk(t, t)] = dgl[t];
+  // X = L^-1, one row per step: row q is final once the steps before it have added
+  // their terms, X[q][c] = -(sum_{c<=p<q} L[q][p] X[p][c]) / L[q][q]; the step then adds
+  // L[i][q] X[q][c] to every later row i. Each sum takes its terms in increasing p,
+  // the order of a column-by-column forward substitution.
+  for (int q = 0; q < k; ++q) {
+    __syncthreads();
+    if (t <= q) Xs[pk(q, t)] = t == q ? 1.0 / Ls[pk(q, q)] : -Xs[pk(q, t)] / Ls[pk(q, q)];
+    __syncthreads();
+    // rows > q are written, row q and L are read: reads first, then writes
+    const int qb = q * (q + 1) / 2;
+    double lv[kM], xv[kM], ov[kM];
+#pragma unroll
+    for (int m = 0; m < kM; ++m) {
+      lv[m] = Ls[rb[m] + min(q, er[m] < 0 ? 0 : er[m])];
+      xv[m] = Xs[qb + min(ec[m], q)];
+      ov[m] = Xs[min(t + 256 * m, kPack - 1)];
     }
+#pragma unroll
+    for (int m = 0; m < kM; ++m)
+      if (er[m] > q && ec[m] <= q) Xs[t + 256 * m] = fma(lv[m], xv[m], ov[m]);
   }
   __syncthreads();
-  double* Li = linv + (int64_t)b * k * k;
-  for (int e = t; e < k * k; e += 256) {
-    const int r = e / k, c = e % k;
-    Li[e] = c <= r ? Xs[pk(r, c)] : 0.0;
-    if (linvT32) linvT32[(int64_t)b * k * k + e] = r <= c ? (float)Xs[pk(c, r)] : 0.f;
-    if (inv64 || inv32) {
-      const int c0 = r > c ? r : c;
-      double acc = 0.0;
-      for (int q = c0; q < k; ++q) acc = fma(Xs[pk(q, r)], Xs[pk(q, c)], acc);
-      if (inv64) inv64[(int64_t)b * k * k + e] = acc;
-      if (inv32) inv32[(int64_t)b * k * k + e] = (float)acc;
+  double* Li = linv + (int64_t)b * kk;
+  for (int e = t; e < kk; e += 256) {
+    const int r = e / k, cc = e % k;
+    Li[e] = cc <= r ? Xs[pk(r, cc)] : 0.0;
+    if (linvT32) linvT32[(int64_t)b * kk + e] = r <= cc ? (float)Xs[pk(cc, r)] : 0.f;
+  }
+  if (inv64 || inv32) {
+    // S^-1 = X^T X: thread owns the 4 x 4 block at rows r0.., columns c0..; entry (r, c)
+    // sums X[q][r] X[q][c] over q >= max(r, c) in increasing q (the block's earlier q
+    // add exact zeros, as X[q][r] = 0 for r > q)
+    const int r0 = 4 * (t >> 4), c0 = 4 * (t & 15);
+    if (r0 < k && c0 < k) {
+      double acc[4][4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int w = 0; w < 4; ++w) acc[u][w] = 0.0;
+#pragma unroll 2
+      for (int q = r0 > c0 ? r0 : c0; q < k; ++q) {
+        double xr[4], xc[4];
+        const int base = q * (q + 1) / 2;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          xr[u] = r0 + u <= q ? Xs[base + r0 + u] : 0.0;
+          xc[u] = c0 + u <= q ? Xs[base + c0 + u] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int w = 0; w < 4; ++w) acc[u][w] = fma(xr[u], xc[w], acc[u][w]);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+          if (r0 + u >= k || c0 + w >= k) continue;
+          const int64_t e = (int64_t)b * kk + (r0 + u) * k + c0 + w;
+          if (inv64) inv64[e] = acc[u][w];
+          if (inv32) inv32[e] = (float)acc[u][w];
+        }
     }
   }
 }
@@ -452,34 +702,53 @@ __global__ __launch_bounds__(256) void energy_kernel(int k, int m, const double*
                                                      float* __restrict__ C1, float* __restrict__ C2,
                                                      double* __restrict__ loss) {
   __shared__ double Ts[kSpdMax * kSpdMax];
-  __shared__ double Sp[kPack];
+  __shared__ double Ss[kSpdMax * kSpdMax];  // S^-1 rebuilt symmetric from its lower triangle
   __shared__ double ws[kSpdMax];
   __shared__ double red[4];
   const int t = threadIdx.x;
-  for (int e = t; e < m * k; e += 256) Ts[e] = T[e];
-  for (int e = t; e < k * k; e += 256) {
-    const int i = e / k, j = e % k;
-    if (j <= i) Sp[pk(i, j)] = Sinv[e];
+  // loads in groups of eight per thread, indexes clamped so each group issues together
+  for (int e0 = 0; e0 < kSpdMax * kSpdMax; e0 += 2048) {
+    double vt[8], vs[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = e0 + t + 256 * q;
+      const int et = min(e, m * k - 1), es = min(e, k * k - 1), i = es / k, j = es % k;
+      vt[q] = T[et];
+      vs[q] = Sinv[i >= j ? es : j * k + i];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int e = e0 + t + 256 * q;
+      if (e < m * k) Ts[e] = vt[q];
+      if (e < k * k) Ss[e] = vs[q];
+    }
   }
-  for (int i = t; i < m; i += 256) ws[i] = w[i];
+  if (t < m) ws[t] = w[t];
   __syncthreads();
   double wsum = 0.0;
   for (int i = 0; i < m; ++i) wsum += ws[i];
   const double c = 2.0 / wsum;
+  // P[i][a] = sum_q T[i][q] S^-1[q][a] for a = g + 4 u: sixteen independent sums per
+  // thread, each over q in increasing order
   const int i = t >> 2, g = t & 3;
   double pv[kSpdMax / 4];
   double f = 0.0;
   if (i < m) {
 #pragma unroll
+    for (int u = 0; u < kSpdMax / 4; ++u) pv[u] = 0.0;
+    for (int q = 0; q < k; ++q) {
+      const double tq = Ts[i * k + q];
+      const double* srow = Ss + q * k;
+#pragma unroll
+      for (int u = 0; u < kSpdMax / 4; ++u)
+        if (g + 4 * u < k) pv[u] = fma(tq, srow[g + 4 * u], pv[u]);
+    }
+#pragma unroll
     for (int u = 0; u < kSpdMax / 4; ++u) {
       const int a = g + 4 * u;
-      pv[u] = 0.0;
       if (a < k) {
-        double acc = 0.0;
-        for (int q = 0; q < k; ++q) acc = fma(Ts[i * k + q], Sp[q >= a ? pk(q, a) : pk(a, q)], acc);
-        pv[u] = acc;
-        C1[i * k + a] = (float)(-c * ws[i] * acc);
-        f = fma(ws[i] * acc, Ts[i * k + a], f);
+        C1[i * k + a] = (float)(-c * ws[i] * pv[u]);
+        f = fma(ws[i] * pv[u], Ts[i * k + a], f);
       }
     }
   }
@@ -492,11 +761,32 @@ __global__ __launch_bounds__(256) void energy_kernel(int k, int m, const double*
     }
   }
   __syncthreads();
-  for (int e = t; e < k * k; e += 256) {
-    const int a = e / k, bb = e % k;
-    double acc = 0.0;
-    for (int r = 0; r < m; ++r) acc = fma(ws[r] * Ts[r * k + a], Ts[r * k + bb], acc);
-    C2[e] = (float)(c * acc);
+  // C2[a][b] = c sum_r (w_r P[r][a]) P[r][b]: thread owns the 4 x 4 block at (a0, b0)
+  const int a0 = 4 * (t >> 4), b0 = 4 * (t & 15);
+  if (a0 < k && b0 < k) {
+    double acc[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+    for (int r = 0; r < m; ++r) {
+      const double* prow = Ts + r * k;
+      double wa[4], pb[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        wa[u] = ws[r] * prow[min(a0 + u, k - 1)];
+        pb[u] = prow[min(b0 + u, k - 1)];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = fma(wa[u], pb[v], acc[u][v]);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (a0 + u < k && b0 + v < k) C2[(a0 + u) * k + b0 + v] = (float)(c * acc[u][v]);
   }
   if (t == 0) loss[0] = 1.0 - f / wsum;
 }
@@ -717,6 +1007,7 @@ int ck_mlp_forward(int64_t n, int32_t din, int32_t dout, const float* X, int64_t
       ldx < din)
     return CK_ERR_BAD_ARG;
   if ((Z1 == nullptr) != (Z2 == nullptr) || (Z1 && ldz < n)) return CK_ERR_BAD_ARG;
+  if (reinterpret_cast<uintptr_t>(params) % 16) return CK_ERR_BAD_ARG;  // weights are staged as float4
   const unsigned grid = (unsigned)cdiv(n, kRows);
   hipStream_t s = (hipStream_t)stream;
   if (Z1)
@@ -735,6 +1026,7 @@ int ck_mlp_backward(int64_t n, int32_t din, int32_t dout, const float* X, int64_
   if (n < 1 || n > 0x7FFFFFFF || din < 1 || din > kMaxIn || dout < 1 || dout > kMaxOut || !X || !params || !Z1 ||
       !Z2 || !dG || !grad || ldx < din || ldz < n)
     return CK_ERR_BAD_ARG;
+  if (reinterpret_cast<uintptr_t>(params) % 16) return CK_ERR_BAD_ARG;  // weights are staged as float4
   if (!work || work_bytes < ck_mlp_grad_workspace_bytes(n, din, dout)) return CK_ERR_WORKSPACE;
   const int nblk = (int)cdiv(n, kRows);
   const int64_t total = mlp_layout(din, dout).total;
@@ -761,18 +1053,19 @@ int ck_adam_step(int64_t count, float* params, const float* grad, float* m, floa
 
 size_t ck_gram_workspace_bytes(int64_t n, int32_t ka, int32_t kb) {
   if (n < 1 || ka < 1 || kb < 1) return 0;
-  return sizeof(double) * (size_t)cdiv(n, kGramRows) * (size_t)ka * (size_t)kb;
+  return sizeof(double) * (size_t)cdiv(n, gram_rows(n)) * (size_t)ka * (size_t)kb;
 }
 
 int ck_gram_f64(int64_t n, int32_t ka, int32_t kb, const float* X, int64_t xsr, int64_t xsc, const float* Y,
                 int64_t ysr, int64_t ysc, double* C, void* work, size_t work_bytes, void* stream) {
   if (n < 1 || ka < 1 || kb < 1 || !X || !Y || !C) return CK_ERR_BAD_ARG;
   if (!work || work_bytes < ck_gram_workspace_bytes(n, ka, kb)) return CK_ERR_WORKSPACE;
-  const int nch = (int)cdiv(n, kGramRows);
+  const int64_t rows = gram_rows(n);
+  const int nch = (int)cdiv(n, rows);
   hipStream_t s = (hipStream_t)stream;
   double* part = static_cast<double*>(work);
-  hipLaunchKernelGGL(gram_partials, dim3(nch, (unsigned)cdiv(ka, 32), (unsigned)cdiv(kb, 32)), dim3(256), 0, s, n, ka,
-                     kb, X, xsr, xsc, Y, ysr, ysc, part);
+  hipLaunchKernelGGL(gram_partials, dim3(nch, (unsigned)cdiv(ka, 32), (unsigned)cdiv(kb, 32)), dim3(256), 0, s, n,
+                     rows, ka, kb, X, xsr, xsc, Y, ysr, ysc, part);
   const int64_t count = (int64_t)ka * kb;
   hipLaunchKernelGGL(gram_reduce, dim3((unsigned)cdiv(count, 256)), dim3(256), 0, s, nch, count, (const double*)part,
                      C);

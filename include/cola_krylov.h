@@ -400,7 +400,8 @@ size_t ck_trace_workspace_bytes(int64_t n);
  *
  * Per-point MLP din -> 128 -> 128 -> dout with exact (erf) GELU after the two hidden
  * layers.  Parameters are one flat fp32 array: W1 (din x 128, row-major, y = x W + b),
- * b1, W2 (128 x 128), b2, W3 (128 x dout), b3.  din <= 8, dout <= 128.
+ * b1, W2 (128 x 128), b2, W3 (128 x dout), b3.  din <= 8, dout <= 128.  params must
+ * be 16-byte aligned (CK_ERR_BAD_ARG otherwise).
  * Points X are row-major (n x din, leading dimension ldx).  Output G[i*gsr + c*gsc].
  * With Z1, Z2 non-NULL the pre-activations are saved feature-major (Z[j*ldz + i]) for
  * ck_mlp_backward. */

@@ -74,7 +74,9 @@ def test_op_workspace_queries(lib):
     assert N.q("ck_mlp_param_count", 4, 64) == 4 * 128 + 128 + 128 * 128 + 128 + 128 * 64 + 64
     assert N.q("ck_mlp_param_count", 9, 64) == -1
     assert N.q("ck_mlp_grad_workspace_bytes", 16384, 4, 64) == 4 * 256 * 25408
-    assert N.q("ck_gram_workspace_bytes", 1025, 64, 32) == 8 * 3 * 64 * 32
+    # one FP64 partial per 64-row chunk, at most 1024 partials (rows per partial grow past that)
+    assert N.q("ck_gram_workspace_bytes", 1025, 64, 32) == 8 * 17 * 64 * 32
+    assert N.q("ck_gram_workspace_bytes", 64 * 3072, 64, 32) == 8 * 1024 * 64 * 32
 
 
 def test_bad_arguments_rejected_without_launch(lib):
