@@ -139,18 +139,55 @@ __device__ __forceinline__ double trace_row(int row, const int32_t* __restrict__
 }
 
 // One partial per 1024 rows: a thread adds its four rows' sums (j ascending), then the block
-// sum (fixed order), so the stored-word and seeded passes give the same bits.
+// sum (fixed order), so the stored-word and seeded passes give the same bits.  With one
+// word group and four short rows (the common case) the four rows' extents are loaded
+// together and then all their entries in one batch (indexes clamped into [0, nnz), the
+// values selected afterwards), so a thread waits for two loads rather than eight; each
+// row's sum is formed as in trace_row.
 template <typename Words>
-__global__ __launch_bounds__(256) void trace_partials(int n, const int32_t* __restrict__ rowptr,
+__global__ __launch_bounds__(256) void trace_partials(int n, int64_t nnz, const int32_t* __restrict__ rowptr,
                                                       const int32_t* __restrict__ colind,
                                                       const float* __restrict__ val, int ngroups, Words words,
                                                       double* __restrict__ part) {
   __shared__ double red[4];
+  constexpr int R = kTraceRowsPerThread;
   double acc = 0.0;
+  int row[R], k0[R], len[R];
+  bool all_short = ngroups == 1;
 #pragma unroll
-  for (int j = 0; j < kTraceRowsPerThread; ++j) {
-    const int row = blockIdx.x * (256 * kTraceRowsPerThread) + j * 256 + threadIdx.x;
-    if (row < n) acc += trace_row(row, rowptr, colind, val, ngroups, words);
+  for (int j = 0; j < R; ++j) {
+    row[j] = blockIdx.x * (256 * R) + j * 256 + threadIdx.x;
+    const int rc = row[j] < n ? row[j] : n - 1;
+    k0[j] = rowptr[rc];
+    len[j] = rowptr[rc + 1] - k0[j];
+    all_short = all_short && (row[j] >= n || len[j] <= kBitsShort);
+  }
+  if (all_short && nnz > 0) {
+    const int64_t last = nnz - 1;
+    int c[R][kBitsShort];
+    float v[R][kBitsShort];
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+#pragma unroll
+      for (int i = 0; i < kBitsShort; ++i) {
+        const int64_t k = (int64_t)k0[j] + i < last ? (int64_t)k0[j] + i : last;
+        c[j][i] = colind[k];
+        v[j][i] = val[k];
+      }
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+      if (row[j] >= n) continue;
+      const uint64_t wr = words(0, row[j]);
+      double s = 0.0;
+#pragma unroll
+      for (int i = 0; i < kBitsShort; ++i)
+        if (i < len[j]) s += (double)v[j][i] * (double)(64 - 2 * (int)__popcll(wr ^ words(0, c[j][i])));
+      acc += s;
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < R; ++j)
+      if (row[j] < n) acc += trace_row(row[j], rowptr, colind, val, ngroups, words);
   }
   acc = block_sum<256>(acc, red);
   if (threadIdx.x == 0) part[blockIdx.x] = acc;
@@ -310,8 +347,8 @@ int ck_trace_csr_rademacher_bits(const ck_op* A, int32_t ngroups, const uint64_t
   hipStream_t s = (hipStream_t)stream;
   const int nparts = (int)cdiv(A->n, 256 * kTraceRowsPerThread);
   double* part = static_cast<double*>(work);
-  hipLaunchKernelGGL(trace_partials<StoredWords>, dim3(nparts), dim3(256), 0, s, A->n, A->rowptr, A->colind,
-                     A->values, ngroups, StoredWords{W, (int64_t)A->n}, part);
+  hipLaunchKernelGGL(trace_partials<StoredWords>, dim3(nparts), dim3(256), 0, s, A->n, A->nnz, A->rowptr,
+                     A->colind, A->values, ngroups, StoredWords{W, (int64_t)A->n}, part);
   hipLaunchKernelGGL(trace_bits_final, dim3(1), dim3(256), 0, s, nparts, (const double*)part, (double)A->scale,
                      (double)A->shift * (double)A->n, 1.0 / (64.0 * ngroups), out);
   CK_CHECK_LAUNCH();
@@ -326,8 +363,8 @@ int ck_trace_csr_rademacher_seeded(const ck_op* A, int32_t ngroups, uint64_t see
   hipStream_t s = (hipStream_t)stream;
   const int nparts = (int)cdiv(A->n, 256 * kTraceRowsPerThread);
   double* part = static_cast<double*>(work);
-  hipLaunchKernelGGL(trace_partials<SeededWords>, dim3(nparts), dim3(256), 0, s, A->n, A->rowptr, A->colind,
-                     A->values, ngroups, SeededWords{seed}, part);
+  hipLaunchKernelGGL(trace_partials<SeededWords>, dim3(nparts), dim3(256), 0, s, A->n, A->nnz, A->rowptr,
+                     A->colind, A->values, ngroups, SeededWords{seed}, part);
   hipLaunchKernelGGL(trace_bits_final, dim3(1), dim3(256), 0, s, nparts, (const double*)part, (double)A->scale,
                      (double)A->shift * (double)A->n, 1.0 / (64.0 * ngroups), out);
   CK_CHECK_LAUNCH();

@@ -45,8 +45,8 @@ const char* ck_status_string(int status);
  *   CK_OP_DENSE_F32  S = A, row-major fp32 with leading dimension lda
  *                    (replaces Dense._matmat, cola/ops/operators.py:26-28)
  *   CK_OP_DENSE_BF16 S = A stored as bf16 (uint16 bit patterns), fp32 accumulate
- *   CK_OP_CSR_F32    S = CSR(rowptr, colind, values), int32 indices
- *                    (replaces Sparse._matmat, cola/ops/operators.py:64-78)
+ *   CK_OP_CSR_F32    S = CSR(rowptr, colind, values), int32 indices, nnz = rowptr[n]
+ *                    stored entries (replaces Sparse._matmat, cola/ops/operators.py:64-78)
  *   CK_OP_KRON_F32   S = F0 (x) F1 (x) ... , 2 or 3 square row-major fp32 factors,
  *                    each of order <= 64 (replaces Kronecker._matmat,
  *                    cola/ops/operators.py:216-223)
