@@ -335,9 +335,11 @@ def bench_lanczos(args, dev):
     dt = (time.perf_counter() - t0) / reps
     exact = laplacian_eigs(1024, 1e-3)[-32:]
     err = float(np.abs(np.sort(vals.cpu().double().numpy()) - exact).max())
-    # algorithmic bytes: SpMV (values, columns, rowptr, x, y) + 2 CGS passes x (project + apply) over i vectors
+    # algorithmic bytes: SpMV (values, columns, rowptr, x, y) + the i basis vectors read three
+    # times per step (project; the fused apply + second project, whose second read is served
+    # by the caches; apply) + the w traffic
     spmv = 4 * (2 * 5238784 + (n + 1) + 2 * n)
-    reorth = sum(2 * 2 * i * 4 * n + 2 * 2 * 4 * n for i in range(1, 121))
+    reorth = sum(3 * i * 4 * n + 2 * 2 * 4 * n for i in range(1, 121))
     total = 120 * spmv + reorth
     return {"metric": "Lanczos eigensolves/sec (config 3a: 2-D Laplacian 1024^2, 120 steps, CGS2, top-32)",
             "value": 1 / dt, "unit": "eigensolves/s", "ms_per_step": dt * 1e3, "dtype": "f32 (+f64 tridiagonal)",

@@ -136,6 +136,104 @@ __device__ __forceinline__ float4 load4_masked(const float* __restrict__ p, int6
   return r;
 }
 
+// hpart[j * P + blk] = V[:, j]^T w over this block's 1024-element tile, j0 <= j <= j1, with
+// the tile of w staged in wl.  Each lane owns 4 consecutive elements of each 256-element
+// quarter; two basis vectors (j, j+4) per pass keep eight 16-byte loads in flight per lane.
+// REV walks the columns from j1 down (the fused kernel: the columns its update read last
+// are the likeliest still in L2); each column's sum is the same either way.
+template <bool REV = false>
+__device__ __forceinline__ void project_tile_vec(const float* __restrict__ Vb, int64_t ldv, int64_t n, int64_t i0,
+                                                 int j0, int j1, const float* wl, float* __restrict__ hpart, int P,
+                                                 int blk) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const float4* wl4 = reinterpret_cast<const float4*>(wl);
+  float4 wv[kTile / 256];
+#pragma unroll
+  for (int q = 0; q < kTile / 256; ++q) wv[q] = wl4[q * 64 + lane];
+  for (int jj = j0 + wave; jj <= j1; jj += 8) {
+    // column pair (j, j + 4), or (j1 + j0 - jj, that - 4) walking down
+    const int j = REV ? j1 + j0 - jj : jj;
+    const int j2 = REV ? j - 4 : j + 4;
+    const bool two = jj + 4 <= j1;
+    const float* v0 = Vb + (int64_t)j * ldv;
+    const float* v1 = Vb + (int64_t)j2 * ldv;
+    float a0 = 0.f, a1 = 0.f;
+#pragma unroll
+    for (int q = 0; q < kTile / 256; ++q) {
+      const int64_t idx = i0 + 256 * q + 4 * lane;
+      const float4 x0 = load4_masked(v0, idx, n);
+      const float4 x1 = two ? load4_masked(v1, idx, n) : make_float4(0.f, 0.f, 0.f, 0.f);
+      a0 = fmaf(x0.x, wv[q].x, a0);
+      a0 = fmaf(x0.y, wv[q].y, a0);
+      a0 = fmaf(x0.z, wv[q].z, a0);
+      a0 = fmaf(x0.w, wv[q].w, a0);
+      a1 = fmaf(x1.x, wv[q].x, a1);
+      a1 = fmaf(x1.y, wv[q].y, a1);
+      a1 = fmaf(x1.z, wv[q].z, a1);
+      a1 = fmaf(x1.w, wv[q].w, a1);
+    }
+    a0 = wave_sum(a0);
+    a1 = wave_sum(a1);
+    if (lane == 0) {
+      hpart[(int64_t)j * P + blk] = a0;
+      if (two) hpart[(int64_t)j2 * P + blk] = a1;
+    }
+  }
+}
+
+// sum_{j0<=j<=j1} V[idx .. idx+3, j] h[j] for one thread's 4 consecutive elements; four
+// basis vectors per pass (sixteen loads in flight)
+__device__ __forceinline__ float4 apply_sum_vec(const float* __restrict__ Vb, int64_t ldv, int64_t n, int64_t idx,
+                                                int j0, int j1, const float* hs) {
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  int j = j0;
+  for (; j + 3 <= j1; j += 4) {
+    float4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = load4_masked(Vb + (int64_t)(j + u) * ldv, idx, n);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float hj = hs[j + u];
+      acc.x = fmaf(v[u].x, hj, acc.x);
+      acc.y = fmaf(v[u].y, hj, acc.y);
+      acc.z = fmaf(v[u].z, hj, acc.z);
+      acc.w = fmaf(v[u].w, hj, acc.w);
+    }
+  }
+  for (; j <= j1; ++j) {
+    const float4 v = load4_masked(Vb + (int64_t)j * ldv, idx, n);
+    const float hj = hs[j];
+    acc.x = fmaf(v.x, hj, acc.x);
+    acc.y = fmaf(v.y, hj, acc.y);
+    acc.z = fmaf(v.z, hj, acc.z);
+    acc.w = fmaf(v.w, hj, acc.w);
+  }
+  return acc;
+}
+
+// w[idx .. idx+3] -= acc (elements below n); returns the new values (zero past n)
+__device__ __forceinline__ float4 apply_store_vec(float* __restrict__ w, int64_t n, int64_t idx, float4 acc) {
+  if (idx + 3 < n) {
+    float4 wv = *reinterpret_cast<const float4*>(w + idx);
+    wv.x -= acc.x;
+    wv.y -= acc.y;
+    wv.z -= acc.z;
+    wv.w -= acc.w;
+    *reinterpret_cast<float4*>(w + idx) = wv;
+    return wv;
+  }
+  const float a4[4] = {acc.x, acc.y, acc.z, acc.w};
+  float r[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (idx + q < n) {
+      r[q] = w[idx + q] - a4[q];
+      w[idx + q] = r[q];
+    }
+  }
+  return make_float4(r[0], r[1], r[2], r[3]);
+}
+
 template <bool FIRST, bool VEC>
 __global__ __launch_bounds__(256) void lz_project(int64_t n, float* __restrict__ V, int64_t ldv, int64_t ldb, int i,
                                                   int j0, int j1, const float* __restrict__ dpart, int P,
@@ -179,38 +277,7 @@ __global__ __launch_bounds__(256) void lz_project(int64_t n, float* __restrict__
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (VEC) {
-    // each lane owns 4 consecutive elements of each 256-element quarter; two basis
-    // vectors (j, j+4) per pass keep eight 16-byte loads in flight per lane
-    const float4* wl4 = reinterpret_cast<const float4*>(wl);
-    float4 wv[kTile / 256];
-#pragma unroll
-    for (int q = 0; q < kTile / 256; ++q) wv[q] = wl4[q * 64 + lane];
-    for (int j = j0 + wave; j <= j1; j += 8) {
-      const bool two = j + 4 <= j1;
-      const float* v0 = Vb + (int64_t)j * ldv;
-      const float* v1 = Vb + (int64_t)(j + 4) * ldv;
-      float a0 = 0.f, a1 = 0.f;
-#pragma unroll
-      for (int q = 0; q < kTile / 256; ++q) {
-        const int64_t idx = i0 + 256 * q + 4 * lane;
-        const float4 x0 = load4_masked(v0, idx, n);
-        const float4 x1 = two ? load4_masked(v1, idx, n) : make_float4(0.f, 0.f, 0.f, 0.f);
-        a0 = fmaf(x0.x, wv[q].x, a0);
-        a0 = fmaf(x0.y, wv[q].y, a0);
-        a0 = fmaf(x0.z, wv[q].z, a0);
-        a0 = fmaf(x0.w, wv[q].w, a0);
-        a1 = fmaf(x1.x, wv[q].x, a1);
-        a1 = fmaf(x1.y, wv[q].y, a1);
-        a1 = fmaf(x1.z, wv[q].z, a1);
-        a1 = fmaf(x1.w, wv[q].w, a1);
-      }
-      a0 = wave_sum(a0);
-      a1 = wave_sum(a1);
-      if (lane == 0) {
-        hpart[((int64_t)b * M2 + j) * P + blk] = a0;
-        if (two) hpart[((int64_t)b * M2 + j + 4) * P + blk] = a1;
-      }
-    }
+    project_tile_vec(Vb, ldv, n, i0, j0, j1, wl, hpart + (int64_t)b * M2 * P, P, blk);
   } else {
     for (int j = j0 + wave; j <= j1; j += 4) {
       const float* vj = Vb + (int64_t)j * ldv;
@@ -254,53 +321,14 @@ __global__ __launch_bounds__(256) void lz_apply(int64_t n, float* __restrict__ V
   const int64_t i0 = (int64_t)blk * kTile;
   float ss = 0.f;
   if (VEC) {
-    // 4 consecutive elements per thread; four basis vectors per pass (sixteen loads in flight)
+    // 4 consecutive elements per thread
     const int64_t idx = i0 + 4 * threadIdx.x;
-    float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    int j = j0;
-    for (; j + 3 <= j1; j += 4) {
-      float4 v[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) v[u] = load4_masked(Vb + (int64_t)(j + u) * ldv, idx, n);
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const float hj = hs[j + u];
-        acc.x = fmaf(v[u].x, hj, acc.x);
-        acc.y = fmaf(v[u].y, hj, acc.y);
-        acc.z = fmaf(v[u].z, hj, acc.z);
-        acc.w = fmaf(v[u].w, hj, acc.w);
-      }
-    }
-    for (; j <= j1; ++j) {
-      const float4 v = load4_masked(Vb + (int64_t)j * ldv, idx, n);
-      const float hj = hs[j];
-      acc.x = fmaf(v.x, hj, acc.x);
-      acc.y = fmaf(v.y, hj, acc.y);
-      acc.z = fmaf(v.z, hj, acc.z);
-      acc.w = fmaf(v.w, hj, acc.w);
-    }
-    if (idx + 3 < n) {
-      float4 wv = *reinterpret_cast<const float4*>(w + idx);
-      wv.x -= acc.x;
-      wv.y -= acc.y;
-      wv.z -= acc.z;
-      wv.w -= acc.w;
-      *reinterpret_cast<float4*>(w + idx) = wv;
-      ss = fmaf(wv.x, wv.x, ss);
-      ss = fmaf(wv.y, wv.y, ss);
-      ss = fmaf(wv.z, wv.z, ss);
-      ss = fmaf(wv.w, wv.w, ss);
-    } else {
-      const float a4[4] = {acc.x, acc.y, acc.z, acc.w};
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (idx + q < n) {
-          const float wv = w[idx + q] - a4[q];
-          w[idx + q] = wv;
-          ss = fmaf(wv, wv, ss);
-        }
-      }
-    }
+    const float4 wv = apply_store_vec(w, n, idx, apply_sum_vec(Vb, ldv, n, idx, j0, j1, hs));
+    // elements past n are zero, so the tail adds nothing
+    ss = fmaf(wv.x, wv.x, ss);
+    ss = fmaf(wv.y, wv.y, ss);
+    ss = fmaf(wv.z, wv.z, ss);
+    ss = fmaf(wv.w, wv.w, ss);
   } else {
     float acc[4] = {0.f, 0.f, 0.f, 0.f};
     for (int j = j0; j <= j1; ++j) {
@@ -326,6 +354,30 @@ __global__ __launch_bounds__(256) void lz_apply(int64_t n, float* __restrict__ V
     ss = block_sum<256>(ss, red);
     if (threadIdx.x == 0) npart[(int64_t)b * P + blk] = ss;
   }
+}
+
+// lz_apply<false, true> then lz_project<false, true> in one launch: w -= V[:, j0..j1] h,
+// then hpart = V[:, j0..j1]^T w for the updated w (the second pass of double Gram-Schmidt
+// right after the first pass's update).  The block's updated tile of w goes to LDS, so
+// the projection's arithmetic is the unfused kernel's; its basis rows are the ones the
+// update just read, served from the caches instead of HBM.
+__global__ __launch_bounds__(256) void lz_apply_project(int64_t n, float* __restrict__ V, int64_t ldv, int64_t ldb,
+                                                        int wcol, int j0, int j1, const float* __restrict__ h,
+                                                        int M2, float* __restrict__ hpart, int P,
+                                                        const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ float hs[1056];
+  __shared__ __align__(16) float wl[kTile];
+  const int b = blockIdx.y, blk = blockIdx.x;
+  for (int j = threadIdx.x; j < M2 && j < 1056; j += 256) hs[j] = h[b * M2 + j];
+  __syncthreads();
+  float* Vb = V + b * ldb;
+  float* w = Vb + (int64_t)wcol * ldv;
+  const int64_t i0 = (int64_t)blk * kTile;
+  const int64_t idx = i0 + 4 * threadIdx.x;
+  reinterpret_cast<float4*>(wl)[threadIdx.x] = apply_store_vec(w, n, idx, apply_sum_vec(Vb, ldv, n, idx, j0, j1, hs));
+  __syncthreads();
+  project_tile_vec<true>(Vb, ldv, n, i0, j0, j1, wl, hpart + (int64_t)b * M2 * P, P, blk);
 }
 
 // beta_i = ||w||, errors and the loop condition (lanczos.py:251-268)
@@ -533,10 +585,14 @@ int lz_post_full(int64_t n, int nb, float* V, int64_t ldv, int64_t ldb, float* d
                      sc.M2, diag, subdiag, m, done);
   const dim3 rgrid((unsigned)cdiv(i, 4), nb);
   hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, 1, i, sc.h, done);
-  launch_apply<false>(vec4, grid, s, n, V, ldv, ldb, i + 1, 1, i, sc.h, sc.M2, sc.dpart,
-                     sc.P, done);
-  launch_project<false>(vec4, grid, s, n, V, ldv, ldb, i, 1, i, sc.dpart, sc.P, sc.hpart,
-                     sc.M2, diag, subdiag, m, done);
+  if (vec4) {
+    hipLaunchKernelGGL(lz_apply_project, grid, dim3(256), 0, s, n, V, ldv, ldb, i + 1, 1, i, (const float*)sc.h,
+                       sc.M2, sc.hpart, sc.P, done);
+  } else {
+    launch_apply<false>(vec4, grid, s, n, V, ldv, ldb, i + 1, 1, i, sc.h, sc.M2, sc.dpart, sc.P, done);
+    launch_project<false>(vec4, grid, s, n, V, ldv, ldb, i, 1, i, sc.dpart, sc.P, sc.hpart, sc.M2, diag, subdiag, m,
+                          done);
+  }
   hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, 1, i, sc.h, done);
   launch_apply<true>(vec4, grid, s, n, V, ldv, ldb, i + 1, 1, i, sc.h, sc.M2, sc.dpart,
                      sc.P, done);
@@ -567,10 +623,14 @@ int ar_post_full(int64_t n, int nb, float* Q, int64_t ldv, int64_t ldb, float* H
     launch_project<false>(vec4, grid, s, n, Q, ldv, ldb, idx, 0, idx, sc.dpart, sc.P,
                        sc.hpart, sc.M2, (float*)nullptr, (const float*)nullptr, m, done);
     hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, 0, idx, sc.h, done);
-    launch_apply<false>(vec4, grid, s, n, Q, ldv, ldb, idx + 1, 0, idx, sc.h, sc.M2,
-                       sc.dpart, sc.P, done);
-    launch_project<false>(vec4, grid, s, n, Q, ldv, ldb, idx, 0, idx, sc.dpart, sc.P,
-                       sc.hpart, sc.M2, (float*)nullptr, (const float*)nullptr, m, done);
+    if (vec4) {
+      hipLaunchKernelGGL(lz_apply_project, grid, dim3(256), 0, s, n, Q, ldv, ldb, idx + 1, 0, idx,
+                         (const float*)sc.h, sc.M2, sc.hpart, sc.P, done);
+    } else {
+      launch_apply<false>(vec4, grid, s, n, Q, ldv, ldb, idx + 1, 0, idx, sc.h, sc.M2, sc.dpart, sc.P, done);
+      launch_project<false>(vec4, grid, s, n, Q, ldv, ldb, idx, 0, idx, sc.dpart, sc.P, sc.hpart, sc.M2,
+                            (float*)nullptr, (const float*)nullptr, m, done);
+    }
     hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, 0, idx, sc.h2, done);
     launch_apply<true>(vec4, grid, s, n, Q, ldv, ldb, idx + 1, 0, idx, sc.h2, sc.M2,
                        sc.dpart, sc.P, done);
