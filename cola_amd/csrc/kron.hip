@@ -101,10 +101,12 @@ __device__ __forceinline__ void tile_product4(const T* a, const T* __restrict__ 
 // scheduling barriers pin the prefetch ahead of the MFMAs and the stores behind them:
 // left to itself the compiler sinks the prefetch into the MFMA stream and then waits for
 // it together with the current slab's rows.  fp32 alternates two LDS intermediates (one
-// barrier per slab); fp64 (34 KB per intermediate) keeps one and a second barrier.
+// barrier per slab); fp64 (34 KB per intermediate) keeps one and a second barrier, reads
+// K3 from LDS instead of registers and walks two slabs per block, so that two blocks (two
+// waves per SIMD, 68 KB of LDS each) share a CU: 40.6 -> 35.3 us for the refinement pass.
 template <typename T>
 constexpr int slabs_per_block() {
-  return 4;
+  return sizeof(T) == 4 ? 4 : 2;
 }
 
 //
@@ -180,13 +182,32 @@ __global__ __launch_bounds__(256) void kron64_pass_a(const T* __restrict__ K2, c
       load16(xp, dst);
     }
   };
+  // fp64 reads K3 from LDS: its 128 registers would leave the wave at the register limit
+  // (an MFMA every 64 cycles leaves the LDS port idle anyway); fp32 keeps it in registers.
+  // Row pitch 65 puts a wave's 16 rows in distinct bank pairs.
+  constexpr bool K3L = sizeof(T) == 8;
+  constexpr int K3P = 65;
+  __shared__ T K3s[K3L ? 64 * K3P : 1];
+  if constexpr (K3L) {
+    T kv[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) kv[q] = K3[threadIdx.x + 256 * q];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int e = threadIdx.x + 256 * q;
+      K3s[(e >> 6) * K3P + (e & 63)] = kv[q];
+    }
+  }
   T a[16];
   fetch(s0, a);
-  T k3[4][16];
+  T k3[K3L ? 1 : 4][16];
+  if constexpr (!K3L) {
 #pragma unroll
-  for (int t = 0; t < 4; ++t) load16(K3 + (16 * t + li) * 64 + 16 * g, k3[t]);
+    for (int t = 0; t < 4; ++t) load16(K3 + (16 * t + li) * 64 + 16 * g, k3[t]);
+  }
   T k2[16];
   load16(K2 + (16 * w + li) * 64 + 16 * g, k2);
+  if constexpr (K3L) __syncthreads();  // K3s complete
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int it = 0; it < SLABS; ++it) {
@@ -201,7 +222,12 @@ __global__ __launch_bounds__(256) void kron64_pass_a(const T* __restrict__ K2, c
 #pragma unroll
     for (int ks = 0; ks < 16; ++ks)
 #pragma unroll
-      for (int t = 0; t < 4; ++t) c[t] = MF<T>::step(a[ks], k3[t][ks], c[t]);
+      for (int t = 0; t < 4; ++t) {
+        if constexpr (K3L)
+          c[t] = MF<T>::step(a[ks], K3s[(16 * t + li) * K3P + 16 * g + ks], c[t]);
+        else
+          c[t] = MF<T>::step(a[ks], k3[t][ks], c[t]);
+      }
     T* tb = Tt[it % NBUF];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
