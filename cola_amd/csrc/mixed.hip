@@ -76,30 +76,46 @@ __global__ __launch_bounds__(256) void ir_load_rhs_kernel(int64_t n, int nrhs, c
   double acc[kLoadCols];
 #pragma unroll
   for (int j = 0; j < kLoadCols; ++j) acc[j] = 0.0;
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
-    // V4: a row's columns are contiguous and 16-byte aligned (cs == 1, rs and c0 multiples of 4)
-    float row[kLoadCols];
-    if (V4) {
+  // rows i, i + 256, ... of this thread, kRowsInFlight at a time: every row's loads are
+  // issued (index clamped into the block's range) before the first store
+  constexpr int kRowsInFlight = 4;
+  for (int64_t ib = i0 + threadIdx.x; ib < i1; ib += 256 * kRowsInFlight) {
+    float row[kRowsInFlight][kLoadCols];
 #pragma unroll
-      for (int j = 0; j < kLoadCols; j += 4)
-        if (j < nc) {
-          const float4 q = *reinterpret_cast<const float4*>(b + i * rs + c0 + j);
-          row[j] = q.x;
-          row[j + 1] = q.y;
-          row[j + 2] = q.z;
-          row[j + 3] = q.w;
-        }
+    for (int u = 0; u < kRowsInFlight; ++u) {
+      const int64_t i = ib + 256 * u < i1 ? ib + 256 * u : i1 - 1;
+      if (V4) {
+        // a row's columns are contiguous and 16-byte aligned (cs == 1, rs and c0 multiples of 4)
+#pragma unroll
+        for (int j = 0; j < kLoadCols; j += 4)
+          if (j < nc) {
+            const float4 q = *reinterpret_cast<const float4*>(b + i * rs + c0 + j);
+            row[u][j] = q.x;
+            row[u][j + 1] = q.y;
+            row[u][j + 2] = q.z;
+            row[u][j + 3] = q.w;
+          }
+      } else {
+#pragma unroll
+        for (int j = 0; j < kLoadCols; ++j)
+          if (j < nc) row[u][j] = b[i * rs + (int64_t)(c0 + j) * cs];
+      }
     }
 #pragma unroll
-    for (int j = 0; j < kLoadCols; ++j) {
-      if (j < nc) {
-        const int c = c0 + j;
-        const float v = V4 ? row[j] : b[i * rs + (int64_t)c * cs];
-        const double dv = (double)v;
-        b64[c * ld + i] = dv;
-        r32[c * ld + i] = v;
-        if (x64) x64[c * ld + i] = 0.0;
-        acc[j] += dv * dv;
+    for (int u = 0; u < kRowsInFlight; ++u) {
+      const int64_t i = ib + 256 * u;
+      if (i >= i1) break;
+#pragma unroll
+      for (int j = 0; j < kLoadCols; ++j) {
+        if (j < nc) {
+          const int c = c0 + j;
+          const float v = row[u][j];
+          const double dv = (double)v;
+          b64[c * ld + i] = dv;
+          r32[c * ld + i] = v;
+          if (x64) x64[c * ld + i] = 0.0;
+          acc[j] += dv * dv;
+        }
       }
     }
   }
