@@ -16,7 +16,9 @@ from oracle import learned as ol
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("n,din,dout", [(1000, 4, 64), (64, 4, 64), (77, 3, 40)])
+# dout = 7 exercises the padded W3 rows (dout not a multiple of 4) and layer 3's uneven
+# split over the four waves; (8, 128) are the largest supported dimensions
+@pytest.mark.parametrize("n,din,dout", [(1000, 4, 64), (64, 4, 64), (77, 3, 40), (150, 2, 7), (65, 8, 128)])
 def test_mlp_forward_matches_oracle(dev, n, din, dout):
     g = torch.Generator().manual_seed(n)
     X = torch.rand(n, din, generator=g)
@@ -30,7 +32,7 @@ def test_mlp_forward_matches_oracle(dev, n, din, dout):
     assert torch.allclose(Z2[:, :n].T.cpu().double(), Z2r, rtol=1e-5, atol=1e-5)
 
 
-@pytest.mark.parametrize("n,din,dout", [(300, 4, 64), (130, 3, 40)])
+@pytest.mark.parametrize("n,din,dout", [(300, 4, 64), (130, 3, 40), (150, 2, 7), (70, 8, 128)])
 def test_mlp_backward_matches_autograd(dev, n, din, dout):
     g = torch.Generator().manual_seed(7)
     X = torch.rand(n, din, generator=g)
@@ -85,6 +87,24 @@ def test_gram_rows_gemm_spd(dev):
     bad = -torch.eye(5, dtype=torch.float64)
     with pytest.raises(np.linalg.LinAlgError):
         L.spd_inverse(bad.to(dev))
+
+
+def test_gram_many_chunks_and_rows_gemm_remainder(dev):
+    # n > 1024 chunks of 64 rows: each partial covers 128 rows; kin = 37 leaves a remainder
+    # after the eight-column groups of rows_gemm, kout = 45 a partial second column slab
+    g = torch.Generator().manual_seed(11)
+    n, ka, kb = 70000, 33, 17
+    X = torch.randn(n, ka, generator=g)
+    Y = torch.randn(n, kb, generator=g)
+    assert N.q("ck_gram_workspace_bytes", n, ka, kb) == 8 * 547 * ka * kb
+    C = L.gram(X.T.contiguous().to(dev), 1, n, Y.T.contiguous().to(dev), 1, n, n, ka, kb).cpu()
+    assert torch.allclose(C, X.double().T @ Y.double(), rtol=1e-12, atol=1e-9)
+    kin, kout, m = 37, 45, 999
+    Xs = torch.randn(m, kin, generator=g)
+    W = torch.randn(kin, kout, generator=g)
+    out = torch.zeros((m, kout), device=dev)
+    L.rows_gemm(Xs.T.contiguous().to(dev), 1, m, W.to(dev), out, kout, 1, m)
+    assert torch.allclose(out.cpu().double(), Xs.double() @ W.double(), rtol=1e-5, atol=1e-4)
 
 
 def test_energy_loss_and_grad_match_autograd(dev):
