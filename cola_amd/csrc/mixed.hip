@@ -63,7 +63,10 @@ __global__ __launch_bounds__(256) void ir_update_kernel(int64_t n, double* __res
 // columns per pass; a thread owns one row.
 constexpr int kLoadCols = 16;
 
-template <bool V4>
+// FULL: the chunk has all kLoadCols columns and ZX: x64 is zeroed; both known on the host,
+// so the per-column and per-store tests compile away (this kernel runs once per launch on
+// a cold instruction cache, where every branch costs).
+template <bool V4, bool FULL, bool ZX>
 __global__ __launch_bounds__(256) void ir_load_rhs_kernel(int64_t n, int nrhs, const float* __restrict__ b,
                                                           int64_t rs, int64_t cs, double* __restrict__ b64,
                                                           float* __restrict__ r32, double* __restrict__ x64,
@@ -72,7 +75,7 @@ __global__ __launch_bounds__(256) void ir_load_rhs_kernel(int64_t n, int nrhs, c
   const int blk = blockIdx.x;
   const int64_t per = cdiv(n, kIrParts);
   const int64_t i0 = blk * per, i1 = i0 + per < n ? i0 + per : n;
-  const int nc = nrhs - c0 < kLoadCols ? nrhs - c0 : kLoadCols;
+  const int nc = FULL ? kLoadCols : (nrhs - c0 < kLoadCols ? nrhs - c0 : kLoadCols);
   double acc[kLoadCols];
 #pragma unroll
   for (int j = 0; j < kLoadCols; ++j) acc[j] = 0.0;
@@ -113,7 +116,7 @@ __global__ __launch_bounds__(256) void ir_load_rhs_kernel(int64_t n, int nrhs, c
           const double dv = (double)v;
           b64[c * ld + i] = dv;
           r32[c * ld + i] = v;
-          if (x64) x64[c * ld + i] = 0.0;
+          if (ZX) x64[c * ld + i] = 0.0;
           acc[j] += dv * dv;
         }
       }
@@ -238,12 +241,16 @@ int ck_ir_load_rhs(int64_t n, int32_t nrhs, const float* b, int64_t rs, int64_t 
   double* part = norms ? static_cast<double*>(work) : nullptr;
   const bool v4 = cs == 1 && rs % 4 == 0 && nrhs % 4 == 0 && (reinterpret_cast<uintptr_t>(b) & 15) == 0;
   for (int c0 = 0; c0 < nrhs; c0 += kLoadCols) {
-    if (v4)
-      hipLaunchKernelGGL(ir_load_rhs_kernel<true>, dim3(kIrParts), dim3(256), 0, s, n, nrhs, b, rs, cs, b64, r32, x64,
-                         ld, c0, part);
+    const bool full = nrhs - c0 >= kLoadCols;
+    auto launch = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(kIrParts), dim3(256), 0, s, n, nrhs, b, rs, cs, b64, r32, x64, ld, c0, part);
+    };
+    if (v4 && full)
+      x64 ? launch(ir_load_rhs_kernel<true, true, true>) : launch(ir_load_rhs_kernel<true, true, false>);
+    else if (v4)
+      x64 ? launch(ir_load_rhs_kernel<true, false, true>) : launch(ir_load_rhs_kernel<true, false, false>);
     else
-      hipLaunchKernelGGL(ir_load_rhs_kernel<false>, dim3(kIrParts), dim3(256), 0, s, n, nrhs, b, rs, cs, b64, r32,
-                         x64, ld, c0, part);
+      x64 ? launch(ir_load_rhs_kernel<false, false, true>) : launch(ir_load_rhs_kernel<false, false, false>);
   }
   if (norms) hipLaunchKernelGGL(ir_norms_kernel, dim3(nrhs, 2), dim3(64), 0, s, (const double*)part, norms);
   CK_CHECK_LAUNCH();

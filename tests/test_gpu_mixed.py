@@ -111,7 +111,7 @@ def test_kron_residual_matches_apply_then_residual(dev):
     assert torch.allclose(norms.view(B, 2)[:, 1].cpu(), Bv.norm(dim=0), rtol=1e-12)
 
 
-@pytest.mark.parametrize("B,transposed", [(3, False), (8, False), (20, False), (5, True)])
+@pytest.mark.parametrize("B,transposed", [(3, False), (8, False), (16, False), (20, False), (5, True)])
 def test_ir_load_rhs(dev, B, transposed):
     """b64 = b, r32 = b, x64 = 0 and both norms = ||b|| from an (n, B) view of any strides;
     more than 16 columns take two passes."""
