@@ -289,9 +289,18 @@ class ScalarMul(LinearOperator):
 
     def __init__(self, c, shape, dtype=None, device=None):
         self.device = device if device is not None else torch.device("cpu")
+        # host copy of c for kernel arguments: reading a device scalar each call would be a
+        # device-to-host copy that synchronises the stream
+        self._c_host = None if isinstance(c, torch.Tensor) else float(c)
         self.c = c if isinstance(c, torch.Tensor) else torch.tensor(c, dtype=dtype or torch.float32,
                                                                     device=self.device)
         super().__init__(dtype or self.c.dtype, shape)
+
+    def c_float(self):
+        """c as a Python float (read from the device once)."""
+        if self._c_host is None:
+            self._c_host = float(self.c)
+        return self._c_host
 
     def _matmat(self, v):
         return self.c.to(v.device) * v
@@ -299,7 +308,7 @@ class ScalarMul(LinearOperator):
     def hip_op(self):
         if self.device.type != "cuda":
             return None
-        return _op(N.CK_OP_IDENTITY, self.shape[0], scale=float(self.c)), []
+        return _op(N.CK_OP_IDENTITY, self.shape[0], scale=self.c_float()), []
 
 
 class Identity(LinearOperator):
@@ -348,7 +357,7 @@ class Product(LinearOperator):
         rest = []
         for M in self.Ms:
             if isinstance(M, ScalarMul):
-                scal *= float(M.c)
+                scal *= M.c_float()
             elif isinstance(M, Identity):
                 continue
             else:
