@@ -219,7 +219,7 @@ def laplacian_eigs(m, shift):
 
 
 def bench_csr(args, dev):
-    """CSR SpMV bandwidth and plain FP32 CG on the 2-D Laplacian + 1e-3 I (config 3's operator, and a
+    """CSR SpMV bandwidth and plain FP32 CG on the 2-D Laplacian + 1e-3 I (the Lanczos and Hutchinson operator, and a
     4096^2 grid whose 0.7 GB of matrix data exceeds the 256 MB Infinity Cache)."""
     from cola_amd.linalg.cg import run_batched_cg as rcg
     s = N.stream_ptr(dev)
@@ -306,7 +306,7 @@ def bench_dense_cg(args, dev):
         xo, io = ocg.cg(lambda V: A @ V, b[:, None], tol=1e-6, max_iters=1000)
         reps += 1
     dto = (time.perf_counter() - t1) / reps
-    return {"metric": "CG solves/sec (config 1: dense N=1024, 1 rhs, reference tol 1e-6)", "value": 1 / dt,
+    return {"metric": "CG solves/sec (dense N=1024, 1 rhs, reference tol 1e-6)", "value": 1 / dt,
             "unit": "solves/s", "ms_per_step": dt * 1e3, "steps": args.steps, "dtype": "f32",
             "config": {"workload": "dense_cg", "cg_steps": int(k), "oracle_steps": int(io["iterations"] - 1),
                        "fp64_rel_residual": res},
@@ -315,7 +315,7 @@ def bench_dense_cg(args, dev):
 
 
 def bench_lanczos(args, dev):
-    """Config 3a: Lanczos top-32 of the 1024^2-grid Laplacian + 1e-3 I, 120 steps, full CGS2."""
+    """Lanczos top-32 of the 1024^2-grid Laplacian + 1e-3 I, 120 steps, full CGS2."""
     from cola_amd.linalg import HipLanczos, eig
     from cola_amd.linalg.lanczos import run_lanczos
     op, A = laplacian_op(dev)
@@ -341,7 +341,7 @@ def bench_lanczos(args, dev):
     spmv = 4 * (2 * 5238784 + (n + 1) + 2 * n)
     reorth = sum(3 * i * 4 * n + 2 * 2 * 4 * n for i in range(1, 121))
     total = 120 * spmv + reorth
-    return {"metric": "Lanczos eigensolves/sec (config 3a: 2-D Laplacian 1024^2, 120 steps, CGS2, top-32)",
+    return {"metric": "Lanczos eigensolves/sec (2-D Laplacian 1024^2, 120 steps, CGS2, top-32)",
             "value": 1 / dt, "unit": "eigensolves/s", "ms_per_step": dt * 1e3, "dtype": "f32 (+f64 tridiagonal)",
             "config": {"workload": "lanczos", "n": n, "nnz": 5238784, "steps": 120,
                        "max_abs_err_top32_vs_closed_form": err,
@@ -354,7 +354,7 @@ def bench_lanczos(args, dev):
 
 
 def bench_hutch(args, dev):
-    """Config 3b: Hutchinson trace of the Laplacian + 1e-3 I with 64 packed Rademacher probes."""
+    """Hutchinson trace of the Laplacian + 1e-3 I with 64 packed Rademacher probes."""
     from cola_amd.linalg import hutch_trace_csr
     op, A = laplacian_op(dev)
     n = op.shape[0]
@@ -379,7 +379,7 @@ def bench_hutch(args, dev):
     def passes():
         N.call("ck_trace_csr_rademacher_seeded", hop, 1, ctypes.c_uint64(7), N.ptr(out), N.ptr(work), wb, s)
     t_k = time_kernel(passes, reps=50)
-    return {"metric": "Hutchinson trace estimates/sec (config 3b: 64 Rademacher probes, CSR 1024^2 Laplacian)",
+    return {"metric": "Hutchinson trace estimates/sec (64 Rademacher probes, CSR 1024^2 Laplacian)",
             "value": 1 / dt, "unit": "estimates/s", "ms_per_step": dt * 1e3, "dtype": "f32 values, popcount probes",
             "config": {"workload": "hutch", "estimate": float(tr), "exact": exact,
                        "rel_err": abs(float(tr) - exact) / exact, "expected_std_rel": 361.9 / exact,
@@ -390,7 +390,7 @@ def bench_hutch(args, dev):
 
 
 def bench_slq(args, dev):
-    """Config 2b: SLQ logdet of K1xK2xK3 + 0.01 I, 32 Rademacher probes x 30 steps, FP64 tridiagonal."""
+    """SLQ logdet of K1xK2xK3 + 0.01 I, 32 Rademacher probes x 30 steps, FP64 tridiagonal."""
     from cola_amd.linalg import HipSLQ, logdet
     A, facs, _ = kron_workload(dev)
     alg = HipSLQ(num_probes=32, max_iters=30, key=5)
@@ -404,7 +404,7 @@ def bench_slq(args, dev):
     dt = (time.perf_counter() - t0) / args.steps
     lam = [torch.linalg.eigvalsh(f.double()) for f in facs]
     exact = float(torch.log((lam[0][:, None, None] * lam[1][None, :, None] * lam[2][None, None, :]) + 0.01).sum())
-    return {"metric": "SLQ log-determinants/sec (config 2b: 64^3 Kronecker, 32 probes x 30 steps)",
+    return {"metric": "SLQ log-determinants/sec (64^3 Kronecker, 32 probes x 30 steps)",
             "value": 1 / dt, "unit": "logdets/s", "ms_per_step": dt * 1e3, "dtype": "f32 (+f64 tridiagonal)",
             "config": {"workload": "slq", "estimate_abs": float(est), "exact": exact,
                        "rel_err_vs_abs_exact": abs(float(est) - abs(exact)) / abs(exact),
@@ -587,7 +587,7 @@ def gp_systems(nsys, dev, bf16=True):
 
 
 def bench_batched_cg(args, dev):
-    """Config 4: 512 uneven dense GP systems; FP64 refinement to 1e-8 around batched PCG (bf16 storage
+    """512 uneven dense GP systems; FP64 refinement to 1e-8 around batched PCG (bf16 storage
     where it contracts, per-system learned / pivoted-Cholesky rank-64 preconditioners) vs plain FP32 CG."""
     from cola_amd.linalg import learned as L
     from cola_amd.linalg.batched import BatchedCG, BatchedMixedSolver, BatchedPrecond, autotune, time_matvec

@@ -1,9 +1,9 @@
 """Writes tests/golden/full_config.json: the oracle's FP64 results on two full-size survey
 workloads, for GPU parity tests that cannot run the oracle within a test's time:
 
-* config 3a: top-32 Ritz values of the 1024^2-grid Laplacian + 1e-3 I after 120 Lanczos
+* Lanczos: top-32 Ritz values of the 1024^2-grid Laplacian + 1e-3 I after 120 Lanczos
   steps (double Gram-Schmidt) from the seed-42 N(0,1) start vector;
-* config 2b: the SLQ log-determinant of K1 x K2 x K3 + 0.01 I (64^3; each Ki an RBF kernel,
+* SLQ: the SLQ log-determinant of K1 x K2 x K3 + 0.01 I (64^3; each Ki an RBF kernel,
   lengthscale 0.1, on 64 sorted U[0,1] points from seed 0) with 32 Rademacher probes of the
   engine stream key 5 and 30 Lanczos steps.
 
@@ -37,7 +37,7 @@ def kron64_factors(seed=0, ell=0.1):
 
 def main():
     out = {}
-    # config 3a
+    # Lanczos on the shifted Laplacian
     m = 1024
     ip, ix, v = oop.laplacian_2d_csr(m, 0.0)
     L = sp.csr_matrix((v.astype(np.float64), ix, ip), shape=(m * m, m * m))
@@ -50,7 +50,7 @@ def main():
     ev, _, _ = olz.lanczos_eigs(mm, m * m, s0.double(), 120, 1e-7, torch.float64)
     out["lanczos_laplacian_1024sq"] = {"start": "torch.randn(1024**2, generator=manual_seed(42))", "steps": 120,
                                        "tol": 1e-7, "top32_ritz_fp64": [float(x) for x in ev[-32:].tolist()]}
-    # config 2b
+    # SLQ on the shifted Kronecker system
     facs = kron64_factors()
     n = 64**3
     Z = orng.rademacher(5, n, 32, torch.float64).T

@@ -145,7 +145,7 @@ def test_arnoldi_eigs_extremes_vs_oracle(dev):
 
 
 def test_lanczos_full_config_matches_golden(dev):
-    """Config 3a at full size (1024^2-grid Laplacian + 1e-3 I, 120 steps, seed-42 start): the
+    """The Lanczos workload at full size (1024^2-grid Laplacian + 1e-3 I, 120 steps, seed-42 start): the
     engine's top-32 Ritz values against the oracle's FP64 Lanczos on the same start vector
     (tests/golden/full_config.json, made by tests/golden/make_full_config.py), 1e-5 relative."""
     import json

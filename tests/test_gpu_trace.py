@@ -196,7 +196,7 @@ def test_diag_structural(dev):
 
 
 def test_slq_kron64_full_config_matches_golden(dev):
-    """Config 2b at full size (64^3 Kronecker + 0.01 I, 32 Rademacher probes of key 5, 30 steps):
+    """The SLQ workload at full size (64^3 Kronecker + 0.01 I, 32 Rademacher probes of key 5, 30 steps):
     the engine's SLQ log-determinant against the oracle's FP64 value on the same probes
     (tests/golden/full_config.json, made by tests/golden/make_full_config.py), 1e-4 relative."""
     import json
