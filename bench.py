@@ -293,7 +293,7 @@ def bench_dense_cg(args, dev):
     A = X @ X.T / 256 + 0.1 * torch.eye(1024)
     op = ca.PSD(ca.Dense(A.to(dev)))
     bd = b.to(dev)[:, None]
-    for _ in range(args.warmup):
+    for _ in range(1 + args.warmup):  # one set-up call (workspaces, first lowering), then the warmup
         x, r, k, info = run_batched_cg(op, bd, None, 1000, 1e-6, None, chunk=32)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -328,7 +328,7 @@ def bench_lanczos(args, dev):
     g = torch.Generator().manual_seed(42)
     s0 = torch.randn(n, generator=g).to(dev)
     alg = HipLanczos(start_vector=s0, max_iters=120, tol=1e-7)
-    for _ in range(args.warmup):
+    for _ in range(1 + args.warmup):  # one set-up call, then the warmup
         vals, vecs = eig(op, 32, "LM", alg)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -362,7 +362,7 @@ def bench_hutch(args, dev):
     from cola_amd.linalg import hutch_trace_csr
     op, A = laplacian_op(dev)
     n = op.shape[0]
-    for _ in range(args.warmup):
+    for _ in range(1 + args.warmup):  # one set-up call (workspace, first lowering), then the warmup
         tr = hutch_trace_csr(op, 64, seed=0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -398,7 +398,7 @@ def bench_slq(args, dev):
     from cola_amd.linalg import HipSLQ, logdet
     A, facs, _ = kron_workload(dev)
     alg = HipSLQ(num_probes=32, max_iters=30, key=5)
-    for _ in range(args.warmup):
+    for _ in range(1 + args.warmup):  # one set-up call, then the warmup
         est = logdet(A, alg)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
