@@ -47,8 +47,17 @@ inline LzScratch scratch_of(void* scratch, int64_t n, int nb, int m) {
 }
 
 __device__ inline double col_total(const float* __restrict__ part, int P, int64_t base) {
+  // each lane adds its partials lane, lane + 64, ... in that order; loads go four at a time
   double acc = 0.0;
-  for (int i = threadIdx.x & 63; i < P; i += 64) acc += (double)part[base + i];
+  int i = threadIdx.x & 63;
+  for (; i + 192 < P; i += 256) {
+    const float v0 = part[base + i], v1 = part[base + i + 64], v2 = part[base + i + 128], v3 = part[base + i + 192];
+    acc += (double)v0;
+    acc += (double)v1;
+    acc += (double)v2;
+    acc += (double)v3;
+  }
+  for (; i < P; i += 64) acc += (double)part[base + i];
   return wave_sum(acc);
 }
 
@@ -113,14 +122,43 @@ __global__ __launch_bounds__(256) void lz_start_ctrl(int nb, int m, float* __res
 }
 
 // V[:,i] /= denom, denom = nrm[b] (first step) or subdiag[b][i-1]
+// V4: four consecutive elements per thread as one 16-byte access (column start and probe
+// stride multiples of 4 floats, V 16-byte aligned); the tail past n is left alone
+template <bool V4>
 __global__ __launch_bounds__(256) void lz_scale_col(int64_t n, float* __restrict__ V, int64_t ldb, int64_t coloff,
                                                     const float* __restrict__ denom, int64_t dstride,
                                                     const int* __restrict__ done) {
   if (done && *done) return;
   const int b = blockIdx.y;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  V[b * ldb + coloff + i] /= denom[b * dstride];
+  const float d = denom[b * dstride];
+  float* col = V + b * ldb + coloff;
+  if (V4) {
+    const int64_t i = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (i + 3 < n) {
+      float4 v = *reinterpret_cast<const float4*>(col + i);
+      v.x /= d;
+      v.y /= d;
+      v.z /= d;
+      v.w /= d;
+      *reinterpret_cast<float4*>(col + i) = v;
+    } else {
+      for (int64_t k = i; k < n; ++k) col[k] /= d;
+    }
+  } else {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    col[i] /= d;
+  }
+}
+
+inline void launch_scale_col(int64_t n, int nb, float* V, int64_t ldb, int64_t coloff, const float* denom,
+                             int64_t dstride, const int* done, hipStream_t s) {
+  if (ldb % 4 == 0 && coloff % 4 == 0 && (reinterpret_cast<uintptr_t>(V) & 15) == 0)
+    hipLaunchKernelGGL(lz_scale_col<true>, dim3((unsigned)cdiv(cdiv(n, 4), 256), nb), dim3(256), 0, s, n, V, ldb,
+                       coloff, denom, dstride, done);
+  else
+    hipLaunchKernelGGL(lz_scale_col<false>, dim3((unsigned)cdiv(n, 256), nb), dim3(256), 0, s, n, V, ldb, coloff,
+                       denom, dstride, done);
 }
 
 // Projection partials h_j = V[:,j] . w over j in [j0, j1] for one tile.
@@ -381,26 +419,35 @@ __global__ __launch_bounds__(256) void lz_apply_project(int64_t n, float* __rest
 }
 
 // beta_i = ||w||, errors and the loop condition (lanczos.py:251-268)
-__global__ __launch_bounds__(256) void lz_finalize(int nb, int m, const float* __restrict__ npart, int P,
-                                                   float* __restrict__ subdiag, void* state, float tol, int i) {
+__global__ __launch_bounds__(1024) void lz_finalize(int nb, int m, const float* __restrict__ npart, int P,
+                                                    float* __restrict__ subdiag, void* state, float tol, int i) {
   ck_lz_ctrl* c = static_cast<ck_lz_ctrl*>(state);
   if (c->done) return;
   float* errors = reinterpret_cast<float*>(c + 1);
-  const int wave = threadIdx.x >> 6;
-  for (int b = wave; b < nb; b += 4) {
+  // each probe's wave forms beta_i, its ratio to beta_1 and its continue flag; thread 0 then
+  // takes the maximum and the any over probes in index order from LDS
+  __shared__ float s_rel[4096];
+  __shared__ unsigned char s_any[4096];
+  const int wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  const int inext = i + 1;
+  for (int b = wave; b < nb; b += nwave) {
     const double s = col_total(npart, P, (int64_t)b * P);
-    if ((threadIdx.x & 63) == 0) subdiag[b * (m + 1) + i] = (float)sqrt(s);
+    if ((threadIdx.x & 63) == 0) {
+      float* sd = subdiag + b * (m + 1);
+      const float beta = (float)sqrt(s);
+      sd[i] = beta;
+      const float beta1 = i == 1 ? beta : sd[1];
+      s_rel[b] = beta / fmaxf(beta1, 1e-30f);
+      s_any[b] = (beta > tol * beta1) || (inext <= 1);
+    }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const int inext = i + 1;
     float err = -INFINITY;
     bool any = false;
     for (int b = 0; b < nb; ++b) {
-      const float* sd = subdiag + b * (m + 1);
-      const float rel = sd[inext - 1] / fmaxf(sd[1], 1e-30f);
-      err = fmaxf(err, rel);
-      any = any || (sd[inext - 1] > tol * sd[1]) || (inext <= 1);
+      err = fmaxf(err, s_rel[b]);
+      any = any || s_any[b];
     }
     errors[inext - 1] = err + (inext <= 1 ? 1.f : 0.f);
     c->i = inext;
@@ -559,12 +606,10 @@ int lz_pre_full(int64_t n, int nb, float* V, int64_t ldv, int64_t ldb, const flo
     hipLaunchKernelGGL(tile_partials<1>, dim3(sc.P, nb), dim3(256), 0, s, n, V + ldv, ldb, V + ldv, ldb, sc.dpart,
                        sc.P, done);
     hipLaunchKernelGGL(lz_norm_final, dim3(1), dim3(256), 0, s, nb, sc.dpart, sc.P, sc.nrm, done);
-    hipLaunchKernelGGL(lz_scale_col, dim3((unsigned)cdiv(n, 256), nb), dim3(256), 0, s, n, V, ldb, ldv,
-                       (const float*)sc.nrm, (int64_t)1, done);
+    launch_scale_col(n, nb, V, ldb, ldv, (const float*)sc.nrm, (int64_t)1, done, s);
   } else {
     // V[:, i] was w_{i-1}; its norm is subdiag[i-1] (lanczos.py:252), the same reduction
-    hipLaunchKernelGGL(lz_scale_col, dim3((unsigned)cdiv(n, 256), nb), dim3(256), 0, s, n, V, ldb,
-                       (int64_t)i * ldv, subdiag + (i - 1), (int64_t)(m + 1), done);
+    launch_scale_col(n, nb, V, ldb, (int64_t)i * ldv, subdiag + (i - 1), (int64_t)(m + 1), done, s);
   }
   CK_CHECK_LAUNCH();
   return CK_OK;
@@ -596,7 +641,7 @@ int lz_post_full(int64_t n, int nb, float* V, int64_t ldv, int64_t ldb, float* d
   hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, 1, i, sc.h, done);
   launch_apply<true>(vec4, grid, s, n, V, ldv, ldb, i + 1, 1, i, sc.h, sc.M2, sc.dpart,
                      sc.P, done);
-  hipLaunchKernelGGL(lz_finalize, dim3(1), dim3(256), 0, s, nb, m, sc.dpart, sc.P, subdiag, state, tol, i);
+  hipLaunchKernelGGL(lz_finalize, dim3(1), dim3(1024), 0, s, nb, m, sc.dpart, sc.P, subdiag, state, tol, i);
   CK_CHECK_LAUNCH();
   return CK_OK;
 }
