@@ -68,23 +68,47 @@ __global__ __launch_bounds__(256) void wb_ftx_tiles(int64_t n, int k, const floa
 }
 
 // per rhs b: t = sum of the tile partials (index order), u = Cinv t (both fp64)
-__global__ __launch_bounds__(128) void wb_core(int k, const double* __restrict__ part, int P,
-                                               const double* __restrict__ Cinv, double* __restrict__ u,
-                                               const int* __restrict__ done) {
+// u_b = C^-1 (F^T x_b): the P tile partials of each of the k entries are summed in
+// kWbSlices fixed slices of consecutive tiles (each slice in tile order, loads eight at a
+// time), the slices then in slice order; then the k x k product, one entry per thread
+constexpr int kWbSlices = 1024 / kMaxRank;
+__global__ __launch_bounds__(1024) void wb_core(int k, const double* __restrict__ part, int P,
+                                                const double* __restrict__ Cinv, double* __restrict__ u,
+                                                const int* __restrict__ done) {
   if (done && *done) return;
+  __shared__ double red[kWbSlices][kMaxRank];
   __shared__ double ts[kMaxRank];
   const int b = blockIdx.x;
-  for (int c = threadIdx.x; c < k; c += 128) {
-    double acc = 0.0;
+  const int c = threadIdx.x % kMaxRank, sl = threadIdx.x / kMaxRank;
+  if (c < k) {
+    const int per = (P + kWbSlices - 1) / kWbSlices;
+    const int p0 = sl * per, p1 = p0 + per < P ? p0 + per : P;
     const double* pc = part + ((int64_t)b * k + c) * P;
-    for (int p = 0; p < P; ++p) acc += pc[p];
-    ts[c] = acc;
+    double acc = 0.0;
+    int p = p0;
+    for (; p + 8 <= p1; p += 8) {
+      double v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = pc[p + q];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) acc += v[q];
+    }
+    for (; p < p1; ++p) acc += pc[p];
+    red[sl][c] = acc;
   }
   __syncthreads();
-  for (int c = threadIdx.x; c < k; c += 128) {
+  if (threadIdx.x < k) {
     double acc = 0.0;
-    for (int j = 0; j < k; ++j) acc = fma(Cinv[c * k + j], ts[j], acc);
-    u[(int64_t)b * k + c] = acc;
+#pragma unroll
+    for (int q = 0; q < kWbSlices; ++q) acc += red[q][threadIdx.x];
+    ts[threadIdx.x] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x < k) {
+    const int cc = threadIdx.x;
+    double acc = 0.0;
+    for (int j = 0; j < k; ++j) acc = fma(Cinv[cc * k + j], ts[j], acc);
+    u[(int64_t)b * k + cc] = acc;
   }
 }
 
@@ -191,7 +215,7 @@ int woodbury_apply(const ck_op* op, int nrhs, const float* X, int64_t ldx, float
   const float* F = static_cast<const float*>(op->a);
   for (int b0 = 0; b0 < nrhs; b0 += kWRhs)
     hipLaunchKernelGGL(wb_ftx_tiles, dim3(P), dim3(256), 0, s, n, k, F, op->lda, X, ldx, nrhs, b0, part, P, done);
-  hipLaunchKernelGGL(wb_core, dim3(nrhs), dim3(128), 0, s, k, (const double*)part, P,
+  hipLaunchKernelGGL(wb_core, dim3(nrhs), dim3(1024), 0, s, k, (const double*)part, P,
                      reinterpret_cast<const double*>(op->aux), u, done);
   for (int b0 = 0; b0 < nrhs; b0 += kWRhs)
     hipLaunchKernelGGL(wb_apply_tiles, dim3(P), dim3(256), 0, s, n, k, F, op->lda, (const double*)u, X, ldx, Y, ldy,
