@@ -62,8 +62,18 @@ __device__ inline double safe_div(double num, double den) {
 
 // Sum of the P partials of column b, fixed order, fp64.
 __device__ inline double column_total(const float* __restrict__ part, int P, int b) {
+  // each lane adds its partials lane, lane + 64, ... in that order; loads go four at a time
+  const float* pb = part + (int64_t)b * P;
   double acc = 0.0;
-  for (int i = threadIdx.x & 63; i < P; i += 64) acc += (double)part[(int64_t)b * P + i];
+  int i = threadIdx.x & 63;
+  for (; i + 192 < P; i += 256) {
+    const float v0 = pb[i], v1 = pb[i + 64], v2 = pb[i + 128], v3 = pb[i + 192];
+    acc += (double)v0;
+    acc += (double)v1;
+    acc += (double)v2;
+    acc += (double)v3;
+  }
+  for (; i < P; i += 64) acc += (double)pb[i];
   return wave_sum(acc);
 }
 
