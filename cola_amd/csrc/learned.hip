@@ -253,9 +253,10 @@ __global__ __launch_bounds__(256) void mlp_bwd_kernel(int n, int din, int dout, 
       }
     }
   }
+  // columns g + 4 i reach 127 < kPitch; those past dp receive zeros and are never read
+  // before dd is rewritten, so the stores need no test
 #pragma unroll
-  for (int i = 0; i < 32; ++i)
-    if (g + 4 * i < dp) dd[r][g + 4 * i] = gv[i];
+  for (int i = 0; i < 32; ++i) dd[r][g + 4 * i] = gv[i];
 #pragma unroll
   for (int jj = 0; jj < 32; ++jj) a[r][c0 + jj] = z2[jj];
 #pragma unroll 1
