@@ -44,7 +44,7 @@ class HipCG(Algorithm):
         return cg(A, b, x0=self.x0, P=self.P, tol=self.tol, max_iters=self.max_iters, chunk=self.chunk)
 
 
-def cg(A, rhs, x0=None, P=None, tol=1e-6, max_iters=1000, pbar=False, chunk=16):
+def cg(A, rhs, x0=None, P=None, tol=1e-6, max_iters=5000, pbar=False, chunk=16):
     """cg.py:39-69 on the engine. rhs (n,) or (n, b) on the HIP device; returns (soln, info)."""
     is_vector = rhs.dim() == 1
     if is_vector:

@@ -52,27 +52,38 @@ class Exact(Algorithm):
     pbar: bool = False
 
     def __call__(self, A, k=0):
-        if k != 0:
-            raise NotImplementedError("Exact diagonal on the engine supports k = 0")
+        # out[t] = A[t + max(0, -k), t + max(0, k)] from blocks of unit columns (exact_diag,
+        # diagonal_estimation.py:117-128, with its chunk shifting written as an index map)
         n = A.shape[0]
+        k = int(k)
+        if abs(k) >= n:
+            raise ValueError(f"diagonal {k} is outside a {n} x {n} operator")
+        m = n - abs(k)
+        r0, c0 = max(0, -k), max(0, k)
         bs = min(100, n)
-        out = torch.empty(n, dtype=A.dtype, device=A.device)
-        for i in range(0, n, bs):
-            cols = torch.arange(i, min(i + bs, n), device=A.device)
-            E = torch.zeros((n, cols.numel()), dtype=A.dtype, device=A.device)
-            E[cols, torch.arange(cols.numel(), device=A.device)] = 1.0
-            out[cols] = (A @ E)[cols, torch.arange(cols.numel(), device=A.device)]
+        out = torch.empty(m, dtype=A.dtype, device=A.device)
+        for t in range(0, m, bs):
+            ts = torch.arange(t, min(t + bs, m), device=A.device)
+            E = torch.zeros((n, ts.numel()), dtype=A.dtype, device=A.device)
+            E[ts + c0, torch.arange(ts.numel(), device=A.device)] = 1.0
+            out[ts] = (A @ E)[ts + r0, torch.arange(ts.numel(), device=A.device)]
         return out
 
 
 def hutchinson_diag_estimate(A, k=0, bs=100, tol=3e-2, max_iters=10000, pbar=False, rand="normal", key=None,
                              bs_override=False):
-    """diagonal_estimation.py:158-210 for k = 0. Returns (mean diagonal (n,), info)."""
-    if k != 0:
-        raise NotImplementedError("the engine's Hutchinson estimator supports the main diagonal (k = 0)")
+    """diagonal_estimation.py:158-210. Returns (mean k-th diagonal (n - |k|,), info).
+
+    For k != 0 the estimator of diagonal entry j is (A z)_j z_{j+k} (the reference's
+    ``roll(z, -k)`` with the wrapped rows zeroed, :192-195): the accumulation kernel reads
+    the probe block k rows further on (k > 0) or A z |k| rows further on (k < 0)."""
     assert tol > 1e-3, "tolerance chosen too high for stochastic diagonal estimation"
     assert rand in ["normal", "rademacher"], "rand must be 'normal' or 'rademacher'"
     n = A.shape[0]
+    k = int(k)
+    if abs(k) >= n:
+        raise ValueError(f"diagonal {k} is outside a {n} x {n} operator")
+    m = n - abs(k)
     dev = A.device
     if dev.type != "cuda":
         raise ValueError("the Krylov engine runs on the HIP device; A is on the CPU")
@@ -81,9 +92,11 @@ def hutchinson_diag_estimate(A, k=0, bs=100, tol=3e-2, max_iters=10000, pbar=Fal
     aop = lower(A)
     work, wb = workspace([aop[0] if aop else None], bs, dev)
     ld = ld_of(n)
-    dsum = torch.zeros(n, dtype=torch.float32, device=dev)
-    dsq = torch.zeros(n, dtype=torch.float32, device=dev)
+    dsum = torch.zeros(m, dtype=torch.float32, device=dev)
+    dsq = torch.zeros(m, dtype=torch.float32, device=dev)
     AZ = torch.empty((bs, ld), dtype=torch.float32, device=dev)
+    # row offsets of the two factors: (A z)_{j + a} * z_{j + c}, j < m
+    a_off, z_off = (0, k) if k >= 0 else (-k, 0)
     s = N.stream_ptr(dev)
     errors = []
     t0 = time.time()
@@ -92,7 +105,8 @@ def hutchinson_diag_estimate(A, k=0, bs=100, tol=3e-2, max_iters=10000, pbar=Fal
         key = next_key(key)
         Z, _ = probes(rand, n, bs, key, dev)
         apply_blocks(A, aop, Z, AZ, n, bs, work, wb)
-        N.call("ck_hutch_accumulate", n, bs, N.ptr(Z), ld, N.ptr(AZ), ld, N.ptr(dsum), N.ptr(dsq), s)
+        N.call("ck_hutch_accumulate", m, bs, N.ptr(Z[:, z_off:]), ld, N.ptr(AZ[:, a_off:]), ld, N.ptr(dsum),
+               N.ptr(dsq), s)
         i += 1
         cnt = i * bs
         mean = dsum / cnt
@@ -101,7 +115,10 @@ def hutchinson_diag_estimate(A, k=0, bs=100, tol=3e-2, max_iters=10000, pbar=Fal
         errors.append(err)
         if not (i < max_iters and err > tol):
             break
-    info = {"iterations": i + 1, "errors": np.array(errors[1:]), "iteration_time": (time.time() - t0) / (i + 1)}
+    # torch_tqdm.py:38-62: one error per condition evaluation (the first, at zero blocks, is
+    # undefined), the final error appended once more, and the first two entries dropped
+    info = {"iterations": i + 1, "errors": np.array(errors[1:] + errors[-1:]),
+            "iteration_time": (time.time() - t0) / (i + 1)}
     return dsum / (i * bs), info
 
 

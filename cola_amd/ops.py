@@ -290,16 +290,24 @@ class ScalarMul(LinearOperator):
     def __init__(self, c, shape, dtype=None, device=None):
         self.device = device if device is not None else torch.device("cpu")
         # host copy of c for kernel arguments: reading a device scalar each call would be a
-        # device-to-host copy that synchronises the stream
+        # device-to-host copy that synchronises the stream.  A tensor c is re-read whenever its
+        # version counter moves (an in-place update such as an optimizer step on a noise
+        # hyperparameter), so the kernels never see a stale coefficient.
         self._c_host = None if isinstance(c, torch.Tensor) else float(c)
+        self._c_version = None
         self.c = c if isinstance(c, torch.Tensor) else torch.tensor(c, dtype=dtype or torch.float32,
                                                                     device=self.device)
+        self._c_is_tensor = isinstance(c, torch.Tensor)
         super().__init__(dtype or self.c.dtype, shape)
 
     def c_float(self):
-        """c as a Python float (read from the device once)."""
-        if self._c_host is None:
-            self._c_host = float(self.c)
+        """c as a Python float: a Python-number coefficient is fixed; a tensor coefficient is
+        read from the device the first time and again after every in-place change."""
+        if self._c_is_tensor:
+            ver = self.c._version
+            if self._c_host is None or ver != self._c_version:
+                self._c_host = float(self.c)
+                self._c_version = ver
         return self._c_host
 
     def _matmat(self, v):

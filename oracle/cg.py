@@ -104,7 +104,7 @@ def run_batched_cg(matmat, b, x0=None, max_iters=1000, tol=1e-6, P=None):
     return state[0] * mult, state[2] * mult, state[1], info
 
 
-def cg(matmat, rhs, x0=None, P=None, tol=1e-6, max_iters=1000):
+def cg(matmat, rhs, x0=None, P=None, tol=1e-6, max_iters=5000):
     """cg.py:39-69: vector rhs is promoted to (n, 1); returns (soln, info)."""
     is_vector = rhs.dim() == 1
     if x0 is None:

@@ -144,3 +144,40 @@ def test_cg_zero_rhs_and_maxiter(dev):
     assert rel(x[:, 0], torch.linalg.solve(A.double(), b[:, 0].double())) < 1e-5
     x, r, k, info = run_batched_cg(ca.Dense(A.to(dev)), b.to(dev), None, 1, 1e-6, None)
     assert k == 1 and info["iterations"] == 2
+
+
+@pytest.mark.parametrize("B", [1, 4])
+def test_cg_warm_start_matches_oracle(dev, B):
+    """cg.py:122-130: r0 = b/||b|| - A x0 with x0 taken as given (not normalised); the returned
+    x is multiplied by ||b|| (cg.py:119).  The engine's x0 path against the oracle's."""
+    g = torch.Generator().manual_seed(40 + B)
+    n = 400
+    M = torch.randn(n, n // 2, generator=g)
+    A = M @ M.T / n + 0.1 * torch.eye(n)
+    b = torch.randn(n, B, generator=g)
+    x0 = 0.01 * torch.randn(n, B, generator=g)
+    xo, ro, ko, io = ocg.run_batched_cg(lambda V: A @ V, b, x0, 500, 1e-6, None)
+    x, r, k, info = run_batched_cg(ca.Dense(A.to(dev)), b.to(dev), x0.to(dev), 500, 1e-6, None)
+    assert abs(k - ko) <= 2
+    assert rel(x, xo) < 1e-5
+    # the same call through the drop-in functional cg() and the dataclass
+    solver = inv(ca.PSD(ca.Dense(A.to(dev))), HipCG(tol=1e-6, max_iters=500, x0=x0.to(dev)))
+    assert rel(solver @ b.to(dev), xo) < 1e-5
+
+
+def test_scalar_shift_updated_in_place_between_solves(dev):
+    """A tensor coefficient changed in place (an optimizer step on a noise parameter) must reach
+    the kernels: inv(K + c I) is solved with the current c each time."""
+    g = torch.Generator().manual_seed(77)
+    n = 300
+    M = torch.randn(n, n // 3, generator=g)
+    K = M @ M.T / n
+    b = torch.randn(n, 2, generator=g)
+    c = torch.tensor(0.5, device=dev)
+    op = ca.PSD(ca.Dense(K.to(dev)) + ca.ScalarMul(c, (n, n), device=dev))
+    for value in (0.5, 0.05):
+        with torch.no_grad():
+            c.fill_(value)
+        x = inv(op, HipCG(tol=1e-7, max_iters=1000)) @ b.to(dev)
+        xd = torch.linalg.solve(K.double() + value * torch.eye(n, dtype=torch.float64), b.double())
+        assert rel(x, xd) < 2e-5, value

@@ -214,3 +214,46 @@ def test_slq_kron64_full_config_matches_golden(dev):
     err = abs(est - abs(ref)) / abs(ref)  # cola's logdet returns |tr log A| (logdet.py:111-117)
     print(f"SLQ logdet {est:.6f} vs FP64 oracle {ref:.6f}: rel err {err:.2e}", flush=True)
     assert err < 1e-4
+
+
+@pytest.mark.parametrize("k", [1, -1, 6, -9])
+def test_hutch_kth_diagonal_matches_oracle_with_same_probes(dev, k):
+    """diagonal_estimation.py:186-195 for k != 0: estimator (A z)_j z_{j+k} on the engine against
+    the oracle restatement with the engine's probe stream, plus the reference's errors length."""
+    n = 300
+    A = spd(np.linspace(0.5, 4, n), 12)
+    op = ca.Dense(A.to(dev))
+    d, info = ca.linalg.hutchinson_diag_estimate(op, k, tol=5e-2, rand="rademacher")
+    probe_fn = lambda key, n_, bs: orng.rademacher(key, n_, bs).T  # noqa: E731
+    do, io = ohu.hutchinson_diag_estimate(lambda V: A @ V, n, k=k, tol=5e-2, rand="rademacher", probe_fn=probe_fn)
+    assert d.shape == (n - abs(k),)
+    assert info["iterations"] == io["iterations"]
+    assert len(info["errors"]) == len(io["errors"])
+    assert torch.allclose(d.cpu(), do, rtol=1e-4, atol=1e-4)
+
+
+def test_hutch_kth_diagonal_exact_through_diag(dev):
+    """diag(A, k, Exact()) returns A[j, j+k] for every k (exact_diag, diagonal_estimation.py:117-128)."""
+    n = 150
+    A = torch.randn(n, n, generator=torch.Generator().manual_seed(4))
+    op = ca.LinearOperator(torch.float32, (n, n), matmat=lambda V: A.to(dev) @ V, device=dev)
+    for k in (0, 2, -3, 149, -149):
+        got = diag(op, k, ca.linalg.Exact())
+        assert torch.allclose(got.cpu(), torch.diagonal(A, k), atol=1e-5), k
+
+
+def test_hutch_stop_rule_block_count_from_known_variance(dev):
+    """Rows of I + c (S + S^T) (S the cyclic shift) have Rademacher estimator variance 2c^2;
+    2c^2 = 0.765 makes the mean relative stderr cross tol = 3e-2 between 8 and 9 blocks of 100
+    probes, so the adaptive stop (diagonal_estimation.py:197-208) runs exactly 9 blocks: a
+    stderr formula with a wrong factor stops elsewhere."""
+    n = 1000
+    c = float(np.sqrt(0.765 / 2))
+    A = torch.eye(n)
+    idx = torch.arange(n)
+    A[idx, (idx + 1) % n] = c
+    A[idx, (idx - 1) % n] = c
+    d, info = ca.linalg.hutchinson_diag_estimate(ca.Dense(A.to(dev)), 0, tol=3e-2, rand="rademacher")
+    assert info["iterations"] == 10
+    assert len(info["errors"]) == 9 and info["errors"][-1] == info["errors"][-2]
+    assert float((d.cpu() - 1).abs().max()) < 6 * np.sqrt(0.765 / 900)

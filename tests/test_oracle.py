@@ -271,3 +271,74 @@ def test_tridiag_vs_dense():
     T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
     assert np.allclose(w, np.linalg.eigvalsh(T))
     assert np.allclose(T @ v, v * w[None, :], atol=1e-12)
+
+
+# ----------------------------------------------------------------------------- pins added for the stop rule,
+# the k-th diagonal and the per-probe quadrature
+def circulant_two_neighbours(n, c):
+    """I + c (S + S^T) with S the cyclic shift: every row has exactly two off-diagonal entries c."""
+    A = torch.eye(n, dtype=torch.float64)
+    idx = torch.arange(n)
+    A[idx, (idx + 1) % n] = c
+    A[idx, (idx - 1) % n] = c
+    return A
+
+
+def test_hutch_stop_rule_block_count_from_known_variance():
+    """Rademacher estimator of row i: (Az)_i z_i = A_ii + sum_{j != i} A_ij z_i z_j, variance
+    sum_{j != i} A_ij^2 = 2c^2.  With A_ii = 1 the mean relative stderr after b blocks of 100
+    probes is sqrt(2c^2 / (100 b)) (diagonal_estimation.py:197-203); 2c^2 = 0.765 puts
+    tol = 3e-2 between b = 8 (0.0309) and b = 9 (0.0292), so the loop runs exactly 9 blocks.
+    A stderr without the 1/(b*bs) factor, or with bs in place of b*bs, stops elsewhere."""
+    n = 1000
+    A = circulant_two_neighbours(n, np.sqrt(0.765 / 2))
+    d, info = ohu.hutchinson_diag_estimate(lambda V: A @ V, n, dtype=torch.float64, tol=3e-2, rand="rademacher")
+    assert info["iterations"] == 10  # 9 blocks + the first condition evaluation
+    # torch_tqdm.py:56-62: errors [e2..e9] plus the final error again
+    assert len(info["errors"]) == 9 and info["errors"][-1] == info["errors"][-2]
+    assert 0.0292 * 0.97 < info["errors"][-1] <= 3e-2
+    assert float((d - 1).abs().max()) < 6 * np.sqrt(0.765 / 900)
+
+
+@pytest.mark.parametrize("k", [0, 1, -1, 5, -7])
+def test_hutch_kth_diagonal_exact_with_hadamard_probes(k):
+    """Hadamard probes satisfy sum_b z_b z_b^T = n I, so one block of n probes gives the k-th
+    diagonal A[j, j+k] exactly: pins the roll direction and the slicing of :192-195."""
+    n = 64
+    H = torch.from_numpy(scipy.linalg.hadamard(n)).double()
+    A = torch.randn(n, n, dtype=torch.float64, generator=torch.Generator().manual_seed(3))
+    d, info = ohu.hutchinson_diag_estimate(lambda V: A @ V, n, dtype=torch.float64, k=k, tol=3e-2,
+                                           max_iters=1, rand="rademacher", probe_fn=lambda key, n_, bs: H)
+    assert d.shape == (n - abs(k),)
+    assert torch.allclose(d, torch.diagonal(A, k), atol=1e-12)
+
+
+def test_slq_quadratic_forms_exact_with_full_lanczos():
+    """With Lanczos run to full dimension the Gauss quadrature is exact: the per-probe form
+    ||z||^2 sum_j tau_j^2 log(theta_j) equals z^T log(A) z from a dense FP64 eigendecomposition,
+    for unit probes (whose sum is log det A) and for Rademacher probes (||z||^2 = n)."""
+    n = 24
+    A = spd(np.linspace(0.4, 6.0, n), 21)
+    w, U = torch.linalg.eigh(A)
+    logA = U @ torch.diag(torch.log(w)) @ U.T
+    for Z in (torch.eye(n, dtype=torch.float64),
+              orng.rademacher(5, n, 6, torch.float64).T):
+        _, alpha, beta, _ = olz.lanczos(lambda V: A @ V, n, Z, n, 1e-13, torch.float64)
+        q, _, _ = oun.slq_quadratic_forms(alpha, beta, (Z**2).sum(0), torch.log)
+        exact = torch.einsum("ib,ij,jb->b", Z, logA, Z)
+        assert torch.allclose(q, exact, rtol=1e-9, atol=1e-9)
+    _, alpha, beta, _ = olz.lanczos(lambda V: A @ V, n, torch.eye(n, dtype=torch.float64), n, 1e-13, torch.float64)
+    q, _, _ = oun.slq_quadratic_forms(alpha, beta, torch.ones(n, dtype=torch.float64), torch.log)
+    assert abs(float(q.sum()) - float(torch.log(w).sum())) < 1e-9
+
+
+def test_slq_quadratic_forms_agrees_with_slq_fwd_for_rademacher():
+    """slq.py:51 scales by n, the per-probe form by ||z||^2: equal for Rademacher probes, also
+    with a truncated (m < n) Lanczos run."""
+    n = 200
+    A = spd(np.linspace(0.5, 9.0, n), 8)
+    Z = orng.rademacher(9, n, 16, torch.float64).T
+    _, alpha, beta, _ = olz.lanczos(lambda V: A @ V, n, Z, 25, 1e-10, torch.float64)
+    q, _, _ = oun.slq_quadratic_forms(alpha, beta, (Z**2).sum(0), torch.log)
+    est = oun.slq_fwd(lambda V: A @ V, n, torch.log, Z, 25, 1e-10, torch.float64, torch.float64)
+    assert abs(float(q.mean()) - float(est)) < 1e-9 * abs(float(est))
