@@ -1,16 +1,19 @@
 """Benchmark of the Krylov engine on one MI355X.
 
-Default workload (the CG-solves/sec metric): batched CG on the Kronecker SPD
-operator K1 x K2 x K3 + 0.01 I, N = 64^3 = 262144, 16 N(0,1) right-hand sides,
-each Ki a 64x64 RBF kernel (lengthscale 0.1) on 64 sorted Uniform[0,1] points.
-A step is one full batched solve to the target relative residual, checked in
-FP64 for every right-hand side; value = solved right-hand sides per second.
+Default workload (the metric: CG solves/sec to 1e-6 relative residual, learned +
+mixed precision vs plain): dense RBF kernel systems K + noise I on 16384 points in
+[0,1]^4 (lengthscale ~ LogU[0.05,1], noise ~ LogU[1e-4,1e-1]), 8 N(0,1) right-hand
+sides each.  A step is one pass of FP64-residual iterative refinement around FP32 PCG
+with the learned rank-64 preconditioner (trained here, on the GPU) over every system;
+value = right-hand sides whose FP64-checked relative residual is <= 1e-6, per second.
+"plain" is the same refinement around unpreconditioned FP32 CG on the same systems.
 
-    python bench.py [--steps K] [--warmup W] [--workload kron_cg|dense_cg|lanczos|hutch|slq]
+    python bench.py [--steps K] [--warmup W]
+                    [--workload learned_cg|kron_cg|dense_cg|batched_cg|lanczos|hutch|slq|csr|copy]
 
 Prints one JSON line with the metric, a roofline object for the dominant kernel
-(measured with HIP events on the engine's stream) and a CPU baseline (the CPU
-torch restatement of the reference algorithm, timed on a bounded sample).
+(measured with HIP events on the engine's stream) and a CPU baseline (the CPU torch
+restatement of the reference algorithm, timed on a bounded sample).
 """
 import argparse
 import json
@@ -34,6 +37,9 @@ HBM_PEAK = 8000.0  # GB/s
 # WRITE_SIZE (x1.0), the factors a 1 GiB stream copy reads on the same counters (profiles/kron64_cg_64cubed_hbm_traffic.md)
 PASS_A_TRAFFIC_BYTES = (16651.2 + 16222.7) * 1024
 FP32_MFMA_PEAK = 157.3  # TFLOP/s
+# HBM bytes per dense_lds_kernel<8,4> launch (16384^2 fp32 matrix, 8 rhs): rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE scaled by the factors a 1 GiB stream copy reads on the same counters; None until measured
+DENSE8_TRAFFIC_BYTES = None
 
 
 def rbf_factor(t, ell):
@@ -443,11 +449,13 @@ def dense_rel_residual(K, shift, x, b):
 
 def bench_learned_cg(args, dev):
     """Dense RBF systems on 16384 points in [0,1]^4 (lengthscale ~ LogU[0.05,1], noise ~
-    LogU[1e-4,1e-1], 8 N(0,1) rhs each): FP64-refined PCG with the learned (or, when it
-    captures less, pivoted-Cholesky) rank-64 preconditioner vs plain FP32 CG."""
+    LogU[1e-4,1e-1], 8 N(0,1) rhs each).  value: solves/s at an FP64-checked relative residual
+    of 1e-6 of the learned (or, when it measurably captures less, pivoted-Cholesky) rank-64
+    preconditioner inside FP64-residual iterative refinement around FP32 PCG.  "plain": the same
+    refinement around unpreconditioned FP32 CG -- the reference's CG, which alone does not reach
+    1e-6 on these systems (its FP32 runs are reported too)."""
     from cola_amd.linalg import learned as L
     from cola_amd.linalg.cg import cg
-    from cola_amd.linalg.mixed import HipMixedCG, MixedSolver
     n, B = args.n, 8
     # --- training: top-64 Lanczos eigenvectors of seeded instances, Adam on the engine
     torch.cuda.synchronize()
@@ -464,80 +472,64 @@ def bench_learned_cg(args, dev):
     t_train = time.perf_counter() - t1
     loss1 = L.mean_loss(net, probe)
     del insts, probe
-    # --- held-out test systems
+    # --- held-out test systems and their preconditioners (construction timed apart)
     systems = []
     for j in range(args.systems):
         X, ell, noise = L.sample_instance(100000 + j, n)
         Xd = X.to(dev)
+        torch.cuda.synchronize()
+        ts = time.perf_counter()
         K = L.rbf_matrix(Xd, ell)
-        s32 = float(np.float32(noise))
         Kop = ca.Dense(K)
         A = ca.PSD(Kop + noise * ca.I_like(Kop))
+        torch.cuda.synchronize()
+        t_op = time.perf_counter() - ts
         b = torch.randn(n, B, generator=torch.Generator().manual_seed(200000 + j)).to(dev)
         torch.cuda.synchronize()
         ts = time.perf_counter()
         Pl = L.LearnedPreconditioner(K, Xd, net, noise)
+        torch.cuda.synchronize()
+        t_pl = time.perf_counter() - ts
         Pc = L.PivotedCholeskyPreconditioner(K, 64, noise)
         P, rep = L.select_preconditioner(K, Xd, noise, candidates={"learned": Pl, "pivoted_cholesky": Pc}, A=A)
         torch.cuda.synchronize()
-        systems.append({"K": K, "A": A, "b": b, "shift": s32, "ell": ell, "noise": noise, "Pl": Pl, "Pc": Pc,
-                        "P": P, "choice": rep["choice"], "setup_s": time.perf_counter() - ts})
+        systems.append({"K": K, "A": A, "b": b, "shift": float(np.float32(noise)), "ell": ell, "noise": noise,
+                        "Pl": Pl, "Pc": Pc, "P": P, "choice": rep["choice"], "operator_s": t_op, "learned_build_s": t_pl,
+                        "select_s": time.perf_counter() - ts})
 
-    def timed_cg(sysd, P):
+    def fp32_run(sd, P):
+        """The reference's CG (FP32, cap 1000) with a 10x tighter recursive target, so that its true
+        residual has a chance to reach the FP64-checked tolerance."""
         torch.cuda.synchronize()
         t = time.perf_counter()
-        # the FP32 solvers get a 10x tighter recursive-residual target so that their true
-        # (FP64-checked) residual can reach args.tol; 1000 is the reference's step cap
-        x, info = cg(sysd["A"], sysd["b"], P=P, tol=args.tol / 10, max_iters=1000, chunk=16)
+        x, info = cg(sd["A"], sd["b"], P=P, tol=args.tol / 10, max_iters=1000, chunk=16)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t
-        res = dense_rel_residual(sysd["K"], sysd["shift"], x, sysd["b"])
+        res = dense_rel_residual(sd["K"], sd["shift"], x, sd["b"])
         return {"steps": int(info["iterations"]) - 1, "s": dt, "max_res": float(res.max()),
                 "solved": int((res <= args.tol).sum())}
 
     per = []
-    tot = {"plain": [0.0, 0], "pivchol": [0.0, 0], "learned": [0.0, 0]}
     for sd in systems:
-        row = {"ell": sd["ell"], "noise": sd["noise"], "choice": sd["choice"], "setup_s": sd["setup_s"],
-               "resid_trace_learned": sd["Pl"].residual_trace, "resid_trace_pivchol": sd["Pc"].residual_trace}
-        for name, P in (("plain", None), ("pivchol", sd["Pc"]), ("learned", sd["Pl"])):
-            r = timed_cg(sd, P)
-            row[name] = r
-            tot[name][0] += r["s"]
-            tot[name][1] += r["solved"]
+        row = {"ell": sd["ell"], "noise": sd["noise"], "choice": sd["choice"], "operator_s": sd["operator_s"],
+               "learned_build_s": sd["learned_build_s"], "select_s": sd["select_s"],
+               "resid_trace_learned": sd["Pl"].residual_trace, "resid_trace_pivchol": sd["Pc"].residual_trace,
+               "fp32_only": {name: fp32_run(sd, P) for name, P in (("plain", None), ("pivchol", sd["Pc"]),
+                                                                   ("learned", sd["Pl"]))}}
         per.append(row)
-    # --- headline: learned/selected preconditioner inside FP64-residual refinement
-    solvers = []
-    for sd in systems:
-        itol = 1e-4 if args.inner_tol is None else args.inner_tol
-        alg = HipMixedCG(tol=args.tol, inner_tol=itol, inner_max_iters=1000, max_refinements=20, P=sd["P"],
-                         chunk=args.chunk)
-        solvers.append(MixedSolver(sd["A"], B, alg))
-    # one-time set-up, whatever --warmup is: the first call on a solver records its schedule
-    for sv, sd in zip(solvers, systems):
-        sv.solve(sd["b"])
-    for _ in range(args.warmup):
-        for sv, sd in zip(solvers, systems):
-            sv.solve(sd["b"])
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    infos = []
-    for _ in range(args.steps):
-        infos = []
-        for sv, sd in zip(solvers, systems):
-            x, info = sv.solve(sd["b"])
-            infos.append((x, info))
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t2) / args.steps
-    solved = 0
-    for (x, info), sd, row in zip(infos, systems, per):
-        res = dense_rel_residual(sd["K"], sd["shift"], x, sd["b"])
-        solved += int((res <= args.tol).sum())
-        row["mixed"] = {"refinements": info["refinements"], "inner_steps": info["inner_iterations"],
-                        "max_res": float(res.max())}
+    itol = 1e-4 if args.inner_tol is None else args.inner_tol
+    refined = {}
+    for name, pick, reps in (("learned", lambda sd: sd["P"], args.steps), ("plain", lambda sd: None, min(args.steps, 3)),
+                             ("pivchol", lambda sd: sd["Pc"], min(args.steps, 3))):
+        refined[name] = timed_refined(systems, [pick(sd) for sd in systems], B, args, itol, reps)
+        for row, r in zip(per, refined[name]["per_system"]):
+            row[name + "_refined"] = r
+    head, plain = refined["learned"], refined["plain"]
     total_rhs = B * len(systems)
-    plain_rate = tot["plain"][1] / tot["plain"][0]
-    # --- roofline: the dense matvec (8 rhs) the solves spend their time in
+    value = head["solved"] / head["s"]
+    plain_rate = plain["solved"] / plain["s"]
+    # --- roofline: the dense 8-rhs matvec (K + noise I) the solves spend their time in, timed with
+    # HIP events on the engine's stream
     from cola_amd._blocks import ld_of
     ld = ld_of(n)
     Xb = torch.randn(B, ld, device=dev)
@@ -548,32 +540,121 @@ def bench_learned_cg(args, dev):
     t_mv = time_kernel(lambda: N.call("ck_op_apply", op, B, N.ptr(Xb), ld, N.ptr(Yb), ld, N.ptr(work), wb,
                                       N.stream_ptr(dev)), reps=20)
     mv_bytes = 4 * n * n + 2 * 4 * n * B
-    roof = {"bound": "hbm", "kernel": "dense_f32_kernel<8,4> (K + noise I, 8 rhs)", "achieved": mv_bytes / t_mv / 1e9,
-            "peak": HBM_PEAK, "unit": "GB/s", "frac": mv_bytes / t_mv / 1e9 / HBM_PEAK, "traffic": None,
-            "launch_us": t_mv * 1e6, "algorithmic_bytes_per_launch": mv_bytes}
-    out = {"metric": METRIC, "value": solved / dt, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f32 (matvec, PCG) + f64 (refinement residuals, Woodbury core)",
-           "data": "synthetic (seeded points, lengthscales, noise, N(0,1) rhs)",
+    roof = {"bound": "hbm", "kernel": "dense_lds_kernel<8,4> (K + noise I, 8 rhs)", "achieved": mv_bytes / t_mv / 1e9,
+            "peak": HBM_PEAK, "unit": "GB/s", "frac": mv_bytes / t_mv / 1e9 / HBM_PEAK,
+            "traffic": DENSE8_TRAFFIC_BYTES, "launch_us": t_mv * 1e6, "algorithmic_bytes_per_launch": mv_bytes}
+    out = {"metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": head["s"] * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None,
+           "dtype": "f32 (matvec, PCG) + f64 (refinement residuals, Woodbury core)",
+           "data": "synthetic (seeded points, lengthscales, noise, N(0,1) rhs; network trained here on seeded instances)",
            "config": {"workload": f"learned_cg: {len(systems)} dense RBF systems, N={n} points in [0,1]^4, "
-                                  f"{B} rhs each, to FP64-checked {args.tol:g}",
+                                  f"lengthscale~LogU[0.05,1], noise~LogU[1e-4,1e-1], {B} rhs each, to FP64-checked "
+                                  f"{args.tol:g}",
                       "global_batch": total_rhs, "seq_len": n, "parallelism": "single GPU",
-                      "rhs_solved": solved, "rhs_total": total_rhs,
+                      "step": f"one pass of refined PCG over all {len(systems)} systems ({total_rhs} rhs)",
+                      "rhs_solved": head["solved"], "rhs_total": total_rhs, "inner_tol": itol,
+                      "plain_refined_solves_per_s": plain_rate, "plain_refined_rhs_solved": plain["solved"],
+                      "ratio_vs_plain": value / plain_rate if plain_rate > 0 else None,
+                      "pivchol_refined_solves_per_s": refined["pivchol"]["solved"] / refined["pivchol"]["s"],
+                      "inner_steps_total": {k: v["inner_steps"] for k, v in refined.items()},
                       "training": {"instances": args.train_instances, "points": args.train_n,
                                    "epochs": args.train_epochs, "lanczos_data_s": t_data, "train_s": t_train,
                                    "energy_loss_before": loss0, "energy_loss_after": loss1,
                                    "last_step_loss": float(losses[-1]) if len(losses) else None},
-                      "plain_fp32_cg_solves_per_s": plain_rate,
-                      "pivchol_pcg_solves_per_s": tot["pivchol"][1] / tot["pivchol"][0],
-                      "learned_pcg_solves_per_s": tot["learned"][1] / tot["learned"][0],
-                      "speedup_vs_plain": (solved / dt) / plain_rate if plain_rate > 0 else None,
-                      "plain_s_per_rhs_attempt": tot["plain"][0] / total_rhs,
-                      "speedup_vs_plain_attempt_time": (tot["plain"][0] / total_rhs) / (dt / max(solved, 1)),
                       "systems": per},
            "roofline": roof}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline_dense(systems[0], n, B, per[0]["plain"]["steps"], args.tol)
+        out["cpu_baseline"] = cpu_baseline_refined(systems, B, args.tol, itol)
+    del systems, per
+    torch.cuda.empty_cache()
+    if not args.no_kron:
+        # the structured Kronecker workload (64^3, 16 rhs) as a labelled sub-object
+        ka = argparse.Namespace(**vars(args))
+        ka.no_cpu_baseline, ka.inner_tol = True, None
+        kr = bench_kron_cg(ka, dev)
+        out["kron_cg"] = {"value": kr["value"], "unit": kr["unit"], "ms_per_step": kr["ms_per_step"],
+                          "config": kr["config"], "roofline": kr["roofline"]}
     return out
+
+
+def timed_refined(systems, precs, B, args, itol, reps):
+    """FP64-residual refinement around FP32 (P)CG on every system: one set-up call per solver
+    (it records the refinement schedule), args.warmup untimed passes, then `reps` timed passes
+    over all systems; every rhs is checked in FP64 afterwards."""
+    from cola_amd.linalg.mixed import HipMixedCG, MixedSolver
+    solvers = []
+    for sd, P in zip(systems, precs):
+        alg = HipMixedCG(tol=args.tol, inner_tol=itol, inner_max_iters=1000, max_refinements=args.max_refinements,
+                         P=P, chunk=args.chunk)
+        solvers.append(MixedSolver(sd["A"], B, alg))
+    for sv, sd in zip(solvers, systems):
+        sv.solve(sd["b"])
+    for _ in range(args.warmup):
+        for sv, sd in zip(solvers, systems):
+            sv.solve(sd["b"])
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    infos = []
+    for _ in range(max(1, reps)):
+        infos = [sv.solve(sd["b"]) for sv, sd in zip(solvers, systems)]
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t) / max(1, reps)
+    solved, steps, per = 0, 0, []
+    for (x, info), sd in zip(infos, systems):
+        res = dense_rel_residual(sd["K"], sd["shift"], x, sd["b"])
+        solved += int((res <= args.tol).sum())
+        steps += int(sum(info["inner_iterations"]))
+        per.append({"refinements": info["refinements"], "inner_steps": [int(k) for k in info["inner_iterations"]],
+                    "max_res": float(res.max()), "solved": int((res <= args.tol).sum())})
+    del solvers
+    return {"s": dt, "solved": solved, "inner_steps": steps, "per_system": per, "timed_passes": max(1, reps)}
+
+
+def cpu_baseline_refined(systems, B, tol, itol, budget_s=25.0, max_refinements=20):
+    """The reference algorithm on the host: the oracle's run_batched_cg (cola's CG restated in
+    torch, FP32) with the same preconditioner (its Woodbury apply in torch), inside the same FP64
+    refinement, on the systems in order until `budget_s` of CPU time is spent.  value = FP64-checked
+    solved rhs per second of that sample."""
+    from oracle import cg as ocg
+    torch.set_num_threads(cpu_threads())
+    used, solved, nsys, steps = 0.0, 0, 0, 0
+    for sd in systems:
+        K = sd["K"].cpu()
+        K64 = K.double()  # residual operator (the matrix as stored, FP64 arithmetic): set-up, not timed
+        b = sd["b"].cpu()
+        b64 = b.double()
+        s = sd["shift"]
+        P = sd["P"]
+        if P is not None:
+            F, F64, core, ps = P.F.cpu(), P.F.cpu().double(), P.core_inv.cpu(), P.s
+
+            def prec(V, F=F, F64=F64, core=core, ps=ps):
+                t = F64.T @ V.double()
+                return ((V.double() - F64 @ (core @ t)) / ps).float()
+        else:
+            prec = None
+        t0 = time.perf_counter()
+        x64 = torch.zeros(K.shape[0], B, dtype=torch.float64)
+        for _ in range(max_refinements):
+            r = b64 - (K64 @ x64 + s * x64)
+            rel = (r.norm(dim=0) / b64.norm(dim=0)).max()
+            if float(rel) <= tol:
+                break
+            d, _, k, _ = ocg.run_batched_cg(lambda V: K @ V + s * V, r.float(), None, 1000, itol, prec)
+            steps += int(k)
+            x64 += d.double()
+        res = ((K64 @ x64 + s * x64) - b64).norm(dim=0) / b64.norm(dim=0)
+        used += time.perf_counter() - t0
+        solved += int((res <= tol).sum())
+        nsys += 1
+        del K64
+        if used > budget_s:
+            break
+    return {"value": solved / used, "unit": "solves/s", "cores": cpu_threads(), "kind": "port",
+            "sample": f"the first {nsys} of {len(systems)} systems ({nsys * B} rhs, {solved} reached {tol:g}) with the "
+                      f"oracle's CG (FP32 torch, same preconditioner, {steps} inner steps) inside FP64 refinement: "
+                      f"{used:.1f} s on {cpu_threads()} host threads"}
 
 
 def gp_systems(nsys, dev, bf16=True):
@@ -731,16 +812,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="kron_cg")
+    ap.add_argument("--workload", default="learned_cg")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--chunk", type=int, default=8)
     ap.add_argument("--inner-tol", type=float, default=None,
                     help="correction-solve tolerance (default 1e-3 for kron_cg, where one eigenbasis-preconditioned "
                          "step reaches it, 3e-5 for batched_cg, 1e-4 for learned_cg)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-kron", action="store_true", help="leave out the Kronecker sub-object of learned_cg")
     ap.add_argument("--no-graph", action="store_true", help="launch the refinement schedule eagerly (no HIP graph)")
     ap.add_argument("--n", type=int, default=16384, help="points per system (learned_cg)")
-    ap.add_argument("--systems", type=int, default=4, help="held-out systems (learned_cg)")
+    ap.add_argument("--systems", type=int, default=8, help="held-out systems (learned_cg)")
+    ap.add_argument("--max-refinements", type=int, default=30, help="refinement cap (learned_cg)")
     ap.add_argument("--train-instances", type=int, default=2000)
     ap.add_argument("--train-n", type=int, default=1024)
     ap.add_argument("--train-epochs", type=int, default=4)

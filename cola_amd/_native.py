@@ -11,7 +11,8 @@ import torch
 
 _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libcolakrylov.so")
 
-CK_OK, CK_ERR_BAD_ARG, CK_ERR_HIP, CK_ERR_UNSUPPORTED, CK_ERR_WORKSPACE, CK_ERR_NOCONV = range(6)
+(CK_OK, CK_ERR_BAD_ARG, CK_ERR_HIP, CK_ERR_UNSUPPORTED, CK_ERR_WORKSPACE, CK_ERR_NOCONV, CK_ERR_MAXITER,
+ CK_ERR_BREAKDOWN, CK_ERR_NAN, CK_ERR_TIMEOUT) = range(10)
 (CK_OP_IDENTITY, CK_OP_DENSE_F32, CK_OP_DENSE_BF16, CK_OP_CSR_F32, CK_OP_KRON_F32, CK_OP_DIAG_F32,
  CK_OP_WOODBURY_INV, CK_OP_RBF_F32, CK_OP_KRON_EIG_F32, CK_OP_DENSE_BATCHED_F32) = range(10)
 CK_CG_MULT, CK_CG_TOL, CK_CG_GAMMA, CK_CG_ALPHA, CK_CG_BETA, CK_CG_RNORM, CK_CG_ERRORS = range(7)
@@ -111,6 +112,10 @@ _PROTOS = {
     "ck_cg_iterate_final_f64": (ctypes.c_int, [_op, _op, _i32, _p, _p, _p, _p, _p, _i64, _p, _p, _p, _sz, _i32, _p,
                                                _i64, _p]),
     "ck_cg_finish": (ctypes.c_int, [_i64, _i32, _p, _p, _i64, _p, _p]),
+    "ck_cg_fused_workspace_bytes": (_sz, [_i64, _i32]),
+    "ck_cg_fused_supported": (ctypes.c_int, [_op, _op, _i32]),
+    "ck_cg_fused": (ctypes.c_int, [_op, _op, _i32, _p, _i64, _i64, _p, _i64, _i64, _p, _i64, _i64, _p, _i64, _i64, _p,
+                                   _f32, _i32, _p, _sz, _i32, _i64, _i32, _p]),
     "ck_lanczos_state_bytes": (_sz, [_i32, _i32]),
     "ck_lanczos_scratch_bytes": (_sz, [_i64, _i32, _i32]),
     "ck_lanczos_start": (ctypes.c_int, [_i64, _i32, _p, _i64, _p, _i64, _i64, _p, _p, _p, _i32, _p, _p]),

@@ -7,6 +7,7 @@
 // recomputes its own column's p.Ap from the partials, so no grid-wide barrier
 // or atomic is needed; the cross-column decisions (k, done, errors) are made by
 // a one-block kernel at the end of each step.
+#include "cg_state.hpp"
 #include "common.hpp"
 #include "ops_internal.hpp"
 
@@ -17,33 +18,9 @@ namespace {
 constexpr double kSmall = (double)(float)1e-40;  // cg.py:8, as an fp32 constant
 constexpr int kChunk = 4096;                      // elements per partial (256 threads x 16)
 
-struct CgView {
-  ck_cg_ctrl* ctrl;
-  float* mult;
-  float* tol;
-  float* gamma;
-  float* alpha;
-  float* beta;
-  float* rnorm;
-  float* errors;
-};
+__host__ __device__ inline int stride_of(int nrhs) { return cg_stride_of(nrhs); }
 
-__host__ __device__ inline int stride_of(int nrhs) { return (nrhs + 63) / 64 * 64; }
-
-__host__ __device__ inline CgView view(void* state, int nrhs) {
-  CgView v;
-  v.ctrl = static_cast<ck_cg_ctrl*>(state);
-  float* f = reinterpret_cast<float*>(static_cast<char*>(state) + sizeof(ck_cg_ctrl));
-  const int s = stride_of(nrhs);
-  v.mult = f + CK_CG_MULT * s;
-  v.tol = f + CK_CG_TOL * s;
-  v.gamma = f + CK_CG_GAMMA * s;
-  v.alpha = f + CK_CG_ALPHA * s;
-  v.beta = f + CK_CG_BETA * s;
-  v.rnorm = f + CK_CG_RNORM * s;
-  v.errors = f + CK_CG_ERRORS * s;
-  return v;
-}
+__host__ __device__ inline CgView view(void* state, int nrhs) { return cg_view(state, nrhs); }
 
 inline int nparts_of(int64_t n) {
   int64_t p = cdiv(n, kChunk);
@@ -191,6 +168,10 @@ __global__ __launch_bounds__(256) void start_final_kernel(int nrhs, const float*
     v.ctrl->nrhs = nrhs;
     v.ctrl->nparts = P;
     v.ctrl->done = (any && 0 < max_iters) ? 0 : 1;  // cg.py:133-138
+    // NaN compares false: a non-finite start stops the loop as in the reference
+    bool finite = true;
+    for (int b = 0; b < nrhs; ++b) finite = finite && isfinite(v.rnorm[b]);
+    v.ctrl->status = !finite ? CK_ERR_NAN : (any && v.ctrl->done) ? CK_ERR_MAXITER : CK_OK;
   }
 }
 
@@ -211,9 +192,14 @@ __global__ __launch_bounds__(256) void update_xr_kernel(int64_t n, const float* 
     const double pap = column_total(part0, np0, b);
     if (threadIdx.x == 0) {
       double a = safe_div((double)v.gamma[b], pap);
-      if ((double)v.rnorm[b] < kSmall) a = 0.0;  // has_converged (cg.py:143-144,161)
+      const bool frozen = (double)v.rnorm[b] < kSmall;
+      if (frozen) a = 0.0;  // has_converged (cg.py:143-144,161)
       alpha_s = (float)a;
-      if (blk == 0) v.alpha[b] = (float)a;
+      if (blk == 0) {
+        v.alpha[b] = (float)a;
+        // the reference divides on regardless; the status records it (same value from every writer)
+        if (!frozen && !(pap > 0.0) && v.ctrl->status == CK_OK) v.ctrl->status = CK_ERR_BREAKDOWN;
+      }
     }
   }
   __syncthreads();
@@ -378,7 +364,15 @@ __global__ __launch_bounds__(256) void step_final_kernel(int nrhs, const float* 
     }
     v.ctrl->k = k;
     v.errors[k] = mean / (float)nrhs;
-    if (!(any && k < v.ctrl->max_iters)) v.ctrl->done = 1;
+    if (!(any && k < v.ctrl->max_iters)) {
+      v.ctrl->done = 1;
+      bool finite = true;
+      for (int b = 0; b < nrhs; ++b) finite = finite && isfinite(v.rnorm[b]);
+      if (!finite)
+        v.ctrl->status = CK_ERR_NAN;
+      else if (any && v.ctrl->status == CK_OK)
+        v.ctrl->status = CK_ERR_MAXITER;
+    }
   }
 }
 

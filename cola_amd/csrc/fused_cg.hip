@@ -1,0 +1,569 @@
+// Persistent fused CG: a whole batched CG solve in one cooperative launch.
+//
+// Arithmetic and stopping rule are those of cola/linalg/inverse/cg.py:94-178 (column
+// normalisation, threshold tol*||r0|| + tol, safe division at 1e-40, frozen columns), the
+// loop of cola/utils/torch_tqdm.py:86-90 runs inside the kernel.  Scope: small dense
+// operators (y = scale*A x + shift*x) whose solves are launch- and latency-bound, with no
+// preconditioner or a diagonal one.
+//
+// Each workgroup owns a contiguous block of R rows.  Its rows of A are staged in LDS once
+// (when they fit), and every workgroup keeps the FULL r and p vectors in LDS.  A step is
+//   Ap_own = A_own p                     (rows from LDS, p from LDS)
+//   publish Ap_own and the partial p.Ap  (one grid-wide exchange)
+//   alpha = gamma / sum_w partial_w      (every workgroup, same order -> same bits)
+//   x_own += alpha p_own;  r -= alpha Ap (all n entries, Ap read back from global)
+//   gamma' = r.z, ||r||, beta, p = z + beta p   (all n entries, redundantly per workgroup)
+// so a step needs exactly one grid-wide synchronisation: the p.Ap all-reduce doubles as
+// the barrier that makes Ap complete.  The exchange buffers (Ap and the partials) are
+// double-buffered by step parity: a workgroup can run at most one exchange ahead of the
+// slowest reader, so the buffer it writes is never one still being read.
+//
+// Synchronisation: workgroup w publishes generation g by an agent-scope release store of
+// gen[w] = g after a fence that orders its Ap and partial stores; readers poll all gen[]
+// with agent-scope loads (wave 0, one wave vote per round) and fence before reading.  The
+// wait is bounded: after spin_limit polling rounds it gives up, records CK_ERR_TIMEOUT in
+// the control block and leaves the loop, so every workgroup ends and the launch drains.
+// Co-residency of all workgroups is guaranteed by hipLaunchCooperativeKernel.
+#include "cg_state.hpp"
+#include "common.hpp"
+
+using namespace ck;
+
+namespace {
+
+constexpr double kSmall = (double)(float)1e-40;  // cg.py:8, as an fp32 constant
+constexpr int kNT = 256;
+constexpr int kMaxRhs = 8;
+constexpr int kMaxBlocks = 256;
+constexpr size_t kLdsBudget = 56 * 1024;  // dynamic LDS; the static arrays take < 5 KB more
+constexpr int64_t kDefaultSpin = int64_t(1) << 22;
+
+struct FusedArgs {
+  int n, nn, nrhs, G, R;  // order, padded order (multiple of 4), columns, workgroups, rows per workgroup
+  const float* A;
+  int64_t lda;
+  float scale, shift;
+  const float* d;  // diagonal preconditioner (nullptr: none): z = dscale * d * r + dshift * r
+  float dscale, dshift;
+  const float* b;
+  int64_t b_rs, b_cs;
+  const float* x0;
+  int64_t x0_rs, x0_cs;
+  float* x;
+  int64_t x_rs, x_cs;
+  float* r;
+  int64_t r_rs, r_cs;
+  void* state;
+  float tol;
+  int max_iters;
+  float* apbuf;    // 2 x nrhs x nn
+  double* part;    // 2 x kMaxBlocks x nrhs
+  uint32_t* gen;   // kMaxBlocks
+  int64_t spin_limit;
+  uint32_t wait_offset;
+};
+
+__device__ inline double safe_div(double num, double den) {
+  if (fabs(den) < kSmall) den = kSmall;
+  return num / den;
+}
+
+// Sum of NV per-thread values over the block, fixed order; every thread gets the totals.
+template <int NV>
+__device__ inline void block_sums(double (&v)[NV], double* lds /* 4 * NV */) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    const double s = wave_sum(v[i]);
+    if (lane == 0) lds[wave * NV + i] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = (lds[i] + lds[NV + i]) + (lds[2 * NV + i] + lds[3 * NV + i]);
+  __syncthreads();
+}
+
+// Make this workgroup's global stores visible at agent scope, then publish generation g.
+__device__ inline void publish(uint32_t* gen, uint32_t g) {
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(gen + blockIdx.x, g, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wait until every workgroup has published generation >= target (bounded); true on success.
+__device__ inline bool wait_all(const uint32_t* gen, int G, uint32_t target, int64_t spin_limit, int* flag) {
+  if (threadIdx.x < 64) {
+    bool ok = false;
+    for (int64_t it = 0;; ++it) {
+      bool mine = true;
+      for (int w = threadIdx.x; w < G; w += 64) {
+        const uint32_t g = __hip_atomic_load(gen + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        mine = mine && (g >= target);
+      }
+      if (__all(mine)) {
+        ok = true;
+        break;
+      }
+      if (it >= spin_limit) break;
+      __builtin_amdgcn_s_sleep(1);
+    }
+    if (threadIdx.x == 0) *flag = ok ? 1 : 0;
+  }
+  __syncthreads();
+  const bool ok = *flag != 0;
+  __threadfence();  // acquire: the published Ap and partials are read after this
+  return ok;
+}
+
+// NB: columns rounded up to 1, 2, 4 or 8 (sizes the per-thread sums); TILE: own rows of A in LDS
+template <bool TILE, int NB>
+__global__ __launch_bounds__(kNT) void fused_cg_kernel(FusedArgs a) {
+  extern __shared__ float lds[];
+  __shared__ double red[4 * 2 * NB];
+  __shared__ double rowprod[NB * 64];
+  __shared__ float s_alpha[NB], s_beta[NB], s_gamma[NB], s_rnorm[NB], s_tol[NB], s_mult[NB];
+  __shared__ int s_flag;
+  const int n = a.n, nn = a.nn, B = a.nrhs, w = blockIdx.x, tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int row0 = w * a.R;
+  const int rows = max(0, min(a.R, n - row0));
+  float* pv = lds;            // B x nn: full p
+  float* rv = pv + B * nn;    // B x nn: full r
+  float* At = rv + B * nn;    // R x nn: own rows of A (TILE)
+  CgView v = cg_view(a.state, B);
+  const bool pre = a.d != nullptr;
+
+  if (TILE) {
+    for (int64_t idx = tid; idx < (int64_t)a.R * nn; idx += kNT) {
+      const int i = (int)(idx / nn), j = (int)(idx % nn);
+      At[idx] = (i < rows && j < n) ? a.A[(int64_t)(row0 + i) * a.lda + j] : 0.f;
+    }
+  }
+  // cg.py:96-97: column norms of b (every workgroup, same order) and r = b / ||b||
+  {
+    double s[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) s[c] = 0.0;
+    for (int j = tid; j < n; j += kNT)
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c < B) {
+          const double bj = (double)a.b[(int64_t)j * a.b_rs + (int64_t)c * a.b_cs];
+          s[c] += bj * bj;
+        }
+    block_sums<NB>(s, red);
+    if (tid < B) s_mult[tid] = (float)sqrt(s[tid]);
+    __syncthreads();
+  }
+  for (int c = 0; c < B; ++c) {
+    float m = s_mult[c];
+    if (fabsf(m) < 1e-40f) m = 1e-40f;
+    for (int j = tid; j < nn; j += kNT) rv[c * nn + j] = j < n ? a.b[(int64_t)j * a.b_rs + (int64_t)c * a.b_cs] / m : 0.f;
+  }
+  uint32_t sync = 0;
+  bool timeout = false;
+  if (a.x0) {
+    // cg.py:123: r0 = b_norm - A x0 (x0 as given); one exchange of A x0 through the odd Ap
+    // buffer (the first step writes the even one, so a fast workgroup never overwrites it)
+    float* ax0 = a.apbuf + (int64_t)B * nn;
+    for (int c = 0; c < B; ++c)
+      for (int j = tid; j < nn; j += kNT) pv[c * nn + j] = j < n ? a.x0[(int64_t)j * a.x0_rs + (int64_t)c * a.x0_cs] : 0.f;
+    __syncthreads();
+    for (int i = wave; i < rows; i += 4) {
+      const float* arow = TILE ? At + (int64_t)i * nn : a.A + (int64_t)(row0 + i) * a.lda;
+      for (int c = 0; c < B; ++c) {
+        float acc = 0.f;
+        for (int j = lane; j < n; j += 64) acc = fmaf(arow[j], pv[c * nn + j], acc);
+        acc = wave_sum(acc);
+        if (lane == 0) {
+          float y = a.scale * acc;
+          if (a.shift != 0.f) y = fmaf(a.shift, pv[c * nn + row0 + i], y);
+          ax0[(int64_t)c * nn + row0 + i] = y;
+        }
+      }
+    }
+    publish(a.gen, ++sync);
+    if (!wait_all(a.gen, a.G, sync + a.wait_offset, a.spin_limit, &s_flag)) timeout = true;
+    if (!timeout) {
+      for (int c = 0; c < B; ++c)
+        for (int j = tid; j < n; j += kNT) rv[c * nn + j] -= ax0[(int64_t)c * nn + j];
+      for (int idx = tid; idx < rows * B; idx += kNT) {
+        const int i = idx / B, c = idx % B;
+        a.x[(int64_t)(row0 + i) * a.x_rs + (int64_t)c * a.x_cs] = pv[c * nn + row0 + i];
+      }
+    }
+  } else {
+    for (int idx = tid; idx < rows * B; idx += kNT) {
+      const int i = idx / B, c = idx % B;
+      a.x[(int64_t)(row0 + i) * a.x_rs + (int64_t)c * a.x_cs] = 0.f;
+    }
+  }
+  __syncthreads();
+  // cg.py:98-101,122-130: z = P r, p = z, gamma = r.z, threshold tol*||r0|| + tol
+  {
+    double s[2 * NB];
+#pragma unroll
+    for (int c = 0; c < 2 * NB; ++c) s[c] = 0.0;
+    for (int c = 0; c < B; ++c)
+      for (int j = tid; j < nn; j += kNT) {
+        const float rj = rv[c * nn + j];
+        const float zj = (pre && j < n) ? fmaf(a.dscale * a.d[j], rj, a.dshift * rj) : rj;
+        pv[c * nn + j] = zj;
+        s[c] += (double)rj * (double)rj;
+        s[NB + c] += (double)rj * (double)zj;
+      }
+    block_sums<2 * NB>(s, red);
+    if (tid < B) {
+      const float rn = (float)sqrt(s[tid]);
+      s_rnorm[tid] = rn;
+      s_gamma[tid] = (float)s[NB + tid];
+      s_tol[tid] = a.tol * rn + a.tol;
+      s_alpha[tid] = 0.f;
+      s_beta[tid] = 0.f;
+    }
+    __syncthreads();
+  }
+  int k = 0, status = CK_OK;
+  bool done;
+  {
+    float mean = 0.f;
+    bool any = false, finite = true;
+    for (int c = 0; c < B; ++c) {
+      mean += s_rnorm[c];
+      any = any || (s_rnorm[c] > s_tol[c]);
+      finite = finite && isfinite(s_rnorm[c]);
+    }
+    if (w == 0 && tid == 0) v.errors[0] = mean / (float)B;
+    done = !(any && 0 < a.max_iters);
+    if (!finite) status = CK_ERR_NAN;
+  }
+  while (!done && !timeout) {
+    const int par = k & 1;
+    float* ap = a.apbuf + (int64_t)par * B * nn;
+    double* part = a.part + (int64_t)par * kMaxBlocks * B;
+    // Ap for the own rows; rowprod = p_i * (Ap)_i
+    for (int i = wave; i < rows; i += 4) {
+      const float* arow = TILE ? At + (int64_t)i * nn : a.A + (int64_t)(row0 + i) * a.lda;
+      float acc[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) acc[c] = 0.f;
+      if (TILE) {
+        for (int j = 4 * lane; j < nn; j += 256) {
+          const float4 av = *reinterpret_cast<const float4*>(arow + j);
+#pragma unroll
+          for (int c = 0; c < NB; ++c)
+            if (c < B) {
+              const float4 p4 = *reinterpret_cast<const float4*>(pv + c * nn + j);
+              acc[c] = fmaf(av.x, p4.x, acc[c]);
+              acc[c] = fmaf(av.y, p4.y, acc[c]);
+              acc[c] = fmaf(av.z, p4.z, acc[c]);
+              acc[c] = fmaf(av.w, p4.w, acc[c]);
+            }
+        }
+      } else {
+        for (int j = lane; j < n; j += 64) {
+          const float aij = arow[j];
+#pragma unroll
+          for (int c = 0; c < NB; ++c)
+            if (c < B) acc[c] = fmaf(aij, pv[c * nn + j], acc[c]);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c < B) {
+          const float s = wave_sum(acc[c]);
+          if (lane == 0) {
+            const float pi = pv[c * nn + row0 + i];
+            float y = a.scale * s;
+            if (a.shift != 0.f) y = fmaf(a.shift, pi, y);
+            ap[(int64_t)c * nn + row0 + i] = y;
+            rowprod[c * 64 + i] = (double)pi * (double)y;
+          }
+        }
+    }
+    __syncthreads();
+    // this workgroup's p.Ap partial per column (its <= 64 rows in order)
+    if (tid < B) {
+      double s = 0.0;
+      for (int i = 0; i < rows; ++i) s += rowprod[tid * 64 + i];
+      part[w * B + tid] = s;
+    }
+    publish(a.gen, ++sync);
+    if (!wait_all(a.gen, a.G, sync + a.wait_offset, a.spin_limit, &s_flag)) {
+      timeout = true;
+      break;
+    }
+    // alpha = gamma / p.Ap (cg.py:143-147); the partials are added in workgroup order
+    if (wave == 0) {
+      for (int c = 0; c < B; ++c) {
+        double s = 0.0;
+        for (int u = lane; u < a.G; u += 64) s += part[u * B + c];
+        s = wave_sum(s);
+        if (lane == 0) {
+          const bool frozen = (double)s_rnorm[c] < kSmall;
+          double al = safe_div((double)s_gamma[c], s);
+          if (frozen) al = 0.0;
+          if (!frozen && !(s > 0.0) && status == CK_OK) status = CK_ERR_BREAKDOWN;
+          s_alpha[c] = (float)al;
+        }
+      }
+    }
+    __syncthreads();
+    // x_own += alpha p_own
+    for (int idx = tid; idx < rows * B; idx += kNT) {
+      const int i = idx / B, c = idx % B;
+      float* xp = a.x + (int64_t)(row0 + i) * a.x_rs + (int64_t)c * a.x_cs;
+      *xp = fmaf(s_alpha[c], pv[c * nn + row0 + i], *xp);
+    }
+    // r -= alpha Ap (all entries), gamma' = r.z, ||r||^2
+    double s[2 * NB];
+#pragma unroll
+    for (int c = 0; c < 2 * NB; ++c) s[c] = 0.0;
+    for (int c = 0; c < B; ++c) {
+      const float al = s_alpha[c];
+      for (int j = tid; j < n; j += kNT) {
+        const float rj = fmaf(-al, ap[(int64_t)c * nn + j], rv[c * nn + j]);
+        rv[c * nn + j] = rj;
+        const float zj = pre ? fmaf(a.dscale * a.d[j], rj, a.dshift * rj) : rj;
+        s[c] += (double)rj * (double)rj;
+        s[NB + c] += (double)rj * (double)zj;
+      }
+    }
+    block_sums<2 * NB>(s, red);
+    // beta = gamma'/gamma (cg.py:150-154), frozen columns keep beta = 0
+    if (tid < B) {
+      const bool frozen = (double)s_rnorm[tid] < kSmall;
+      double be = safe_div(s[NB + tid], (double)s_gamma[tid]);
+      if (frozen) be = 0.0;
+      s_beta[tid] = (float)be;
+      s_gamma[tid] = (float)s[NB + tid];
+      s_rnorm[tid] = (float)sqrt(s[tid]);
+    }
+    __syncthreads();
+    for (int c = 0; c < B; ++c) {
+      const float be = s_beta[c];
+      for (int j = tid; j < n; j += kNT) {
+        const float rj = rv[c * nn + j];
+        const float zj = pre ? fmaf(a.dscale * a.d[j], rj, a.dshift * rj) : rj;
+        pv[c * nn + j] = fmaf(be, pv[c * nn + j], zj);
+      }
+    }
+    // k, errors[k], loop condition (cg.py:133-138)
+    ++k;
+    float mean = 0.f;
+    bool any = false, finite = true;
+    for (int c = 0; c < B; ++c) {
+      mean += s_rnorm[c];
+      any = any || (s_rnorm[c] > s_tol[c]);
+      finite = finite && isfinite(s_rnorm[c]);
+    }
+    if (w == 0 && tid == 0) v.errors[k] = mean / (float)B;
+    done = !(any && k < a.max_iters);
+    if (done) {
+      if (!finite)
+        status = CK_ERR_NAN;
+      else if (any && status == CK_OK)
+        status = CK_ERR_MAXITER;
+    }
+    __syncthreads();
+  }
+  if (timeout) {
+    if (tid == 0) __hip_atomic_store(&v.ctrl->status, (int32_t)CK_ERR_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (w == 0 && tid == 0) {
+      v.ctrl->k = k;
+      v.ctrl->done = 1;
+    }
+    return;
+  }
+  // cg.py:119: x and r back to the scale of b (own rows)
+  for (int idx = tid; idx < rows * B; idx += kNT) {
+    const int i = idx / B, c = idx % B;
+    const float m = s_mult[c];
+    float* xp = a.x + (int64_t)(row0 + i) * a.x_rs + (int64_t)c * a.x_cs;
+    *xp = *xp * m;
+    if (a.r) a.r[(int64_t)(row0 + i) * a.r_rs + (int64_t)c * a.r_cs] = rv[c * nn + row0 + i] * m;
+  }
+  if (w == 0) {
+    if (tid < B) {
+      v.mult[tid] = s_mult[tid];
+      v.tol[tid] = s_tol[tid];
+      v.gamma[tid] = s_gamma[tid];
+      v.alpha[tid] = s_alpha[tid];
+      v.beta[tid] = s_beta[tid];
+      v.rnorm[tid] = s_rnorm[tid];
+    }
+    if (tid == 0) {
+      v.ctrl->k = k;
+      v.ctrl->done = 1;
+      v.ctrl->max_iters = a.max_iters;
+      v.ctrl->nrhs = B;
+      v.ctrl->nparts = a.G;
+      __hip_atomic_fetch_max(&v.ctrl->status, (int32_t)status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// Clears the generation words and the status before a solve.
+__global__ void fused_reset_kernel(uint32_t* gen, int G, void* state) {
+  for (int i = threadIdx.x; i < G; i += blockDim.x) gen[i] = 0u;
+  if (threadIdx.x == 0) static_cast<ck_cg_ctrl*>(state)->status = CK_OK;
+}
+
+struct Plan {
+  bool ok, tile;
+  int G, R, nn;
+  size_t lds;
+};
+
+size_t lds_bytes(int nn, int B, int R, bool tile) { return sizeof(float) * ((size_t)2 * B * nn + (tile ? (size_t)R * nn : 0)); }
+
+template <bool TILE>
+const void* kernel_nb(int nb) {
+  switch (nb) {
+    case 1:
+      return reinterpret_cast<const void*>(&fused_cg_kernel<TILE, 1>);
+    case 2:
+      return reinterpret_cast<const void*>(&fused_cg_kernel<TILE, 2>);
+    case 4:
+      return reinterpret_cast<const void*>(&fused_cg_kernel<TILE, 4>);
+    default:
+      return reinterpret_cast<const void*>(&fused_cg_kernel<TILE, 8>);
+  }
+}
+
+const void* kernel_of(bool tile, int nb) { return tile ? kernel_nb<true>(nb) : kernel_nb<false>(nb); }
+
+int nb_of(int B) { return B <= 1 ? 1 : B <= 2 ? 2 : B <= 4 ? 4 : 8; }
+
+// Workgroups that can be resident at once for (kernel, LDS bytes), cached per device.
+int max_coresident(bool tile, int nb, size_t lds) {
+  struct Entry {
+    int dev, key;
+    size_t lds;
+    int val;
+  };
+  static Entry cache[64];
+  static int used = 0;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  const int key = (tile ? 16 : 0) + nb;
+  for (int i = 0; i < used && i < 64; ++i)
+    if (cache[i].dev == dev && cache[i].key == key && cache[i].lds == lds) return cache[i].val;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  int per = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel_of(tile, nb), kNT, lds) != hipSuccess) return 0;
+  const int val = per * cus;
+  if (used < 64) cache[used++] = Entry{dev, key, lds, val};
+  return val;
+}
+
+// Workgroups and layout for order n, B columns; nblocks > 0 asks for that many workgroups.
+Plan make_plan(int n, int B, int nblocks) {
+  Plan p{};
+  p.nn = (n + 3) / 4 * 4;
+  if (n < 1 || B < 1 || B > kMaxRhs) return p;
+  if (lds_bytes(p.nn, B, 0, false) > kLdsBudget) return p;
+  // each wave handles rows i, i + 4, ...: at most 64 rows per workgroup (the p.Ap slots)
+  const int gmin = (int)cdiv(n, 64);
+  int G = nblocks > 0 ? nblocks : (int)cdiv(n, 8);
+  if (G > kMaxBlocks) G = kMaxBlocks;
+  if (G < gmin) G = gmin;
+  if (G > kMaxBlocks || G < 1) return p;
+  int R = (int)cdiv(n, G);
+  G = (int)cdiv(n, R);
+  p.tile = lds_bytes(p.nn, B, R, true) <= kLdsBudget;
+  p.G = G;
+  p.R = R;
+  p.lds = lds_bytes(p.nn, B, R, p.tile);
+  p.ok = true;
+  return p;
+}
+
+size_t work_bytes_of(int nn, int B) {
+  return sizeof(float) * (size_t)2 * B * nn + sizeof(double) * (size_t)2 * kMaxBlocks * B +
+         sizeof(uint32_t) * kMaxBlocks + 256;
+}
+
+bool fused_eligible(const ck_op* A, const ck_op* P, int nrhs) {
+  if (!A || A->kind != CK_OP_DENSE_F32 || !A->a || A->lda < A->n) return false;
+  if (P && !(P->kind == CK_OP_DIAG_F32 && P->a && P->n == A->n)) return false;
+  return nrhs >= 1 && nrhs <= kMaxRhs;
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t ck_cg_fused_workspace_bytes(int64_t n, int32_t nrhs) {
+  if (n < 1 || nrhs < 1) return 0;
+  return work_bytes_of((int)((n + 3) / 4 * 4), nrhs);
+}
+
+int ck_cg_fused_supported(const ck_op* A, const ck_op* P, int32_t nrhs) {
+  if (!fused_eligible(A, P, nrhs)) return CK_ERR_UNSUPPORTED;
+  Plan p = make_plan(A->n, nrhs, 0);
+  return p.ok ? CK_OK : CK_ERR_UNSUPPORTED;
+}
+
+int ck_cg_fused(const ck_op* A, const ck_op* P, int32_t nrhs, const float* b, int64_t b_rs, int64_t b_cs,
+                const float* x0, int64_t x0_rs, int64_t x0_cs, float* x, int64_t x_rs, int64_t x_cs, float* r,
+                int64_t r_rs, int64_t r_cs, void* state, float tol, int32_t max_iters, void* work,
+                size_t work_bytes, int32_t nblocks, int64_t spin_limit, int32_t wait_offset, void* stream) {
+  if (!A || !b || !x || !state || !work || max_iters < 0 || nblocks < 0 || wait_offset < 0) return CK_ERR_BAD_ARG;
+  if (!fused_eligible(A, P, nrhs)) return CK_ERR_UNSUPPORTED;
+  const int n = A->n;
+  Plan pl = make_plan(n, nrhs, nblocks);
+  if (!pl.ok) return CK_ERR_UNSUPPORTED;
+  if (work_bytes < work_bytes_of(pl.nn, nrhs)) return CK_ERR_WORKSPACE;
+  const int nb = nb_of(nrhs);
+  if (pl.G > max_coresident(pl.tile, nb, pl.lds)) return CK_ERR_UNSUPPORTED;
+  FusedArgs a{};
+  a.n = n;
+  a.nn = pl.nn;
+  a.nrhs = nrhs;
+  a.G = pl.G;
+  a.R = pl.R;
+  a.A = static_cast<const float*>(A->a);
+  a.lda = A->lda;
+  a.scale = A->scale;
+  a.shift = A->shift;
+  if (P) {
+    a.d = static_cast<const float*>(P->a);
+    a.dscale = P->scale;
+    a.dshift = P->shift;
+  }
+  a.b = b;
+  a.b_rs = b_rs;
+  a.b_cs = b_cs;
+  a.x0 = x0;
+  a.x0_rs = x0_rs;
+  a.x0_cs = x0_cs;
+  a.x = x;
+  a.x_rs = x_rs;
+  a.x_cs = x_cs;
+  a.r = r;
+  a.r_rs = r_rs;
+  a.r_cs = r_cs;
+  a.state = state;
+  a.tol = tol;
+  a.max_iters = max_iters;
+  char* wp = static_cast<char*>(work);
+  a.apbuf = reinterpret_cast<float*>(wp);
+  wp += sizeof(float) * (size_t)2 * nrhs * pl.nn;
+  wp = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(wp) + 255) & ~uintptr_t(255));
+  a.part = reinterpret_cast<double*>(wp);
+  wp += sizeof(double) * (size_t)2 * kMaxBlocks * nrhs;
+  a.gen = reinterpret_cast<uint32_t*>(wp);
+  a.spin_limit = spin_limit > 0 ? spin_limit : kDefaultSpin;
+  a.wait_offset = (uint32_t)wait_offset;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(fused_reset_kernel, dim3(1), dim3(256), 0, s, a.gen, pl.G, state);
+  CK_CHECK_LAUNCH();
+  void* args[] = {&a};
+  if (hipLaunchCooperativeKernel(kernel_of(pl.tile, nb), dim3(pl.G), dim3(kNT), args, (unsigned)pl.lds, s) != hipSuccess)
+    return CK_ERR_HIP;
+  return CK_OK;
+}
+
+}  // extern "C"

@@ -17,6 +17,14 @@ extern "C" const char* ck_status_string(int status) {
       return "workspace too small";
     case CK_ERR_NOCONV:
       return "tridiagonal QL iteration did not converge";
+    case CK_ERR_MAXITER:
+      return "iteration cap reached before the tolerance";
+    case CK_ERR_BREAKDOWN:
+      return "breakdown: p.Ap <= 0 (operator or preconditioner not positive definite)";
+    case CK_ERR_NAN:
+      return "non-finite residual";
+    case CK_ERR_TIMEOUT:
+      return "grid-wide wait timed out";
     default:
       return "unknown status";
   }

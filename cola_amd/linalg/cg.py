@@ -32,6 +32,8 @@ class HipCG(Algorithm):
         x0 (Tensor): (n,) or (n, b) initial guess.
         P (LinearOperator): preconditioner, applied as z = P r (an approximation of A^-1).
         chunk (int): steps enqueued between two reads of the device loop flag.
+        fused (bool): run small dense systems (with no or a diagonal preconditioner) as one
+            persistent kernel launch (ck_cg_fused); other systems use chained launches.
     """
     tol: float = 1e-6
     max_iters: int = 1000
@@ -39,20 +41,76 @@ class HipCG(Algorithm):
     x0: object = None
     P: object = None
     chunk: int = 16
+    fused: bool = True
 
     def __call__(self, A, b):
-        return cg(A, b, x0=self.x0, P=self.P, tol=self.tol, max_iters=self.max_iters, chunk=self.chunk)
+        return cg(A, b, x0=self.x0, P=self.P, tol=self.tol, max_iters=self.max_iters, chunk=self.chunk,
+                  fused=self.fused)
 
 
-def cg(A, rhs, x0=None, P=None, tol=1e-6, max_iters=5000, pbar=False, chunk=16):
+def cg(A, rhs, x0=None, P=None, tol=1e-6, max_iters=5000, pbar=False, chunk=16, fused=True):
     """cg.py:39-69 on the engine. rhs (n,) or (n, b) on the HIP device; returns (soln, info)."""
     is_vector = rhs.dim() == 1
     if is_vector:
         rhs = rhs[:, None]
         if x0 is not None:
             x0 = x0[:, None]
-    X, _, _, info = run_batched_cg(A, rhs, x0, max_iters, tol, P, chunk=chunk)
+    X, _, _, info = run_batched_cg(A, rhs, x0, max_iters, tol, P, chunk=chunk, fused=fused)
     return (X.reshape(-1) if is_vector else X), info
+
+
+STATUS_NAMES = {N.CK_OK: "ok", N.CK_ERR_MAXITER: "maxiter", N.CK_ERR_BREAKDOWN: "breakdown", N.CK_ERR_NAN: "nan",
+                N.CK_ERR_TIMEOUT: "timeout"}
+
+
+def _status_name(code):
+    return STATUS_NAMES.get(int(code), f"status {int(code)}")
+
+
+def fused_solve(A, b, x0=None, max_iters=1000, tol=1e-6, P=None, nblocks=0, spin_limit=0, wait_offset=0):
+    """cg.py:94-119 as one persistent launch (ck_cg_fused) when A lowers to a dense fp32
+    operator and P is None or diagonal; returns (x, r, k, info) or None when unsupported.
+    b (n, B) is read in place (any strides), x and r are written in the same layout."""
+    aop = lower(A)
+    pop = lower(P) if P is not None else None
+    if aop is None or (P is not None and pop is None):
+        return None
+    n, B = b.shape
+    if N.q("ck_cg_fused_supported", aop[0], pop[0] if pop else None, B) != N.CK_OK:
+        return None
+    dev = b.device
+    bf = b if b.dtype == torch.float32 else b.float()
+    x = torch.empty((n, B), dtype=torch.float32, device=dev)
+    r = torch.empty((n, B), dtype=torch.float32, device=dev)
+    nbytes = N.q("ck_cg_state_bytes", B, max_iters)
+    state = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    wb = N.q("ck_cg_fused_workspace_bytes", n, B)
+    work = torch.empty((wb + 7) // 8, dtype=torch.float64, device=dev)
+    x0f = None
+    if x0 is not None:
+        x0f = x0.reshape(n, B)
+        x0f = x0f if x0f.dtype == torch.float32 else x0f.float()
+    t0 = time.time()
+    N.call("ck_cg_fused", aop[0], pop[0] if pop else None, B, N.ptr(bf), bf.stride(0), bf.stride(1), N.ptr(x0f),
+           x0f.stride(0) if x0f is not None else 0, x0f.stride(1) if x0f is not None else 0, N.ptr(x), x.stride(0),
+           x.stride(1), N.ptr(r), r.stride(0), r.stride(1), N.ptr(state), float(tol), int(max_iters), N.ptr(work),
+           work.numel() * 8, int(nblocks), int(spin_limit), int(wait_offset), N.stream_ptr(dev))
+    host = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+    host.copy_(state, non_blocking=True)
+    torch.cuda.current_stream(dev).synchronize()
+    ctrl = host[:N.CTRL_BYTES].view(torch.int32).tolist()
+    k, status = ctrl[0], ctrl[5]
+    if status == N.CK_ERR_TIMEOUT:
+        raise N.KrylovError("ck_cg_fused", status, "a grid-wide wait of the persistent CG timed out")
+    stride = N.q("ck_cg_stride", B)
+    f = host[N.CTRL_BYTES:].view(torch.float32)
+    errs = f[N.CK_CG_ERRORS * stride:N.CK_CG_ERRORS * stride + k + 1].numpy().astype(np.float64)
+    rn = f[N.CK_CG_RNORM * stride:N.CK_CG_RNORM * stride + B].numpy()
+    tl = f[N.CK_CG_TOL * stride:N.CK_CG_TOL * stride + B].numpy()
+    elapsed = time.time() - t0
+    info = {"iterations": k + 1, "errors": np.concatenate([errs, errs[-1:]])[2:], "iteration_time": elapsed / (k + 1),
+            "converged": bool((rn <= tl).all()), "status": _status_name(status), "fused": True}
+    return x, r, k, info
 
 
 class CGState:
@@ -73,7 +131,7 @@ class CGState:
         return self.state[off:off + 4 * count].view(torch.float32)
 
 
-def run_batched_cg(A, b, x0=None, max_iters=1000, tol=1e-6, P=None, chunk=16, return_state=False):
+def run_batched_cg(A, b, x0=None, max_iters=1000, tol=1e-6, P=None, chunk=16, return_state=False, fused=True):
     """cg.py:94-119. b is (n, B) on the HIP device. Returns (x, r, k, info)."""
     N.require_device(b, "right-hand side")
     if b.dim() == 1:
@@ -81,6 +139,10 @@ def run_batched_cg(A, b, x0=None, max_iters=1000, tol=1e-6, P=None, chunk=16, re
     n, B = b.shape
     if B > 1024:
         raise ValueError("at most 1024 right-hand sides per solve")
+    if fused and not return_state:
+        out = fused_solve(A, b, x0, max_iters, tol, P)
+        if out is not None:
+            return out
     st = CGState(n, B, max_iters, b.device, P is not None)
     Bb = to_blocks(b, st.ld)
     x0b = to_blocks(x0.reshape(n, B), st.ld) if x0 is not None else None
@@ -169,9 +231,9 @@ def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, l
             if finish and not x64_updated:
                 N.call("ck_cg_finish", n, B, N.ptr(st.x), N.ptr(st.r), ld, N.ptr(st.state), s)
             return None, None, {"x64_updated": x64_updated}
-        k, done = read_ints(st.state, 2, dev)
+        k, done, _, _, _, status = read_ints(st.state, 6, dev)
     if steps == 0:
-        k, done = read_ints(st.state, 2, dev)
+        k, done, _, _, _, status = read_ints(st.state, 6, dev)
     if finish:
         N.call("ck_cg_finish", n, B, N.ptr(st.x), N.ptr(st.r), ld, N.ptr(st.state), s)
     if not want_info:
@@ -180,7 +242,8 @@ def cg_blocks(A, Bb, st, x0b=None, max_iters=1000, tol=1e-6, P=None, chunk=16, l
     elapsed = time.time() - t0
     # torch_tqdm.py:38-62: one error per cond evaluation (k+1) plus the final one, first two dropped
     info = {"iterations": k + 1, "errors": np.concatenate([errs, errs[-1:]])[2:],
-            "iteration_time": elapsed / (k + 1), "converged": bool(done and k < max_iters) or _all_below(st, B)}
+            "iteration_time": elapsed / (k + 1), "converged": bool(done and k < max_iters) or _all_below(st, B),
+            "status": _status_name(status), "fused": False}
     return k, done, info
 
 

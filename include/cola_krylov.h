@@ -31,7 +31,18 @@ enum {
   CK_ERR_HIP = 2,         /* a HIP runtime call or launch failed                */
   CK_ERR_UNSUPPORTED = 3, /* operator structure this entry point does not cover */
   CK_ERR_WORKSPACE = 4,   /* workspace smaller than the *_workspace_bytes query */
-  CK_ERR_NOCONV = 5       /* tridiagonal QL did not converge in 60 sweeps/value  */
+  CK_ERR_NOCONV = 5,      /* tridiagonal QL did not converge in 60 sweeps/value  */
+  /* Solver outcomes.  The enqueueing calls return before the solve runs, so these are
+   * written by the kernels into the solver's control block (ck_cg_ctrl.status) and read
+   * by the host with the step count; ck_cg_fused also returns them when it is asked to
+   * wait for the result.  The reference records none of them: it stops on NaN (the loop
+   * condition is false), divides safely on breakdown and returns at max_iters
+   * (cola/linalg/inverse/cg.py:133-178), and so does the engine -- the status only says
+   * which of these happened. */
+  CK_ERR_MAXITER = 6,     /* the step cap was reached with a column above its tolerance */
+  CK_ERR_BREAKDOWN = 7,   /* p.Ap <= 0 in an unfrozen column: A (or P) not positive definite */
+  CK_ERR_NAN = 8,         /* a residual norm became NaN or infinite                     */
+  CK_ERR_TIMEOUT = 9      /* a grid-wide wait of the persistent solve gave up (bounded spin) */
 };
 
 /* Library version string, e.g. "cola_krylov 0.1.0 gfx950". */
@@ -218,8 +229,9 @@ typedef struct ck_cg_ctrl {
   int32_t done;      /* 1 once the loop condition is false               */
   int32_t max_iters;
   int32_t nrhs;
-  int32_t nparts;    /* partial sums per column                          */
-  int32_t pad[11];
+  int32_t nparts;    /* partial sums per column (workgroups of the fused solve) */
+  int32_t status;    /* CK_OK or CK_ERR_MAXITER / BREAKDOWN / NAN / TIMEOUT */
+  int32_t pad[10];
 } ck_cg_ctrl;
 /* followed by fp32 arrays, each of ck_cg_stride(nrhs) floats, at these indices:
  * 0 mult(||b||) 1 tol 2 gamma 3 alpha 4 beta 5 rnorm, then errors[max_iters + 2]:
@@ -278,6 +290,30 @@ int ck_pcg_step1_f64(const ck_op* A, const ck_op* P, int32_t nrhs, const float* 
                      double* x64, int64_t ldx64, float* partials, void* work, size_t work_bytes, void* stream);
 /* x *= mult; r *= mult  (cg.py:119). */
 int ck_cg_finish(int64_t n, int32_t nrhs, float* x, float* r, int64_t ld, const void* state, void* stream);
+
+/* ---------------------------------------------------------------- persistent fused CG
+ * The whole batched CG solve of cola/linalg/inverse/cg.py:94-178 -- normalisation,
+ * start, every step and the loop of cola/utils/torch_tqdm.py:86-90, un-normalisation --
+ * in ONE cooperative launch (plus a one-block reset), for a dense fp32 operator
+ * A = CK_OP_DENSE_F32 (y = scale*A x + shift*x) and P = NULL or CK_OP_DIAG_F32, 1 <= nrhs
+ * <= 8, n small enough that two n x nrhs fp32 vectors fit in 56 KB of LDS (n*nrhs <= 7168).
+ * Each workgroup keeps its rows of A in LDS when they fit and the full r and p vectors;
+ * a step costs one grid-wide exchange (the p.Ap all-reduce, which is also the barrier
+ * that completes Ap).  b, x0 (optional) and x, r (r optional) are (n x nrhs) arrays with
+ * element strides (row stride, column stride), i.e. the reference's layout directly.
+ * state: ck_cg_state_bytes(nrhs, max_iters), laid out and filled as by the chained
+ * entry points (k, done, status, errors[0..k], per-column scalars); work:
+ * ck_cg_fused_workspace_bytes.  nblocks = 0 picks the workgroup count.  Every grid-wide
+ * wait gives up after spin_limit polling rounds (<= 0: about a second), records
+ * CK_ERR_TIMEOUT in ctrl.status and ends the kernel; wait_offset (0 in normal use) is
+ * added to the generation each wait expects, so a host can make every wait time out
+ * and exercise that exit.  Returns CK_ERR_UNSUPPORTED for other operators or sizes. */
+size_t ck_cg_fused_workspace_bytes(int64_t n, int32_t nrhs);
+int ck_cg_fused_supported(const ck_op* A, const ck_op* P, int32_t nrhs);
+int ck_cg_fused(const ck_op* A, const ck_op* P, int32_t nrhs, const float* b, int64_t b_rs, int64_t b_cs,
+                const float* x0, int64_t x0_rs, int64_t x0_cs, float* x, int64_t x_rs, int64_t x_cs, float* r,
+                int64_t r_rs, int64_t r_cs, void* state, float tol, int32_t max_iters, void* work,
+                size_t work_bytes, int32_t nblocks, int64_t spin_limit, int32_t wait_offset, void* stream);
 
 /* ---------------------------------------------------------------- Lanczos
  * Batched Lanczos with double classical Gram-Schmidt against every filled basis
