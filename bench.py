@@ -657,6 +657,35 @@ def cpu_baseline_refined(systems, B, tol, itol, budget_s=25.0, max_refinements=2
                       f"{used:.1f} s on {cpu_threads()} host threads"}
 
 
+def bench_dense_matvec(args, dev):
+    """The dense 8-rhs matvec of the learned_cg workload alone (one 16384-point RBF system of
+    it, K + noise I): the launch rocprofv3 counter passes measure."""
+    from cola_amd._blocks import ld_of
+    from cola_amd.linalg import learned as L
+    n, B = args.n, 8
+    X, ell, noise = L.sample_instance(100000, n)
+    K = L.rbf_matrix(X.to(dev), ell)
+    Kop = ca.Dense(K)
+    A = ca.PSD(Kop + noise * ca.I_like(Kop))
+    ld = ld_of(n)
+    Xb = torch.randn(B, ld, device=dev)
+    Yb = torch.empty_like(Xb)
+    op, keep = A.hip_op()
+    wb = N.q("ck_op_workspace_bytes", op, B)
+    work = torch.empty(max(wb, 4) // 4, device=dev)
+    reps = max(args.steps, 10)
+    t = time_kernel(lambda: N.call("ck_op_apply", op, B, N.ptr(Xb), ld, N.ptr(Yb), ld, N.ptr(work), wb,
+                                   N.stream_ptr(dev)), reps=reps)
+    nbytes = 4 * n * n + 2 * 4 * n * B
+    return {"metric": "dense 8-rhs matvec bandwidth (K + noise I, K 16384^2 fp32)", "value": nbytes / t / 1e9,
+            "unit": "GB/s", "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": t * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"dense_matvec: {n}^2 fp32 RBF matrix + noise I, {B} rhs",
+                       "algorithmic_bytes_per_launch": nbytes},
+            "roofline": {"bound": "hbm", "achieved": nbytes / t / 1e9, "peak": HBM_PEAK, "unit": "GB/s",
+                         "frac": nbytes / t / 1e9 / HBM_PEAK, "traffic": DENSE8_TRAFFIC_BYTES}}
+
+
 def gp_systems(nsys, dev, bf16=True):
     """512 (nsys) RBF systems: n_i ~ U{256..4096} (generator seed 0), system i from manual_seed(i):
     points U[0,1]^3, lengthscale LogU[0.05,0.5], one N(0,1) rhs; noise 1e-2."""
@@ -839,7 +868,7 @@ def main():
     N.load()
     fn = {"kron_cg": bench_kron_cg, "copy": bench_copy, "dense_cg": bench_dense_cg, "lanczos": bench_lanczos,
           "hutch": bench_hutch, "slq": bench_slq, "learned_cg": bench_learned_cg,
-          "batched_cg": bench_batched_cg, "csr": bench_csr}[args.workload]
+          "batched_cg": bench_batched_cg, "csr": bench_csr, "dense_matvec": bench_dense_matvec}[args.workload]
     out = fn(args, dev)
     print(json.dumps(out), flush=True)
 

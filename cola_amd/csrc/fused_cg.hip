@@ -6,24 +6,26 @@
 // operators (y = scale*A x + shift*x) whose solves are launch- and latency-bound, with no
 // preconditioner or a diagonal one.
 //
-// Each workgroup owns a contiguous block of R rows.  Its rows of A are staged in LDS once
-// (when they fit), and every workgroup keeps the FULL r and p vectors in LDS.  A step is
-//   Ap_own = A_own p                     (rows from LDS, p from LDS)
-//   publish Ap_own and the partial p.Ap  (one grid-wide exchange)
-//   alpha = gamma / sum_w partial_w      (every workgroup, same order -> same bits)
-//   x_own += alpha p_own;  r -= alpha Ap (all n entries, Ap read back from global)
-//   gamma' = r.z, ||r||, beta, p = z + beta p   (all n entries, redundantly per workgroup)
-// so a step needs exactly one grid-wide synchronisation: the p.Ap all-reduce doubles as
-// the barrier that makes Ap complete.  The exchange buffers (Ap and the partials) are
-// double-buffered by step parity: a workgroup can run at most one exchange ahead of the
-// slowest reader, so the buffer it writes is never one still being read.
+// Each workgroup owns a contiguous block of R <= 64 rows.  Its rows of A are staged in LDS
+// once (when they fit), and every workgroup keeps the FULL r and p vectors in LDS.  A step:
+//   Ap_own = A_own p                   (rows from LDS, p from LDS)
+//   publish Ap_own                     (the step's one grid-wide exchange)
+//   read all of Ap; p.Ap, alpha        (every workgroup, same order -> same bits)
+//   x_own += alpha p_own;  r -= alpha Ap;  gamma' = r.z, ||r||, beta, p = z + beta p
+// with the vector updates done redundantly by every workgroup, so a step needs exactly one
+// grid-wide exchange and no separate barrier.
 //
-// Synchronisation: workgroup w publishes generation g by an agent-scope release store of
-// gen[w] = g after a fence that orders its Ap and partial stores; readers poll all gen[]
-// with agent-scope loads (wave 0, one wave vote per round) and fence before reading.  The
-// wait is bounded: after spin_limit polling rounds it gives up, records CK_ERR_TIMEOUT in
-// the control block and leaves the loop, so every workgroup ends and the launch drains.
-// Co-residency of all workgroups is guaranteed by hipLaunchCooperativeKernel.
+// The exchange: every Ap entry travels as one 64-bit word (value bits, step tag) written and
+// read with agent-scope atomic stores and loads, which the compiler emits as coherent (sc1)
+// accesses that do not stop in a per-XCD L2.  A reader knows an entry is the current step's
+// when its tag matches, so no flag, fence or cache maintenance is needed: one store from
+// the owner, loads (re-issued until the tag matches) from everyone.  The buffers are doubled
+// by step parity: a workgroup can run at most one step ahead of the slowest reader (it needs
+// every entry of the current step, written after each writer finished reading the previous
+// one), so it never overwrites a word that is still to be read.  Every wait is bounded:
+// after spin_limit polling rounds it gives up, records CK_ERR_TIMEOUT in the control block
+// and leaves the loop, so every workgroup ends and the launch drains.  Co-residency of all
+// workgroups is guaranteed by hipLaunchCooperativeKernel.
 #include "cg_state.hpp"
 #include "common.hpp"
 
@@ -35,8 +37,13 @@ constexpr double kSmall = (double)(float)1e-40;  // cg.py:8, as an fp32 constant
 constexpr int kNT = 256;
 constexpr int kMaxRhs = 8;
 constexpr int kMaxBlocks = 256;
-constexpr size_t kLdsBudget = 56 * 1024;  // dynamic LDS; the static arrays take < 5 KB more
+constexpr int kMaxRows = 64;              // rows per workgroup
+constexpr int kMaxEntries = 7168;         // n * nrhs
+constexpr size_t kLdsBudget = 56 * 1024;  // dynamic LDS; the static arrays take < 1 KB more
 constexpr int64_t kDefaultSpin = int64_t(1) << 22;
+
+// entries of one column a thread reads back per step (n <= 256 * max_t_of(NB))
+__host__ __device__ constexpr int max_t_of(int nb) { return nb == 1 ? 28 : nb == 2 ? 14 : nb == 4 ? 10 : 6; }
 
 struct FusedArgs {
   int n, nn, nrhs, G, R;  // order, padded order (multiple of 4), columns, workgroups, rows per workgroup
@@ -56,9 +63,7 @@ struct FusedArgs {
   void* state;
   float tol;
   int max_iters;
-  float* apbuf;    // 2 x nrhs x nn
-  double* part;    // 2 x kMaxBlocks x nrhs
-  uint32_t* gen;   // kMaxBlocks
+  uint64_t* xbuf;  // 2 x nrhs x nn tagged words
   int64_t spin_limit;
   uint32_t wait_offset;
 };
@@ -67,6 +72,16 @@ __device__ inline double safe_div(double num, double den) {
   if (fabs(den) < kSmall) den = kSmall;
   return num / den;
 }
+
+__device__ inline void put(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline uint64_t get(const uint64_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline void put(int32_t* p, int32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+__device__ inline uint64_t pack(float v, uint32_t tag) { return ((uint64_t)tag << 32) | (uint64_t)__float_as_uint(v); }
+__device__ inline uint32_t tag_of(uint64_t w) { return (uint32_t)(w >> 32); }
+__device__ inline float value_of(uint64_t w) { return __uint_as_float((uint32_t)w); }
 
 // Sum of NV per-thread values over the block, fixed order; every thread gets the totals.
 template <int NV>
@@ -83,46 +98,50 @@ __device__ inline void block_sums(double (&v)[NV], double* lds /* 4 * NV */) {
   __syncthreads();
 }
 
-// Make this workgroup's global stores visible at agent scope, then publish generation g.
-__device__ inline void publish(uint32_t* gen, uint32_t g) {
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) __hip_atomic_store(gen + blockIdx.x, g, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Wait until every workgroup has published generation >= target (bounded); true on success.
-__device__ inline bool wait_all(const uint32_t* gen, int G, uint32_t target, int64_t spin_limit, int* flag) {
-  if (threadIdx.x < 64) {
-    bool ok = false;
-    for (int64_t it = 0;; ++it) {
-      bool mine = true;
-      for (int w = threadIdx.x; w < G; w += 64) {
-        const uint32_t g = __hip_atomic_load(gen + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        mine = mine && (g >= target);
-      }
-      if (__all(mine)) {
-        ok = true;
-        break;
-      }
-      if (it >= spin_limit) break;
-      __builtin_amdgcn_s_sleep(1);
+// This thread's entries j = tid + 256 t of every column from the tagged buffer xb (nn words
+// per column): all loads are issued first, then a word whose tag is not yet `want` is
+// re-read until it is (bounded by spin rounds; after one timeout the rest are not waited
+// for).  Returns false on timeout.
+template <int NB, int KT>
+__device__ inline bool collect(const uint64_t* xb, int n, int nn, int B, uint32_t want, int64_t spin,
+                               float (&out)[NB][KT]) {
+  uint64_t raw[NB][KT];
+  const uint64_t ready = (uint64_t)want << 32;
+#pragma unroll
+  for (int c = 0; c < NB; ++c)
+#pragma unroll
+    for (int t = 0; t < KT; ++t) {
+      const int j = threadIdx.x + kNT * t;
+      raw[c][t] = (c < B && j < n) ? get(xb + (int64_t)c * nn + j) : ready;
     }
-    if (threadIdx.x == 0) *flag = ok ? 1 : 0;
-  }
-  __syncthreads();
-  const bool ok = *flag != 0;
-  __threadfence();  // acquire: the published Ap and partials are read after this
+  bool ok = true;
+#pragma unroll
+  for (int c = 0; c < NB; ++c)
+#pragma unroll
+    for (int t = 0; t < KT; ++t) {
+      const int j = threadIdx.x + kNT * t;
+      int64_t it = 0;
+      while (ok && tag_of(raw[c][t]) != want) {
+        if (++it > spin) {
+          ok = false;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        raw[c][t] = get(xb + (int64_t)c * nn + j);
+      }
+      out[c][t] = value_of(raw[c][t]);
+    }
   return ok;
 }
 
-// NB: columns rounded up to 1, 2, 4 or 8 (sizes the per-thread sums); TILE: own rows of A in LDS
+// NB: columns rounded up to 1, 2, 4 or 8; TILE: own rows of A in LDS
 template <bool TILE, int NB>
 __global__ __launch_bounds__(kNT) void fused_cg_kernel(FusedArgs a) {
+  constexpr int KT = max_t_of(NB);
   extern __shared__ float lds[];
   __shared__ double red[4 * 2 * NB];
-  __shared__ double rowprod[NB * 64];
   __shared__ float s_alpha[NB], s_beta[NB], s_gamma[NB], s_rnorm[NB], s_tol[NB], s_mult[NB];
-  __shared__ int s_flag;
+  __shared__ int s_fail;
   const int n = a.n, nn = a.nn, B = a.nrhs, w = blockIdx.x, tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int row0 = w * a.R;
@@ -132,6 +151,7 @@ __global__ __launch_bounds__(kNT) void fused_cg_kernel(FusedArgs a) {
   float* At = rv + B * nn;    // R x nn: own rows of A (TILE)
   CgView v = cg_view(a.state, B);
   const bool pre = a.d != nullptr;
+  if (tid == 0) s_fail = 0;
 
   if (TILE) {
     for (int64_t idx = tid; idx < (int64_t)a.R * nn; idx += kNT) {
@@ -162,31 +182,67 @@ __global__ __launch_bounds__(kNT) void fused_cg_kernel(FusedArgs a) {
   }
   uint32_t sync = 0;
   bool timeout = false;
+  // A times the vectors in pv for the own rows, published with tag tg into the buffer xb
+  auto gemv_publish = [&](uint64_t* xb, uint32_t tg) {
+    for (int i = wave; i < rows; i += 4) {
+      const float* arow = TILE ? At + (int64_t)i * nn : a.A + (int64_t)(row0 + i) * a.lda;
+      float acc[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) acc[c] = 0.f;
+      if (TILE) {
+        for (int j = 4 * lane; j < nn; j += 256) {
+          const float4 av = *reinterpret_cast<const float4*>(arow + j);
+#pragma unroll
+          for (int c = 0; c < NB; ++c)
+            if (c < B) {
+              const float4 p4 = *reinterpret_cast<const float4*>(pv + c * nn + j);
+              acc[c] = fmaf(av.x, p4.x, acc[c]);
+              acc[c] = fmaf(av.y, p4.y, acc[c]);
+              acc[c] = fmaf(av.z, p4.z, acc[c]);
+              acc[c] = fmaf(av.w, p4.w, acc[c]);
+            }
+        }
+      } else {
+        for (int j = lane; j < n; j += 64) {
+          const float aij = arow[j];
+#pragma unroll
+          for (int c = 0; c < NB; ++c)
+            if (c < B) acc[c] = fmaf(aij, pv[c * nn + j], acc[c]);
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c < B) {
+          const float s = wave_sum(acc[c]);
+          if (lane == 0) {
+            float y = a.scale * s;
+            if (a.shift != 0.f) y = fmaf(a.shift, pv[c * nn + row0 + i], y);
+            put(xb + (int64_t)c * nn + row0 + i, pack(y, tg));
+          }
+        }
+    }
+  };
+  float ap[NB][KT];
   if (a.x0) {
-    // cg.py:123: r0 = b_norm - A x0 (x0 as given); one exchange of A x0 through the odd Ap
-    // buffer (the first step writes the even one, so a fast workgroup never overwrites it)
-    float* ax0 = a.apbuf + (int64_t)B * nn;
+    // cg.py:123: r0 = b_norm - A x0 (x0 as given); one exchange of A x0 through the odd buffer
+    // (the first step writes the even one, so a fast workgroup never overwrites it)
+    uint64_t* xb = a.xbuf + (int64_t)B * nn;
     for (int c = 0; c < B; ++c)
       for (int j = tid; j < nn; j += kNT) pv[c * nn + j] = j < n ? a.x0[(int64_t)j * a.x0_rs + (int64_t)c * a.x0_cs] : 0.f;
     __syncthreads();
-    for (int i = wave; i < rows; i += 4) {
-      const float* arow = TILE ? At + (int64_t)i * nn : a.A + (int64_t)(row0 + i) * a.lda;
-      for (int c = 0; c < B; ++c) {
-        float acc = 0.f;
-        for (int j = lane; j < n; j += 64) acc = fmaf(arow[j], pv[c * nn + j], acc);
-        acc = wave_sum(acc);
-        if (lane == 0) {
-          float y = a.scale * acc;
-          if (a.shift != 0.f) y = fmaf(a.shift, pv[c * nn + row0 + i], y);
-          ax0[(int64_t)c * nn + row0 + i] = y;
-        }
-      }
-    }
-    publish(a.gen, ++sync);
-    if (!wait_all(a.gen, a.G, sync + a.wait_offset, a.spin_limit, &s_flag)) timeout = true;
+    ++sync;
+    gemv_publish(xb, sync);
+    if (!collect<NB, KT>(xb, n, nn, B, sync + a.wait_offset, a.spin_limit, ap)) s_fail = 1;
+    __syncthreads();
+    timeout = s_fail != 0;
     if (!timeout) {
-      for (int c = 0; c < B; ++c)
-        for (int j = tid; j < n; j += kNT) rv[c * nn + j] -= ax0[(int64_t)c * nn + j];
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+          const int j = tid + kNT * t;
+          if (c < B && j < n) rv[c * nn + j] -= ap[c][t];
+        }
       for (int idx = tid; idx < rows * B; idx += kNT) {
         const int i = idx / B, c = idx % B;
         a.x[(int64_t)(row0 + i) * a.x_rs + (int64_t)c * a.x_cs] = pv[c * nn + row0 + i];
@@ -238,76 +294,35 @@ __global__ __launch_bounds__(kNT) void fused_cg_kernel(FusedArgs a) {
     if (!finite) status = CK_ERR_NAN;
   }
   while (!done && !timeout) {
-    const int par = k & 1;
-    float* ap = a.apbuf + (int64_t)par * B * nn;
-    double* part = a.part + (int64_t)par * kMaxBlocks * B;
-    // Ap for the own rows; rowprod = p_i * (Ap)_i
-    for (int i = wave; i < rows; i += 4) {
-      const float* arow = TILE ? At + (int64_t)i * nn : a.A + (int64_t)(row0 + i) * a.lda;
-      float acc[NB];
+    uint64_t* xb = a.xbuf + (int64_t)(k & 1) * B * nn;
+    ++sync;
+    gemv_publish(xb, sync);
+    if (!collect<NB, KT>(xb, n, nn, B, sync + a.wait_offset, a.spin_limit, ap)) s_fail = 1;
+    // p.Ap over all n entries (every workgroup, same order) -> alpha (cg.py:143-147)
+    double s[NB];
 #pragma unroll
-      for (int c = 0; c < NB; ++c) acc[c] = 0.f;
-      if (TILE) {
-        for (int j = 4 * lane; j < nn; j += 256) {
-          const float4 av = *reinterpret_cast<const float4*>(arow + j);
+    for (int c = 0; c < NB; ++c) {
+      s[c] = 0.0;
 #pragma unroll
-          for (int c = 0; c < NB; ++c)
-            if (c < B) {
-              const float4 p4 = *reinterpret_cast<const float4*>(pv + c * nn + j);
-              acc[c] = fmaf(av.x, p4.x, acc[c]);
-              acc[c] = fmaf(av.y, p4.y, acc[c]);
-              acc[c] = fmaf(av.z, p4.z, acc[c]);
-              acc[c] = fmaf(av.w, p4.w, acc[c]);
-            }
-        }
-      } else {
-        for (int j = lane; j < n; j += 64) {
-          const float aij = arow[j];
-#pragma unroll
-          for (int c = 0; c < NB; ++c)
-            if (c < B) acc[c] = fmaf(aij, pv[c * nn + j], acc[c]);
-        }
+      for (int t = 0; t < KT; ++t) {
+        const int j = tid + kNT * t;
+        if (c < B && j < n) s[c] += (double)pv[c * nn + j] * (double)ap[c][t];
       }
-#pragma unroll
-      for (int c = 0; c < NB; ++c)
-        if (c < B) {
-          const float s = wave_sum(acc[c]);
-          if (lane == 0) {
-            const float pi = pv[c * nn + row0 + i];
-            float y = a.scale * s;
-            if (a.shift != 0.f) y = fmaf(a.shift, pi, y);
-            ap[(int64_t)c * nn + row0 + i] = y;
-            rowprod[c * 64 + i] = (double)pi * (double)y;
-          }
-        }
     }
-    __syncthreads();
-    // this workgroup's p.Ap partial per column (its <= 64 rows in order)
-    if (tid < B) {
-      double s = 0.0;
-      for (int i = 0; i < rows; ++i) s += rowprod[tid * 64 + i];
-      part[w * B + tid] = s;
-    }
-    publish(a.gen, ++sync);
-    if (!wait_all(a.gen, a.G, sync + a.wait_offset, a.spin_limit, &s_flag)) {
+    block_sums<NB>(s, red);
+    if (s_fail) {
       timeout = true;
       break;
     }
-    // alpha = gamma / p.Ap (cg.py:143-147); the partials are added in workgroup order
-    if (wave == 0) {
-      for (int c = 0; c < B; ++c) {
-        double s = 0.0;
-        for (int u = lane; u < a.G; u += 64) s += part[u * B + c];
-        s = wave_sum(s);
-        if (lane == 0) {
-          const bool frozen = (double)s_rnorm[c] < kSmall;
-          double al = safe_div((double)s_gamma[c], s);
-          if (frozen) al = 0.0;
-          if (!frozen && !(s > 0.0) && status == CK_OK) status = CK_ERR_BREAKDOWN;
-          s_alpha[c] = (float)al;
-        }
-      }
+    if (tid < B) {
+      const bool frozen = (double)s_rnorm[tid] < kSmall;
+      double al = safe_div((double)s_gamma[tid], s[tid]);
+      if (frozen) al = 0.0;
+      s_alpha[tid] = (float)al;
     }
+    if (tid == 0)
+      for (int c = 0; c < B; ++c)
+        if (!((double)s_rnorm[c] < kSmall) && !(s[c] > 0.0) && status == CK_OK) status = CK_ERR_BREAKDOWN;
     __syncthreads();
     // x_own += alpha p_own
     for (int idx = tid; idx < rows * B; idx += kNT) {
@@ -316,28 +331,32 @@ __global__ __launch_bounds__(kNT) void fused_cg_kernel(FusedArgs a) {
       *xp = fmaf(s_alpha[c], pv[c * nn + row0 + i], *xp);
     }
     // r -= alpha Ap (all entries), gamma' = r.z, ||r||^2
-    double s[2 * NB];
+    double q[2 * NB];
 #pragma unroll
-    for (int c = 0; c < 2 * NB; ++c) s[c] = 0.0;
-    for (int c = 0; c < B; ++c) {
-      const float al = s_alpha[c];
-      for (int j = tid; j < n; j += kNT) {
-        const float rj = fmaf(-al, ap[(int64_t)c * nn + j], rv[c * nn + j]);
-        rv[c * nn + j] = rj;
-        const float zj = pre ? fmaf(a.dscale * a.d[j], rj, a.dshift * rj) : rj;
-        s[c] += (double)rj * (double)rj;
-        s[NB + c] += (double)rj * (double)zj;
+    for (int c = 0; c < NB; ++c) {
+      q[c] = 0.0;
+      q[NB + c] = 0.0;
+#pragma unroll
+      for (int t = 0; t < KT; ++t) {
+        const int j = tid + kNT * t;
+        if (c < B && j < n) {
+          const float rj = fmaf(-s_alpha[c], ap[c][t], rv[c * nn + j]);
+          rv[c * nn + j] = rj;
+          const float zj = pre ? fmaf(a.dscale * a.d[j], rj, a.dshift * rj) : rj;
+          q[c] += (double)rj * (double)rj;
+          q[NB + c] += (double)rj * (double)zj;
+        }
       }
     }
-    block_sums<2 * NB>(s, red);
+    block_sums<2 * NB>(q, red);
     // beta = gamma'/gamma (cg.py:150-154), frozen columns keep beta = 0
     if (tid < B) {
       const bool frozen = (double)s_rnorm[tid] < kSmall;
-      double be = safe_div(s[NB + tid], (double)s_gamma[tid]);
+      double be = safe_div(q[NB + tid], (double)s_gamma[tid]);
       if (frozen) be = 0.0;
       s_beta[tid] = (float)be;
-      s_gamma[tid] = (float)s[NB + tid];
-      s_rnorm[tid] = (float)sqrt(s[tid]);
+      s_gamma[tid] = (float)q[NB + tid];
+      s_rnorm[tid] = (float)sqrt(q[tid]);
     }
     __syncthreads();
     for (int c = 0; c < B; ++c) {
@@ -368,7 +387,7 @@ __global__ __launch_bounds__(kNT) void fused_cg_kernel(FusedArgs a) {
     __syncthreads();
   }
   if (timeout) {
-    if (tid == 0) __hip_atomic_store(&v.ctrl->status, (int32_t)CK_ERR_TIMEOUT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (tid == 0) put(&v.ctrl->status, (int32_t)CK_ERR_TIMEOUT);
     if (w == 0 && tid == 0) {
       v.ctrl->k = k;
       v.ctrl->done = 1;
@@ -403,10 +422,11 @@ __global__ __launch_bounds__(kNT) void fused_cg_kernel(FusedArgs a) {
   }
 }
 
-// Clears the generation words and the status before a solve.
-__global__ void fused_reset_kernel(uint32_t* gen, int G, void* state) {
-  for (int i = threadIdx.x; i < G; i += blockDim.x) gen[i] = 0u;
-  if (threadIdx.x == 0) static_cast<ck_cg_ctrl*>(state)->status = CK_OK;
+// Clears the exchange tags and the status before a solve.
+__global__ void fused_reset_kernel(uint64_t* xbuf, int64_t words, void* state) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x)
+    xbuf[i] = 0ull;
+  if (blockIdx.x == 0 && threadIdx.x == 0) static_cast<ck_cg_ctrl*>(state)->status = CK_OK;
 }
 
 struct Plan {
@@ -462,10 +482,12 @@ int max_coresident(bool tile, int nb, size_t lds) {
 Plan make_plan(int n, int B, int nblocks) {
   Plan p{};
   p.nn = (n + 3) / 4 * 4;
-  if (n < 1 || B < 1 || B > kMaxRhs) return p;
+  if (n < 1 || B < 1 || B > kMaxRhs || (int64_t)n * B > kMaxEntries) return p;
+  // the per-thread read-back holds max_t_of(nb) entries of each column
+  if (n > kNT * max_t_of(nb_of(B))) return p;
   if (lds_bytes(p.nn, B, 0, false) > kLdsBudget) return p;
-  // each wave handles rows i, i + 4, ...: at most 64 rows per workgroup (the p.Ap slots)
-  const int gmin = (int)cdiv(n, 64);
+  // each wave handles rows i, i + 4, ...: at most kMaxRows rows per workgroup
+  const int gmin = (int)cdiv(n, kMaxRows);
   int G = nblocks > 0 ? nblocks : (int)cdiv(n, 8);
   if (G > kMaxBlocks) G = kMaxBlocks;
   if (G < gmin) G = gmin;
@@ -480,10 +502,7 @@ Plan make_plan(int n, int B, int nblocks) {
   return p;
 }
 
-size_t work_bytes_of(int nn, int B) {
-  return sizeof(float) * (size_t)2 * B * nn + sizeof(double) * (size_t)2 * kMaxBlocks * B +
-         sizeof(uint32_t) * kMaxBlocks + 256;
-}
+size_t work_bytes_of(int nn, int B) { return sizeof(uint64_t) * (size_t)2 * B * nn; }
 
 bool fused_eligible(const ck_op* A, const ck_op* P, int nrhs) {
   if (!A || A->kind != CK_OP_DENSE_F32 || !A->a || A->lda < A->n) return false;
@@ -515,7 +534,7 @@ int ck_cg_fused(const ck_op* A, const ck_op* P, int32_t nrhs, const float* b, in
   const int n = A->n;
   Plan pl = make_plan(n, nrhs, nblocks);
   if (!pl.ok) return CK_ERR_UNSUPPORTED;
-  if (work_bytes < work_bytes_of(pl.nn, nrhs)) return CK_ERR_WORKSPACE;
+  if (work_bytes < work_bytes_of(pl.nn, nrhs) || (reinterpret_cast<uintptr_t>(work) & 7) != 0) return CK_ERR_WORKSPACE;
   const int nb = nb_of(nrhs);
   if (pl.G > max_coresident(pl.tile, nb, pl.lds)) return CK_ERR_UNSUPPORTED;
   FusedArgs a{};
@@ -548,17 +567,13 @@ int ck_cg_fused(const ck_op* A, const ck_op* P, int32_t nrhs, const float* b, in
   a.state = state;
   a.tol = tol;
   a.max_iters = max_iters;
-  char* wp = static_cast<char*>(work);
-  a.apbuf = reinterpret_cast<float*>(wp);
-  wp += sizeof(float) * (size_t)2 * nrhs * pl.nn;
-  wp = reinterpret_cast<char*>((reinterpret_cast<uintptr_t>(wp) + 255) & ~uintptr_t(255));
-  a.part = reinterpret_cast<double*>(wp);
-  wp += sizeof(double) * (size_t)2 * kMaxBlocks * nrhs;
-  a.gen = reinterpret_cast<uint32_t*>(wp);
+  a.xbuf = static_cast<uint64_t*>(work);
   a.spin_limit = spin_limit > 0 ? spin_limit : kDefaultSpin;
   a.wait_offset = (uint32_t)wait_offset;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(fused_reset_kernel, dim3(1), dim3(256), 0, s, a.gen, pl.G, state);
+  const int64_t words = (int64_t)2 * nrhs * pl.nn;
+  const unsigned rblocks = (unsigned)(cdiv(words, 1024) < 64 ? cdiv(words, 1024) : 64);
+  hipLaunchKernelGGL(fused_reset_kernel, dim3(rblocks), dim3(256), 0, s, a.xbuf, words, state);
   CK_CHECK_LAUNCH();
   void* args[] = {&a};
   if (hipLaunchCooperativeKernel(kernel_of(pl.tile, nb), dim3(pl.G), dim3(kNT), args, (unsigned)pl.lds, s) != hipSuccess)

@@ -7,6 +7,7 @@ from .arnoldi import HipArnoldi, arnoldi, arnoldi_eigs
 from .batched import BatchedMixedSolver, BatchedPrecond, BatchedSystems, HipBatchedCG, autotune
 from .cg import HipCG, cg, run_batched_cg
 from .eig import eig, eigmax, eigmin, get_slice
+from .gmres import HipGMRES, gmres
 from .inv import inv, solve
 from .keys import PRNGKey, next_key
 from .lanczos import HipLanczos, lanczos, lanczos_eigs
@@ -22,8 +23,9 @@ CG = HipCG
 Lanczos = HipLanczos
 Arnoldi = HipArnoldi
 Hutch = HipHutch
+GMRES = HipGMRES
 
-__all__ = ["Algorithm", "Auto", "IterativeOperatorWInfo", "HipCG", "CG", "cg", "run_batched_cg", "inv", "solve",
+__all__ = ["Algorithm", "Auto", "IterativeOperatorWInfo", "HipCG", "CG", "cg", "HipGMRES", "GMRES", "gmres", "run_batched_cg", "inv", "solve",
            "HipLanczos", "Lanczos", "lanczos", "lanczos_eigs", "HipArnoldi", "Arnoldi", "arnoldi", "arnoldi_eigs",
            "eig", "eigmax", "eigmin", "get_slice", "logdet", "slogdet", "HipSLQ", "stochastic_lanczos_quad",
            "HipHutch", "Hutch", "Exact", "diag", "trace", "hutch_trace_csr", "hutchinson_diag_estimate",

@@ -16,6 +16,7 @@ import torch
 
 from .. import _native as N
 from .._blocks import from_blocks, ld_of, to_blocks
+from ..autodiff import GradOperator, iterative_autograd, vjp_derivs
 from ._engine import apply_blocks, lower, read_ints, workspace
 from .algorithm_base import Algorithm
 
@@ -49,14 +50,41 @@ class HipCG(Algorithm):
 
 
 def cg(A, rhs, x0=None, P=None, tol=1e-6, max_iters=5000, pbar=False, chunk=16, fused=True):
-    """cg.py:39-69 on the engine. rhs (n,) or (n, b) on the HIP device; returns (soln, info)."""
+    """cg.py:39-69 on the engine. rhs (n,) or (n, b) on the HIP device; returns (soln, info).
+    Differentiable with respect to the operator's tensors (cg_bwd, cg.py:72-86)."""
     is_vector = rhs.dim() == 1
     if is_vector:
         rhs = rhs[:, None]
         if x0 is not None:
             x0 = x0[:, None]
-    X, _, _, info = run_batched_cg(A, rhs, x0, max_iters, tol, P, chunk=chunk, fused=fused)
+    X, _, _, info = run_cg(A, rhs, x0, max_iters, tol, P, chunk, fused)
     return (X.reshape(-1) if is_vector else X), info
+
+
+def cg_bwd(res, grads, unflatten, *args, **kwargs):
+    """cg.py:72-86: the output gradient is solved with the same CG settings (including the
+    forward's x0, as the reference does), db = A^-1 dx, and the operator's gradient is the
+    vector-Jacobian product of theta -> A(theta) @ x with -db."""
+    x0, max_iters, tol, P, chunk, fused = args[1:7]
+    y_grads = grads[0]
+    op_args, output = res
+    soln = output[0]
+    A = unflatten(op_args)
+    if y_grads is None:
+        return GradOperator([None] * len(op_args)), None
+    db, *_ = run_batched_cg(A, y_grads.contiguous(), x0, max_iters, tol, P, chunk=chunk, fused=fused)
+
+    def fun(*theta):
+        return unflatten(theta) @ soln
+
+    d_params = vjp_derivs(fun, op_args, -db)
+    return GradOperator(d_params), db
+
+
+@iterative_autograd(cg_bwd)
+def run_cg(A, b, x0, max_iters, tol, P, chunk=16, fused=True):
+    """cg.py:89-91: run_batched_cg with the backward rule cg_bwd."""
+    return run_batched_cg(A, b, x0, max_iters, tol, P, chunk=chunk, fused=fused)
 
 
 STATUS_NAMES = {N.CK_OK: "ok", N.CK_ERR_MAXITER: "maxiter", N.CK_ERR_BREAKDOWN: "breakdown", N.CK_ERR_NAN: "nan",

@@ -7,6 +7,7 @@ from ..ops import BlockDiag, Diagonal, Identity, Kronecker, LinearOperator, Prod
 from .algorithm_base import Algorithm, Auto, IterativeOperatorWInfo
 from .batched import BatchedInverse, HipBatchedCG
 from .cg import HipCG
+from .gmres import HipGMRES
 from .mixed import HipMixedCG
 
 
@@ -34,15 +35,19 @@ def inv(A: LinearOperator, alg: HipMixedCG):
     return IterativeOperatorWInfo(A, alg)
 
 
+@dispatch
+def inv(A: LinearOperator, alg: HipGMRES):
+    """inv.py:61-63."""
+    return IterativeOperatorWInfo(A, alg)
+
+
 @dispatch(precedence=-1)
 def inv(A: LinearOperator, alg: Auto):
-    """inv.py:73-92 restricted to the engine: PSD operators go to CG at every size.
-
-    The reference's small-size dense factorisations (Cholesky/LU) and GMRES for
-    non-PSD operators are outside the engine; those cases raise."""
+    """inv.py:73-92 on the engine: PSD operators go to CG and the others to GMRES at every
+    size (the reference's dense Cholesky and LU for N^2 <= 1e6 are outside the engine)."""
     if A.isa(PSD):
         return inv(A, HipCG(**alg.__dict__))
-    raise NotImplementedError("the engine solves PSD operators with CG; wrap A in PSD(...) or use a dense solver")
+    return inv(A, HipGMRES(**alg.__dict__))
 
 
 @dispatch(cond=lambda A, *_: A.isa(Unitary))

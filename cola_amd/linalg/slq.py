@@ -56,7 +56,33 @@ def slq_quadratures(A, Z, max_iters, tol, f_kind=0, eps=None, max_mode=0, scale=
 
 
 def slq_logdet(A, alg):
-    """Mean over probes of ||z||^2 sum_j tau_j^2 log(theta_j) (or n * ..., slq.py:51)."""
+    """Mean over probes of ||z||^2 sum_j tau_j^2 log(theta_j) (or n * ..., slq.py:51).
+    Differentiable with respect to the operator's tensors: the backward is the rule of
+    slq.py:10-31 on this estimator's own probes, (1/m) sum_b (A^-1 z_b)^T (dA/dtheta) z_b."""
+    from ..autodiff import iterative_autograd
+    return iterative_autograd(_slq_logdet_bwd)(_slq_logdet)(A, alg)
+
+
+def _slq_logdet_bwd(res, grads, unflatten, alg):
+    from ..autodiff import GradOperator, vjp_derivs
+    from .cg import cg
+    op_args, _ = res
+    A = unflatten(op_args)
+    n = A.shape[0]
+    seed = PRNGKey(0) if alg.key is None else int(alg.key)
+    Zb, _ = probes(alg.rand, n, alg.num_probes, seed, A.device)
+    Z = Zb[:, :n].T.contiguous()
+    solves, _ = cg(A, Z, tol=1e-6, max_iters=100)
+    g = grads[0]
+    d_solves = (1.0 / alg.num_probes) * g.to(solves.dtype) * solves
+
+    def fun(*theta):
+        return unflatten(theta) @ Z
+
+    return (GradOperator(vjp_derivs(fun, op_args, d_solves)), )
+
+
+def _slq_logdet(A, alg):
     n = A.shape[0]
     seed = PRNGKey(0) if alg.key is None else int(alg.key)
     Zb, _ = probes(alg.rand, n, alg.num_probes, seed, A.device)
@@ -72,14 +98,45 @@ def slq_logdet(A, alg):
 def stochastic_lanczos_quad(A, fun, max_iters=100, tol=1e-5, vtol=0.1, pbar=False, key=None):
     """slq.py:55-75: Gaussian probes, num_samples = max(int(1/vtol^2), 1), estimate
     n * mean_b sum_j tau_bj^2 f(theta_bj) with the signed-max threshold of slq.py:48.
+    Differentiable with respect to the operator's tensors by slq_bwd (slq.py:10-31), which,
+    like the reference, assumes f = log.
 
     ``fun`` may be one of the names in F_KIND or a torch function applied on the device."""
     num = max(int(1 / vtol**2), 1)
+    return slq_fwd(A, fun, num_samples=num, max_iters=max_iters, tol=tol, pbar=pbar, key=key)
+
+
+def _gaussian_probes(A, num, key):
     n = A.shape[0]
     seed = PRNGKey(0) if key is None else int(key)
     Zb, _ = probes("normal", n, num, seed, A.device)
-    Z = Zb[:, :n].T
-    sc = torch.full((num, ), float(n), dtype=torch.float64, device=A.device)
+    return Zb[:, :n].T
+
+
+def slq_bwd(res, grads, unflatten, *args, **kwargs):
+    """slq.py:10-31: d logdet/d theta ~ (1/m) sum_j (A^-1 z_j)^T (dA/dtheta) z_j with m Gaussian
+    probes from key (default PRNGKey(0)), solved by CG at tol 1e-6 with at most 100 steps,
+    scaled by the output gradient."""
+    from ..autodiff import GradOperator, vjp_derivs
+    from .cg import cg
+    op_args, _ = res
+    num_samples = kwargs["num_samples"]
+    A = unflatten(op_args)
+    Z = _gaussian_probes(A, num_samples, kwargs.get("key"))
+    solves, _ = cg(A, Z.contiguous(), tol=1e-6, max_iters=100)
+    g = grads[0] if isinstance(grads, (tuple, list)) else grads
+    d_solves = (1.0 / num_samples) * g.to(solves.dtype) * solves
+
+    def fun(*theta):
+        return unflatten(theta) @ Z
+
+    return (GradOperator(vjp_derivs(fun, op_args, d_solves)), )
+
+
+def _slq_fwd(A, fun, num_samples, max_iters, tol, pbar, key):
+    n = A.shape[0]
+    Z = _gaussian_probes(A, num_samples, key)
+    sc = torch.full((num_samples, ), float(n), dtype=torch.float64, device=A.device)
     if isinstance(fun, str):
         q, _ = slq_quadratures(A, Z, max_iters, tol, F_KIND[fun], None, 1, sc)
         return q.mean()
@@ -88,6 +145,13 @@ def stochastic_lanczos_quad(A, fun, max_iters=100, tol=1e-5, vtol=0.1, pbar=Fals
     const = 10 * torch.finfo(A.dtype).eps * ev.max(dim=1, keepdim=True).values
     fv = torch.where(ev.abs() > const, fun(ev), torch.zeros_like(ev))
     return (n * (fr**2 * fv).sum(-1)).mean()
+
+
+def slq_fwd(A, fun, num_samples, max_iters, tol, pbar, key):
+    """slq.py:36-52 with the backward rule slq_bwd (keyword arguments as in the reference)."""
+    from ..autodiff import iterative_autograd
+    return iterative_autograd(slq_bwd)(_slq_fwd)(A, fun, num_samples=num_samples, max_iters=max_iters, tol=tol,
+                                                   pbar=pbar, key=key)
 
 
 _ = ctypes

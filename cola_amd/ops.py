@@ -21,6 +21,67 @@ from . import annotations as ann
 from ._blocks import from_blocks, ld_of, to_blocks
 
 
+_ENGINE = {"enabled": True}
+
+
+class engine_disabled:
+    """Context in which operators on HIP tensors use their torch arithmetic instead of the
+    engine's kernels, so that torch autograd can differentiate A(theta) @ X with respect to the
+    operator's tensors (the vector-Jacobian products of the iterative solvers' backward rules)."""
+
+    def __enter__(self):
+        self._prev = _ENGINE["enabled"]
+        _ENGINE["enabled"] = False
+        return self
+
+    def __exit__(self, *exc):
+        _ENGINE["enabled"] = self._prev
+        return False
+
+
+class _Leaf:
+    __slots__ = ("i", )
+
+    def __init__(self, i):
+        self.i = i
+
+
+class _OpTemplate:
+    __slots__ = ("cls", "fields")
+
+    def __init__(self, cls, fields):
+        self.cls, self.fields = cls, fields
+
+
+def _flatten(obj, leaves):
+    if isinstance(obj, torch.Tensor):
+        leaves.append(obj)
+        return _Leaf(len(leaves) - 1)
+    if isinstance(obj, LinearOperator):
+        return _OpTemplate(type(obj), {k: _flatten(v, leaves) for k, v in sorted(vars(obj).items())})
+    if type(obj) in (list, tuple):
+        return type(obj)(_flatten(v, leaves) for v in obj)
+    if type(obj) is dict:
+        return {k: _flatten(v, leaves) for k, v in obj.items()}
+    return obj
+
+
+def _rebuild(t, params):
+    if isinstance(t, _Leaf):
+        return params[t.i]
+    if isinstance(t, _OpTemplate):
+        obj = object.__new__(t.cls)
+        for k, v in t.fields.items():
+            object.__setattr__(obj, k, _rebuild(v, params))
+        obj._after_unflatten()
+        return obj
+    if type(t) in (list, tuple):
+        return type(t)(_rebuild(v, params) for v in t)
+    if type(t) is dict:
+        return {k: _rebuild(v, params) for k, v in t.items()}
+    return t
+
+
 def _find_device(objs):
     for o in objs:
         if isinstance(o, torch.Tensor):
@@ -51,6 +112,21 @@ class LinearOperator:
     def isa(self, annotation):
         return any(issubclass(a, annotation) for a in self.annotations)
 
+    def flatten(self):
+        """(tensors, unflatten): every tensor reachable from the operator's attributes (nested
+        operators, lists, tuples, dicts included) in a fixed order, and the function that builds
+        the same operator around replacement tensors (operator_base.py:89-95, 191-219)."""
+        leaves = []
+        tmpl = _flatten(self, leaves)
+
+        def unflatten(params):
+            return _rebuild(tmpl, list(params))
+
+        return leaves, unflatten
+
+    def _after_unflatten(self):
+        """Hook for operators that cache values derived from their tensors."""
+
     def _matmat(self, X):
         raise NotImplementedError
 
@@ -73,7 +149,8 @@ class LinearOperator:
         return from_blocks(Yb, n).to(X.dtype)
 
     def _use_hip(self, X):
-        return isinstance(X, torch.Tensor) and X.is_cuda and X.dtype == torch.float32 and self.hip_op() is not None
+        return (_ENGINE["enabled"] and isinstance(X, torch.Tensor) and X.is_cuda and X.dtype == torch.float32
+                and self.hip_op() is not None)
 
     # ---- algebra ------------------------------------------------------------
     def __matmul__(self, X):
@@ -312,6 +389,12 @@ class ScalarMul(LinearOperator):
 
     def _matmat(self, v):
         return self.c.to(v.device) * v
+
+    def _after_unflatten(self):
+        # the coefficient is now whatever tensor was passed in: read it again when needed
+        self._c_is_tensor = True
+        self._c_host = None
+        self._c_version = None
 
     def hip_op(self):
         if self.device.type != "cuda":

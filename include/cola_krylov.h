@@ -401,6 +401,20 @@ int ck_basis_combine(int64_t n, int32_t m, int32_t k, const float* V, int64_t ld
 int ck_basis_combine_batched(int64_t n, int32_t nb, int32_t k, const float* V, int64_t ldv, int64_t ldb,
                              const double* C, int64_t ldc, float* out, int64_t ldo, void* stream);
 
+/* ---------------------------------------------------------------- GMRES least squares
+ * The small solve of GMRES (cola/linalg/inverse/gmres.py:99-121) for nb Arnoldi blocks H
+ * ((m+1) x m fp32 row-major each, block b at H + b*ldh): y_b minimises
+ * ||H_b[:rows, :cols] y - beta_b e1||^2 + sum_{padded i} y_i^2, then y_i = 0 on padded rows,
+ * where a padded row's largest |entry| is below rtol * max|H_b| -- the normal equations
+ * (H^T H + D) y = H^T beta e1 the reference solves (with rtol = 10 * tol), solved by Givens
+ * rotations in FP64.  rows = cols + 1 after an Arnoldi run that stopped at step cols < m,
+ * rows = cols = m after a full run; rows <= 4096.  y (nb x cols fp64, leading dimension ldy);
+ * work >= ck_hessenberg_lstsq_workspace_bytes.  Replaces the xnp.solve of gmres.py:116. */
+size_t ck_hessenberg_lstsq_workspace_bytes(int32_t nb, int32_t rows, int32_t cols);
+int ck_hessenberg_lstsq_f64(int32_t nb, int32_t m, const float* H, int64_t ldh, int32_t rows, int32_t cols,
+                            const float* beta, double rtol, void* work, size_t work_bytes, double* y, int64_t ldy,
+                            void* stream);
+
 /* ---------------------------------------------------------------- quadrature / trace
  * Stochastic Lanczos quadrature: per probe b, q_b = scale_b * sum_c tau_bc^2 f(theta_bc)
  * with f(theta) = 0 where |theta| <= 10*eps*max|theta| (cola/linalg/unary/unary.py:55-57,

@@ -123,3 +123,37 @@ def test_algorithm_fields_match_reference_defaults():
 def test_keys_match_reference_chain():
     from cola_amd.linalg.keys import PRNGKey, next_key
     assert PRNGKey(42) == 3300923923 and next_key(3300923923) == 1419105812
+
+
+def test_flatten_unflatten_round_trip():
+    """operator_base.py:89-95: flatten lists every tensor of a composite operator; unflatten builds
+    the same structure around new tensors (here on the host, where operators use torch arithmetic)."""
+    g = torch.Generator().manual_seed(0)
+    M = torch.randn(5, 5, generator=g)
+    c = torch.tensor(0.3)
+    d = torch.rand(5, generator=g) + 1.0
+    A = ca.Dense(M) + ca.ScalarMul(c, (5, 5)) + ca.Diagonal(d)
+    leaves, unflatten = A.flatten()
+    assert any(t is M for t in leaves) and any(t is c for t in leaves) and any(t is d for t in leaves)
+    X = torch.randn(5, 2, generator=g)
+    B = unflatten([2 * t if t.is_floating_point() else t for t in leaves])
+    assert type(B) is type(A)
+    assert torch.allclose(B @ X, 2 * (A @ X), atol=1e-5)
+    # the original is untouched
+    assert torch.allclose(A @ X, M @ X + 0.3 * X + d[:, None] * X, atol=1e-5)
+
+
+def test_vjp_derivs_of_operator_products():
+    """d/dtheta <A(theta) X, D> for A = Dense(M) + c I: D X^T for M and <X, D> for c."""
+    from cola_amd.autodiff import vjp_derivs
+    g = torch.Generator().manual_seed(1)
+    M = torch.randn(4, 4, generator=g)
+    c = torch.tensor(0.7)
+    A = ca.Dense(M) + ca.ScalarMul(c, (4, 4))
+    leaves, unflatten = A.flatten()
+    X = torch.randn(4, 3, generator=g)
+    D = torch.randn(4, 3, generator=g)
+    grads = vjp_derivs(lambda *th: unflatten(th) @ X, leaves, D)
+    byid = {id(t): gr for t, gr in zip(leaves, grads)}
+    assert torch.allclose(byid[id(M)], D @ X.T, atol=1e-5)
+    assert torch.allclose(byid[id(c)], (X * D).sum(), atol=1e-5)

@@ -342,3 +342,74 @@ def test_slq_quadratic_forms_agrees_with_slq_fwd_for_rademacher():
     q, _, _ = oun.slq_quadratic_forms(alpha, beta, (Z**2).sum(0), torch.log)
     est = oun.slq_fwd(lambda V: A @ V, n, torch.log, Z, 25, 1e-10, torch.float64, torch.float64)
     assert abs(float(q.mean()) - float(est)) < 1e-9 * abs(float(est))
+
+
+# ------------------------------------------------------------------------------- GMRES
+def _pd_from_diag(diag, seed):
+    """cola/utils/utils_for_tests.py:164-173 with a seeded generator: Q^T diag Q, Q from QR of a normal matrix."""
+    rs = np.random.RandomState(seed)
+    Q, _ = np.linalg.qr(rs.normal(size=(len(diag), len(diag))))
+    return Q.T @ np.diag(diag) @ Q
+
+
+def test_oracle_gmres_easy_known_answer():
+    """tests/algorithms/test_gmres.py:72-88 (easy): diag(3, 4, 5) x = 1 with 3 steps -> (1/3, 1/4, 1/5)."""
+    from oracle import gmres as og
+    A = torch.diag(torch.tensor([3., 4., 5.]))
+    x, info = og.gmres(lambda V: A @ V, 3, torch.ones(3, 1), None, 3, 1e-8)
+    assert torch.allclose(x[:, 0], torch.tensor([1 / 3, 1 / 4, 1 / 5]), rtol=1e-6)
+
+
+def test_oracle_gmres_random_spectrum():
+    """tests/algorithms/test_gmres.py (random): spectrum 1 - x^0.5 on 25 points, max_iters = n - 5,
+    tol 1e-8, three rhs of ones: relative error below 5e-4."""
+    from oracle import gmres as og
+    x_ = np.linspace(0, 1, num=26)[:-1]
+    diag = (1 - x_**0.5).astype(np.float32)
+    A = torch.from_numpy(_pd_from_diag(diag, 7)).float()
+    rhs = torch.ones(25, 3)
+    soln = torch.linalg.solve(A.double(), rhs.double())
+    x, _ = og.gmres(lambda V: A @ V, 25, rhs, None, 20, 1e-8)
+    assert float((x.double() - soln).norm() / soln.norm()) < 5e-4
+
+
+def test_oracle_gmres_equals_fom_from_independent_arnoldi():
+    """With m < n steps and no padded rows, the reference's normal equations of the square m x m
+    Hessenberg block give x0 + Q_m H_m^-1 beta e1: against that formula built from an independent
+    FP64 Arnoldi with full re-orthogonalisation (numpy)."""
+    from oracle import gmres as og
+    rs = np.random.RandomState(11)
+    n, m = 60, 15
+    A = rs.normal(size=(n, n)) / np.sqrt(n) + 3 * np.eye(n)
+    b = rs.normal(size=n)
+    x, _ = og.gmres(lambda V: torch.from_numpy(A) @ V, n, torch.from_numpy(b), None, m, 1e-12, torch.float64)
+    Q = np.zeros((n, m + 1))
+    H = np.zeros((m + 1, m))
+    Q[:, 0] = b / np.linalg.norm(b)
+    for j in range(m):
+        w = A @ Q[:, j]
+        for _ in range(2):
+            h = Q[:, :j + 1].T @ w
+            w -= Q[:, :j + 1] @ h
+            H[:j + 1, j] += h
+        H[j + 1, j] = np.linalg.norm(w)
+        Q[:, j + 1] = w / H[j + 1, j]
+    e1 = np.zeros(m)
+    e1[0] = np.linalg.norm(b)
+    fom = Q[:, :m] @ np.linalg.solve(H[:m, :m], e1)
+    assert np.linalg.norm(x.numpy() - fom) / np.linalg.norm(fom) < 1e-10
+
+
+def test_oracle_gmres_early_stop_least_squares():
+    """A diagonalisable non-symmetric matrix with three distinct eigenvalues: Arnoldi stops after three
+    steps, the padded tail drops out, and x is the exact solution."""
+    from oracle import gmres as og
+    rs = np.random.RandomState(3)
+    n = 12
+    V = rs.normal(size=(n, n)) + 4 * np.eye(n)
+    lam = np.repeat([1.0, 2.0, 5.0], 4)
+    A = V @ np.diag(lam) @ np.linalg.inv(V)
+    b = rs.normal(size=(n, 1))
+    x, info = og.gmres(lambda W: torch.from_numpy(A) @ W, n, torch.from_numpy(b), None, 10, 1e-10, torch.float64)
+    assert info["iterations"] <= 5
+    assert np.linalg.norm(A @ x.numpy() - b) / np.linalg.norm(b) < 1e-8
