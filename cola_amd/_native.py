@@ -117,7 +117,7 @@ _PROTOS = {
     "ck_cg_fused_workspace_bytes": (_sz, [_i64, _i32]),
     "ck_cg_fused_supported": (ctypes.c_int, [_op, _op, _i32]),
     "ck_cg_fused": (ctypes.c_int, [_op, _op, _i32, _p, _i64, _i64, _p, _i64, _i64, _p, _i64, _i64, _p, _i64, _i64, _p,
-                                   _f32, _i32, _p, _sz, _i32, _i64, _i32, _p]),
+                                   _f32, _i32, _p, _sz, _i32, _i64, _i32, _i32, _p]),
     "ck_lanczos_state_bytes": (_sz, [_i32, _i32]),
     "ck_lanczos_scratch_bytes": (_sz, [_i64, _i32, _i32]),
     "ck_lanczos_start": (ctypes.c_int, [_i64, _i32, _p, _i64, _p, _i64, _i64, _p, _p, _p, _i32, _p, _p]),

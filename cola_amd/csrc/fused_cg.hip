@@ -17,15 +17,16 @@
 //
 // The exchange: every Ap entry travels as one 64-bit word (value bits, step tag) written and
 // read with agent-scope atomic stores and loads, which the compiler emits as coherent (sc1)
-// accesses that do not stop in a per-XCD L2.  A reader knows an entry is the current step's
-// when its tag matches, so no flag, fence or cache maintenance is needed: one store from
-// the owner, loads (re-issued until the tag matches) from everyone.  The buffers are doubled
-// by step parity: a workgroup can run at most one step ahead of the slowest reader (it needs
-// every entry of the current step, written after each writer finished reading the previous
-// one), so it never overwrites a word that is still to be read.  Every wait is bounded:
-// after spin_limit polling rounds it gives up, records CK_ERR_TIMEOUT in the control block
-// and leaves the loop, so every workgroup ends and the launch drains.  Co-residency of all
-// workgroups is guaranteed by hipLaunchCooperativeKernel.
+// accesses that do not stop in a per-XCD L2.  Readers load their words and re-read, all
+// together, the ones whose tag is still old: the tag is the flag, so a step costs the
+// writers' stores landing plus the readers' polling round trips, with no fence, no separate
+// flag and no cache maintenance.  The buffers are doubled by step parity: a workgroup can run
+// at most one step ahead of the slowest reader (it needs every entry of the current step,
+// written after each writer finished reading the previous one), so it never overwrites a word
+// still to be read.  Every wait is bounded: after spin_limit polling rounds it gives up,
+// records CK_ERR_TIMEOUT in the control block and leaves the loop, so every workgroup ends and
+// the launch drains.  All workgroups fit on the device at once (checked by occupancy before
+// the launch; a cooperative launch guarantees co-residency, a plain one is the default).
 #include "cg_state.hpp"
 #include "common.hpp"
 
@@ -35,7 +36,7 @@ namespace {
 
 constexpr double kSmall = (double)(float)1e-40;  // cg.py:8, as an fp32 constant
 constexpr int kNT = 256;
-constexpr int kMaxRhs = 8;
+constexpr int kMaxRhs = 4;
 constexpr int kMaxBlocks = 256;
 constexpr int kMaxRows = 64;              // rows per workgroup
 constexpr int kMaxEntries = 7168;         // n * nrhs
@@ -43,7 +44,7 @@ constexpr size_t kLdsBudget = 56 * 1024;  // dynamic LDS; the static arrays take
 constexpr int64_t kDefaultSpin = int64_t(1) << 22;
 
 // entries of one column a thread reads back per step (n <= 256 * max_t_of(NB))
-__host__ __device__ constexpr int max_t_of(int nb) { return nb == 1 ? 28 : nb == 2 ? 14 : nb == 4 ? 10 : 6; }
+__host__ __device__ constexpr int max_t_of(int nb) { return nb == 1 ? 28 : nb == 2 ? 14 : 10; }
 
 struct FusedArgs {
   int n, nn, nrhs, G, R;  // order, padded order (multiple of 4), columns, workgroups, rows per workgroup
@@ -78,6 +79,8 @@ __device__ inline void put(uint64_t* p, uint64_t v) {
 }
 __device__ inline uint64_t get(const uint64_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 __device__ inline void put(int32_t* p, int32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline void put(uint32_t* p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+__device__ inline uint32_t get(const uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 __device__ inline uint64_t pack(float v, uint32_t tag) { return ((uint64_t)tag << 32) | (uint64_t)__float_as_uint(v); }
 __device__ inline uint32_t tag_of(uint64_t w) { return (uint32_t)(w >> 32); }
@@ -99,48 +102,62 @@ __device__ inline void block_sums(double (&v)[NV], double* lds /* 4 * NV */) {
 }
 
 // This thread's entries j = tid + 256 t of every column from the tagged buffer xb (nn words
-// per column): all loads are issued first, then a word whose tag is not yet `want` is
-// re-read until it is (bounded by spin rounds; after one timeout the rest are not waited
-// for).  Returns false on timeout.
+// per column): the loads of a chunk of sixteen words are issued together, and the words still
+// carrying an old tag are re-read together, one round trip per round, until every tag is
+// `want` (bounded by spin rounds; after a timeout nothing more is waited for).  Returns false
+// on timeout (and sets *fail).
 template <int NB, int KT>
 __device__ inline bool collect(const uint64_t* xb, int n, int nn, int B, uint32_t want, int64_t spin,
-                               float (&out)[NB][KT]) {
-  uint64_t raw[NB][KT];
-  const uint64_t ready = (uint64_t)want << 32;
-#pragma unroll
-  for (int c = 0; c < NB; ++c)
-#pragma unroll
-    for (int t = 0; t < KT; ++t) {
-      const int j = threadIdx.x + kNT * t;
-      raw[c][t] = (c < B && j < n) ? get(xb + (int64_t)c * nn + j) : ready;
-    }
+                               float (&out)[NB][KT], int* fail) {
   bool ok = true;
+  // entries e = c * KT + t, sixteen at a time: all loads of a chunk are issued together, and
+  // the words still carrying an old tag are re-read together, one round trip per round
+  constexpr int E = NB * KT, CH = 16;
 #pragma unroll
-  for (int c = 0; c < NB; ++c)
+  for (int e0 = 0; e0 < E; e0 += CH) {
+    uint64_t raw[CH];
+    uint32_t pending = 0;
 #pragma unroll
-    for (int t = 0; t < KT; ++t) {
+    for (int u = 0; u < CH; ++u) {
+      const int e = e0 + u, c = e / KT, t = e % KT;
       const int j = threadIdx.x + kNT * t;
-      int64_t it = 0;
-      while (ok && tag_of(raw[c][t]) != want) {
-        if (++it > spin) {
-          ok = false;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-        raw[c][t] = get(xb + (int64_t)c * nn + j);
-      }
-      out[c][t] = value_of(raw[c][t]);
+      if (e < E && c < B && j < n) pending |= 1u << u;
     }
+    if (!ok) pending = 0;
+    for (int64_t it = 0; pending; ++it) {
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int e = e0 + u, c = e / KT, t = e % KT;
+        if (pending & (1u << u)) raw[u] = get(xb + (int64_t)c * nn + threadIdx.x + kNT * t);
+      }
+#pragma unroll
+      for (int u = 0; u < CH; ++u) {
+        const int e = e0 + u, c = e / KT, t = e % KT;
+        if ((pending & (1u << u)) && tag_of(raw[u]) == want) {
+          out[c][t] = value_of(raw[u]);
+          pending &= ~(1u << u);
+        }
+      }
+      if (!pending) break;
+      if (it >= spin) {
+        ok = false;
+        *fail = 1;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
   return ok;
 }
 
-// NB: columns rounded up to 1, 2, 4 or 8; TILE: own rows of A in LDS
+// NB: columns rounded up to 1, 2 or 4; TILE: own rows of A in LDS
 template <bool TILE, int NB>
 __global__ __launch_bounds__(kNT) void fused_cg_kernel(FusedArgs a) {
   constexpr int KT = max_t_of(NB);
   extern __shared__ float lds[];
   __shared__ double red[4 * 2 * NB];
   __shared__ float s_alpha[NB], s_beta[NB], s_gamma[NB], s_rnorm[NB], s_tol[NB], s_mult[NB];
+  __shared__ float xo[kMaxRows * NB];  // own rows of x, [i * NB + c]
   __shared__ int s_fail;
   const int n = a.n, nn = a.nn, B = a.nrhs, w = blockIdx.x, tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -154,9 +171,43 @@ __global__ __launch_bounds__(kNT) void fused_cg_kernel(FusedArgs a) {
   if (tid == 0) s_fail = 0;
 
   if (TILE) {
-    for (int64_t idx = tid; idx < (int64_t)a.R * nn; idx += kNT) {
-      const int i = (int)(idx / nn), j = (int)(idx % nn);
-      At[idx] = (i < rows && j < n) ? a.A[(int64_t)(row0 + i) * a.lda + j] : 0.f;
+    // own rows of A into LDS: eight float4 loads in flight per thread when rows are 16-byte
+    // aligned and n is a multiple of 4, else eight scalar loads
+    const bool v4 = (n & 3) == 0 && (a.lda & 3) == 0 && (reinterpret_cast<uintptr_t>(a.A) & 15) == 0;
+    if (v4) {
+      const int q = nn / 4;  // float4 per row
+      const int64_t total = (int64_t)a.R * q;
+      for (int64_t base = tid; base < total; base += 8 * kNT) {
+        float4 buf[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int64_t idx = base + (int64_t)u * kNT;
+          const int i = (int)(idx / q), j4 = (int)(idx % q);
+          buf[u] = (idx < total && i < rows) ? *reinterpret_cast<const float4*>(a.A + (int64_t)(row0 + i) * a.lda + 4 * j4)
+                                              : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int64_t idx = base + (int64_t)u * kNT;
+          if (idx < total) *reinterpret_cast<float4*>(At + 4 * idx) = buf[u];
+        }
+      }
+    } else {
+      const int64_t total = (int64_t)a.R * nn;
+      for (int64_t base = tid; base < total; base += 8 * kNT) {
+        float buf[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int64_t idx = base + (int64_t)u * kNT;
+          const int i = (int)(idx / nn), j = (int)(idx % nn);
+          buf[u] = (idx < total && i < rows && j < n) ? a.A[(int64_t)(row0 + i) * a.lda + j] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int64_t idx = base + (int64_t)u * kNT;
+          if (idx < total) At[idx] = buf[u];
+        }
+      }
     }
   }
   // cg.py:96-97: column norms of b (every workgroup, same order) and r = b / ||b||
@@ -232,7 +283,7 @@ __global__ __launch_bounds__(kNT) void fused_cg_kernel(FusedArgs a) {
     __syncthreads();
     ++sync;
     gemv_publish(xb, sync);
-    if (!collect<NB, KT>(xb, n, nn, B, sync + a.wait_offset, a.spin_limit, ap)) s_fail = 1;
+    collect<NB, KT>(xb, n, nn, B, sync + a.wait_offset, a.spin_limit, ap, &s_fail);
     __syncthreads();
     timeout = s_fail != 0;
     if (!timeout) {
@@ -245,14 +296,11 @@ __global__ __launch_bounds__(kNT) void fused_cg_kernel(FusedArgs a) {
         }
       for (int idx = tid; idx < rows * B; idx += kNT) {
         const int i = idx / B, c = idx % B;
-        a.x[(int64_t)(row0 + i) * a.x_rs + (int64_t)c * a.x_cs] = pv[c * nn + row0 + i];
+        xo[i * NB + c] = pv[c * nn + row0 + i];
       }
     }
   } else {
-    for (int idx = tid; idx < rows * B; idx += kNT) {
-      const int i = idx / B, c = idx % B;
-      a.x[(int64_t)(row0 + i) * a.x_rs + (int64_t)c * a.x_cs] = 0.f;
-    }
+    for (int idx = tid; idx < rows * B; idx += kNT) xo[(idx / B) * NB + idx % B] = 0.f;
   }
   __syncthreads();
   // cg.py:98-101,122-130: z = P r, p = z, gamma = r.z, threshold tol*||r0|| + tol
@@ -297,7 +345,7 @@ __global__ __launch_bounds__(kNT) void fused_cg_kernel(FusedArgs a) {
     uint64_t* xb = a.xbuf + (int64_t)(k & 1) * B * nn;
     ++sync;
     gemv_publish(xb, sync);
-    if (!collect<NB, KT>(xb, n, nn, B, sync + a.wait_offset, a.spin_limit, ap)) s_fail = 1;
+    collect<NB, KT>(xb, n, nn, B, sync + a.wait_offset, a.spin_limit, ap, &s_fail);
     // p.Ap over all n entries (every workgroup, same order) -> alpha (cg.py:143-147)
     double s[NB];
 #pragma unroll
@@ -327,8 +375,7 @@ __global__ __launch_bounds__(kNT) void fused_cg_kernel(FusedArgs a) {
     // x_own += alpha p_own
     for (int idx = tid; idx < rows * B; idx += kNT) {
       const int i = idx / B, c = idx % B;
-      float* xp = a.x + (int64_t)(row0 + i) * a.x_rs + (int64_t)c * a.x_cs;
-      *xp = fmaf(s_alpha[c], pv[c * nn + row0 + i], *xp);
+      xo[i * NB + c] = fmaf(s_alpha[c], pv[c * nn + row0 + i], xo[i * NB + c]);
     }
     // r -= alpha Ap (all entries), gamma' = r.z, ||r||^2
     double q[2 * NB];
@@ -398,8 +445,7 @@ __global__ __launch_bounds__(kNT) void fused_cg_kernel(FusedArgs a) {
   for (int idx = tid; idx < rows * B; idx += kNT) {
     const int i = idx / B, c = idx % B;
     const float m = s_mult[c];
-    float* xp = a.x + (int64_t)(row0 + i) * a.x_rs + (int64_t)c * a.x_cs;
-    *xp = *xp * m;
+    a.x[(int64_t)(row0 + i) * a.x_rs + (int64_t)c * a.x_cs] = xo[i * NB + c] * m;
     if (a.r) a.r[(int64_t)(row0 + i) * a.r_rs + (int64_t)c * a.r_cs] = rv[c * nn + row0 + i] * m;
   }
   if (w == 0) {
@@ -444,16 +490,14 @@ const void* kernel_nb(int nb) {
       return reinterpret_cast<const void*>(&fused_cg_kernel<TILE, 1>);
     case 2:
       return reinterpret_cast<const void*>(&fused_cg_kernel<TILE, 2>);
-    case 4:
-      return reinterpret_cast<const void*>(&fused_cg_kernel<TILE, 4>);
     default:
-      return reinterpret_cast<const void*>(&fused_cg_kernel<TILE, 8>);
+      return reinterpret_cast<const void*>(&fused_cg_kernel<TILE, 4>);
   }
 }
 
 const void* kernel_of(bool tile, int nb) { return tile ? kernel_nb<true>(nb) : kernel_nb<false>(nb); }
 
-int nb_of(int B) { return B <= 1 ? 1 : B <= 2 ? 2 : B <= 4 ? 4 : 8; }
+int nb_of(int B) { return B <= 1 ? 1 : B <= 2 ? 2 : 4; }
 
 // Workgroups that can be resident at once for (kernel, LDS bytes), cached per device.
 int max_coresident(bool tile, int nb, size_t lds) {
@@ -528,7 +572,8 @@ int ck_cg_fused_supported(const ck_op* A, const ck_op* P, int32_t nrhs) {
 int ck_cg_fused(const ck_op* A, const ck_op* P, int32_t nrhs, const float* b, int64_t b_rs, int64_t b_cs,
                 const float* x0, int64_t x0_rs, int64_t x0_cs, float* x, int64_t x_rs, int64_t x_cs, float* r,
                 int64_t r_rs, int64_t r_cs, void* state, float tol, int32_t max_iters, void* work,
-                size_t work_bytes, int32_t nblocks, int64_t spin_limit, int32_t wait_offset, void* stream) {
+                size_t work_bytes, int32_t nblocks, int64_t spin_limit, int32_t wait_offset, int32_t cooperative,
+                void* stream) {
   if (!A || !b || !x || !state || !work || max_iters < 0 || nblocks < 0 || wait_offset < 0) return CK_ERR_BAD_ARG;
   if (!fused_eligible(A, P, nrhs)) return CK_ERR_UNSUPPORTED;
   const int n = A->n;
@@ -576,8 +621,14 @@ int ck_cg_fused(const ck_op* A, const ck_op* P, int32_t nrhs, const float* b, in
   hipLaunchKernelGGL(fused_reset_kernel, dim3(rblocks), dim3(256), 0, s, a.xbuf, words, state);
   CK_CHECK_LAUNCH();
   void* args[] = {&a};
-  if (hipLaunchCooperativeKernel(kernel_of(pl.tile, nb), dim3(pl.G), dim3(kNT), args, (unsigned)pl.lds, s) != hipSuccess)
-    return CK_ERR_HIP;
+  // every workgroup fits at once (checked above); a cooperative launch also guarantees that
+  // they are resident together, a plain one costs about 20 us less per solve -- with the
+  // bounded waits, a workgroup that started late makes the others wait, never hang
+  const hipError_t e = cooperative
+                           ? hipLaunchCooperativeKernel(kernel_of(pl.tile, nb), dim3(pl.G), dim3(kNT), args,
+                                                        (unsigned)pl.lds, s)
+                           : hipLaunchKernel(kernel_of(pl.tile, nb), dim3(pl.G), dim3(kNT), args, (unsigned)pl.lds, s);
+  if (e != hipSuccess) return CK_ERR_HIP;
   return CK_OK;
 }
 

@@ -95,7 +95,29 @@ def _status_name(code):
     return STATUS_NAMES.get(int(code), f"status {int(code)}")
 
 
-def fused_solve(A, b, x0=None, max_iters=1000, tol=1e-6, P=None, nblocks=0, spin_limit=0, wait_offset=0):
+_FUSED_BUFFERS = {}
+
+
+def _fused_buffers(dev, n, B, max_iters):
+    """State, workspace and pinned host copy of the state for ck_cg_fused, kept per (device,
+    stream, shape): solves enqueued on one stream run in order, so they can share them."""
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    key = (dev.index, stream, n, B, int(max_iters))
+    bufs = _FUSED_BUFFERS.get(key)
+    if bufs is None:
+        if len(_FUSED_BUFFERS) >= 32:
+            _FUSED_BUFFERS.clear()
+        nbytes = N.q("ck_cg_state_bytes", B, max_iters)
+        wb = N.q("ck_cg_fused_workspace_bytes", n, B)
+        bufs = (torch.empty(nbytes, dtype=torch.uint8, device=dev),
+                torch.empty((wb + 7) // 8, dtype=torch.float64, device=dev),
+                torch.empty(nbytes, dtype=torch.uint8, pin_memory=True), N.q("ck_cg_stride", B))
+        _FUSED_BUFFERS[key] = bufs
+    return bufs
+
+
+def fused_solve(A, b, x0=None, max_iters=1000, tol=1e-6, P=None, nblocks=0, spin_limit=0, wait_offset=0,
+                cooperative=False):
     """cg.py:94-119 as one persistent launch (ck_cg_fused) when A lowers to a dense fp32
     operator and P is None or diagonal; returns (x, r, k, info) or None when unsupported.
     b (n, B) is read in place (any strides), x and r are written in the same layout."""
@@ -108,12 +130,9 @@ def fused_solve(A, b, x0=None, max_iters=1000, tol=1e-6, P=None, nblocks=0, spin
         return None
     dev = b.device
     bf = b if b.dtype == torch.float32 else b.float()
-    x = torch.empty((n, B), dtype=torch.float32, device=dev)
-    r = torch.empty((n, B), dtype=torch.float32, device=dev)
-    nbytes = N.q("ck_cg_state_bytes", B, max_iters)
-    state = torch.empty(nbytes, dtype=torch.uint8, device=dev)
-    wb = N.q("ck_cg_fused_workspace_bytes", n, B)
-    work = torch.empty((wb + 7) // 8, dtype=torch.float64, device=dev)
+    xr = torch.empty((2, n, B), dtype=torch.float32, device=dev)
+    x, r = xr[0], xr[1]
+    state, work, host, stride = _fused_buffers(dev, n, B, max_iters)
     x0f = None
     if x0 is not None:
         x0f = x0.reshape(n, B)
@@ -122,19 +141,18 @@ def fused_solve(A, b, x0=None, max_iters=1000, tol=1e-6, P=None, nblocks=0, spin
     N.call("ck_cg_fused", aop[0], pop[0] if pop else None, B, N.ptr(bf), bf.stride(0), bf.stride(1), N.ptr(x0f),
            x0f.stride(0) if x0f is not None else 0, x0f.stride(1) if x0f is not None else 0, N.ptr(x), x.stride(0),
            x.stride(1), N.ptr(r), r.stride(0), r.stride(1), N.ptr(state), float(tol), int(max_iters), N.ptr(work),
-           work.numel() * 8, int(nblocks), int(spin_limit), int(wait_offset), N.stream_ptr(dev))
-    host = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+           work.numel() * 8, int(nblocks), int(spin_limit), int(wait_offset), int(bool(cooperative)),
+           N.stream_ptr(dev))
     host.copy_(state, non_blocking=True)
     torch.cuda.current_stream(dev).synchronize()
-    ctrl = host[:N.CTRL_BYTES].view(torch.int32).tolist()
-    k, status = ctrl[0], ctrl[5]
+    ctrl = host[:N.CTRL_BYTES].view(torch.int32)
+    k, status = int(ctrl[0]), int(ctrl[5])
     if status == N.CK_ERR_TIMEOUT:
         raise N.KrylovError("ck_cg_fused", status, "a grid-wide wait of the persistent CG timed out")
-    stride = N.q("ck_cg_stride", B)
-    f = host[N.CTRL_BYTES:].view(torch.float32)
-    errs = f[N.CK_CG_ERRORS * stride:N.CK_CG_ERRORS * stride + k + 1].numpy().astype(np.float64)
-    rn = f[N.CK_CG_RNORM * stride:N.CK_CG_RNORM * stride + B].numpy()
-    tl = f[N.CK_CG_TOL * stride:N.CK_CG_TOL * stride + B].numpy()
+    f = host[N.CTRL_BYTES:].view(torch.float32).numpy()
+    errs = f[N.CK_CG_ERRORS * stride:N.CK_CG_ERRORS * stride + k + 1].astype(np.float64)
+    rn = f[N.CK_CG_RNORM * stride:N.CK_CG_RNORM * stride + B]
+    tl = f[N.CK_CG_TOL * stride:N.CK_CG_TOL * stride + B]
     elapsed = time.time() - t0
     info = {"iterations": k + 1, "errors": np.concatenate([errs, errs[-1:]])[2:], "iteration_time": elapsed / (k + 1),
             "converged": bool((rn <= tl).all()), "status": _status_name(status), "fused": True}

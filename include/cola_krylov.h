@@ -296,24 +296,28 @@ int ck_cg_finish(int64_t n, int32_t nrhs, float* x, float* r, int64_t ld, const 
  * start, every step and the loop of cola/utils/torch_tqdm.py:86-90, un-normalisation --
  * in ONE cooperative launch (plus a one-block reset), for a dense fp32 operator
  * A = CK_OP_DENSE_F32 (y = scale*A x + shift*x) and P = NULL or CK_OP_DIAG_F32, 1 <= nrhs
- * <= 8, n small enough that two n x nrhs fp32 vectors fit in 56 KB of LDS (n*nrhs <= 7168).
- * Each workgroup keeps its rows of A in LDS when they fit and the full r and p vectors;
- * a step costs one grid-wide exchange (the p.Ap all-reduce, which is also the barrier
- * that completes Ap).  b, x0 (optional) and x, r (r optional) are (n x nrhs) arrays with
+ * <= 4, n*nrhs <= 7168 (and n <= 2560 for 3 or 4 columns): two n x nrhs fp32 vectors in LDS
+ * and each thread's share of A p in registers.  Each workgroup keeps its rows of A in LDS
+ * when they fit and the full r and p vectors; a step costs one grid-wide exchange: each
+ * workgroup publishes its rows of A p as words tagged with the step, and every workgroup
+ * reads all of A p back.  b, x0 (optional) and x, r (r optional) are (n x nrhs) arrays with
  * element strides (row stride, column stride), i.e. the reference's layout directly.
  * state: ck_cg_state_bytes(nrhs, max_iters), laid out and filled as by the chained
  * entry points (k, done, status, errors[0..k], per-column scalars); work:
  * ck_cg_fused_workspace_bytes.  nblocks = 0 picks the workgroup count.  Every grid-wide
  * wait gives up after spin_limit polling rounds (<= 0: about a second), records
  * CK_ERR_TIMEOUT in ctrl.status and ends the kernel; wait_offset (0 in normal use) is
- * added to the generation each wait expects, so a host can make every wait time out
- * and exercise that exit.  Returns CK_ERR_UNSUPPORTED for other operators or sizes. */
+ * added to the tag each wait expects, so a host can make every wait time out and exercise
+ * that exit.  cooperative = 1 launches with hipLaunchCooperativeKernel (co-residency
+ * guaranteed), 0 with a plain launch after the same occupancy check (about 20 us cheaper).
+ * Returns CK_ERR_UNSUPPORTED for other operators or sizes. */
 size_t ck_cg_fused_workspace_bytes(int64_t n, int32_t nrhs);
 int ck_cg_fused_supported(const ck_op* A, const ck_op* P, int32_t nrhs);
 int ck_cg_fused(const ck_op* A, const ck_op* P, int32_t nrhs, const float* b, int64_t b_rs, int64_t b_cs,
                 const float* x0, int64_t x0_rs, int64_t x0_cs, float* x, int64_t x_rs, int64_t x_cs, float* r,
                 int64_t r_rs, int64_t r_cs, void* state, float tol, int32_t max_iters, void* work,
-                size_t work_bytes, int32_t nblocks, int64_t spin_limit, int32_t wait_offset, void* stream);
+                size_t work_bytes, int32_t nblocks, int64_t spin_limit, int32_t wait_offset, int32_t cooperative,
+                void* stream);
 
 /* ---------------------------------------------------------------- Lanczos
  * Batched Lanczos with double classical Gram-Schmidt against every filled basis
