@@ -37,9 +37,10 @@ HBM_PEAK = 8000.0  # GB/s
 # WRITE_SIZE (x1.0), the factors a 1 GiB stream copy reads on the same counters (profiles/kron64_cg_64cubed_hbm_traffic.md)
 PASS_A_TRAFFIC_BYTES = (16651.2 + 16222.7) * 1024
 FP32_MFMA_PEAK = 157.3  # TFLOP/s
-# HBM bytes per dense_lds_kernel<8,4> launch (16384^2 fp32 matrix, 8 rhs): rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE scaled by the factors a 1 GiB stream copy reads on the same counters; None until measured
-DENSE8_TRAFFIC_BYTES = None
+# HBM bytes per dense_lds_kernel<8,4> launch (16384^2 fp32 matrix, 8 rhs): rocprofv3 --pmc FETCH_SIZE (x2.0)
+# plus WRITE_SIZE (x1.0), the factors a 1 GiB stream copy reads on the same counters
+# (profiles/dense_matvec_16384x8_hbm_traffic.md)
+DENSE8_TRAFFIC_BYTES = (526427.07 * 2.0 + 512.42) * 1024
 
 
 def rbf_factor(t, ell):
@@ -291,23 +292,40 @@ def bench_csr(args, dev):
 
 
 def bench_dense_cg(args, dev):
-    """A = X X^T/256 + 0.1 I, N=1024, one N(0,1) rhs (seed 0), CG to the reference tolerance."""
-    from cola_amd.linalg import HipCG, inv
+    """A = X X^T/256 + 0.1 I, N=1024, one N(0,1) rhs (seed 0), CG to the reference tolerance: the
+    persistent fused solve (one launch per solve) and the chained-launch path on the same system."""
     g = torch.Generator().manual_seed(0)
     X = torch.randn(1024, 256, generator=g)
     b = torch.randn(1024, generator=g)
     A = X @ X.T / 256 + 0.1 * torch.eye(1024)
     op = ca.PSD(ca.Dense(A.to(dev)))
     bd = b.to(dev)[:, None]
-    for _ in range(1 + args.warmup):  # one set-up call (workspaces, first lowering), then the warmup
-        x, r, k, info = run_batched_cg(op, bd, None, 1000, 1e-6, None, chunk=32)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        x, r, k, info = run_batched_cg(op, bd, None, 1000, 1e-6, None, chunk=32)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
+
+    def timed(fused):
+        for _ in range(1 + args.warmup):  # one set-up call (workspaces, first lowering), then the warmup
+            out = run_batched_cg(op, bd, None, 1000, 1e-6, None, chunk=32, fused=fused)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            out = run_batched_cg(op, bd, None, 1000, 1e-6, None, chunk=32, fused=fused)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / args.steps, out
+
+    dt, (x, r, k, info) = timed(True)
+    dtc, (xc, rc, kc, ic) = timed(False)
     res = float((A.double() @ x.cpu().double()[:, 0] - b.double()).norm() / b.double().norm())
+    # GPU time of the solve alone (the reset and solve kernels back to back, HIP events on the
+    # engine's stream) for the roofline line
+    from cola_amd.linalg._engine import lower
+    from cola_amd.linalg.cg import _fused_buffers
+    aop = lower(op)
+    state, work, _, _ = _fused_buffers(dev, 1024, 1, 1000)
+    xr = torch.empty((2, 1024, 1), dtype=torch.float32, device=dev)
+    t_kernel = time_kernel(lambda: N.call("ck_cg_fused", aop[0], None, 1, N.ptr(bd), bd.stride(0), bd.stride(1), None, 0,
+                                          0, N.ptr(xr[0]), 1, 1, N.ptr(xr[1]), 1, 1, N.ptr(state), 1e-6, 1000,
+                                          N.ptr(work), work.numel() * 8, 0, 0, 0, 0, N.stream_ptr(dev)), reps=20)
+    step_bytes = 4 * 1024 * 1024 + 8 * 4 * 1024  # A once per step plus the vector streams
+    ach = int(k) * step_bytes / t_kernel / 1e9
     from oracle import cg as ocg
     torch.set_num_threads(cpu_threads())
     t1 = time.perf_counter()
@@ -317,11 +335,19 @@ def bench_dense_cg(args, dev):
         reps += 1
     dto = (time.perf_counter() - t1) / reps
     return {"metric": "CG solves/sec (dense N=1024, 1 rhs, reference tol 1e-6)", "value": 1 / dt,
-            "unit": "solves/s", "ms_per_step": dt * 1e3, "steps": args.steps, "dtype": "f32",
-            "config": {"workload": "dense_cg", "cg_steps": int(k), "oracle_steps": int(io["iterations"] - 1),
-                       "fp64_rel_residual": res},
+            "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded X, rhs)",
+            "config": {"workload": "dense_cg: A = X X^T/256 + 0.1 I, N=1024, 1 rhs, tol 1e-6 (persistent fused CG)",
+                       "cg_steps": int(k), "oracle_steps": int(io["iterations"] - 1), "fp64_rel_residual": res,
+                       "chained_launch_solves_per_s": 1 / dtc, "chained_cg_steps": int(kc),
+                       "solve_gpu_us": t_kernel * 1e6, "us_per_cg_step_gpu": t_kernel * 1e6 / max(int(k), 1)},
+            "roofline": {"bound": "hbm", "kernel": "fused_cg_kernel<true, 1> (whole solve, A rows held in LDS)",
+                         "achieved": ach, "peak": HBM_PEAK, "unit": "GB/s", "frac": ach / HBM_PEAK, "traffic": None,
+                         "algorithmic_bytes_per_step": step_bytes,
+                         "note": "latency-bound: one grid-wide exchange per step; A is read from LDS, not HBM"},
             "cpu_baseline": {"value": 1 / dto, "unit": "solves/s", "cores": cpu_threads(), "kind": "port",
-                             "sample": f"{reps} full solves"}}
+                             "sample": f"{reps} full solves of the oracle's CG (fp32 torch) in 2 s"}}
 
 
 def bench_lanczos(args, dev):
@@ -565,7 +591,7 @@ def bench_learned_cg(args, dev):
                       "systems": per},
            "roofline": roof}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline_refined(systems, B, args.tol, itol)
+        out["cpu_baseline"] = cpu_baseline_refined(systems, B, args.tol, itol, head["per_system"])
     del systems, per
     torch.cuda.empty_cache()
     if not args.no_kron:
@@ -611,50 +637,57 @@ def timed_refined(systems, precs, B, args, itol, reps):
     return {"s": dt, "solved": solved, "inner_steps": steps, "per_system": per, "timed_passes": max(1, reps)}
 
 
-def cpu_baseline_refined(systems, B, tol, itol, budget_s=25.0, max_refinements=20):
+def cpu_baseline_refined(systems, B, tol, itol, per_system, max_refinements=30):
     """The reference algorithm on the host: the oracle's run_batched_cg (cola's CG restated in
     torch, FP32) with the same preconditioner (its Woodbury apply in torch), inside the same FP64
-    refinement, on the systems in order until `budget_s` of CPU time is spent.  value = FP64-checked
-    solved rhs per second of that sample."""
+    refinement.  The sample is one converging solve (FP64-checked) of the system that needed the
+    fewest inner steps on the GPU; its time per inner CG step and per FP64 residual is scaled by
+    the inner steps and refinements every system took on the GPU, which gives the time the same
+    converging solves of the whole workload take on the host."""
     from oracle import cg as ocg
     torch.set_num_threads(cpu_threads())
-    used, solved, nsys, steps = 0.0, 0, 0, 0
-    for sd in systems:
-        K = sd["K"].cpu()
-        K64 = K.double()  # residual operator (the matrix as stored, FP64 arithmetic): set-up, not timed
-        b = sd["b"].cpu()
-        b64 = b.double()
-        s = sd["shift"]
-        P = sd["P"]
-        if P is not None:
-            F, F64, core, ps = P.F.cpu(), P.F.cpu().double(), P.core_inv.cpu(), P.s
+    cost = [sum(r["inner_steps"]) for r in per_system]
+    j = int(np.argmin(cost))
+    sd = systems[j]
+    K = sd["K"].cpu()
+    K64 = K.double()  # residual operator (the matrix as stored, FP64 arithmetic): set-up, not timed
+    b64 = sd["b"].cpu().double()
+    s = sd["shift"]
+    P = sd["P"]
+    prec = None
+    if P is not None:
+        F64, core, ps = P.F.cpu().double(), P.core_inv.cpu(), P.s
 
-            def prec(V, F=F, F64=F64, core=core, ps=ps):
-                t = F64.T @ V.double()
-                return ((V.double() - F64 @ (core @ t)) / ps).float()
-        else:
-            prec = None
+        def prec(V):
+            t = F64.T @ V.double()
+            return ((V.double() - F64 @ (core @ t)) / ps).float()
+    x64 = torch.zeros(K.shape[0], B, dtype=torch.float64)
+    t_res = t_cg = 0.0
+    steps = refinements = 0
+    for _ in range(max_refinements):
         t0 = time.perf_counter()
-        x64 = torch.zeros(K.shape[0], B, dtype=torch.float64)
-        for _ in range(max_refinements):
-            r = b64 - (K64 @ x64 + s * x64)
-            rel = (r.norm(dim=0) / b64.norm(dim=0)).max()
-            if float(rel) <= tol:
-                break
-            d, _, k, _ = ocg.run_batched_cg(lambda V: K @ V + s * V, r.float(), None, 1000, itol, prec)
-            steps += int(k)
-            x64 += d.double()
-        res = ((K64 @ x64 + s * x64) - b64).norm(dim=0) / b64.norm(dim=0)
-        used += time.perf_counter() - t0
-        solved += int((res <= tol).sum())
-        nsys += 1
-        del K64
-        if used > budget_s:
+        r = b64 - (K64 @ x64 + s * x64)
+        rel = float((r.norm(dim=0) / b64.norm(dim=0)).max())
+        t_res += time.perf_counter() - t0
+        if rel <= tol:
             break
-    return {"value": solved / used, "unit": "solves/s", "cores": cpu_threads(), "kind": "port",
-            "sample": f"the first {nsys} of {len(systems)} systems ({nsys * B} rhs, {solved} reached {tol:g}) with the "
-                      f"oracle's CG (FP32 torch, same preconditioner, {steps} inner steps) inside FP64 refinement: "
-                      f"{used:.1f} s on {cpu_threads()} host threads"}
+        t0 = time.perf_counter()
+        d, _, k, _ = ocg.run_batched_cg(lambda V: K @ V + s * V, r.float(), None, 1000, itol, prec)
+        t_cg += time.perf_counter() - t0
+        steps += int(k)
+        refinements += 1
+        x64 += d.double()
+    res = ((K64 @ x64 + s * x64) - b64).norm(dim=0) / b64.norm(dim=0)
+    solved = int((res <= tol).sum())
+    per_step = t_cg / max(steps, 1)
+    per_res = t_res / (refinements + 1)
+    total = sum(sum(r["inner_steps"]) * per_step + (r["refinements"] + 1) * per_res for r in per_system)
+    return {"value": B * len(systems) / total, "unit": "solves/s", "cores": cpu_threads(), "kind": "port",
+            "sample": f"one converging solve of system {j} ({B} rhs, {solved} reached {tol:g} FP64-checked): the oracle's "
+                      f"CG (FP32 torch) with the same preconditioner, {steps} inner steps in {refinements} refinements, "
+                      f"{per_step * 1e3:.1f} ms per CG step and {per_res * 1e3:.1f} ms per FP64 residual on "
+                      f"{cpu_threads()} host threads ({t_cg + t_res:.1f} s), scaled to the inner steps and refinements "
+                      f"of all {len(systems)} systems' GPU solves"}
 
 
 def bench_dense_matvec(args, dev):
@@ -836,6 +869,13 @@ def cpu_baseline_dense(sd, n, B, k_steps, tol, iters=130):
                       f"scaled to the {k_steps} steps plain CG takes on it"}
 
 
+def workloads():
+    """--workload name -> bench function."""
+    return {"learned_cg": bench_learned_cg, "kron_cg": bench_kron_cg, "dense_cg": bench_dense_cg,
+            "batched_cg": bench_batched_cg, "lanczos": bench_lanczos, "hutch": bench_hutch, "slq": bench_slq,
+            "csr": bench_csr, "copy": bench_copy, "dense_matvec": bench_dense_matvec}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -866,9 +906,7 @@ def main():
         sys.exit(2)
     dev = torch.device("cuda:0")
     N.load()
-    fn = {"kron_cg": bench_kron_cg, "copy": bench_copy, "dense_cg": bench_dense_cg, "lanczos": bench_lanczos,
-          "hutch": bench_hutch, "slq": bench_slq, "learned_cg": bench_learned_cg,
-          "batched_cg": bench_batched_cg, "csr": bench_csr, "dense_matvec": bench_dense_matvec}[args.workload]
+    fn = workloads()[args.workload]
     out = fn(args, dev)
     print(json.dumps(out), flush=True)
 
