@@ -116,6 +116,9 @@ _PROTOS = {
     "ck_hessenberg_lstsq_f64": (ctypes.c_int, [_i32, _i32, _p, _i64, _i32, _i32, _p, _f64, _p, _sz, _p, _i64, _p]),
     "ck_cg_fused_workspace_bytes": (_sz, [_i64, _i32]),
     "ck_cg_fused_supported": (ctypes.c_int, [_op, _op, _i32]),
+    "ck_cg_fused_noalloc": (ctypes.c_int, [_op, _op, _i32, _p, _i64, _i64, _p, _i64, _i64, _p, _i64, _i64, _p, _i64,
+                                           _i64, _p, _f32, _i32, _i32, _i64, _i32, _i32, _p]),
+    "ck_cg_fused_noalloc_max_bytes": (_sz, []),
     "ck_cg_fused": (ctypes.c_int, [_op, _op, _i32, _p, _i64, _i64, _p, _i64, _i64, _p, _i64, _i64, _p, _i64, _i64, _p,
                                    _f32, _i32, _p, _sz, _i32, _i64, _i32, _i32, _p]),
     "ck_lanczos_state_bytes": (_sz, [_i32, _i32]),

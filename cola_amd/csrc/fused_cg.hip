@@ -27,6 +27,8 @@
 // records CK_ERR_TIMEOUT in the control block and leaves the loop, so every workgroup ends and
 // the launch drains.  All workgroups fit on the device at once (checked by occupancy before
 // the launch; a cooperative launch guarantees co-residency, a plain one is the default).
+#include <mutex>
+
 #include "cg_state.hpp"
 #include "common.hpp"
 
@@ -554,32 +556,44 @@ bool fused_eligible(const ck_op* A, const ck_op* P, int nrhs) {
   return nrhs >= 1 && nrhs <= kMaxRhs;
 }
 
-}  // namespace
+// The allocation-free entry point's exchange buffer: a file-scope device array, one copy per
+// GPU, held until the process ends.  Host code takes its device address once per GPU with
+// hipGetSymbolAddress (with that GPU current) and passes that address to the kernels; the
+// array's name itself is only a host-side stand-in.  Every call on one GPU shares the array,
+// so calls that overlap on one GPU race; the reset kernel clears its tags before each solve.
+constexpr int kScratchWords = 8192;  // 64 KiB: 2 * nrhs * (n rounded up to 4) words, i.e. about n * nrhs <= 4096
+__device__ __align__(256) uint64_t scratch[kScratchWords];
 
-extern "C" {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+uint64_t* g_scratch_ptr[kMaxDevices] = {};
 
-size_t ck_cg_fused_workspace_bytes(int64_t n, int32_t nrhs) {
-  if (n < 1 || nrhs < 1) return 0;
-  return work_bytes_of((int)((n + 3) / 4 * 4), nrhs);
+int scratch_of_current_device(uint64_t** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return CK_ERR_HIP;
+  if (dev < 0 || dev >= kMaxDevices) return CK_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return CK_ERR_HIP;
+    g_scratch_ptr[dev] = static_cast<uint64_t*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return CK_OK;
 }
 
-int ck_cg_fused_supported(const ck_op* A, const ck_op* P, int32_t nrhs) {
-  if (!fused_eligible(A, P, nrhs)) return CK_ERR_UNSUPPORTED;
-  Plan p = make_plan(A->n, nrhs, 0);
-  return p.ok ? CK_OK : CK_ERR_UNSUPPORTED;
-}
-
-int ck_cg_fused(const ck_op* A, const ck_op* P, int32_t nrhs, const float* b, int64_t b_rs, int64_t b_cs,
-                const float* x0, int64_t x0_rs, int64_t x0_cs, float* x, int64_t x_rs, int64_t x_cs, float* r,
-                int64_t r_rs, int64_t r_cs, void* state, float tol, int32_t max_iters, void* work,
-                size_t work_bytes, int32_t nblocks, int64_t spin_limit, int32_t wait_offset, int32_t cooperative,
-                void* stream) {
-  if (!A || !b || !x || !state || !work || max_iters < 0 || nblocks < 0 || wait_offset < 0) return CK_ERR_BAD_ARG;
+// The solve with its exchange buffer at xbuf (xbuf_bytes available); shared by the entry points.
+int fused_launch(const ck_op* A, const ck_op* P, int32_t nrhs, const float* b, int64_t b_rs, int64_t b_cs,
+                 const float* x0, int64_t x0_rs, int64_t x0_cs, float* x, int64_t x_rs, int64_t x_cs, float* r,
+                 int64_t r_rs, int64_t r_cs, void* state, float tol, int32_t max_iters, uint64_t* xbuf,
+                 size_t xbuf_bytes, int32_t nblocks, int64_t spin_limit, int32_t wait_offset, int32_t cooperative,
+                 void* stream) {
+  if (!A || !b || !x || !state || !xbuf || max_iters < 0 || nblocks < 0 || wait_offset < 0) return CK_ERR_BAD_ARG;
   if (!fused_eligible(A, P, nrhs)) return CK_ERR_UNSUPPORTED;
   const int n = A->n;
   Plan pl = make_plan(n, nrhs, nblocks);
   if (!pl.ok) return CK_ERR_UNSUPPORTED;
-  if (work_bytes < work_bytes_of(pl.nn, nrhs) || (reinterpret_cast<uintptr_t>(work) & 7) != 0) return CK_ERR_WORKSPACE;
+  if (xbuf_bytes < work_bytes_of(pl.nn, nrhs)) return CK_ERR_WORKSPACE;
   const int nb = nb_of(nrhs);
   if (pl.G > max_coresident(pl.tile, nb, pl.lds)) return CK_ERR_UNSUPPORTED;
   FusedArgs a{};
@@ -612,7 +626,7 @@ int ck_cg_fused(const ck_op* A, const ck_op* P, int32_t nrhs, const float* b, in
   a.state = state;
   a.tol = tol;
   a.max_iters = max_iters;
-  a.xbuf = static_cast<uint64_t*>(work);
+  a.xbuf = xbuf;
   a.spin_limit = spin_limit > 0 ? spin_limit : kDefaultSpin;
   a.wait_offset = (uint32_t)wait_offset;
   hipStream_t s = (hipStream_t)stream;
@@ -631,5 +645,43 @@ int ck_cg_fused(const ck_op* A, const ck_op* P, int32_t nrhs, const float* b, in
   if (e != hipSuccess) return CK_ERR_HIP;
   return CK_OK;
 }
+
+}  // namespace
+
+extern "C" {
+
+size_t ck_cg_fused_workspace_bytes(int64_t n, int32_t nrhs) {
+  if (n < 1 || nrhs < 1) return 0;
+  return work_bytes_of((int)((n + 3) / 4 * 4), nrhs);
+}
+
+int ck_cg_fused_supported(const ck_op* A, const ck_op* P, int32_t nrhs) {
+  if (!fused_eligible(A, P, nrhs)) return CK_ERR_UNSUPPORTED;
+  Plan p = make_plan(A->n, nrhs, 0);
+  return p.ok ? CK_OK : CK_ERR_UNSUPPORTED;
+}
+
+int ck_cg_fused(const ck_op* A, const ck_op* P, int32_t nrhs, const float* b, int64_t b_rs, int64_t b_cs,
+                const float* x0, int64_t x0_rs, int64_t x0_cs, float* x, int64_t x_rs, int64_t x_cs, float* r,
+                int64_t r_rs, int64_t r_cs, void* state, float tol, int32_t max_iters, void* work,
+                size_t work_bytes, int32_t nblocks, int64_t spin_limit, int32_t wait_offset, int32_t cooperative,
+                void* stream) {
+  if (!work || (reinterpret_cast<uintptr_t>(work) & 7) != 0) return !work ? CK_ERR_BAD_ARG : CK_ERR_WORKSPACE;
+  return fused_launch(A, P, nrhs, b, b_rs, b_cs, x0, x0_rs, x0_cs, x, x_rs, x_cs, r, r_rs, r_cs, state, tol, max_iters,
+                      static_cast<uint64_t*>(work), work_bytes, nblocks, spin_limit, wait_offset, cooperative, stream);
+}
+
+int ck_cg_fused_noalloc(const ck_op* A, const ck_op* P, int32_t nrhs, const float* b, int64_t b_rs, int64_t b_cs,
+                        const float* x0, int64_t x0_rs, int64_t x0_cs, float* x, int64_t x_rs, int64_t x_cs, float* r,
+                        int64_t r_rs, int64_t r_cs, void* state, float tol, int32_t max_iters, int32_t nblocks,
+                        int64_t spin_limit, int32_t wait_offset, int32_t cooperative, void* stream) {
+  uint64_t* ptr = nullptr;
+  const int st = scratch_of_current_device(&ptr);
+  if (st != CK_OK) return st;
+  return fused_launch(A, P, nrhs, b, b_rs, b_cs, x0, x0_rs, x0_cs, x, x_rs, x_cs, r, r_rs, r_cs, state, tol, max_iters,
+                      ptr, sizeof(scratch), nblocks, spin_limit, wait_offset, cooperative, stream);
+}
+
+size_t ck_cg_fused_noalloc_max_bytes(void) { return sizeof(scratch); }
 
 }  // extern "C"

@@ -319,6 +319,20 @@ int ck_cg_fused(const ck_op* A, const ck_op* P, int32_t nrhs, const float* b, in
                 size_t work_bytes, int32_t nblocks, int64_t spin_limit, int32_t wait_offset, int32_t cooperative,
                 void* stream);
 
+/* ck_cg_fused without a workspace argument and without any allocation: the exchange buffer
+ * is a file-scope device array of ck_cg_fused_noalloc_max_bytes() (64 KiB) per GPU, held
+ * until the process ends.  No counterpart in the reference: for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * caller still passes b, x, r and state.  Sizes whose exchange buffer exceeds the array
+ * (ck_cg_fused_workspace_bytes(n, nrhs) > ck_cg_fused_noalloc_max_bytes(), i.e. about
+ * n * nrhs > 4096) return CK_ERR_WORKSPACE.  All calls on one GPU share the array: calls that overlap on one GPU
+ * (other streams, other threads) race and must be serialised by the caller. */
+int ck_cg_fused_noalloc(const ck_op* A, const ck_op* P, int32_t nrhs, const float* b, int64_t b_rs, int64_t b_cs,
+                        const float* x0, int64_t x0_rs, int64_t x0_cs, float* x, int64_t x_rs, int64_t x_cs, float* r,
+                        int64_t r_rs, int64_t r_cs, void* state, float tol, int32_t max_iters, int32_t nblocks,
+                        int64_t spin_limit, int32_t wait_offset, int32_t cooperative, void* stream);
+size_t ck_cg_fused_noalloc_max_bytes(void);
+
 /* ---------------------------------------------------------------- Lanczos
  * Batched Lanczos with double classical Gram-Schmidt against every filled basis
  * vector (cola/linalg/decompositions/lanczos.py:235-296).  For nb start vectors
