@@ -3,7 +3,7 @@
 Keeps cola's LinearOperator / annotation / cola.linalg API; the solvers run as
 hand-written gfx950 HIP kernels behind a C ABI (include/cola_krylov.h).
 """
-from . import annotations, linalg, ops
+from . import annotations, io, linalg, ops
 from .annotations import PSD, Hermitian, SelfAdjoint, Stiefel, Unitary
 from .fns import add, densify, dot, kron, lazify, mul
 from .ops import (BlockDiag, Dense, DenseBF16, Diagonal, I_like, Identity, Kernel, Kronecker, LinearOperator, Product,

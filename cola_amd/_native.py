@@ -114,6 +114,11 @@ _PROTOS = {
     "ck_cg_finish": (ctypes.c_int, [_i64, _i32, _p, _p, _i64, _p, _p]),
     "ck_hessenberg_lstsq_workspace_bytes": (_sz, [_i32, _i32, _i32]),
     "ck_hessenberg_lstsq_f64": (ctypes.c_int, [_i32, _i32, _p, _i64, _i32, _i32, _p, _f64, _p, _sz, _p, _i64, _p]),
+    "ck_sym_eigh_f64": (ctypes.c_int, [_i32, _i32, _p, _i64, _i64, _p, _i64, _p, _i64, _i64, _i32, _p, _p]),
+    "ck_coo_to_csr_workspace_bytes": (_sz, [_i64, _i64]),
+    "ck_coo_to_csr_f32": (ctypes.c_int, [_i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _sz, _p]),
+    "ck_coo_to_csr_f64": (ctypes.c_int, [_i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _sz, _p]),
+    "ck_csr_apply_f64acc": (ctypes.c_int, [_i64, _p, _p, _p, _f64, _f64, _i32, _p, _i64, _p, _i64, _p]),
     "ck_cg_fused_workspace_bytes": (_sz, [_i64, _i32]),
     "ck_cg_fused_supported": (ctypes.c_int, [_op, _op, _i32]),
     "ck_cg_fused_noalloc": (ctypes.c_int, [_op, _op, _i32, _p, _i64, _i64, _p, _i64, _i64, _p, _i64, _i64, _p, _i64,

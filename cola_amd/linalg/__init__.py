@@ -15,6 +15,7 @@ from .learned import (LearnedPreconditioner, PivotedCholeskyPreconditioner, Poin
                       rbf_matrix, select_preconditioner, train)
 from .logdet import logdet, slogdet
 from .mixed import HipMixedCG, KroneckerEigInverse, mixed_cg
+from .preconditioning import NystromPrecond, NystromPrecondLazy, get_nys_approx
 from .slq import HipSLQ, stochastic_lanczos_quad
 from .trace import Exact, HipHutch, diag, hutch_trace_csr, hutchinson_diag_estimate, trace
 from .unary import LanczosUnary, apply_unary, exp, isqrt, log, pow, sqrt
@@ -32,4 +33,5 @@ __all__ = ["Algorithm", "Auto", "IterativeOperatorWInfo", "HipCG", "CG", "cg", "
            "HipMixedCG", "KroneckerEigInverse", "mixed_cg", "LanczosUnary", "apply_unary", "log", "exp", "sqrt", "isqrt", "pow", "PRNGKey", "next_key",
            "PointMLP", "LearnedPreconditioner", "PivotedCholeskyPreconditioner", "select_preconditioner",
            "make_training_instance", "train", "rbf_matrix",
-           "HipBatchedCG", "BatchedSystems", "BatchedMixedSolver", "BatchedPrecond", "autotune"]
+           "HipBatchedCG", "BatchedSystems", "BatchedMixedSolver", "BatchedPrecond", "autotune",
+           "NystromPrecond", "NystromPrecondLazy", "get_nys_approx"]

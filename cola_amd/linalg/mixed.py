@@ -98,7 +98,8 @@ class KroneckerEigInverse(LinearOperator):
 class _F64Apply:
     """FP64 application of A for refinement residuals (the matrix as stored):
     64^3 Kronecker operators through ck_kron_apply_f64, dense fp32 matrices (with
-    scale and shift) through ck_dense_apply_f64acc."""
+    scale and shift) through ck_dense_apply_f64acc, CSR matrices through
+    ck_csr_apply_f64acc."""
 
     def __init__(self, A, nrhs):
         ks = kron_structure(A)
@@ -114,10 +115,13 @@ class _F64Apply:
         elif got is not None and got[0].kind == N.CK_OP_DENSE_F32:
             self.kind = "dense"
             self.op, self.keep = got
+        elif got is not None and got[0].kind == N.CK_OP_CSR_F32:
+            self.kind = "csr"
+            self.op, self.keep = got
         else:
             raise N.UnsupportedOperator("mixed-precision refinement", N.CK_ERR_UNSUPPORTED,
-                                        "FP64 residuals need a dense fp32 matrix or a 64^3 Kronecker operator "
-                                        "(plus shift)")
+                                        "FP64 residuals need a dense or CSR fp32 matrix or a 64^3 Kronecker "
+                                        "operator (plus shift)")
 
     def residual(self, X, Bv, R32, norms, nrhs):
         """R32 = float(Bv - A X), norms[2c] = ||Bv_c - A X_c||, norms[2c+1] = ||Bv_c|| in one
@@ -135,6 +139,12 @@ class _F64Apply:
             op = self.op
             N.call("ck_dense_apply_f64acc", op.n, ctypes.c_void_p(op.a), op.lda, float(op.scale), float(op.shift),
                    nrhs, N.ptr(X), ld, N.ptr(Y), Y.stride(0), N.stream_ptr(X.device))
+            return
+        if self.kind == "csr":
+            op = self.op
+            N.call("ck_csr_apply_f64acc", op.n, ctypes.c_void_p(op.rowptr), ctypes.c_void_p(op.colind),
+                   ctypes.c_void_p(op.values), float(op.scale), float(op.shift), nrhs, N.ptr(X), ld, N.ptr(Y),
+                   Y.stride(0), N.stream_ptr(X.device))
             return
         dims = (ctypes_int32_array(self.dims))
         N.call("ck_kron_apply_f64", 3, dims, N.ptr(self.f64[0]), N.ptr(self.f64[1]), N.ptr(self.f64[2]),

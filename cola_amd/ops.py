@@ -272,10 +272,14 @@ class DenseBF16(LinearOperator):
 
 
 def coo_to_csr(data, rows, cols, shape):
-    """Device-side COO -> CSR: sort by (row, col), sum duplicates, int32 indices.
+    """COO -> CSR: rows in order, columns ascending inside each row, duplicates summed,
+    int32 indices -- the arrays the reference keeps after argsort + scipy's
+    ``coo_array(...).tocsr()`` (operators.py:64-75).
 
-    The reference does argsort + scipy ``coo_array(...).tocsr()`` on the host
-    (operators.py:64-75); the resulting arrays are the same."""
+    Device tensors go through the engine's kernels (``ck_coo_to_csr_f32``/``_f64``: row
+    counts, scan, scatter, per-row sort, duplicate merge); CPU tensors through torch."""
+    if data.is_cuda:
+        return _coo_to_csr_device(data, rows, cols, shape)
     dev = data.device
     rows = rows.to(device=dev, dtype=torch.int64)
     cols = cols.to(device=dev, dtype=torch.int64)
@@ -290,6 +294,34 @@ def coo_to_csr(data, rows, cols, shape):
     indptr = torch.zeros(int(shape[0]) + 1, dtype=torch.int64, device=dev)
     indptr[1:] = torch.cumsum(counts, 0)
     return indptr.to(torch.int32), c, summed
+
+
+def _coo_to_csr_device(data, rows, cols, shape):
+    fn = {torch.float32: "ck_coo_to_csr_f32", torch.float64: "ck_coo_to_csr_f64"}.get(data.dtype)
+    if fn is None:
+        raise TypeError(f"sparse values on the device must be float32 or float64, got {data.dtype}")
+    dev = data.device
+    n, m = int(shape[0]), int(shape[1])
+    data = data.contiguous()
+    rows = rows.to(device=dev, dtype=torch.int64).contiguous()
+    cols = cols.to(device=dev, dtype=torch.int64).contiguous()
+    if rows.numel() != data.numel() or cols.numel() != data.numel():
+        raise ValueError("data, row and column index arrays differ in length")
+    if n >= 2**31 or m >= 2**31 or data.numel() >= 2**31:
+        raise ValueError("CSR with int32 indices needs n, nnz < 2**31")
+    nnz = data.numel()
+    indptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    indices = torch.empty(max(nnz, 1), dtype=torch.int32, device=dev)
+    values = torch.empty(max(nnz, 1), dtype=data.dtype, device=dev)
+    info = torch.zeros(2, dtype=torch.int32, device=dev)  # [unique count, index status]
+    wb = N.q("ck_coo_to_csr_workspace_bytes", n, nnz)
+    work = torch.empty(max(wb, 1), dtype=torch.uint8, device=dev)
+    N.call(fn, n, m, nnz, N.ptr(data), N.ptr(rows), N.ptr(cols), N.ptr(indptr), N.ptr(indices), N.ptr(values),
+           N.ptr(info), N.ptr(info[1:]), N.ptr(work), wb, N.stream_ptr(dev))
+    k, status = info.tolist()
+    if status:
+        raise ValueError("sparse indices out of range for shape")
+    return indptr, indices[:k], values[:k]
 
 
 def _check_coo(rows, cols, shape):
@@ -320,7 +352,8 @@ class Sparse(LinearOperator):
     def __init__(self, data, row_indices, col_indices, shape):
         self.device = data.device
         super().__init__(data.dtype, shape)
-        _check_coo(row_indices, col_indices, shape)
+        if not data.is_cuda:  # the device kernels check the indices themselves
+            _check_coo(row_indices, col_indices, shape)
         self.indptr, self.indices, self.data = coo_to_csr(data, row_indices, col_indices, shape)
 
     @classmethod

@@ -433,6 +433,40 @@ int ck_hessenberg_lstsq_f64(int32_t nb, int32_t m, const float* H, int64_t ldh, 
                             const float* beta, double rtol, void* work, size_t work_bytes, double* y, int64_t ldy,
                             void* stream);
 
+/* Eigenpairs of nb small dense symmetric FP64 matrices (k <= 64), one workgroup each, by
+ * parallel cyclic Jacobi: w (descending, stride_w apart) and E (row j = unit eigenvector of
+ * w[j], leading dimension lde, stride_e apart).  Only the symmetric part of A is used.
+ * sweeps[b] (optional) = sweeps taken, or -1 if max_sweeps ended before convergence.
+ * Replaces the xnp.svd of the Nystrom factor (cola/linalg/preconditioning/
+ * preconditioners.py:150), whose squared singular values and right vectors are the
+ * eigenpairs of its Gram matrix. */
+int ck_sym_eigh_f64(int32_t nb, int32_t k, const double* A, int64_t lda, int64_t stride_a, double* w,
+                    int64_t stride_w, double* E, int64_t lde, int64_t stride_e, int32_t max_sweeps, int32_t* sweeps,
+                    void* stream);
+
+/* ---------------------------------------------------------------- sparse construction
+ * COO -> CSR on the device (replaces the argsort + scipy coo_array(...).tocsr() of
+ * Sparse.__init__, cola/ops/operators.py:64-75): rows in order, column indices ascending
+ * inside each row, duplicate (row, col) entries summed in input order.  rows/cols are
+ * int64 (nnz each); indptr (nrows + 1), indices and values (nnz each, an upper bound) are
+ * written on the device, with the unique count in *nnz_out (device int32, optional).
+ * *status (device int32) becomes 1 if any index lies outside [0, nrows) x [0, ncols);
+ * those entries are dropped.  nrows, ncols, nnz < 2^31.  The call is asynchronous on
+ * `stream`; work >= ck_coo_to_csr_workspace_bytes(nrows, nnz) (about 24 bytes/entry). */
+size_t ck_coo_to_csr_workspace_bytes(int64_t nrows, int64_t nnz);
+int ck_coo_to_csr_f32(int64_t nrows, int64_t ncols, int64_t nnz, const float* data, const int64_t* rows,
+                      const int64_t* cols, int32_t* indptr, int32_t* indices, float* values, int32_t* nnz_out,
+                      int32_t* status, void* work, size_t work_bytes, void* stream);
+int ck_coo_to_csr_f64(int64_t nrows, int64_t ncols, int64_t nnz, const double* data, const int64_t* rows,
+                      const int64_t* cols, int32_t* indptr, int32_t* indices, double* values, int32_t* nnz_out,
+                      int32_t* status, void* work, size_t work_bytes, void* stream);
+/* Y = scale A X + shift X with A a CSR matrix (int32 indices, fp32 values) and X, Y fp64
+ * column blocks, FP64 accumulation: refinement residuals of sparse systems (Sparse._matmat,
+ * cola/ops/operators.py:77-78, at higher precision). */
+int ck_csr_apply_f64acc(int64_t n, const int32_t* rowptr, const int32_t* colind, const float* values, double scale,
+                        double shift, int32_t nrhs, const double* X, int64_t ldx, double* Y, int64_t ldy,
+                        void* stream);
+
 /* ---------------------------------------------------------------- quadrature / trace
  * Stochastic Lanczos quadrature: per probe b, q_b = scale_b * sum_c tau_bc^2 f(theta_bc)
  * with f(theta) = 0 where |theta| <= 10*eps*max|theta| (cola/linalg/unary/unary.py:55-57,
