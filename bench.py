@@ -519,7 +519,7 @@ def bench_learned_cg(args, dev):
         Pc = L.PivotedCholeskyPreconditioner(K, 64, noise)
         P, rep = L.select_preconditioner(K, Xd, noise, candidates={"learned": Pl, "pivoted_cholesky": Pc}, A=A)
         torch.cuda.synchronize()
-        systems.append({"K": K, "A": A, "b": b, "shift": float(np.float32(noise)), "ell": ell, "noise": noise,
+        systems.append({"K": K, "X": Xd, "A": A, "b": b, "shift": float(np.float32(noise)), "ell": ell, "noise": noise,
                         "Pl": Pl, "Pc": Pc, "P": P, "choice": rep["choice"], "operator_s": t_op, "learned_build_s": t_pl,
                         "select_s": time.perf_counter() - ts})
 
@@ -569,10 +569,11 @@ def bench_learned_cg(args, dev):
     roof = {"bound": "hbm", "kernel": "dense_lds_kernel<8,4> (K + noise I, 8 rhs)", "achieved": mv_bytes / t_mv / 1e9,
             "peak": HBM_PEAK, "unit": "GB/s", "frac": mv_bytes / t_mv / 1e9 / HBM_PEAK,
             "traffic": DENSE8_TRAFFIC_BYTES, "launch_us": t_mv * 1e6, "algorithmic_bytes_per_launch": mv_bytes}
+    sketch = bench_sketch(systems[0], net, n, dev)
     out = {"metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": head["s"] * 1e3, "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None,
-           "dtype": "f32 (matvec, PCG) + f64 (refinement residuals, Woodbury core)",
+           "dtype": "f32 (matvec, PCG) + f64 (refinement residuals, Woodbury core) + bf16 MFMA (preconditioner sketch)",
            "data": "synthetic (seeded points, lengthscales, noise, N(0,1) rhs; network trained here on seeded instances)",
            "config": {"workload": f"learned_cg: {len(systems)} dense RBF systems, N={n} points in [0,1]^4, "
                                   f"lengthscale~LogU[0.05,1], noise~LogU[1e-4,1e-1], {B} rhs each, to FP64-checked "
@@ -589,7 +590,7 @@ def bench_learned_cg(args, dev):
                                    "energy_loss_before": loss0, "energy_loss_after": loss1,
                                    "last_step_loss": float(losses[-1]) if len(losses) else None},
                       "systems": per},
-           "roofline": roof}
+           "roofline": roof, "sketch_bf16": sketch}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline_refined(systems, B, args.tol, itol, head["per_system"])
     del systems, per
@@ -602,6 +603,42 @@ def bench_learned_cg(args, dev):
         out["kron_cg"] = {"value": kr["value"], "unit": kr["unit"], "ms_per_step": kr["ms_per_step"],
                           "config": kr["config"], "roofline": kr["roofline"]}
     return out
+
+
+def bench_sketch(sd, net, n, dev):
+    """The preconditioner's sketch product K G (n x n by n x 64) on BF16 MFMA (K as a bf16
+    pair, ck_sketch_bf16) against the FP32 operator apply on 64 columns, both timed with HIP
+    events, and the learned preconditioner's build time either way."""
+    from cola_amd._blocks import ld_of
+    from cola_amd.linalg import learned as L
+    k, ld = 64, ld_of(n)
+    K = sd["K"]
+    G = torch.randn(k, ld, device=dev)
+    G16 = torch.empty((k, ld), dtype=torch.int16, device=dev)
+    Gr = torch.zeros((k, ld), device=dev)
+    Y = torch.zeros((k, ld), device=dev)
+    s = N.stream_ptr(dev)
+    N.call("ck_round_bf16", n, k, N.ptr(G), ld, N.ptr(G16), ld, N.ptr(Gr), ld, s)
+    t16 = time_kernel(lambda: N.call("ck_sketch_bf16", n, k, N.ptr(K), K.stride(0), 1.0, 0.0, N.ptr(G16), ld, None, 0,
+                                     N.ptr(Y), ld, 1, s), reps=10)
+    op, keep = ca.Dense(K).hip_op()
+    wb = N.q("ck_op_workspace_bytes", op, k)
+    work = torch.empty(max(wb, 4) // 4, device=dev)
+    t32 = time_kernel(lambda: N.call("ck_op_apply", op, k, N.ptr(G), ld, N.ptr(Y), ld, N.ptr(work), wb, s), reps=5)
+    builds = {}
+    for name, flag in (("bf16_mfma", True), ("fp32", False)):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        L.LearnedPreconditioner(K, sd["X"], net, sd["noise"], bf16_sketch=flag)
+        torch.cuda.synchronize()
+        builds[name] = (time.perf_counter() - t) * 1e3
+    by = 4 * n * n + 2 * k * n + 4 * k * n
+    flops = 2 * 2 * n * n * k  # two bf16 products (hi, lo) per element
+    return {"kernel": "sketch_bf16_kernel<64, split> (K G, K as bf16 hi + lo, FP32 accumulation)",
+            "launch_us": t16 * 1e6, "achieved_GBps": by / t16 / 1e9, "frac_hbm": by / t16 / 1e9 / HBM_PEAK,
+            "mfma_TFLOPs": flops / t16 / 1e12, "frac_bf16_mfma": flops / t16 / 1e12 / 2500.0,
+            "fp32_apply_64cols_us": t32 * 1e6, "speedup_vs_fp32": t32 / t16,
+            "learned_build_ms": builds}
 
 
 def timed_refined(systems, precs, B, args, itol, reps):

@@ -19,6 +19,7 @@ No reference counterpart: the result is an ordinary preconditioner ``P`` for col
   that error exceeds the largest captured eigenvalue, i.e. when the bound
   kappa(P A) <= 1 + tr(E) / noise promises no gain over kappa(A).
 """
+import ctypes
 import math
 import time
 
@@ -353,6 +354,30 @@ def mean_loss(net, instances, ridge_rel=1e-6):
 
 
 # ----------------------------------------------------------------------------- preconditioners
+def dense_sketch_bf16(lowered, G, n, split=True):
+    """Y = A G on BF16 MFMA (``ck_sketch_bf16``) for a lowered dense fp32 operator A and a
+    feature-major sketch block G (k, ld) with k = 32 or 64; G is first rounded to bf16, A to
+    a bf16 pair hi + lo (split, the default) or to one bf16 value.
+    Returns (Y, G~) with G~ the rounded sketch as fp32 (the one the product used, so that a
+    Gram matrix G~^T Y stays consistent with Y), or None when the shapes fall outside the
+    kernel (then the caller's FP32 path runs)."""
+    if lowered is None or lowered[0].kind != N.CK_OP_DENSE_F32:
+        return None
+    op = lowered[0]
+    k, ld = G.shape
+    dev = G.device
+    G16 = torch.empty((k, ld), dtype=torch.int16, device=dev)
+    if k not in (32, 64) or not N.q("ck_sketch_bf16_supported", n, k, op.lda, ctypes.c_void_p(op.a), ld, N.ptr(G16)):
+        return None
+    Gr = torch.zeros((k, ld), dtype=torch.float32, device=dev)
+    Y = torch.zeros((k, ld), dtype=torch.float32, device=dev)
+    s = N.stream_ptr(dev)
+    N.call("ck_round_bf16", n, k, N.ptr(G), ld, N.ptr(G16), ld, N.ptr(Gr), ld, s)
+    N.call("ck_sketch_bf16", n, k, ctypes.c_void_p(op.a), op.lda, float(op.scale), float(op.shift), N.ptr(G16), ld,
+           N.ptr(Gr), ld, N.ptr(Y), ld, 1 if split else 0, s)
+    return Y, Gr
+
+
 def woodbury_core(F, noise):
     """(noise I + F^T F)^-1 (FP64) for a row-major factor F (n, k), with tr(F^T F) and
     max diag(F^T F), all formed on the engine."""
@@ -374,7 +399,7 @@ def _dense_matrix(K):
 class LearnedPreconditioner(WoodburyInverse):
     """P = (noise I + F F^T)^-1 with F the Nystrom completion of the network's sketch of K."""
 
-    def __init__(self, K, points, net, noise, jitter_rel=1e-6):
+    def __init__(self, K, points, net, noise, jitter_rel=1e-6, bf16_sketch=True):
         t0 = time.time()
         K = _dense_matrix(K)
         n = K.shape[0]
@@ -383,9 +408,16 @@ class LearnedPreconditioner(WoodburyInverse):
         ld = G.stride(0)
         Kop = Dense(K)
         aop = lower(Kop)
-        work, wb = workspace([aop[0]], k, K.device)
-        Y = torch.zeros_like(G)
-        apply_blocks(Kop, aop, G, Y, n, k, work, wb)
+        # the sketch product K G: one BF16 MFMA pass over K where the shapes allow (the
+        # rounding perturbs the preconditioner only), else the FP32 apply
+        got = dense_sketch_bf16(aop, G, n) if bf16_sketch else None
+        self.sketch = "bf16_mfma" if got is not None else "fp32"
+        if got is not None:
+            Y, G = got
+        else:
+            work, wb = workspace([aop[0]], k, K.device)
+            Y = torch.zeros_like(G)
+            apply_blocks(Kop, aop, G, Y, n, k, work, wb)
         C = gram_blocks(G, Y, n)
         fac = spd_inverse(C, jitter_rel=jitter_rel, linvT32=True)
         F = torch.empty((n, k), dtype=torch.float32, device=K.device)

@@ -23,7 +23,7 @@ from .._blocks import ld_of, to_blocks
 from ..ops import LinearOperator, _op
 from ._engine import apply_blocks, lower, workspace
 from .keys import PRNGKey
-from .learned import gram_blocks, rows_gemm, spd_inverse
+from .learned import dense_sketch_bf16, gram_blocks, rows_gemm, spd_inverse
 
 MAX_RANK = 64
 
@@ -59,8 +59,10 @@ def sym_eigh(S, max_sweeps=40):
     return w, E, sweeps
 
 
-def _nys_factor(A, Omega, eps):
-    """get_nys_approx on the engine; returns (Lambda FP64, U (n, rank) fp32, nu FP64)."""
+def _nys_factor(A, Omega, eps, bf16_sketch=False):
+    """get_nys_approx on the engine; returns (Lambda FP64, U (n, rank) fp32, nu FP64).
+    bf16_sketch: form A Omega on BF16 MFMA for a dense fp32 A (rank 32 or 64, n a multiple
+    of 256) from the bf16-rounded basis, which then serves as Omega throughout."""
     N.require_device(Omega, "sketch")
     n, r = Omega.shape
     if r > MAX_RANK:
@@ -69,9 +71,13 @@ def _nys_factor(A, Omega, eps):
     ld = ld_of(n)
     Qb = _cholqr_blocks(to_blocks(Omega.to(torch.float32)), n)
     lowered = lower(A)
-    work, wb = workspace([lowered[0]] if lowered is not None else [], r, dev)
-    Yb = torch.zeros_like(Qb)
-    apply_blocks(A, lowered, Qb, Yb, n, r, work, wb)
+    got = dense_sketch_bf16(lowered, Qb, n) if bf16_sketch else None
+    if got is not None:
+        Yb, Qb = got
+    else:
+        work, wb = workspace([lowered[0]] if lowered is not None else [], r, dev)
+        Yb = torch.zeros_like(Qb)
+        apply_blocks(A, lowered, Qb, Yb, n, r, work, wb)
     nu = eps * torch.sqrt(torch.diagonal(gram_blocks(Yb, Yb, n)).sum())
     Yb[:, :n] += nu.to(torch.float32) * Qb[:, :n]
     fac = spd_inverse(gram_blocks(Qb, Yb, n), linvT32=True)
@@ -135,11 +141,14 @@ class NystromPrecond(_NystromApply):
         adjust_mu (bool): scale mu by the largest captured eigenvalue.
         key (int): seed of the Gaussian sketch (default PRNGKey(42)), drawn as the
             reference's torch backend draws it (torch_fns.py:233-241).
+        bf16_sketch (bool): form A Omega on BF16 MFMA when A is dense fp32 (engine option;
+            the reference multiplies in the operator's dtype).
     """
 
-    def __init__(self, A, rank, mu=1e-7, eps=1e-8, adjust_mu=True, key=None):
+    def __init__(self, A, rank, mu=1e-7, eps=1e-8, adjust_mu=True, key=None, bf16_sketch=False):
         self.device = A.device
         super().__init__(A.dtype, A.shape)
+        self.bf16_sketch = bf16_sketch
         key = PRNGKey(42) if key is None else key
         g = torch.Generator(device=A.device).manual_seed(int(key))
         Omega = torch.randn(A.shape[0], rank, generator=g, dtype=torch.float32, device=A.device)
@@ -147,7 +156,7 @@ class NystromPrecond(_NystromApply):
 
     def _create_approx(self, A, Omega, mu, eps, adjust_mu):
         """preconditioners.py:111-120."""
-        Lam, U, _ = _nys_factor(A, Omega, eps)
+        Lam, U, _ = _nys_factor(A, Omega, eps, getattr(self, "bf16_sketch", False))
         self.Lambda = Lam.to(torch.float32)
         self.adjusted_mu = amu = mu * Lam.max() if adjust_mu else torch.tensor(float(mu), dtype=torch.float64,
                                                                                  device=Lam.device)

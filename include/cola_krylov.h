@@ -558,6 +558,24 @@ int ck_rbf_dense_f32(int64_t n, int32_t dim, const float* P, int64_t ldp, float 
 int ck_dense_apply_f64acc(int64_t n, const float* A, int64_t lda, double scale, double shift, int32_t nrhs,
                           const double* X, int64_t ldx, double* Y, int64_t ldy, void* stream);
 
+/* ---------------------------------------------------------------- BF16 MFMA sketch products
+ * Yt[c][i] = scale * sum_l A[i][l] G[l][c] + shift * Gs[c][i], c < k (k = 32 or 64): the
+ * sketch product K G that starts a low-rank preconditioner (the learned factor, a
+ * Nystrom sketch: cola/linalg/preconditioning/preconditioners.py:147).  A is fp32 (n x n,
+ * row-major, lda), read once and rounded to bf16 in registers; Gt is the sketch as k
+ * feature-major bf16 rows (ldg), Gs its fp32 copy for the shift term; Yt fp32 feature-major.
+ * v_mfma_f32_32x32x16_bf16 with FP32 accumulation; split = 1 rounds A to a bf16 pair
+ * A_hi + A_lo (16 significant bits, two MFMAs per tile), split = 0 to one bf16 value.
+ * ck_sketch_bf16_supported: n a multiple
+ * of 256, lda % 4 == 0, ldg % 8 == 0, 16-byte aligned A and Gt. */
+int ck_sketch_bf16_supported(int64_t n, int32_t k, int64_t lda, const void* A, int64_t ldg, const void* G);
+int ck_sketch_bf16(int64_t n, int32_t k, const float* A, int64_t lda, float scale, float shift, const uint16_t* Gt,
+                   int64_t ldg, const float* Gs, int64_t ldgs, float* Yt, int64_t ldy, int32_t split, void* stream);
+/* Round rows x n fp32 values (row stride ldx) to bf16, round to nearest even: the bits to Xb
+ * (ldb) and/or the rounded values back as fp32 to Xr (ldr); either output may be NULL. */
+int ck_round_bf16(int64_t n, int32_t rows, const float* X, int64_t ldx, uint16_t* Xb, int64_t ldb, float* Xr,
+                  int64_t ldr, void* stream);
+
 /* ---------------------------------------------------------------- batched uneven systems
  * CG on many independent dense SPD systems (K_s + shift_s I) x_s = b_s of uneven order,
  * one rhs each: the blocks of a block-diagonal operator.  Replaces the per-block loop
