@@ -380,17 +380,18 @@ __global__ __launch_bounds__(256) void step_final_kernel(int nrhs, const float* 
 template <bool V4>
 __global__ __launch_bounds__(256) void update_p_kernel(int64_t n, const float* __restrict__ z, float* __restrict__ p,
                                                        int64_t ld, void* state, int nrhs) {
-  CgView v = view(state, nrhs);
-  if (v.ctrl->done) return;
   const int b = blockIdx.y;
-  const float beta = v.beta[b];
   const float* zb = z + b * ld;
   float* pb = p + b * ld;
   if (V4) {
     const int64_t i = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
     if (i >= n) return;
+    // the vector loads go out before the loop flag and beta are read
     const float4 zv = *reinterpret_cast<const float4*>(zb + i);
     float4 pv = *reinterpret_cast<float4*>(pb + i);
+    CgView v = view(state, nrhs);
+    if (v.ctrl->done) return;
+    const float beta = v.beta[b];
     pv.x = fmaf(beta, pv.x, zv.x);
     pv.y = fmaf(beta, pv.y, zv.y);
     pv.z = fmaf(beta, pv.z, zv.z);
@@ -399,7 +400,9 @@ __global__ __launch_bounds__(256) void update_p_kernel(int64_t n, const float* _
   } else {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    pb[i] = fmaf(beta, pb[i], zb[i]);
+    CgView v = view(state, nrhs);
+    if (v.ctrl->done) return;
+    pb[i] = fmaf(v.beta[b], pb[i], zb[i]);
   }
 }
 
@@ -547,6 +550,263 @@ int ck_cg_step_p(int64_t n, int32_t nrhs, const float* r, const float* z, float*
 
 namespace {
 
+// ------------------------------------------------ PCG step with a Woodbury preconditioner
+// For A dense fp32 and P = CK_OP_WOODBURY_INV (rank <= 64, 3..8 rhs) a step runs as
+//   dense matvec (p.Ap partials in its epilogue) -> wbf_xr (alpha, x/r update, r.r and
+//   F^T r tile partials) -> wbf_core (u = C^-1 F^T r) -> wbf_z (z = scale (r - F u) +
+//   shift r, r.z partials) -> step_final -> update_p
+// instead of separate partial, F^T r, core, apply and r.z passes.  The Woodbury arithmetic
+// after the fp32 loads stays FP64 as in the unfused apply.
+struct WbFused {
+  float* pap;
+  float* rr;
+  float* rz;
+  double* ftr;
+  double* u;
+  int T;
+  int npap;
+};
+
+WbFused wb_fused_carve(void* work, int64_t n, int nrhs, int k) {
+  auto al = [](size_t v) { return (v + 255) / 256 * 256; };
+  WbFused w;
+  w.T = (int)cdiv(n, kWbFusedRows);
+  w.npap = dense_xy_nparts(n);
+  char* b = static_cast<char*>(work);
+  w.pap = reinterpret_cast<float*>(b);
+  b += al(4 * (size_t)nrhs * w.npap);
+  w.rr = reinterpret_cast<float*>(b);
+  b += al(4 * (size_t)nrhs * w.T);
+  w.rz = reinterpret_cast<float*>(b);
+  b += al(4 * (size_t)nrhs * w.T);
+  w.ftr = reinterpret_cast<double*>(b);
+  b += al(8 * (size_t)nrhs * k * w.T);
+  w.u = reinterpret_cast<double*>(b);
+  return w;
+}
+
+constexpr int kWbfRows = kWbFusedRows;  // 128: two rows per lane of a wave
+
+// one block per kWbfRows-row tile, all rhs; every global load of the block is issued before
+// the first wait (F rows, then p, Ap, x, r), so the tile costs about one memory latency
+__global__ __launch_bounds__(256) void wbf_xr_kernel(int64_t n, int nrhs, const float* __restrict__ p,
+                                                     const float* __restrict__ Ap, float* __restrict__ x,
+                                                     float* __restrict__ r, int64_t ld, const float* __restrict__ pap,
+                                                     int npap, float* __restrict__ rr_part,
+                                                     const float* __restrict__ F, int64_t lda, int k,
+                                                     double* __restrict__ ftr, int T, void* state) {
+  constexpr int kRowsPerWave = kWbfRows / 4;
+  constexpr int kPer = 8 * kWbfRows / 256;  // (rhs, row) pairs per thread at 8 rhs
+  __shared__ float alpha_s[8];
+  __shared__ float rs[8][kWbfRows];
+  __shared__ double red[4][64][9];
+  const int tile = blockIdx.x, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t i0 = (int64_t)tile * kWbfRows;
+  // F rows of this wave's quarter of the tile, column = lane.  Every load below reads a valid
+  // (clamped) address and its value is used as loaded -- no branch or select waits on it, so
+  // all of them are in flight together.  A clamped F value never counts: lanes >= k write no
+  // output, and rows >= n meet r = 0 in the product.
+  float fv[kRowsPerWave];
+  const int cl = lane < k ? lane : k - 1;
+#pragma unroll
+  for (int j = 0; j < kRowsPerWave; ++j) {
+    const int64_t row = i0 + wave * kRowsPerWave + j;
+    fv[j] = F[(row < n ? row : n - 1) * lda + cl];
+  }
+  float pv[kPer], av[kPer], xv[kPer], rv[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int e = threadIdx.x + 256 * j, b = e / kWbfRows, i = e % kWbfRows;
+    const int64_t row = i0 + i;
+    const int64_t o = (int64_t)(b < nrhs ? b : nrhs - 1) * ld + (row < n ? row : n - 1);
+    pv[j] = p[o];
+    av[j] = Ap[o];
+    xv[j] = x[o];
+    rv[j] = r[o];
+  }
+  // the loop flag is read after the loads are issued (they read valid memory either way)
+  CgView v = view(state, nrhs);
+  if (v.ctrl->done) return;
+  for (int b = wave; b < nrhs; b += 4) {
+    const double pap_b = column_total(pap, npap, b);
+    if (lane == 0) {
+      double a = safe_div((double)v.gamma[b], pap_b);
+      const bool frozen = (double)v.rnorm[b] < kSmall;  // has_converged (cg.py:143-144,161)
+      if (frozen) a = 0.0;
+      alpha_s[b] = (float)a;
+      if (tile == 0) {
+        v.alpha[b] = (float)a;
+        if (!frozen && !(pap_b > 0.0) && v.ctrl->status == CK_OK) v.ctrl->status = CK_ERR_BREAKDOWN;
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int e = threadIdx.x + 256 * j, b = e / kWbfRows, i = e % kWbfRows;
+    const int64_t row = i0 + i;
+    float rn = 0.f;  // rows >= n and rhs >= nrhs stay zero in the product below
+    if (b < nrhs && row < n) {
+      const float alpha = alpha_s[b];
+      const int64_t o = (int64_t)b * ld + row;
+      x[o] = fmaf(alpha, pv[j], xv[j]);
+      rn = fmaf(-alpha, av[j], rv[j]);
+      r[o] = rn;
+    }
+    rs[b][i] = rn;
+  }
+  __syncthreads();
+  // r.r partials: wave w sums rhs w and w + 4 over the tile
+  for (int b = wave; b < nrhs; b += 4) {
+    const float a0 = rs[b][lane], a1 = rs[b][lane + 64];
+    const float t = wave_sum(fmaf(a0, a0, a1 * a1));
+    if (lane == 0) rr_part[(int64_t)b * T + tile] = t;
+  }
+  // F^T r over the tile: column c = lane, rows split over the four waves, FP64
+  double acc[8];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) acc[b] = 0.0;
+#pragma unroll
+  for (int j = 0; j < kRowsPerWave; ++j) {
+    const double f = (double)fv[j];
+    const int i = wave * kRowsPerWave + j;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) acc[b] = fma(f, (double)rs[b][i], acc[b]);
+  }
+#pragma unroll
+  for (int b = 0; b < 8; ++b) red[wave][lane][b] = acc[b];
+  __syncthreads();
+  for (int o = threadIdx.x; o < nrhs * 64; o += 256) {
+    const int b = o / 64, c = o % 64;
+    if (c >= k) continue;
+    const double t = (red[0][c][b] + red[1][c][b]) + (red[2][c][b] + red[3][c][b]);
+    ftr[((int64_t)b * k + c) * T + tile] = t;
+  }
+}
+
+// u_b = C^-1 sum_tiles F^T r (tiles in index order within four fixed slices), one block per rhs
+__global__ __launch_bounds__(256) void wbf_core_kernel(int k, int nrhs, const double* __restrict__ ftr, int T,
+                                                       const double* __restrict__ Cinv, double* __restrict__ u,
+                                                       void* state) {
+  CgView v = view(state, nrhs);
+  if (v.ctrl->done) return;
+  __shared__ double red[4][64];
+  __shared__ double ts[64];
+  const int b = blockIdx.x, c = threadIdx.x & 63, sl = threadIdx.x >> 6;
+  if (c < k) {
+    const int per = (T + 3) / 4;
+    const int t0 = sl * per, t1 = t0 + per < T ? t0 + per : T;
+    const double* pc = ftr + ((int64_t)b * k + c) * T;
+    double acc = 0.0;
+    int t = t0;
+    for (; t + 4 <= t1; t += 4) {
+      const double a0 = pc[t], a1 = pc[t + 1], a2 = pc[t + 2], a3 = pc[t + 3];
+      acc += a0;
+      acc += a1;
+      acc += a2;
+      acc += a3;
+    }
+    for (; t < t1; ++t) acc += pc[t];
+    red[sl][c] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x < k)
+    ts[threadIdx.x] = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+  __syncthreads();
+  if (threadIdx.x < k) {
+    const int cc = threadIdx.x;
+    double acc = 0.0;
+    for (int j = 0; j < k; ++j) acc = fma(Cinv[cc * k + j], ts[j], acc);
+    u[(int64_t)b * k + cc] = acc;
+  }
+}
+
+// z = scale (r - F u) + shift r over the tile (F tile staged in LDS), r.z partials; the F
+// tile (float4 loads when rank and row stride allow) and the r values are loaded up front
+__global__ __launch_bounds__(256) void wbf_z_kernel(int64_t n, int nrhs, const float* __restrict__ r,
+                                                    float* __restrict__ z, int64_t ld, const float* __restrict__ F,
+                                                    int64_t lda, int k, const double* __restrict__ u, double scale,
+                                                    double shift, float* __restrict__ rz_part, int T, void* state) {
+  constexpr int kPer = 8 * kWbfRows / 256;
+  __shared__ float Fs[kWbfRows * 65];
+  __shared__ double us[8][64];
+  __shared__ float prod[8][kWbfRows];
+  const int tile = blockIdx.x, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t i0 = (int64_t)tile * kWbfRows;
+  float rv[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {  // clamped addresses: no branch around the loads
+    const int e = threadIdx.x + 256 * j, b = e / kWbfRows, i = e % kWbfRows;
+    const int64_t row = i0 + i;
+    rv[j] = r[(int64_t)(b < nrhs ? b : nrhs - 1) * ld + (row < n ? row : n - 1)];
+  }
+  const bool v4 = (k % 4 == 0) && (lda % 4 == 0) && ((reinterpret_cast<uintptr_t>(F) & 15) == 0);
+  if (v4) {
+    const int k4 = k / 4;
+    constexpr int kMaxF4 = kWbfRows * 16 / 256;  // float4 per thread at rank 64
+    float4 f4[kMaxF4];
+#pragma unroll
+    for (int j = 0; j < kMaxF4; ++j) {
+      const int e = threadIdx.x + 256 * j, i = e / k4, c4 = e % k4;
+      // clamped address, value used as loaded (rows >= n and entries past the tile are never
+      // read back), so the loads stay in flight together
+      int64_t row = i0 + (i < kWbfRows ? i : kWbfRows - 1);
+      row = row < n ? row : n - 1;
+      f4[j] = *reinterpret_cast<const float4*>(F + row * lda + 4 * c4);
+    }
+#pragma unroll
+    for (int j = 0; j < kMaxF4; ++j) {
+      const int e = threadIdx.x + 256 * j, i = e / k4, c4 = e % k4;
+      if (i < kWbfRows) {
+        float* d = Fs + i * 65 + 4 * c4;
+        d[0] = f4[j].x;
+        d[1] = f4[j].y;
+        d[2] = f4[j].z;
+        d[3] = f4[j].w;
+      }
+    }
+  } else {
+    for (int e = threadIdx.x; e < kWbfRows * k; e += 256) {
+      const int i = e / k, c = e % k;
+      const int64_t row = i0 + i;
+      Fs[i * 65 + c] = row < n ? F[row * lda + c] : 0.f;
+    }
+  }
+  for (int e = threadIdx.x; e < nrhs * k; e += 256) us[e / k][e % k] = u[e];
+  CgView v = view(state, nrhs);
+  if (v.ctrl->done) return;  // uniform across the block: no thread is left at the barrier
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int e = threadIdx.x + 256 * j, b = e / kWbfRows, i = e % kWbfRows;
+    if (b >= nrhs) continue;
+    const int64_t row = i0 + i;
+    float pr = 0.f;
+    if (row < n) {
+      double acc = 0.0;
+      const float* fr = Fs + i * 65;
+      for (int c = 0; c < k; ++c) acc = fma((double)fr[c], us[b][c], acc);
+      const double xd = (double)rv[j];
+      const float zv = (float)(scale * (xd - acc) + shift * xd);
+      z[(int64_t)b * ld + row] = zv;
+      pr = rv[j] * zv;
+    }
+    prod[b][i] = pr;
+  }
+  __syncthreads();
+  for (int b = wave; b < nrhs; b += 4) {
+    const float t = wave_sum(prod[b][lane] + prod[b][lane + 64]);
+    if (lane == 0) rz_part[(int64_t)b * T + tile] = t;
+  }
+}
+
+bool wb_fused_ok(const ck_op* A, const ck_op* P, int nrhs, const float* p, int64_t ld, size_t work_bytes) {
+  auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
+  return P && A->kind == CK_OP_DENSE_F32 && P->kind == CK_OP_WOODBURY_INV && nrhs >= 3 && nrhs <= 8 &&
+         P->rank >= 1 && P->rank <= 64 && P->lda >= P->rank && P->a && P->aux && A->n % 4 == 0 && A->lda % 4 == 0 &&
+         ld % 4 == 0 && al16(A->a) && al16(p) && work_bytes >= wb_fused_bytes(A->n, nrhs, P->rank);
+}
+
 // nsteps steps; with last_partial the final one stops after the x/r update
 int cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r, float* p, float* Ap, float* z,
                int64_t ld, void* state, float* partials, void* work, size_t work_bytes, int32_t nsteps,
@@ -566,9 +826,48 @@ int cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r,
   // slab: the same chunking as the generic partial sums when n = 64^3 (P = 64)
   const bool fuse_eig = P && P->kind == CK_OP_KRON_EIG_F32 && q.P == 64 && q.per == 4096 &&
                         kron64_eligible(P, r, ld, z, ld);
+  // dense A with a Woodbury P: the fused step (dense_apply_xy, wbf_*) in P's workspace
+  const bool fuse_wb = !fuse_kron && wb_fused_ok(A, P, nrhs, p, ld, work_bytes);
+  WbFused W{};
+  if (fuse_wb) W = wb_fused_carve(work, n, nrhs, P->rank);
   for (int it = 0; it < nsteps; ++it) {
     int st;
     const bool last = last_partial && it + 1 == nsteps;
+    if (fuse_wb) {
+      const float* F = static_cast<const float*>(P->a);
+      const int k = P->rank;
+      st = dense_apply_xy(A, nrhs, p, ld, Ap, ld, done, W.pap, s);
+      if (st) return st;
+      if (last && x64) {
+        dim3 grid(q.P, nrhs);
+        if (use_v4(n, ld, p, x, nullptr, nullptr) && ldx64 % 4 == 0 && (reinterpret_cast<uintptr_t>(x64) & 31) == 0)
+          hipLaunchKernelGGL(update_x64_kernel<true>, grid, dim3(256), 0, s, n, p, x, ld, x64, ldx64,
+                             (const float*)W.pap, W.npap, q.per, state, nrhs);
+        else
+          hipLaunchKernelGGL(update_x64_kernel<false>, grid, dim3(256), 0, s, n, p, x, ld, x64, ldx64,
+                             (const float*)W.pap, W.npap, q.per, state, nrhs);
+        CK_CHECK_LAUNCH();
+        break;
+      }
+      hipLaunchKernelGGL(wbf_xr_kernel, dim3(W.T), dim3(256), 0, s, n, (int)nrhs, (const float*)p, (const float*)Ap, x,
+                         r, ld, (const float*)W.pap, W.npap, W.rr, F, P->lda, k, W.ftr, W.T, state);
+      CK_CHECK_LAUNCH();
+      if (last) break;
+      hipLaunchKernelGGL(wbf_core_kernel, dim3(nrhs), dim3(256), 0, s, k, (int)nrhs, (const double*)W.ftr, W.T,
+                         reinterpret_cast<const double*>(P->aux), W.u, state);
+      hipLaunchKernelGGL(wbf_z_kernel, dim3(W.T), dim3(256), 0, s, n, (int)nrhs, (const float*)r, z, ld, F, P->lda, k,
+                         (const double*)W.u, (double)P->scale, (double)P->shift, W.rz, W.T, state);
+      hipLaunchKernelGGL(step_final_kernel, dim3(1), dim3(256), 0, s, (int)nrhs, (const float*)W.rr,
+                         (const float*)W.rz, W.T, 1, state);
+      if (use_v4(n, ld, z, p, nullptr, nullptr))
+        hipLaunchKernelGGL(update_p_kernel<true>, dim3((unsigned)cdiv(n / 4, 256), nrhs), dim3(256), 0, s, n, z, p,
+                           ld, state, nrhs);
+      else
+        hipLaunchKernelGGL(update_p_kernel<false>, dim3((unsigned)cdiv(n, 256), nrhs), dim3(256), 0, s, n, z, p, ld,
+                           state, nrhs);
+      CK_CHECK_LAUNCH();
+      continue;
+    }
     int np0 = q.P;
     if (fuse_kron) {
       // p.Ap partials come out of the Kronecker epilogue

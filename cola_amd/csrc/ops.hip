@@ -176,7 +176,8 @@ __global__ __launch_bounds__(64 * kDenseWaves) void dense_lds_kernel(int n, cons
                                                                      float* __restrict__ Y, int64_t ldy, int nbr,
                                                                      float scale, float shift,
                                                                      const int* __restrict__ done, int64_t astride,
-                                                                     int64_t xbs, int64_t ybs) {
+                                                                     int64_t xbs, int64_t ybs,
+                                                                     float* __restrict__ xy_part = nullptr) {
   if (done && *done) return;
   {
     const int64_t pb = blockIdx.y;
@@ -272,17 +273,37 @@ __global__ __launch_bounds__(64 * kDenseWaves) void dense_lds_kernel(int n, cons
       }
     }
   }
+  // X . Y partials: lane L < ROWS * NB owns (row0 + L / NB, rhs L % NB) and loads its X value
+  // once; the wave-wide sums give it Y
+  const int my_r = lane / NB, my_b = lane % NB;
+  const bool mine = xy_part && lane < ROWS * NB && my_b < nbr && row0 + my_r < n;
+  const float my_x = mine ? X[(int64_t)my_b * ldx + row0 + my_r] : 0.f;
+  float my_y = 0.f;
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
       const float sum = wave_sum(acc[r][b]);
       const int row = row0 + r;
+      float y = scale * sum;
       if (lane == 0 && b < nbr && row < n) {
-        float y = scale * sum;
         if (shift != 0.f) y = fmaf(shift, X[(int64_t)b * ldx + row], y);
         Y[(int64_t)b * ldy + row] = y;
       }
+      if (lane == r * NB + b) my_y = shift != 0.f ? fmaf(shift, my_x, scale * sum) : y;
+    }
+  }
+  if (xy_part) {
+    // per-block partial sums of X . Y (the CG step's p.Ap), rows then waves in index order
+    __shared__ float xys[kDenseWaves][ROWS * NB];
+    if (lane < ROWS * NB) xys[wave][lane] = mine ? my_x * my_y : 0.f;
+    __syncthreads();
+    if (threadIdx.x < nbr) {
+      float t = 0.f;
+      for (int w = 0; w < kDenseWaves; ++w)
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) t += xys[w][r * NB + threadIdx.x];
+      xy_part[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
     }
   }
 }
@@ -616,6 +637,27 @@ __global__ __launch_bounds__(256) void kron_pass_b(int n1, int plane, const floa
 
 // ============================================================== internal API
 namespace ck {
+
+// Y = op(X) for a dense fp32 operator with 3..8 rhs on the LDS-staged kernel, writing the
+// per-block partial sums of X . Y to xy_part (nrhs x dense_xy_nparts(n), column-major by
+// rhs); returns CK_ERR_UNSUPPORTED when the shapes fall outside that kernel
+int dense_apply_xy(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, const int* done,
+                   float* xy_part, hipStream_t s) {
+  auto aligned16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
+  const int n = op->n;
+  if (op->kind != CK_OP_DENSE_F32 || nrhs < 3 || nrhs > 8 || n % 4 || op->lda % 4 || ldx % 4 || !aligned16(op->a) ||
+      !aligned16(X))
+    return CK_ERR_UNSUPPORTED;
+  const int grid = dense_xy_nparts(n);
+  if (nrhs <= 4)
+    hipLaunchKernelGGL((dense_lds_kernel<4, 4, false>), dim3(grid), dim3(64 * kDenseWaves), 0, s, n, op->a, op->lda, X,
+                       ldx, Y, ldy, nrhs, op->scale, op->shift, done, (int64_t)0, (int64_t)0, (int64_t)0, xy_part);
+  else
+    hipLaunchKernelGGL((dense_lds_kernel<8, 4, false>), dim3(grid), dim3(64 * kDenseWaves), 0, s, n, op->a, op->lda, X,
+                       ldx, Y, ldy, nrhs, op->scale, op->shift, done, (int64_t)0, (int64_t)0, (int64_t)0, xy_part);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
 
 size_t kron_workspace_bytes(const ck_op* op, int nrhs) {
   if (op->nfactors == 3) return (size_t)nrhs * op->n * sizeof(float);

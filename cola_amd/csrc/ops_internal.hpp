@@ -26,9 +26,19 @@ int kron64_apply_f32(const ck_op* op, int nrhs, const float* X, int64_t ldx, flo
 int kron64_eig_apply_f32(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,
                          size_t work_bytes, const int* done, hipStream_t s, float* rz_partials = nullptr);
 
+// Dense fp32 apply (3..8 rhs) that also writes per-block partial sums of X . Y:
+// dense_xy_nparts(n) partials per rhs; CK_ERR_UNSUPPORTED outside that kernel's shapes.
+inline int dense_xy_nparts(int64_t n) { return (int)((n + 31) / 32); }
+int dense_apply_xy(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, const int* done,
+                   float* xy_part, hipStream_t s);
+
 int woodbury_apply(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,
                    size_t work_bytes, const int* done, hipStream_t s);
 size_t woodbury_workspace_bytes(const ck_op* op, int nrhs);
+// Scratch of CG's fused Woodbury step (rows per tile kWbFusedRows): p.Ap, r.r and r.z
+// partials, per-tile F^T r partials and the core products u.
+constexpr int kWbFusedRows = 128;
+size_t wb_fused_bytes(int64_t n, int nrhs, int k);
 int rbf_apply(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, const int* done,
               hipStream_t s);
 

@@ -30,13 +30,25 @@ __host__ __device__ inline int wb_tile_rows(int k) {
   return r;
 }
 
-// stage rows [i0, i0 + tr) of F (row-major, lda) into Fs[row * (k+1) + c]; zero past n
+// stage rows [i0, i0 + tr) of F (row-major, lda) into Fs[row * (k+1) + c].  Rows past n are
+// loaded from row n - 1 (a valid address) and never count: the F^T x pass meets x = 0 there
+// and the update pass skips them.  Eight loads per thread go out before the first LDS store.
 __device__ __forceinline__ void wb_stage(const float* __restrict__ F, int64_t lda, int64_t n, int k, int64_t i0,
                                          int tr, float* Fs) {
-  for (int e = threadIdx.x; e < tr * k; e += 256) {
-    const int r = e / k, c = e % k;
-    const int64_t i = i0 + r;
-    Fs[r * (k + 1) + c] = i < n ? F[i * lda + c] : 0.f;
+  const int total = tr * k;
+  for (int e0 = threadIdx.x; e0 < total; e0 += 8 * 256) {
+    float v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = e0 + 256 * j < total ? e0 + 256 * j : total - 1;
+      const int64_t i = i0 + e / k;
+      v[j] = F[(i < n ? i : n - 1) * lda + e % k];
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int e = e0 + 256 * j;
+      if (e < total) Fs[(e / k) * (k + 1) + e % k] = v[j];
+    }
   }
 }
 
@@ -196,10 +208,23 @@ __global__ __launch_bounds__(256) void rbf_kernel(int n, int dim, const float* _
 
 namespace ck {
 
+size_t wb_fused_bytes(int64_t n, int nrhs, int k) {
+  auto al = [](size_t v) { return (v + 255) / 256 * 256; };
+  const size_t T = cdiv(n, kWbFusedRows), np = dense_xy_nparts(n);
+  return al(4 * (size_t)nrhs * np) + 2 * al(4 * (size_t)nrhs * T) + al(8 * (size_t)nrhs * k * T) +
+         al(8 * (size_t)nrhs * k);
+}
+
 size_t woodbury_workspace_bytes(const ck_op* op, int nrhs) {
   const int k = op->rank < 1 ? 1 : op->rank;
   const size_t P = cdiv(op->n, wb_tile_rows(k));
-  return sizeof(double) * ((size_t)nrhs * k * P + (size_t)nrhs * k);
+  size_t b = sizeof(double) * ((size_t)nrhs * k * P + (size_t)nrhs * k);
+  // room for the PCG step's fused Woodbury passes (cg.hip) as well
+  if (nrhs >= 3 && nrhs <= 8 && k <= 64) {
+    const size_t f = wb_fused_bytes(op->n, nrhs, k);
+    if (f > b) b = f;
+  }
+  return b;
 }
 
 int woodbury_apply(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,

@@ -15,8 +15,8 @@
 // ||E|| 256 times smaller at the same single pass over K.
 //
 // Work split: a workgroup owns 32 rows of A and all k output columns; its four waves take
-// quarters of the reduction index, each issuing 4 MFMA k-steps (64 indices) per round with
-// all loads of the round in flight; the four partial tiles are summed through LDS.
+// quarters of the reduction index, each issuing 4 MFMA k-steps (64 indices) per round while
+// the next round's loads are in flight; the four partial tiles are summed through LDS.
 // Operand layout of v_mfma_f32_32x32x16_bf16: lane l holds A[l % 32][8 (l / 32) + 0..7]
 // and B[8 (l / 32) + 0..7][l % 32]; accumulator v holds D[8 (v / 4) + 4 (l / 32) + v % 4][l % 32].
 #include "common.hpp"
@@ -105,17 +105,29 @@ __global__ __launch_bounds__(64 * kWaves) void sketch_bf16_kernel(int n, const f
   for (int t = 0; t < NT; ++t)
 #pragma unroll
     for (int v = 0; v < 16; ++v) acc[t][v] = 0.f;
-  for (int l0 = l_begin; l0 < l_end; l0 += 16 * kUnroll) {
-    float4 a[kUnroll][2];
-    uint4 g[kUnroll][NT];
+  // software pipeline: the next round's loads are in flight while this round's MFMAs run
+  float4 a[kUnroll][2], an[kUnroll][2];
+  uint4 g[kUnroll][NT], gn[kUnroll][NT];
+  auto load = [&](int l0, float4 (&aa)[kUnroll][2], uint4 (&gg)[kUnroll][NT]) {
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       const float4* ap = reinterpret_cast<const float4*>(Ar + l0 + 16 * u);
-      a[u][0] = ap[0];
-      a[u][1] = ap[1];
+      aa[u][0] = ap[0];
+      aa[u][1] = ap[1];
 #pragma unroll
-      for (int t = 0; t < NT; ++t) g[u][t] = *reinterpret_cast<const uint4*>(Gc[t] + l0 + 16 * u);
+      for (int t = 0; t < NT; ++t) gg[u][t] = *reinterpret_cast<const uint4*>(Gc[t] + l0 + 16 * u);
     }
+  };
+  load(l_begin, an, gn);
+  for (int l0 = l_begin; l0 < l_end; l0 += 16 * kUnroll) {
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      a[u][0] = an[u][0];
+      a[u][1] = an[u][1];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) g[u][t] = gn[u][t];
+    }
+    if (l0 + 16 * kUnroll < l_end) load(l0 + 16 * kUnroll, an, gn);
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       const bf16x8 av = pack8(a[u][0], a[u][1]);
