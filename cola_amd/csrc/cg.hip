@@ -828,6 +828,9 @@ int cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r,
                         kron64_eligible(P, r, ld, z, ld);
   // dense A with a Woodbury P: the fused step (dense_apply_xy, wbf_*) in P's workspace
   const bool fuse_wb = !fuse_kron && wb_fused_ok(A, P, nrhs, p, ld, work_bytes);
+  const bool dense_xy = !fuse_kron && !fuse_wb && A->kind == CK_OP_DENSE_F32 && nrhs >= 3 && nrhs <= 8 && work &&
+                        work_bytes >= dense_xy_bytes(n, nrhs) && n % 4 == 0 && A->lda % 4 == 0 && ld % 4 == 0 &&
+                        (reinterpret_cast<uintptr_t>(A->a) & 15) == 0 && (reinterpret_cast<uintptr_t>(p) & 15) == 0;
   WbFused W{};
   if (fuse_wb) W = wb_fused_carve(work, n, nrhs, P->rank);
   for (int it = 0; it < nsteps; ++it) {
@@ -869,10 +872,18 @@ int cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r,
       continue;
     }
     int np0 = q.P;
+    const float* part0 = q.p0;
     if (fuse_kron) {
       // p.Ap partials come out of the Kronecker epilogue
       np0 = kron_nparts(A);
       st = kron_dispatch(A, nrhs, p, ld, Ap, ld, work, work_bytes, done, q.p0, np0, s);
+      if (st) return st;
+    } else if (dense_xy) {
+      // ... or out of the dense matvec's epilogue, into the head of the workspace (P's apply,
+      // which reuses it, runs after the x/r update has read them)
+      np0 = dense_xy_nparts(n);
+      part0 = static_cast<const float*>(work);
+      st = dense_apply_xy(A, nrhs, p, ld, Ap, ld, done, static_cast<float*>(work), s);
       if (st) return st;
     } else {
       st = op_apply_internal(A, nrhs, p, ld, Ap, ld, work, work_bytes, done, s);
@@ -883,15 +894,15 @@ int cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r,
     if (last && x64) {
       dim3 grid(q.P, nrhs);
       if (use_v4(n, ld, p, x, nullptr, nullptr) && ldx64 % 4 == 0 && (reinterpret_cast<uintptr_t>(x64) & 31) == 0)
-        hipLaunchKernelGGL(update_x64_kernel<true>, grid, dim3(256), 0, s, n, p, x, ld, x64, ldx64, q.p0, np0, q.per,
+        hipLaunchKernelGGL(update_x64_kernel<true>, grid, dim3(256), 0, s, n, p, x, ld, x64, ldx64, part0, np0, q.per,
                            state, nrhs);
       else
-        hipLaunchKernelGGL(update_x64_kernel<false>, grid, dim3(256), 0, s, n, p, x, ld, x64, ldx64, q.p0, np0, q.per,
-                           state, nrhs);
+        hipLaunchKernelGGL(update_x64_kernel<false>, grid, dim3(256), 0, s, n, p, x, ld, x64, ldx64, part0, np0,
+                           q.per, state, nrhs);
       CK_CHECK_LAUNCH();
       break;
     }
-    st = step_xr(n, nrhs, p, Ap, x, r, ld, state, q, q.p0, np0, s);
+    st = step_xr(n, nrhs, p, Ap, x, r, ld, state, q, part0, np0, s);
     if (st) return st;
     if (last) break;
     if (fuse_eig) {

@@ -759,6 +759,9 @@ size_t op_workspace_internal(const ck_op* op, int nrhs) {
       return woodbury_workspace_bytes(op, nrhs);
     case CK_OP_KRON_EIG_F32:
       return 2 * (size_t)nrhs * op->n * sizeof(float);
+    case CK_OP_DENSE_F32:
+      // room for the p.Ap partials a CG step takes from the matvec epilogue (dense_apply_xy)
+      return (nrhs >= 3 && nrhs <= 8) ? dense_xy_bytes(op->n, nrhs) : 0;
     default:
       return 0;
   }

@@ -29,6 +29,7 @@ int kron64_eig_apply_f32(const ck_op* op, int nrhs, const float* X, int64_t ldx,
 // Dense fp32 apply (3..8 rhs) that also writes per-block partial sums of X . Y:
 // dense_xy_nparts(n) partials per rhs; CK_ERR_UNSUPPORTED outside that kernel's shapes.
 inline int dense_xy_nparts(int64_t n) { return (int)((n + 31) / 32); }
+inline size_t dense_xy_bytes(int64_t n, int nrhs) { return ((size_t)4 * nrhs * dense_xy_nparts(n) + 255) / 256 * 256; }
 int dense_apply_xy(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, const int* done,
                    float* xy_part, hipStream_t s);
 

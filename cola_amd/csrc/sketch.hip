@@ -5,7 +5,8 @@
 // cola/linalg/preconditioning/preconditioners.py:147 for the latter).  At k = 64 the
 // product carries 64 multiply-adds per matrix element: FP32 vector code is compute bound
 // there (eight 8-column launches re-stream K), while v_mfma_f32_32x32x16_bf16 keeps one
-// pass over K at the HBM rate.  K is read as fp32 and rounded to bf16 in registers; G
+// pass over K at the HBM rate.  K is read as fp32 and rounded to bf16 in registers
+// (v_cvt_pk_bf16_f32); G
 // arrives already rounded (ck_round_bf16).  Rounding the sketch is free: with G~ the
 // rounded sketch, Y (G~^T Y)^-1 Y^T is an exact Nystrom approximation for the sketch G~.
 // Rounding K is not: the preconditioner then approximates K~ = K + E, and the preconditioned
@@ -40,23 +41,16 @@ __device__ __forceinline__ uint32_t bf16_bits(float f) {
   return (u + 0x7FFFu + ((u >> 16) & 1u)) >> 16;
 }
 
-__device__ __forceinline__ bf16x8 pack8(const float4 a, const float4 b) {
-  union {
-    uint32_t w[4];
-    bf16x8 v;
-  } u;
-  u.w[0] = bf16_bits(a.x) | (bf16_bits(a.y) << 16);
-  u.w[1] = bf16_bits(a.z) | (bf16_bits(a.w) << 16);
-  u.w[2] = bf16_bits(b.x) | (bf16_bits(b.y) << 16);
-  u.w[3] = bf16_bits(b.z) | (bf16_bits(b.w) << 16);
-  return u.v;
-}
-
-// the bf16 rounding error of each value, itself rounded to bf16: hi + lo carries 16
-// significant bits of the fp32 input
-__device__ __forceinline__ bf16x8 pack8_lo(const float4 a, const float4 b) {
-  auto lo = [](float x) { return __uint_as_float(bf16_bits(x - __uint_as_float(bf16_bits(x) << 16)) << 16); };
-  return pack8(make_float4(lo(a.x), lo(a.y), lo(a.z), lo(a.w)), make_float4(lo(b.x), lo(b.y), lo(b.z), lo(b.w)));
+// eight fp32 values as bf16 (v_cvt_pk_bf16_f32: round to nearest even), and optionally the
+// rounding errors, themselves rounded to bf16: hi + lo carries 16 significant bits
+__device__ __forceinline__ void split8(const float4 a, const float4 b, bf16x8& hi, bf16x8* lo) {
+  const float x[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const __bf16 h = (__bf16)x[i];
+    hi[i] = h;
+    if (lo) (*lo)[i] = (__bf16)(x[i] - (float)h);
+  }
 }
 
 __device__ __forceinline__ bf16x8 as_bf16x8(const uint4 q) {
@@ -118,8 +112,16 @@ __global__ __launch_bounds__(64 * kWaves) void sketch_bf16_kernel(int n, const f
       for (int t = 0; t < NT; ++t) gg[u][t] = *reinterpret_cast<const uint4*>(Gc[t] + l0 + 16 * u);
     }
   };
-  load(l_begin, an, gn);
-  for (int l0 = l_begin; l0 < l_end; l0 += 16 * kUnroll) {
+  // The rows of A (and of G) lie a whole row apart -- the same offset modulo a large power of
+  // two -- so workgroups that walked their ranges in step would all hit the same HBM channels;
+  // each workgroup starts its walk at its own rotation of the range instead.
+  constexpr int kStep = 16 * kUnroll;
+  const int rounds = span / kStep;
+  const int rot = (int)((blockIdx.x * 37u) % (unsigned)rounds);
+  auto l_of = [&](int q) { return l_begin + ((q + rot) % rounds) * kStep; };
+  (void)l_end;
+  load(l_of(0), an, gn);
+  for (int q = 0; q < rounds; ++q) {
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       a[u][0] = an[u][0];
@@ -127,15 +129,15 @@ __global__ __launch_bounds__(64 * kWaves) void sketch_bf16_kernel(int n, const f
 #pragma unroll
       for (int t = 0; t < NT; ++t) g[u][t] = gn[u][t];
     }
-    if (l0 + 16 * kUnroll < l_end) load(l0 + 16 * kUnroll, an, gn);
+    if (q + 1 < rounds) load(l_of(q + 1), an, gn);
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
-      const bf16x8 av = pack8(a[u][0], a[u][1]);
+      bf16x8 av, al;
+      split8(a[u][0], a[u][1], av, SPLIT ? &al : nullptr);
 #pragma unroll
       for (int t = 0; t < NT; ++t)
         acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, as_bf16x8(g[u][t]), acc[t], 0, 0, 0);
       if (SPLIT) {
-        const bf16x8 al = pack8_lo(a[u][0], a[u][1]);
 #pragma unroll
         for (int t = 0; t < NT; ++t)
           acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, as_bf16x8(g[u][t]), acc[t], 0, 0, 0);
