@@ -119,6 +119,9 @@ _PROTOS = {
     "ck_coo_to_csr_f32": (ctypes.c_int, [_i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _sz, _p]),
     "ck_coo_to_csr_f64": (ctypes.c_int, [_i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _sz, _p]),
     "ck_csr_apply_f64acc": (ctypes.c_int, [_i64, _p, _p, _p, _f64, _f64, _i32, _p, _i64, _p, _i64, _p]),
+    "ck_lanczos_block_scratch_bytes": (_sz, [_i64, _i32, _i32]),
+    "ck_lanczos_iterate_block": (ctypes.c_int, [_op, _i64, _i32, _i32, _p, _i64, _i64, _p, _p, _p, _f32, _i32, _i32,
+                                                _i32, _p, _p, _sz, _p, _sz, _p]),
     "ck_sketch_bf16_supported": (ctypes.c_int, [_i64, _i32, _i64, _p, _i64, _p]),
     "ck_sketch_bf16": (ctypes.c_int, [_i64, _i32, _p, _i64, _f32, _f32, _p, _i64, _p, _i64, _p, _i64, _i32, _p]),
     "ck_round_bf16": (ctypes.c_int, [_i64, _i32, _p, _i64, _p, _i64, _p, _i64, _p]),

@@ -646,6 +646,219 @@ int lz_post_full(int64_t n, int nb, float* V, int64_t ldv, int64_t ldb, float* d
   return CK_OK;
 }
 
+// ---------------------------------------------------------------- block re-orthogonalisation
+// Opt-in mode of ck_lanczos_iterate_block (no counterpart in the reference, whose Lanczos
+// re-orthogonalises every new vector against the whole basis, lanczos.py:287-296).  The
+// basis is cut into blocks of S columns.  Each step orthogonalises w (CGS2) only against the
+// columns of its own block and the column before it; when a block is complete it is made
+// orthogonal to all older columns at once, two classical passes of
+//   H = V_old^T W  (blk_project)   and   W -= V_old H  (blk_apply)
+// on v_mfma_f32_16x16x4f32.  V_old streams from HBM once per pass for S columns instead of
+// three times per column, so the basis traffic drops by about 3S/4; between two block
+// passes the new columns carry only the components a few recurrence steps let grow.
+constexpr int kBlkJ = 256;        // old columns per blk_project / blk_apply launch
+constexpr int kBlkTile = 4096;    // vector elements per blk_project block
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+
+// hpart[b][(jj * 16 + c) * P2 + tile] = sum over the tile of V[j0 + jj] . V[w0 + c] for
+// jj < nj, c < S.  A lane of a 16x16x4 step holds operand k = 16 (lane / 16) + ks of the
+// 64-element chunk (the sum over k is order-free), so its A and B values are 16 contiguous
+// floats of one column: four float4 loads each.
+__global__ __launch_bounds__(256) void blk_project(int64_t ld, const float* __restrict__ V, int64_t ldv,
+                                                   int64_t ldb, int j0, int nj, int w0, int S,
+                                                   float* __restrict__ hpart, int P2, const int* __restrict__ done) {
+  if (done && *done) return;
+  const int b = blockIdx.y, tile = blockIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const float* Vb = V + b * ldb;
+  const int64_t i_begin = (int64_t)tile * kBlkTile;
+  const int64_t i_end = i_begin + kBlkTile < ld ? i_begin + kBlkTile : ld;
+  const int njt = (nj + 15) / 16;
+  const int cl = lane & 15, slot = lane >> 4;
+  const float* wcol = Vb + (int64_t)(w0 + (cl < S ? cl : S - 1)) * ldv + 16 * slot;
+  const float* vcol[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int jt = wave + 4 * t;
+    const int jj = 16 * jt + cl;
+    vcol[t] = Vb + (int64_t)(j0 + (jj < nj ? jj : nj - 1)) * ldv + 16 * slot;
+  }
+  f32x4v acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) acc[t] = f32x4v{0.f, 0.f, 0.f, 0.f};
+  for (int64_t i0 = i_begin; i0 < i_end; i0 += 64) {
+    float4 wv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) wv[q] = *reinterpret_cast<const float4*>(wcol + i0 + 4 * q);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (wave + 4 * t >= njt) break;
+      float4 av[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) av[q] = *reinterpret_cast<const float4*>(vcol[t] + i0 + 4 * q);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q].x, wv[q].x, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q].y, wv[q].y, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q].z, wv[q].z, acc[t], 0, 0, 0);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[q].w, wv[q].w, acc[t], 0, 0, 0);
+      }
+    }
+  }
+  // D[m][n]: lane holds m = 4 (lane / 16) + v (old column within the tile), n = lane % 16
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int jt = wave + 4 * t;
+    if (jt >= njt) break;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int jj = 16 * jt + 4 * slot + v;
+      if (jj < nj && cl < S) hpart[((int64_t)b * kBlkJ * 16 + jj * 16 + cl) * P2 + tile] = acc[t][v];
+    }
+  }
+}
+
+// H[b][jj * 16 + c] = sum over tiles (fixed order, FP64) for jj < nj, c < S; zero for c >= S
+__global__ __launch_bounds__(256) void blk_reduce(const float* __restrict__ hpart, int P2, int nj, int S,
+                                                  float* __restrict__ H, const int* __restrict__ done) {
+  if (done && *done) return;
+  const int b = blockIdx.y;
+  const int e = blockIdx.x * 4 + (threadIdx.x >> 6);  // (jj, c) entry
+  if (e >= nj * 16) return;
+  const int c = e % 16;
+  double s = 0.0;
+  if (c < S) s = col_total(hpart, P2, ((int64_t)b * kBlkJ * 16 + e) * P2);
+  if ((threadIdx.x & 63) == 0) H[(int64_t)b * kBlkJ * 16 + e] = (float)s;
+}
+
+// W[c] -= sum_{jj < nj} V[j0 + jj] H[jj][c] over 64 elements per wave: a lane of a 16x16x4
+// step holds V[j0 + 4 ks + lane / 16][i0 + 4 (lane % 16) + q] as A (row m = lane % 16 of
+// sub-tile q, i.e. element 4 m + q) and H[4 ks + lane / 16][lane % 16] as B; one float4 load
+// per lane per step serves the four sub-tiles.
+__global__ __launch_bounds__(256) void blk_apply(int64_t ld, float* __restrict__ V, int64_t ldv, int64_t ldb, int j0,
+                                                 int nj, int w0, int S, const float* __restrict__ H,
+                                                 const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ float Hs[kBlkJ][16];
+  const int b = blockIdx.y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int e = threadIdx.x; e < nj * 16; e += 256) Hs[e / 16][e % 16] = H[(int64_t)b * kBlkJ * 16 + e];
+  __syncthreads();
+  float* Vb = V + b * ldb;
+  const int64_t i0 = (int64_t)blockIdx.x * 256 + 64 * wave;
+  if (i0 >= ld) return;
+  const int m = lane & 15, slot = lane >> 4;
+  f32x4v acc[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) acc[q] = f32x4v{0.f, 0.f, 0.f, 0.f};
+  const int nks = (nj + 3) / 4;
+  int ks = 0;
+  for (; ks + 4 <= nks; ks += 4) {
+    float4 a[4];
+    float hb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int jj = 4 * (ks + u) + slot;
+      a[u] = *reinterpret_cast<const float4*>(Vb + (int64_t)(j0 + (jj < nj ? jj : nj - 1)) * ldv + i0 + 4 * m);
+      hb[u] = jj < nj ? Hs[jj][m] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].x, hb[u], acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].y, hb[u], acc[1], 0, 0, 0);
+      acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].z, hb[u], acc[2], 0, 0, 0);
+      acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[u].w, hb[u], acc[3], 0, 0, 0);
+    }
+  }
+  for (; ks < nks; ++ks) {
+    const int jj = 4 * ks + slot;
+    const float4 a = *reinterpret_cast<const float4*>(Vb + (int64_t)(j0 + (jj < nj ? jj : nj - 1)) * ldv + i0 + 4 * m);
+    const float hb = jj < nj ? Hs[jj][m] : 0.f;
+    acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, hb, acc[0], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, hb, acc[1], 0, 0, 0);
+    acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.z, hb, acc[2], 0, 0, 0);
+    acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(a.w, hb, acc[3], 0, 0, 0);
+  }
+  // sub-tile q, D[m'][n] at lane: m' = 4 slot + v (element i0 + 4 m' + q), n = lane % 16 (column)
+  const int c = m;
+  if (c >= S) return;
+  float* wc = Vb + (int64_t)(w0 + c) * ldv + i0;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int base = 4 * (4 * slot + v);
+    float4 wv = *reinterpret_cast<float4*>(wc + base);
+    wv.x -= acc[0][v];
+    wv.y -= acc[1][v];
+    wv.z -= acc[2][v];
+    wv.w -= acc[3][v];
+    *reinterpret_cast<float4*>(wc + base) = wv;
+  }
+}
+
+struct BlkScratch {
+  float* hpart;  // nb x kBlkJ x 16 x P2
+  float* H;      // nb x kBlkJ x 16
+  int P2;
+};
+
+inline BlkScratch blk_scratch_of(void* scratch, int64_t ld, int nb) {
+  BlkScratch s;
+  s.P2 = (int)cdiv(ld, kBlkTile);
+  s.hpart = static_cast<float*>(scratch);
+  s.H = s.hpart + (size_t)nb * kBlkJ * 16 * s.P2;
+  return s;
+}
+
+// columns [w0, w0 + S) orthogonal to columns [1, w0): two classical passes, old columns in
+// chunks of kBlkJ
+int blk_orthogonalize(int64_t ld, int nb, float* V, int64_t ldv, int64_t ldb, int w0, int S, const BlkScratch& bs,
+                      const int* done, hipStream_t s) {
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int j0 = 1; j0 < w0; j0 += kBlkJ) {
+      const int nj = w0 - j0 < kBlkJ ? w0 - j0 : kBlkJ;
+      hipLaunchKernelGGL(blk_project, dim3(bs.P2, nb), dim3(256), 0, s, ld, (const float*)V, ldv, ldb, j0, nj, w0, S,
+                         bs.hpart, bs.P2, done);
+      hipLaunchKernelGGL(blk_reduce, dim3((unsigned)cdiv(nj * 16, 4), nb), dim3(256), 0, s, (const float*)bs.hpart,
+                         bs.P2, nj, S, bs.H, done);
+      hipLaunchKernelGGL(blk_apply, dim3((unsigned)cdiv(ld, 256), nb), dim3(256), 0, s, ld, V, ldv, ldb, j0, nj, w0,
+                         S, (const float*)bs.H, done);
+    }
+  }
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+// lz_post_full with the re-orthogonalisation restricted to columns [jlo, i]
+int lz_post_window(int64_t n, int nb, float* V, int64_t ldv, int64_t ldb, float* diag, float* subdiag, void* state,
+                   int m, float tol, int step, int jlo, void* scratch, hipStream_t s) {
+  const int i = step + 1;
+  if (i > m) return CK_ERR_BAD_ARG;
+  const int* done = &static_cast<ck_lz_ctrl*>(state)->done;
+  LzScratch sc = scratch_of(scratch, n, nb, m);
+  const float* w = V + (int64_t)(i + 1) * ldv;
+  const float* vi = V + (int64_t)i * ldv;
+  dim3 grid(sc.P, nb);
+  const bool vec4 = basis_vec4(V, ldv, ldb);
+  hipLaunchKernelGGL(tile_partials<0>, grid, dim3(256), 0, s, n, w, ldb, vi, ldb, sc.dpart, sc.P, done);
+  launch_project<true>(vec4, grid, s, n, V, ldv, ldb, i, jlo, i, sc.dpart, sc.P, sc.hpart, sc.M2, diag, subdiag, m,
+                       done);
+  const dim3 rgrid((unsigned)cdiv(i - jlo + 1, 4), nb);
+  hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, jlo, i, sc.h, done);
+  if (vec4) {
+    hipLaunchKernelGGL(lz_apply_project, grid, dim3(256), 0, s, n, V, ldv, ldb, i + 1, jlo, i, (const float*)sc.h,
+                       sc.M2, sc.hpart, sc.P, done);
+  } else {
+    launch_apply<false>(vec4, grid, s, n, V, ldv, ldb, i + 1, jlo, i, sc.h, sc.M2, sc.dpart, sc.P, done);
+    launch_project<false>(vec4, grid, s, n, V, ldv, ldb, i, jlo, i, sc.dpart, sc.P, sc.hpart, sc.M2, diag, subdiag,
+                          m, done);
+  }
+  hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, jlo, i, sc.h, done);
+  launch_apply<true>(vec4, grid, s, n, V, ldv, ldb, i + 1, jlo, i, sc.h, sc.M2, sc.dpart, sc.P, done);
+  hipLaunchKernelGGL(lz_finalize, dim3(1), dim3(1024), 0, s, nb, m, sc.dpart, sc.P, subdiag, state, tol, i);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
 int ar_post_full(int64_t n, int nb, float* Q, int64_t ldv, int64_t ldb, float* H, void* state, int m, float tol,
                  int step, int mgs, void* scratch, hipStream_t s) {
   const int idx = step;
@@ -751,6 +964,49 @@ int ck_lanczos_iterate(const ck_op* A, int64_t n, int32_t nb, int32_t max_iters,
                            s);
     if (st) return st;
     st = lz_post_full(n, nb, V, ldv, ldb, diag, subdiag, state, max_iters, tol, step, scratch, s);
+    if (st) return st;
+  }
+  return CK_OK;
+}
+
+size_t ck_lanczos_block_scratch_bytes(int64_t n, int32_t nb, int32_t block) {
+  if (block < 2 || block > 16 || nb < 1) return 0;
+  const int64_t ld = cdiv(n, 64) * 64;
+  const size_t P2 = cdiv(ld, kBlkTile);
+  return sizeof(float) * ((size_t)nb * kBlkJ * 16 * P2 + (size_t)nb * kBlkJ * 16) + 256;
+}
+
+int ck_lanczos_iterate_block(const ck_op* A, int64_t n, int32_t nb, int32_t max_iters, float* V, int64_t ldv,
+                             int64_t ldb, float* diag, float* subdiag, void* state, float tol, int32_t first_step,
+                             int32_t nsteps, int32_t block, void* scratch, void* block_scratch,
+                             size_t block_scratch_bytes, void* work, size_t work_bytes, void* stream) {
+  if (!A || A->n != n || first_step < 0 || nsteps < 0 || block < 2 || block > 16) return CK_ERR_BAD_ARG;
+  if (int st = check_basis(n, nb, V, ldv, ldb, max_iters, 2)) return st;
+  if (!diag || !subdiag || !state || !scratch || !block_scratch) return CK_ERR_BAD_ARG;
+  const int64_t ld = cdiv(n, 64) * 64;
+  // the block kernels read whole 64-element chunks of every column: zero padding up to ld
+  if (ldv < ld || (ldv % 4) || (reinterpret_cast<uintptr_t>(V) & 15)) return CK_ERR_UNSUPPORTED;
+  if (block_scratch_bytes < ck_lanczos_block_scratch_bytes(n, nb, block)) return CK_ERR_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  const int* done = &static_cast<ck_lz_ctrl*>(state)->done;
+  const BlkScratch bs = blk_scratch_of(block_scratch, ld, nb);
+  const int S = block;
+  for (int step = first_step; step < first_step + nsteps && step < max_iters; ++step) {
+    const int i = step + 1;
+    int st = lz_pre_full(n, nb, V, ldv, ldb, subdiag, state, max_iters, step, scratch, s);
+    if (st) return st;
+    // column i completes block [i - S + 1, i]: orthogonalise it against every older column
+    if (i % S == 0 && i > S) {
+      st = blk_orthogonalize(ld, nb, V, ldv, ldb, i - S + 1, S, bs, done, s);
+      if (st) return st;
+    }
+    st = op_apply_internal(A, nb, V + (int64_t)i * ldv, ldb, V + (int64_t)(i + 1) * ldv, ldb, work, work_bytes, done,
+                           s);
+    if (st) return st;
+    // w (column i + 1) against its own block and the column before it
+    const int c0 = (i / S) * S + 1;
+    const int jlo = c0 - 1 > 1 ? c0 - 1 : 1;
+    st = lz_post_window(n, nb, V, ldv, ldb, diag, subdiag, state, max_iters, tol, step, jlo, scratch, s);
     if (st) return st;
   }
   return CK_OK;

@@ -42,7 +42,8 @@ def eig(A: LinearOperator, k: int, which: str, alg: HipLanczos):
     kw = dict(alg.__dict__)
     kw.pop("pbar", None)
     vals, vecs, _ = lanczos_eigs(A, kw.pop("start_vector"), kw["max_iters"], kw["tol"], key=kw["key"],
-                                 eig_slice=eig_slice, chunk=kw["chunk"])
+                                 eig_slice=eig_slice, chunk=kw["chunk"], reorth=kw.get("reorth", "full"),
+                                 block=kw.get("block", 16))
     return vals, vecs
 
 

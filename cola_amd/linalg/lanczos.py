@@ -38,6 +38,11 @@ class HipLanczos(Algorithm):
         pbar: accepted for compatibility.
         key: seed of the default start vector.
         chunk: steps enqueued between reads of the device loop flag.
+        reorth: "full" (the reference's re-orthogonalisation of every new vector against the
+            whole basis) or "block" (engine option: against its own block of ``block``
+            columns each step, and each completed block against all older columns at once on
+            FP32 MFMA; ``ck_lanczos_iterate_block``).
+        block: block size of the "block" mode (2..16).
     """
     start_vector: Any = None
     max_iters: int = 1_000
@@ -45,9 +50,12 @@ class HipLanczos(Algorithm):
     pbar: bool = False
     key: Optional[int] = None
     chunk: int = 32
+    reorth: str = "full"
+    block: int = 16
 
     def __call__(self, A):
-        return lanczos(A, self.start_vector, self.max_iters, self.tol, self.pbar, self.key, chunk=self.chunk)
+        return lanczos(A, self.start_vector, self.max_iters, self.tol, self.pbar, self.key, chunk=self.chunk,
+                       reorth=self.reorth, block=self.block)
 
 
 class LanczosRun:
@@ -83,8 +91,10 @@ def default_start(A, key, nb=1):
     return Z[:, :A.shape[0]].T
 
 
-def run_lanczos(A, start, max_iters, tol, chunk=32):
+def run_lanczos(A, start, max_iters, tol, chunk=32, reorth="full", block=16):
     """Batched Lanczos on the device; start is (n, b). Returns a LanczosRun."""
+    if reorth not in ("full", "block"):
+        raise ValueError(f"reorth must be 'full' or 'block', got {reorth!r}")
     N.require_device(start, "start vector")
     n = A.shape[0]
     m = min(int(max_iters), n)
@@ -96,6 +106,15 @@ def run_lanczos(A, start, max_iters, tol, chunk=32):
     aop = lower(A)
     work, wb = workspace([aop[0] if aop else None], nb, dev)
     s = N.stream_ptr(dev)
+    bscr, bsb = None, 0
+    if reorth == "block":
+        if aop is None:
+            raise N.UnsupportedOperator("block re-orthogonalisation", N.CK_ERR_UNSUPPORTED,
+                                        "the block mode runs with library operators only (hip_op)")
+        bsb = N.q("ck_lanczos_block_scratch_bytes", n, nb, int(block))
+        if bsb == 0:
+            raise ValueError("block must lie in 2..16")
+        bscr = torch.empty((bsb + 3) // 4, dtype=torch.float32, device=dev)
     t0 = time.time()
     S = to_blocks(start, run.ld)
     N.call("ck_lanczos_start", n, nb, N.ptr(S), run.ld, N.ptr(run.V), run.ld, run.ldb, N.ptr(run.diag),
@@ -104,7 +123,11 @@ def run_lanczos(A, start, max_iters, tol, chunk=32):
     i, done = read_ints(run.state, 2, dev)
     while not done and step < m:
         nst = min(int(chunk), m - step)
-        if aop is not None:
+        if bscr is not None:
+            N.call("ck_lanczos_iterate_block", aop[0], n, nb, m, N.ptr(run.V), run.ld, run.ldb, N.ptr(run.diag),
+                   N.ptr(run.subdiag), N.ptr(run.state), float(tol), step, nst, int(block), N.ptr(run.scratch),
+                   N.ptr(bscr), bsb, N.ptr(work), wb, s)
+        elif aop is not None:
             N.call("ck_lanczos_iterate", aop[0], n, nb, m, N.ptr(run.V), run.ld, run.ldb, N.ptr(run.diag),
                    N.ptr(run.subdiag), N.ptr(run.state), float(tol), step, nst, N.ptr(run.scratch), N.ptr(work), wb, s)
         else:
@@ -126,12 +149,12 @@ def run_lanczos(A, start, max_iters, tol, chunk=32):
     return run
 
 
-def lanczos(A, start_vector=None, max_iters=100, tol=1e-7, pbar=False, key=None, chunk=32):
+def lanczos(A, start_vector=None, max_iters=100, tol=1e-7, pbar=False, key=None, chunk=32, reorth="full", block=16):
     """lanczos.py:185-232: returns (Q, T, info) with Q (n, k) [or (b, n, k)] and
     T = Tridiagonal(alpha, beta, alpha)."""
     if start_vector is None:
         start_vector = default_start(A, key)[:, 0]
-    run = run_lanczos(A, start_vector, max_iters, tol, chunk)
+    run = run_lanczos(A, start_vector, max_iters, tol, chunk, reorth, block)
     alpha, beta = run.alpha(), run.beta()
     if start_vector.dim() == 1:
         T = Tridiagonal(alpha[0], beta[0], alpha[0])
@@ -168,13 +191,14 @@ def ritz_vectors(run, S, b=0):
     return out[:, :run.n].T
 
 
-def lanczos_eigs(A, start_vector=None, max_iters=100, tol=1e-7, pbar=False, key=None, eig_slice=None, chunk=32):
+def lanczos_eigs(A, start_vector=None, max_iters=100, tol=1e-7, pbar=False, key=None, eig_slice=None, chunk=32,
+                 reorth="full", block=16):
     """lanczos.py:34-61: ascending Ritz values and (lazy in the reference) Ritz vectors.
 
     ``eig_slice`` limits which Ritz vectors are formed."""
     if start_vector is None:
         start_vector = default_start(A, key)[:, 0]
-    run = run_lanczos(A, start_vector, max_iters, tol, chunk)
+    run = run_lanczos(A, start_vector, max_iters, tol, chunk, reorth, block)
     ev, vecs, _, st = tridiag_eigh(run, vectors=True)
     if int(st.max()) != 0:
         raise RuntimeError("tridiagonal eigensolve did not converge")

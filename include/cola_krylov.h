@@ -368,6 +368,19 @@ int ck_lanczos_post(int64_t n, int32_t nb, int32_t max_iters, float* V, int64_t 
 int ck_lanczos_iterate(const ck_op* A, int64_t n, int32_t nb, int32_t max_iters, float* V, int64_t ldv, int64_t ldb,
                        float* diag, float* subdiag, void* state, float tol, int32_t first_step, int32_t nsteps,
                        void* scratch, void* work, size_t work_bytes, void* stream);
+/* ck_lanczos_iterate with block re-orthogonalisation (block = S in 2..16; no counterpart in
+ * the reference, which re-orthogonalises every new vector against the whole basis,
+ * lanczos.py:287-296): each step orthogonalises w (two classical passes) against its own
+ * block of S columns and the column before it; when column i completes a block (i % S == 0,
+ * i > S) the block [i-S+1, i] is made orthogonal to columns [1, i-S] by two passes of
+ * H = V_old^T W, W -= V_old H on FP32 MFMA.  The recurrence, stop rule and outputs are those of
+ * ck_lanczos_iterate.  ldv >= n rounded up to 64 (zero padding), 16-byte aligned V;
+ * block_scratch >= ck_lanczos_block_scratch_bytes. */
+size_t ck_lanczos_block_scratch_bytes(int64_t n, int32_t nb, int32_t block);
+int ck_lanczos_iterate_block(const ck_op* A, int64_t n, int32_t nb, int32_t max_iters, float* V, int64_t ldv,
+                             int64_t ldb, float* diag, float* subdiag, void* state, float tol, int32_t first_step,
+                             int32_t nsteps, int32_t block, void* scratch, void* block_scratch,
+                             size_t block_scratch_bytes, void* work, size_t work_bytes, void* stream);
 
 /* ---------------------------------------------------------------- Arnoldi
  * Arnoldi with Gram-Schmidt against columns 0..idx (cola/linalg/decompositions/
