@@ -288,7 +288,82 @@ def bench_csr(args, dev):
                                      "step_effective_gbs": step_bytes / (dt / max(steps, 1)) / 1e9}},
             "roofline": {"bound": "hbm", "kernel": "CSR SpMV, 4096^2 grid, 16 vectors (matrix data > Infinity Cache)",
                          "achieved": big["gbs"], "peak": HBM_PEAK, "unit": "GB/s", "frac": big["frac"],
-                         "traffic": None, "launch_us": big["us"], "algorithmic_bytes_per_launch": big["algorithmic_bytes"]}}
+                         "traffic": None, "launch_us": big["us"], "algorithmic_bytes_per_launch": big["algorithmic_bytes"]},
+            **({} if args.no_cpu_baseline else {"cpu_baseline": cpu_baseline_csr_cg(A, b, steps)})}
+
+
+def _cpu_laplacian_matmat(A, shift=1e-3):
+    """The reference's CPU operator apply for the shifted CSR Laplacian: torch's sparse CSR
+    product (Sparse._matmat, operators.py:77-78; torch_fns.py:79-80) plus shift * V."""
+    torch.set_num_threads(cpu_threads())
+    n = A.shape[0]
+    M = torch.sparse_csr_tensor(A.indptr.cpu().to(torch.int64), A.indices.cpu().to(torch.int64), A.data.cpu(),
+                                size=(n, n))
+    return lambda V: M @ V + shift * V
+
+
+def cpu_baseline_csr_cg(A, b, steps_gpu):
+    """The oracle's run_batched_cg (cola's CG restated, FP32 torch) on the CPU with the CSR
+    apply, the same rhs, tolerance 1e-6 and cap 1000: one whole solve (a few seconds)."""
+    from oracle import cg as ocg
+    mm = _cpu_laplacian_matmat(A)
+    bc = b.cpu()
+    ocg.run_batched_cg(mm, bc, None, 2, 1e-6, None)
+    t0 = time.perf_counter()
+    _, _, k, _ = ocg.run_batched_cg(mm, bc, None, 1000, 1e-6, None)
+    dt = time.perf_counter() - t0
+    return {"value": 1 / dt, "unit": "solves/s", "cores": cpu_threads(), "kind": "port",
+            "sample": f"one whole plain CG solve on the 1024^2 Laplacian ({int(k)} steps, the GPU run took "
+                      f"{int(steps_gpu)}), {dt * 1e3 / max(int(k), 1):.2f} ms per step, torch sparse CSR"}
+
+
+def cpu_baseline_lanczos(A, s0):
+    """The oracle's Lanczos with full double Gram-Schmidt (lanczos.py:235-296 restated, FP32
+    torch) and its eigh/Ritz-vector step, the whole 120-step run on the 1024^2 Laplacian, on the
+    CPU (about half a minute to a minute of host time: its steps sweep all m + 2 basis columns,
+    as the reference's do)."""
+    from oracle import lanczos as olz
+    mm = _cpu_laplacian_matmat(A)
+    n = A.shape[0]
+    t0 = time.perf_counter()
+    olz.lanczos_eigs(mm, n, s0.cpu(), 120, 1e-7, torch.float32, torch.float64)
+    dt = time.perf_counter() - t0
+    return {"value": 1 / dt, "unit": "eigensolves/s", "cores": cpu_threads(), "kind": "port",
+            "sample": f"one whole 120-step eigensolve ({dt:.1f} s), torch sparse CSR apply, FP64 eigh of T"}
+
+
+def cpu_baseline_hutch(A, reps=3):
+    """The oracle's Hutchinson estimate with 64 Rademacher probes (diag_trace.py restated) on the
+    CPU: one 64-column CSR product and the probe sums, `reps` estimates."""
+    from oracle import hutch as oh
+    mm = _cpu_laplacian_matmat(A)
+    n = A.shape[0]
+    g = torch.Generator().manual_seed(0)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        Z = (torch.randint(0, 2, (n, 64), generator=g) * 2 - 1).float()
+        oh.trace_from_probes(mm, Z)
+    dt = (time.perf_counter() - t0) / reps
+    return {"value": 1 / dt, "unit": "estimates/s", "cores": cpu_threads(), "kind": "port",
+            "sample": f"{reps} estimates of 64 probes each ({dt * 1e3:.0f} ms per estimate, probes drawn on the host)"}
+
+
+def cpu_baseline_slq(facs, probes=8):
+    """The oracle's SLQ (slq.py:37-52 restated: batched Lanczos with full re-orthogonalisation,
+    eigh of T, quadrature) with the Kronecker apply on the CPU, for `probes` of the 32 Rademacher
+    probes, scaled to 32."""
+    from oracle import operators as oop
+    from oracle import unary as ou
+    torch.set_num_threads(cpu_threads())
+    mm = oop.shifted(lambda V: oop.kron_matmat(facs, V), 0.01)
+    n = 64**3
+    g = torch.Generator().manual_seed(5)
+    Z = (torch.randint(0, 2, (n, probes), generator=g) * 2 - 1).float()
+    t0 = time.perf_counter()
+    ou.slq_fwd(mm, n, torch.log, Z, 30, 1e-6, torch.float32, torch.float64)
+    dt = (time.perf_counter() - t0) * 32 / probes
+    return {"value": 1 / dt, "unit": "logdets/s", "cores": cpu_threads(), "kind": "port",
+            "sample": f"{probes} of the 32 probes x 30 steps ({dt * probes / 32:.1f} s), scaled to 32 probes"}
 
 
 def bench_dense_cg(args, dev):
@@ -377,8 +452,36 @@ def bench_lanczos(args, dev):
     spmv = 4 * (2 * 5238784 + (n + 1) + 2 * n)
     reorth = sum(3 * i * 4 * n + 2 * 2 * 4 * n for i in range(1, 121))
     total = 120 * spmv + reorth
+    # the engine's block re-orthogonalisation (S = 16): same start, same accuracy bar
+    S = 16
+    balg = HipLanczos(start_vector=s0, max_iters=120, tol=1e-7, reorth="block", block=S)
+    for _ in range(1 + args.warmup):
+        bvals, _ = eig(op, 32, "LM", balg)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        bvals, _ = eig(op, 32, "LM", balg)
+    torch.cuda.synchronize()
+    bdt = (time.perf_counter() - t0) / reps
+    # bytes: SpMV + the window passes (three reads of the window's columns) + per completed block
+    # two passes of (project: old + S columns; apply: old columns + S columns read and written)
+    bwin = 0
+    for i in range(1, 121):
+        c0 = (i // S) * S + 1
+        jlo = max(1, c0 - 1)
+        bwin += 3 * (i - jlo + 1) * 4 * n + 2 * 2 * 4 * n
+    bblk = sum(2 * 4 * n * (2 * (i - S) + 3 * S) for i in range(2 * S, 121, S))
+    bflops = sum(2 * 2 * 2 * (i - S) * 16 * n for i in range(2 * S, 121, S))
+    btotal = 120 * spmv + bwin + bblk
+    block_line = {"value": 1 / bdt, "unit": "eigensolves/s", "ms_per_step": bdt * 1e3, "block": S,
+                  "max_rel_err_top32_vs_fp64_oracle_same_start": _golden_rel(
+                      "lanczos_laplacian_1024sq", "top32_ritz_fp64", bvals.cpu().numpy()),
+                  "algorithmic_bytes": btotal, "block_pass_mfma_flops": bflops,
+                  "roofline": {"bound": "hbm", "achieved": btotal / bdt / 1e9, "peak": HBM_PEAK, "unit": "GB/s",
+                               "frac": btotal / bdt / 1e9 / HBM_PEAK, "traffic": None}}
     return {"metric": "Lanczos eigensolves/sec (2-D Laplacian 1024^2, 120 steps, CGS2, top-32)",
             "value": 1 / dt, "unit": "eigensolves/s", "ms_per_step": dt * 1e3, "dtype": "f32 (+f64 tridiagonal)",
+            "block_reorth": block_line,
             "config": {"workload": "lanczos", "n": n, "nnz": 5238784, "steps": 120,
                        "max_abs_err_top32_vs_closed_form": err,
                        "max_rel_err_top32_vs_fp64_oracle_same_start": _golden_rel(
@@ -386,7 +489,8 @@ def bench_lanczos(args, dev):
                        "algorithmic_bytes": total,
                        "effective_gbs": total / dt / 1e9},
             "roofline": {"bound": "hbm", "achieved": total / dt / 1e9, "peak": HBM_PEAK, "unit": "GB/s",
-                         "frac": total / dt / 1e9 / HBM_PEAK, "traffic": None}}
+                         "frac": total / dt / 1e9 / HBM_PEAK, "traffic": None},
+            **({} if args.no_cpu_baseline else {"cpu_baseline": cpu_baseline_lanczos(A, s0)})}
 
 
 def bench_hutch(args, dev):
@@ -422,7 +526,8 @@ def bench_hutch(args, dev):
                        "algorithmic_bytes": byts, "device_us": t_k * 1e6},
             "roofline": {"bound": "hbm", "kernel": "trace_partials<SeededWords> + trace_bits_final",
                          "achieved": byts / t_k / 1e9, "peak": HBM_PEAK, "unit": "GB/s",
-                         "frac": byts / t_k / 1e9 / HBM_PEAK, "traffic": None}}
+                         "frac": byts / t_k / 1e9 / HBM_PEAK, "traffic": None},
+            **({} if args.no_cpu_baseline else {"cpu_baseline": cpu_baseline_hutch(A)})}
 
 
 def bench_slq(args, dev):
@@ -440,12 +545,20 @@ def bench_slq(args, dev):
     dt = (time.perf_counter() - t0) / args.steps
     lam = [torch.linalg.eigvalsh(f.double()) for f in facs]
     exact = float(torch.log((lam[0][:, None, None] * lam[1][None, :, None] * lam[2][None, None, :]) + 0.01).sum())
+    # algorithmic bytes: per step the 32-probe Kronecker apply (read and write 32 vectors of 64^3)
+    # and each probe's re-orthogonalisation (three reads of its i basis vectors, w traffic)
+    vb = 4 * 64**3
+    slq_bytes = sum(2 * 32 * vb + 32 * (3 * i + 4) * vb for i in range(1, 31))
     return {"metric": "SLQ log-determinants/sec (64^3 Kronecker, 32 probes x 30 steps)",
             "value": 1 / dt, "unit": "logdets/s", "ms_per_step": dt * 1e3, "dtype": "f32 (+f64 tridiagonal)",
             "config": {"workload": "slq", "estimate_abs": float(est), "exact": exact,
                        "rel_err_vs_abs_exact": abs(float(est) - abs(exact)) / abs(exact),
                        "rel_err_vs_fp64_oracle_same_probes": _golden_rel("slq_kron64", "logdet_fp64", float(est)),
-                       "note": "cola's logdet returns |tr log A| (logdet.py:111-117)"}}
+                       "note": "cola's logdet returns |tr log A| (logdet.py:111-117)", "algorithmic_bytes": slq_bytes},
+            "roofline": {"bound": "hbm", "kernel": "whole SLQ run (Kronecker applies + re-orthogonalisation), wall time",
+                         "achieved": slq_bytes / dt / 1e9, "peak": HBM_PEAK, "unit": "GB/s",
+                         "frac": slq_bytes / dt / 1e9 / HBM_PEAK, "traffic": None},
+            **({} if args.no_cpu_baseline else {"cpu_baseline": cpu_baseline_slq([f.cpu() for f in facs])})}
 
 
 def _golden_rel(section, key, got):
