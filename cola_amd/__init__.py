@@ -7,6 +7,7 @@ from . import annotations, io, linalg, ops
 from .annotations import PSD, Hermitian, SelfAdjoint, Stiefel, Unitary
 from .fns import add, densify, dot, kron, lazify, mul
 from .ops import (BlockDiag, Dense, DenseBF16, Diagonal, I_like, Identity, Kernel, Kronecker, LinearOperator, Product,
-                  RBFKernel, ScalarMul, Sparse, Sum, Tridiagonal, WoodburyInverse)
+                  RBFKernel, ScalarMul, Sparse, StationaryKernelFn, Sum, Tridiagonal, WoodburyInverse, matern12,
+                  matern32, matern52, rbf)
 
 __version__ = "0.1.0"

@@ -158,10 +158,27 @@ constexpr int kRbfTile = 256;
 constexpr int kRbfMaxDim = 8;
 constexpr int kRbfNB = 8;
 
+// the stationary kernel k(d2) for d2 = |p_i - p_j|^2: FAMILY 0 exp(-d2 / (2 l^2)) (RBF),
+// 1 exp(-r / l) (Matern 1/2), 2 (1 + a) exp(-a) with a = sqrt(3) r / l (Matern 3/2),
+// 3 (1 + a + a^2 / 3) exp(-a) with a = sqrt(5) r / l (Matern 5/2)
+template <int FAMILY>
+__device__ __forceinline__ float stationary(float d2, float inv2l2, float inv_l) {
+  if (FAMILY == 0) return expf(-d2 * inv2l2);
+  const float r = sqrtf(d2);
+  if (FAMILY == 1) return expf(-r * inv_l);
+  if (FAMILY == 2) {
+    const float a = 1.7320508075688772f * r * inv_l;
+    return (1.f + a) * expf(-a);
+  }
+  const float a = 2.23606797749979f * r * inv_l;
+  return (1.f + a + a * a * (1.f / 3.f)) * expf(-a);
+}
+
+template <int FAMILY>
 __global__ __launch_bounds__(256) void rbf_kernel(int n, int dim, const float* __restrict__ P, int64_t ldp,
-                                                  float inv2l2, const float* __restrict__ X, int64_t ldx,
-                                                  float* __restrict__ Y, int64_t ldy, int nrhs, float scale,
-                                                  float shift, const int* __restrict__ done) {
+                                                  float inv2l2, float inv_l, const float* __restrict__ X,
+                                                  int64_t ldx, float* __restrict__ Y, int64_t ldy, int nrhs,
+                                                  float scale, float shift, const int* __restrict__ done) {
   if (done && *done) return;
   __shared__ float pj[kRbfTile][kRbfMaxDim];
   __shared__ float xj[kRbfNB][kRbfTile];
@@ -192,7 +209,7 @@ __global__ __launch_bounds__(256) void rbf_kernel(int n, int dim, const float* _
         const float df = pi[d] - pj[jj][d];
         d2 = fmaf(df, df, d2);
       }
-      const float kv = expf(-d2 * inv2l2);
+      const float kv = stationary<FAMILY>(d2, inv2l2, inv_l);
 #pragma unroll
       for (int b = 0; b < kRbfNB; ++b) acc[b] = fmaf(kv, xj[b][jj], acc[b]);
     }
@@ -251,11 +268,30 @@ int woodbury_apply(const ck_op* op, int nrhs, const float* X, int64_t ldx, float
 
 int rbf_apply(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, const int* done,
               hipStream_t s) {
-  if (!op->a || op->dim < 1 || op->dim > kRbfMaxDim || op->lda < op->dim || !(op->aux_f > 0.f)) return CK_ERR_BAD_ARG;
-  const float inv2l2 = 1.f / (2.f * op->aux_f * op->aux_f);
-  hipLaunchKernelGGL(rbf_kernel, dim3((unsigned)cdiv(op->n, 256), (unsigned)cdiv(nrhs, kRbfNB)), dim3(256), 0, s,
-                     op->n, op->dim, static_cast<const float*>(op->a), op->lda, inv2l2, X, ldx, Y, ldy, nrhs,
-                     op->scale, op->shift, done);
+  if (!op->a || op->dim < 1 || op->dim > kRbfMaxDim || op->lda < op->dim || !(op->aux_f > 0.f) || op->rank < 0 ||
+      op->rank > 3)
+    return CK_ERR_BAD_ARG;
+  const float inv2l2 = 1.f / (2.f * op->aux_f * op->aux_f), inv_l = 1.f / op->aux_f;
+  const dim3 grid((unsigned)cdiv(op->n, 256), (unsigned)cdiv(nrhs, kRbfNB));
+  const float* P = static_cast<const float*>(op->a);
+#define CK_RBF(F_)                                                                                               \
+  hipLaunchKernelGGL(rbf_kernel<F_>, grid, dim3(256), 0, s, op->n, op->dim, P, op->lda, inv2l2, inv_l, X, ldx, Y, \
+                     ldy, nrhs, op->scale, op->shift, done)
+  switch (op->rank) {
+    case 0:
+      CK_RBF(0);
+      break;
+    case 1:
+      CK_RBF(1);
+      break;
+    case 2:
+      CK_RBF(2);
+      break;
+    default:
+      CK_RBF(3);
+      break;
+  }
+#undef CK_RBF
   CK_CHECK_LAUNCH();
   return CK_OK;
 }

@@ -662,8 +662,55 @@ class Tridiagonal(LinearOperator):
         return self.to_dense().to(X.dtype) @ X
 
 
+class StationaryKernelFn:
+    """A stationary kernel k(x1_i, x2_j) = g(|x1_i - x2_j|) usable as the ``fn`` of ``Kernel``:
+    called on two point blocks it returns the dense block (torch, as cola's Kernel expects),
+    and its ``ck_family`` lets ``Kernel`` lower to the engine's on-the-fly kernel op
+    (CK_OP_RBF_F32).  Families: 0 RBF exp(-r^2/(2 l^2)), 1 Matern 1/2, 2 Matern 3/2,
+    3 Matern 5/2."""
+
+    def __init__(self, family, lengthscale):
+        if family not in (0, 1, 2, 3):
+            raise ValueError("family must be 0 (RBF), 1, 2 or 3 (Matern 1/2, 3/2, 5/2)")
+        self.ck_family, self.lengthscale = int(family), float(lengthscale)
+
+    def __call__(self, x1, x2):
+        # direct differences: the matmul form of cdist loses the small distances to cancellation
+        r = torch.cdist(x1, x2, compute_mode="donot_use_mm_for_euclid_dist")
+        ell = self.lengthscale
+        if self.ck_family == 0:
+            return torch.exp(-r**2 / (2 * ell * ell))
+        if self.ck_family == 1:
+            return torch.exp(-r / ell)
+        a = (3.0**0.5 if self.ck_family == 2 else 5.0**0.5) * r / ell
+        if self.ck_family == 2:
+            return (1 + a) * torch.exp(-a)
+        return (1 + a + a * a / 3) * torch.exp(-a)
+
+
+def rbf(lengthscale):
+    return StationaryKernelFn(0, lengthscale)
+
+
+def matern12(lengthscale):
+    return StationaryKernelFn(1, lengthscale)
+
+
+def matern32(lengthscale):
+    return StationaryKernelFn(2, lengthscale)
+
+
+def matern52(lengthscale):
+    return StationaryKernelFn(3, lengthscale)
+
+
 class Kernel(LinearOperator):
-    """Blocked on-the-fly kernel matvec [Kv]_i = sum_j f(x1_i, x2_j) v_j (operators.py:609-646)."""
+    """Blocked on-the-fly kernel matvec [Kv]_i = sum_j f(x1_i, x2_j) v_j (operators.py:609-646).
+
+    With x2 the same points as x1 (at most 8 coordinates, fp32 on the device) and ``fn`` a
+    ``StationaryKernelFn`` (``rbf``, ``matern12``, ``matern32``, ``matern52``) the product runs
+    in the engine's kernel, which forms each entry inside the matvec; any other ``fn`` is
+    evaluated block by block as in the reference."""
 
     def __init__(self, x1, x2, fn, block_size1, block_size2):
         self.x1, self.x2, self.fn = x1, x2, fn
@@ -671,7 +718,20 @@ class Kernel(LinearOperator):
         self.device = x1.device
         super().__init__(x1.dtype, (x1.shape[0], x2.shape[0]))
 
+    def hip_op(self):
+        fam = getattr(self.fn, "ck_family", None)
+        x1, x2 = self.x1, self.x2
+        same = x2 is x1 or (x1.shape == x2.shape and x1.data_ptr() == x2.data_ptr() and x1.stride() == x2.stride())
+        if fam is None or not same or not (x1.is_cuda and x1.dtype == torch.float32 and x1.dim() == 2
+                                           and x1.shape[1] <= 8 and x1.stride(1) == 1):
+            return None
+        op = _op(N.CK_OP_RBF_F32, x1.shape[0])
+        op.a, op.lda, op.dim, op.aux_f, op.rank = x1.data_ptr(), x1.stride(0), x1.shape[1], self.fn.lengthscale, fam
+        return op, [x1]
+
     def _matmat(self, V):
+        if self._use_hip(V):
+            return self._hip_apply(V)
         out = torch.zeros((self.shape[0], V.shape[1]), dtype=V.dtype, device=V.device)
         for i0 in range(0, self.shape[0], self.block_size1):
             blk = slice(i0, i0 + self.block_size1)

@@ -66,8 +66,11 @@ const char* ck_status_string(int status);
  *                    s I + F F^T when scale = 1/s and C^-1 = (s I + F^T F)^-1;
  *                    a = F (n x rank, row-major fp32, lda), aux = C^-1 (rank x rank
  *                    fp64); arithmetic after the loads in fp64, rank <= 128
- *   CK_OP_RBF_F32    S_ij = exp(-|p_i - p_j|^2 / (2 l^2)) built on the fly from
- *                    points a (n x dim fp32, row-major, lda) with l = aux_f
+ *   CK_OP_RBF_F32    S_ij = k(|p_i - p_j|) built on the fly from points a (n x dim fp32,
+ *                    row-major, lda, dim <= 8) with lengthscale l = aux_f; rank selects
+ *                    the stationary kernel: 0 exp(-r^2 / (2 l^2)), 1 exp(-r / l) (Matern
+ *                    1/2), 2 (1 + a) exp(-a), a = sqrt(3) r / l (Matern 3/2), 3 (1 + a +
+ *                    a^2/3) exp(-a), a = sqrt(5) r / l (Matern 5/2)
  *                    (replaces Kernel._matmat, cola/ops/operators.py:631-643)
  *   CK_OP_KRON_EIG_F32  S = Q diag(aux) Q^T with Q = factor[0] (x) factor[1] (x) factor[2]
  *                    (three 64x64 orthogonal factors) and factor2[i] = factor[i]^T:

@@ -157,3 +157,29 @@ def test_vjp_derivs_of_operator_products():
     byid = {id(t): gr for t, gr in zip(leaves, grads)}
     assert torch.allclose(byid[id(M)], D @ X.T, atol=1e-5)
     assert torch.allclose(byid[id(c)], (X * D).sum(), atol=1e-5)
+
+
+def test_stationary_kernel_fns_closed_forms():
+    """The kernel callables of Kernel(x1, x2, fn): value 1 at r = 0 and the closed forms at r = l."""
+    import math
+    x = torch.tensor([[0.0, 0.0], [0.3, 0.4]], dtype=torch.float64)  # |x1 - x0| = 0.5
+    ell = 0.5
+    want = {0: math.exp(-0.5), 1: math.exp(-1.0), 2: (1 + math.sqrt(3)) * math.exp(-math.sqrt(3)),
+            3: (1 + math.sqrt(5) + 5.0 / 3.0) * math.exp(-math.sqrt(5))}
+    for fam, fn in ((0, ca.rbf(ell)), (1, ca.matern12(ell)), (2, ca.matern32(ell)), (3, ca.matern52(ell))):
+        K = fn(x, x)
+        assert abs(float(K[0, 0]) - 1.0) < 1e-12 and abs(float(K[0, 1]) - want[fam]) < 1e-12
+    with pytest.raises(ValueError):
+        ca.StationaryKernelFn(7, 1.0)
+
+
+def test_kernel_blocks_match_dense_on_cpu():
+    """Kernel's block loop (operators.py:631-643) on the CPU equals fn on the whole point set; a CPU
+    operator is not lowered to the engine."""
+    x = torch.rand(50, 3, generator=torch.Generator().manual_seed(0))
+    v = torch.randn(50, 2, generator=torch.Generator().manual_seed(1))
+    for fn in (ca.matern32(0.3),
+               lambda a, b: torch.exp(-torch.cdist(a, b, compute_mode="donot_use_mm_for_euclid_dist"))):
+        K = ca.Kernel(x, x, fn, 16, 16)
+        assert K.hip_op() is None
+        assert torch.allclose(K @ v, fn(x, x) @ v, rtol=1e-5, atol=1e-4)
