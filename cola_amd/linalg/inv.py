@@ -1,9 +1,11 @@
 """inv / solve dispatch (cola/linalg/inverse/inv.py:24-165) onto the engine."""
 import numpy as np
+import torch
 
 from ..annotations import PSD, Unitary
 from ..dispatch import dispatch
-from ..ops import BlockDiag, Diagonal, Identity, Kronecker, LinearOperator, Product, ScalarMul
+from ..ops import (BlockDiag, Diagonal, Identity, Kronecker, LinearOperator, Permutation, Product, ScalarMul, Triangular,
+                   TriangularInv)
 from .algorithm_base import Algorithm, Auto, IterativeOperatorWInfo
 from .batched import BatchedInverse, HipBatchedCG
 from .cg import HipCG
@@ -97,6 +99,18 @@ def inv(A: BlockDiag, alg: HipBatchedCG):
 def inv(A: Diagonal, alg: Algorithm):
     """inv.py:144-146."""
     return Diagonal(1. / A.diag)
+
+
+@dispatch
+def inv(A: Permutation, alg: Algorithm):
+    """inv.py:123-125."""
+    return Permutation(torch.argsort(A.perm), A.dtype)
+
+
+@dispatch
+def inv(A: Triangular, alg: Algorithm):
+    """inv.py:149-151: triangular solves (TriangularInv, inv.py:154-165)."""
+    return TriangularInv(A)
 
 
 _ = np

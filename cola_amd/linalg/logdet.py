@@ -2,7 +2,7 @@
 
 Iterative rule (logdet.py:111-117): log(A) through Lanczos, then its trace by
 Hutchinson; plus the structural rules for Kronecker, Diagonal, Identity,
-ScalarMul and square Products.  ``HipSLQ`` is the engine's direct stochastic
+ScalarMul, square Products, BlockDiag, Triangular and Permutation.  ``HipSLQ`` is the engine's direct stochastic
 Lanczos quadrature log-determinant (explicit probe count, FP64 quadrature)."""
 from dataclasses import dataclass
 from typing import Optional
@@ -12,7 +12,7 @@ import torch
 
 from ..annotations import PSD
 from ..dispatch import dispatch
-from ..ops import Diagonal, Identity, Kronecker, LinearOperator, Product, ScalarMul
+from ..ops import BlockDiag, Diagonal, Identity, Kronecker, LinearOperator, Permutation, Product, ScalarMul, Triangular
 from .algorithm_base import Algorithm, Auto
 from .lanczos import HipLanczos
 from .slq import HipSLQ, slq_logdet
@@ -94,3 +94,29 @@ def slogdet(A: Kronecker, log_alg: Algorithm, trace_alg: Algorithm):
     for s_, n_ in zip(signs, sizes):
         out_s = out_s * s_**(prod / n_)
     return out_s, sum(ld_ * prod / n_ for ld_, n_ in zip(lds, sizes))
+
+
+@dispatch
+def slogdet(A: BlockDiag, log_alg: Algorithm, trace_alg: Algorithm):
+    """logdet.py:161-167: sum of the blocks' log-determinants, weighted by multiplicity."""
+    signs, lds = zip(*[slogdet(Ai, log_alg, trace_alg) for Ai in A.Ms])
+    out_s = 1.0
+    for s_, m in zip(signs, A.multiplicities):
+        out_s = out_s * s_**m
+    return out_s, sum(ld_ * m for ld_, m in zip(lds, A.multiplicities))
+
+
+@dispatch
+def slogdet(A: Triangular, log_alg: Algorithm, trace_alg: Algorithm):
+    """logdet.py:170-176: the diagonal's product."""
+    d = torch.diagonal(A.A)
+    mag = torch.abs(d)
+    return torch.prod(d / mag), torch.sum(torch.log(mag))
+
+
+@dispatch
+def slogdet(A: Permutation, log_alg: Algorithm, trace_alg: Algorithm):
+    """logdet.py:179-184: (1, 0), as the reference returns (the permutation's parity is not
+    counted there either)."""
+    zero = torch.zeros((), dtype=A.dtype, device=A.device)
+    return 1. + zero, zero

@@ -24,7 +24,7 @@ import torch
 from .. import _native as N
 from .._blocks import ld_of
 from ..dispatch import dispatch
-from ..ops import Dense, Diagonal, Identity, Kronecker, LinearOperator, ScalarMul, Sum
+from ..ops import BlockDiag, Dense, Diagonal, Identity, Kronecker, KronSum, LinearOperator, ScalarMul, Sum
 from ._engine import apply_blocks, lower, probes, workspace
 from .algorithm_base import Algorithm, Auto
 from .keys import PRNGKey, next_key
@@ -195,6 +195,24 @@ def diag(A: Sum, k: int, alg: Algorithm):
 @dispatch
 def diag(A: ScalarMul, k: int, alg: Algorithm):
     return A.c * diag(Identity(A.shape, A.dtype, A.device), k, alg)
+
+
+@dispatch
+def diag(A: BlockDiag, k: int, alg: Algorithm):
+    """diag_trace.py:87-91: the blocks' diagonals in order, multiplicities repeated (k = 0)."""
+    assert k == 0, "only the main diagonal of a BlockDiag is formed"
+    return torch.cat([diag(M, k, alg) for M, m in zip(A.Ms, A.multiplicities) for _ in range(m)])
+
+
+@dispatch
+def diag(A: KronSum, k: int, alg: Algorithm):
+    """diag_trace.py:112-118: outer sum of the factors' diagonals (k = 0)."""
+    assert k == 0
+    ds = [diag(M, k, alg) for M in A.Ms]
+    out = ds[0]
+    for d in ds[1:]:
+        out = (out[:, None] + d[None, :]).reshape(-1)
+    return out
 
 
 @dispatch

@@ -541,6 +541,75 @@ class Sum(LinearOperator):
         return op, keep
 
 
+class Triangular(Dense):
+    """operators.py:41-45: a dense triangular matrix (lower or upper)."""
+
+    def __init__(self, A, lower=True):
+        super().__init__(A)
+        self.lower = lower
+
+
+class TriangularInv(LinearOperator):
+    """inv.py:154-165: A^-1 for a Triangular A by triangular solves."""
+
+    def __init__(self, A):
+        self.A = A.to_dense()
+        self.lower = A.lower
+        self.device = self.A.device
+        super().__init__(A.dtype, A.shape)
+
+    def _matmat(self, X):
+        return torch.linalg.solve_triangular(self.A.to(X.dtype), X, upper=not self.lower)
+
+    def to_dense(self):
+        return torch.linalg.solve_triangular(self.A, torch.eye(self.shape[0], dtype=self.A.dtype, device=self.device),
+                                             upper=not self.lower)
+
+
+class Permutation(LinearOperator):
+    """operators.py:530-546: (P v)_i = v_{perm[i]}."""
+
+    def __init__(self, perm, dtype=None):
+        self.perm = perm
+        self.device = perm.device
+        super().__init__(torch.float32 if dtype is None else dtype, (len(perm), len(perm)))
+
+    def _matmat(self, v):
+        return v[self.perm]
+
+    def to_dense(self):
+        n = self.shape[0]
+        return torch.eye(n, dtype=self.dtype, device=self.device)[self.perm]
+
+
+class KronSum(LinearOperator):
+    """operators.py:241-273: A (+) B = A (x) I + I (x) B, applied factor by factor."""
+
+    def __init__(self, *Ms):
+        from .fns import lazify
+        self.Ms = tuple(lazify(M) for M in Ms)
+        self.device = self.Ms[0].device
+        shape = (int(np.prod([M.shape[-2] for M in self.Ms])), int(np.prod([M.shape[-1] for M in self.Ms])))
+        dtype = reduce(torch.promote_types, (M.dtype for M in self.Ms))
+        super().__init__(dtype, shape)
+
+    def _matmat(self, v):
+        ev = v.reshape(*[M.shape[-1] for M in self.Ms], -1)
+        out = torch.zeros_like(ev)
+        for i, M in enumerate(self.Ms):
+            front = torch.movedim(ev, i, 0)
+            Mf = (M @ front.reshape(M.shape[-1], -1)).reshape(M.shape[0], *front.shape[1:])
+            out = out + torch.movedim(Mf, 0, i)
+        return out.reshape(self.shape[-2], ev.shape[-1])
+
+    def to_dense(self):
+        def ks(A, B):
+            IA = torch.eye(A.shape[0], dtype=A.dtype, device=A.device)
+            IB = torch.eye(B.shape[0], dtype=B.dtype, device=B.device)
+            return torch.kron(A, IB) + torch.kron(IA, B)
+        return reduce(ks, [M.to_dense() for M in self.Ms])
+
+
 class BlockDiag(LinearOperator):
     """operators.py:278-315: [M_0 0 ...; 0 M_1 ...] with optional multiplicities (block i
     repeated multiplicities[i] times along the diagonal)."""

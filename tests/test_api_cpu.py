@@ -183,3 +183,43 @@ def test_kernel_blocks_match_dense_on_cpu():
         K = ca.Kernel(x, x, fn, 16, 16)
         assert K.hip_op() is None
         assert torch.allclose(K @ v, fn(x, x) @ v, rtol=1e-5, atol=1e-4)
+
+
+def test_structural_rules_triangular_permutation_kronsum_blockdiag():
+    """inv/slogdet/diag rules of inv.py:123-165, logdet.py:161-184 and diag_trace.py:87-118 on the
+    CPU against dense torch results."""
+    from cola_amd.linalg import diag, slogdet
+    g = torch.Generator().manual_seed(0)
+    L = torch.tril(torch.randn(6, 6, generator=g, dtype=torch.float64)) + 4 * torch.eye(6, dtype=torch.float64)
+    T = ca.Triangular(L, lower=True)
+    Ti = inv(T)
+    v = torch.randn(6, 2, generator=g, dtype=torch.float64)
+    assert torch.allclose(L @ (Ti @ v), v)
+    U = ca.Triangular(L.T.contiguous(), lower=False)
+    assert torch.allclose(L.T @ (inv(U) @ v), v)
+    s, ld = slogdet(T)
+    s0, ld0 = torch.linalg.slogdet(L)
+    assert torch.allclose(ld, ld0) and float(s) == float(s0)
+    perm = torch.tensor([2, 0, 3, 1])
+    P = ca.Permutation(perm, torch.float64)
+    w = torch.randn(4, 3, generator=g, dtype=torch.float64)
+    assert torch.equal(P @ w, w[perm]) and torch.equal(inv(P) @ (P @ w), w)
+    assert float(slogdet(P)[1]) == 0.0
+    A = torch.randn(3, 3, generator=g, dtype=torch.float64)
+    B = torch.randn(4, 4, generator=g, dtype=torch.float64)
+    KS = ca.KronSum(ca.Dense(A), ca.Dense(B))
+    dense = torch.kron(A, torch.eye(4, dtype=torch.float64)) + torch.kron(torch.eye(3, dtype=torch.float64), B)
+    x = torch.randn(12, 2, generator=g, dtype=torch.float64)
+    assert torch.allclose(KS @ x, dense @ x) and torch.allclose(KS.to_dense(), dense)
+    assert torch.allclose(diag(KS), torch.diagonal(dense))
+    M1 = torch.randn(2, 2, generator=g, dtype=torch.float64) + 3 * torch.eye(2, dtype=torch.float64)
+    M2 = torch.randn(3, 3, generator=g, dtype=torch.float64) + 3 * torch.eye(3, dtype=torch.float64)
+    BD = ca.BlockDiag(ca.Dense(M1), ca.Dense(M2), multiplicities=[2, 1])
+    D = torch.block_diag(M1, M1, M2)
+    assert torch.allclose(diag(BD), torch.diagonal(D))
+    d1 = torch.tensor([2.0, -3.0], dtype=torch.float64)
+    d2 = torch.tensor([0.5, 4.0, 1.5], dtype=torch.float64)
+    BDd = ca.BlockDiag(ca.Diagonal(d1), ca.Diagonal(d2), multiplicities=[2, 1])
+    sb, lb = slogdet(BDd)
+    s_ref, l_ref = torch.linalg.slogdet(torch.diag(torch.cat([d1, d1, d2])))
+    assert abs(float(lb) - float(l_ref)) < 1e-12 and float(sb) == float(s_ref)
