@@ -24,7 +24,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-Wall", "-Wno
           "-Wno-unused-variable", "-Wno-unused-but-set-variable"]
 DEVICE = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = ["ops.hip", "kron.hip", "cg.hip", "lanczos.hip", "tridiag.hip", "trace.hip", "precond.hip", "mixed.hip",
+SOURCES = ["ops.hip", "kron.hip", "cg.hip", "lanczos.hip", "tridiag.hip", "tridiag_bisect.hip", "trace.hip", "precond.hip", "mixed.hip",
            "learned.hip", "batched.hip", "fused_cg.hip", "gmres.hip", "symeig.hip", "coo.hip", "sketch.hip", "version.cpp"]
 
 

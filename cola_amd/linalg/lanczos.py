@@ -163,22 +163,42 @@ def lanczos(A, start_vector=None, max_iters=100, tol=1e-7, pbar=False, key=None,
     return Q, Tridiagonal(alpha, beta, alpha), run.info
 
 
-def tridiag_eigh(run, vectors=True):
-    """FP64 eigen-decomposition of each probe's T on the device.
+def tridiag_eigh_batch(d, e, vectors=True, method=None):
+    """FP64 eigen-decompositions of nb tridiagonals: d (nb, k) and e (nb, >= k-1) fp32 rows.
 
-    Returns (evals (nb, k) f64 ascending, evecs (nb, k, k) f64 or None, first_row (nb, k) f64)."""
-    k = run.iters
-    nb = run.nb
-    dev = run.device
-    d = run.diag[:, :k].contiguous()
-    e = run.subdiag[:, 1:max(k, 2)].contiguous()
+    ``method`` "bisect" (Sturm bisection + inverse iteration, ck_tridiag_eigh_bisect_f64) or
+    "ql" (implicit QL, ck_tridiag_eigh_f64); by default bisection when eigenvectors are
+    wanted (its vectors are computed in parallel, the QL rotations are serial) and QL for
+    eigenvalues with first components only.
+    Returns (evals (nb, k) f64 ascending, evecs (nb, k, k) f64 or None, first_row (nb, k) f64,
+    status (nb,) int32)."""
+    nb, k = d.shape
+    dev = d.device
+    method = method or ("bisect" if vectors else "ql")
     ev = torch.empty((nb, k), dtype=torch.float64, device=dev)
     vecs = torch.empty((nb, k, k), dtype=torch.float64, device=dev) if vectors else None
     fr = torch.empty((nb, k), dtype=torch.float64, device=dev)
     st = torch.empty(nb, dtype=torch.int32, device=dev)
-    N.call("ck_tridiag_eigh_f64", nb, k, N.ptr(d), d.stride(0), N.ptr(e), e.stride(0), N.ptr(ev), N.ptr(vecs),
-           N.ptr(fr), N.ptr(st), N.stream_ptr(dev))
+    if method == "bisect":
+        tmp = vecs if vectors else torch.empty((nb, k, k), dtype=torch.float64, device=dev)
+        wb = N.q("ck_tridiag_eigh_bisect_workspace_bytes", nb, k)
+        work = torch.empty(wb, dtype=torch.uint8, device=dev)
+        N.call("ck_tridiag_eigh_bisect_f64", nb, k, N.ptr(d), d.stride(0), N.ptr(e), e.stride(0), N.ptr(ev),
+               N.ptr(tmp), N.ptr(fr), N.ptr(st), N.ptr(work), wb, N.stream_ptr(dev))
+    elif method == "ql":
+        N.call("ck_tridiag_eigh_f64", nb, k, N.ptr(d), d.stride(0), N.ptr(e), e.stride(0), N.ptr(ev), N.ptr(vecs),
+               N.ptr(fr), N.ptr(st), N.stream_ptr(dev))
+    else:
+        raise ValueError(f"unknown tridiagonal eigensolver {method!r}")
     return ev, vecs, fr, st
+
+
+def tridiag_eigh(run, vectors=True, method=None):
+    """FP64 eigen-decomposition of each probe's T on the device (tridiag_eigh_batch)."""
+    k = run.iters
+    d = run.diag[:, :k].contiguous()
+    e = run.subdiag[:, 1:max(k, 2)].contiguous()
+    return tridiag_eigh_batch(d, e, vectors, method)
 
 
 def ritz_vectors(run, S, b=0):

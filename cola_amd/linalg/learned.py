@@ -30,7 +30,7 @@ from .. import _native as N
 from .._blocks import ld_of
 from ..ops import Dense, LinearOperator, WoodburyInverse
 from ._engine import apply_blocks, lower, workspace
-from .lanczos import default_start, run_lanczos, tridiag_eigh
+from .lanczos import default_start, run_lanczos, tridiag_eigh, tridiag_eigh_batch
 
 HIDDEN = 128
 
@@ -265,7 +265,7 @@ def _lanczos_instances(seeds, n, din, ell_range, steps, device):
 def make_training_set(seeds, n, m=64, din=4, ell_range=(0.05, 1.0), steps=None, device="cuda", batch=256):
     """Training instances for many seeds: batches of up to ``batch`` instances run one
     Lanczos together (CK_OP_DENSE_BATCHED_F32, every launch serves all of them), and their
-    tridiagonal eigenproblems are solved in one ck_tridiag_eigh_f64 launch."""
+    tridiagonal eigenproblems are solved in one ck_tridiag_eigh_bisect_f64 launch."""
     steps = steps or (2 * m + 32)
     out = []
     seeds = list(seeds)
@@ -298,11 +298,7 @@ def make_training_set(seeds, n, m=64, din=4, ell_range=(0.05, 1.0), steps=None, 
             nb = len(grp)
             d = torch.stack([it[3].diag[it[4], :k] for it in grp]).contiguous()
             e = torch.stack([it[3].subdiag[it[4], 1:max(k, 2)] for it in grp]).contiguous()
-            ev = torch.empty((nb, k), dtype=torch.float64, device=device)
-            vecs = torch.empty((nb, k, k), dtype=torch.float64, device=device)
-            st = torch.empty(nb, dtype=torch.int32, device=device)
-            N.call("ck_tridiag_eigh_f64", nb, k, N.ptr(d), d.stride(0), N.ptr(e), e.stride(0), N.ptr(ev), N.ptr(vecs),
-                   None, N.ptr(st), N.stream_ptr(device))
+            ev, vecs, _, st = tridiag_eigh_batch(d, e, vectors=True)
             if int(st.max()) != 0:
                 raise RuntimeError("tridiagonal eigensolve did not converge")
             mm = min(m, k)

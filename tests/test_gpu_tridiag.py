@@ -1,4 +1,5 @@
-"""Device FP64 implicit-QL tridiagonal eigensolver against LAPACK (scipy)."""
+"""Device FP64 tridiagonal eigensolvers against LAPACK (scipy): implicit QL (ck_tridiag_eigh_f64) and
+Sturm bisection + inverse iteration (ck_tridiag_eigh_bisect_f64)."""
 import numpy as np
 import pytest
 import torch
@@ -65,3 +66,139 @@ def test_tridiag_eigh_split_blocks(dev, m):
     T = np.diag(d[0].astype(np.float64)) + np.diag(e[0].astype(np.float64), 1) + np.diag(e[0].astype(np.float64), -1)
     assert np.abs(T @ V - V * w[None, :]).max() < 1e-11
     assert np.abs(V.T @ V - np.eye(m)).max() < 1e-12
+
+
+def _bisect(dev, d, e):
+    """ck_tridiag_eigh_bisect_f64 on fp32 rows d (nb, m), e (nb, m-1); returns evals, evecs, first_row, status."""
+    nb, m = d.shape
+    dd, ee = torch.from_numpy(d).to(dev), torch.from_numpy(np.ascontiguousarray(e)).to(dev)
+    ev = torch.empty(nb, m, dtype=torch.float64, device=dev)
+    vecs = torch.empty(nb, m, m, dtype=torch.float64, device=dev)
+    fr = torch.empty(nb, m, dtype=torch.float64, device=dev)
+    st = torch.full((nb,), -1, dtype=torch.int32, device=dev)
+    wb = N.q("ck_tridiag_eigh_bisect_workspace_bytes", nb, m)
+    work = torch.empty(wb, dtype=torch.uint8, device=dev)
+    N.call("ck_tridiag_eigh_bisect_f64", nb, m, N.ptr(dd), m, N.ptr(ee), max(e.shape[1], 1), N.ptr(ev), N.ptr(vecs),
+           N.ptr(fr), N.ptr(st), N.ptr(work), wb, N.stream_ptr(dev))
+    torch.cuda.synchronize()
+    return ev.cpu().numpy(), vecs.cpu().numpy(), fr.cpu().numpy(), st.cpu().numpy()
+
+
+def _check_decomposition(d, e, w, V, fr, what):
+    """Eigenvalues against LAPACK to 1e-12 of ||T||, residual ||T v - lambda v|| and orthogonality
+    to 1e-11 of ||T||, the sign rule (largest-magnitude component positive), first_row = V[0]."""
+    m = d.shape[0]
+    d64, e64 = d.astype(np.float64), e.astype(np.float64)
+    T = np.diag(d64) + np.diag(e64, 1) + np.diag(e64, -1)
+    tn = max(np.abs(np.linalg.eigvalsh(T)).max(), 1e-300)
+    wl, _ = eigh_tridiag(d, e)
+    res = np.abs(T @ V - V * w[None, :]).max() / tn
+    orth = np.abs(V.T @ V - np.eye(m)).max()
+    print(f"{what}: m={m} |w - LAPACK|/||T|| {np.abs(w - wl).max() / tn:.1e} residual {res:.1e} "
+          f"orthogonality {orth:.1e}", flush=True)
+    assert np.all(np.diff(w) >= 0)
+    assert np.abs(w - wl).max() <= 1e-12 * tn
+    assert res < 1e-11 and orth < 1e-11
+    # sign rule, up to rounding ties (symmetric vectors have equal-magnitude components)
+    assert np.all(V.max(axis=0) >= np.abs(V).max(axis=0) - 1e-9)
+    assert np.array_equal(fr, V[0])
+
+
+@pytest.mark.parametrize("m,nb", [(1, 1), (2, 3), (10, 4), (30, 32), (120, 1), (120, 6), (200, 2), (700, 1)])
+def test_tridiag_bisect_random(dev, m, nb):
+    """Bisection + inverse iteration on random tridiagonals: against LAPACK, residual and
+    orthogonality bars, and the same eigenvalues as the QL kernel to 1e-12."""
+    rs = np.random.RandomState(100 + m + nb)
+    d = rs.normal(size=(nb, m)).astype(np.float32)
+    e = rs.uniform(0.1, 1.0, size=(nb, max(m - 1, 1))).astype(np.float32)[:, :m - 1]
+    w, V, fr, st = _bisect(dev, d, e if m > 1 else np.zeros((nb, 1), np.float32))
+    assert (st == 0).all()
+    for b in range(nb):
+        _check_decomposition(d[b], e[b], w[b], V[b], fr[b], "random")
+
+
+def _wilkinson(m):
+    """W_m^+: d = |i - (m-1)/2|, e = 1; its largest eigenvalues come in pairs that agree to ~1e-14."""
+    return np.abs(np.arange(m) - (m - 1) / 2.0).astype(np.float32), np.ones(m - 1, np.float32)
+
+
+def _repeated(m):
+    """Diagonal with each value three times and zero off-diagonals: exact triple eigenvalues."""
+    rs = np.random.RandomState(m)
+    return np.repeat(rs.normal(size=(m + 2) // 3), 3)[:m].astype(np.float32), np.zeros(m - 1, np.float32)
+
+
+def _graded(m):
+    """Entries graded over 12 decades: eigenvalues from ~1 down to ~1e-12 of it."""
+    i = np.arange(m)
+    d = 10.0 ** (-12.0 * i / (m - 1))
+    e = 0.1 * 10.0 ** (-12.0 * (i[:-1] + 0.5) / (m - 1))
+    return d.astype(np.float32), e.astype(np.float32)
+
+
+def _split(m):
+    rs = np.random.RandomState(m)
+    d = rs.normal(size=m).astype(np.float32)
+    e = rs.uniform(0.1, 1.0, size=m - 1).astype(np.float32)
+    e[::17] = 0.0
+    e[5::13] = 1e-30
+    return d, e
+
+
+def _lanczos_laplacian(m):
+    """T of m unreorthogonalised Lanczos steps on a 1-D Laplacian (n = 100 < m): converged Ritz
+    values reappear as ghost copies (45 gaps below 1e-8 for m = 250), tight clusters the
+    inverse iteration has to separate."""
+    n = 100
+    A = 2 * np.eye(n) - np.eye(n, k=1) - np.eye(n, k=-1)
+    q = np.random.RandomState(3).normal(size=n)
+    q /= np.linalg.norm(q)
+    qp, beta = np.zeros(n), 0.0
+    al, be = [], []
+    for _ in range(m):
+        w = A @ q - beta * qp
+        a = q @ w
+        w -= a * q
+        beta = np.linalg.norm(w)
+        al.append(a)
+        be.append(beta)
+        qp, q = q, w / beta
+    return np.array(al, np.float32), np.array(be[:-1], np.float32)
+
+
+@pytest.mark.parametrize("kind", ["wilkinson", "repeated", "graded", "split", "lanczos_ghosts"])
+def test_tridiag_bisect_hard_spectra(dev, kind):
+    """Clustered, repeated, graded and split spectra: the cluster rule and the perturbed shifts
+    keep the vectors orthogonal and their residuals at FP64 level."""
+    gen = {"wilkinson": lambda: _wilkinson(41), "repeated": lambda: _repeated(60), "graded": lambda: _graded(50),
+           "split": lambda: _split(127), "lanczos_ghosts": lambda: _lanczos_laplacian(250)}[kind]
+    d, e = gen()
+    w, V, fr, st = _bisect(dev, d[None], e[None])
+    assert st[0] == 0
+    _check_decomposition(d, e, w[0], V[0], fr[0], kind)
+
+
+def test_tridiag_bisect_matches_ql_on_lanczos_run(dev):
+    """The eigenvector solver the Lanczos wrappers use by default against the QL kernel on one
+    FP32 Lanczos tridiagonal (a 64^2-grid Laplacian, 120 steps): eigenvalues to 1e-12 of ||T||,
+    Ritz vectors equal up to sign, to 1e-9."""
+    import cola_amd as ca
+    from cola_amd.linalg.lanczos import run_lanczos, tridiag_eigh
+    from oracle import operators as oop
+    ip, ix, v = oop.laplacian_2d_csr(64, 0.0)
+    A = ca.Sparse.from_csr(torch.from_numpy(ip).to(dev), torch.from_numpy(ix).to(dev), torch.from_numpy(v).to(dev),
+                           (4096, 4096))
+    run = run_lanczos(ca.SelfAdjoint(A), torch.randn(4096, generator=torch.Generator().manual_seed(1)).to(dev), 120,
+                      0.0)
+    wq, Vq, _, sq = tridiag_eigh(run, vectors=True, method="ql")
+    wb, Vb, _, sb = tridiag_eigh(run, vectors=True)
+    assert int(sq.max()) == 0 and int(sb.max()) == 0
+    wq, Vq, wb, Vb = wq[0].cpu().numpy(), Vq[0].cpu().numpy(), wb[0].cpu().numpy(), Vb[0].cpu().numpy()
+    tn = np.abs(wq).max()
+    assert np.abs(wq - wb).max() <= 1e-12 * tn
+    # distinct eigenvalues: vectors agree up to sign
+    gap = np.minimum(np.diff(wq, prepend=-np.inf), np.diff(wq, append=np.inf))
+    ok = gap > 1e-6 * tn
+    sgn = np.sign(np.sum(Vq * Vb, axis=0))
+    print("vector diff", np.abs(Vq * sgn - Vb)[:, ok].max(), "distinct", int(ok.sum()), flush=True)
+    assert np.abs(Vq * sgn - Vb)[:, ok].max() < 1e-9
