@@ -524,7 +524,7 @@ def bench_hutch(args, dev):
             "config": {"workload": "hutch", "estimate": float(tr), "exact": exact,
                        "rel_err": abs(float(tr) - exact) / exact, "expected_std_rel": 361.9 / exact,
                        "algorithmic_bytes": byts, "device_us": t_k * 1e6},
-            "roofline": {"bound": "hbm", "kernel": "trace_partials<SeededWords> + trace_bits_final",
+            "roofline": {"bound": "hbm", "kernel": "trace_stream<SeededWords> + trace_bits_final",
                          "achieved": byts / t_k / 1e9, "peak": HBM_PEAK, "unit": "GB/s",
                          "frac": byts / t_k / 1e9 / HBM_PEAK, "traffic": None},
             **({} if args.no_cpu_baseline else {"cpu_baseline": cpu_baseline_hutch(A)})}

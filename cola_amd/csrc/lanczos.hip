@@ -174,6 +174,16 @@ __device__ __forceinline__ float4 load4_masked(const float* __restrict__ p, int6
   return r;
 }
 
+// elements idx..idx+3 of p: a plain 16-byte load when the caller knows they are all below n
+// (every tile but a vector's last), so no load sits behind a bounds branch that would make
+// the compiler wait for the loads before it.  (The two-sweep kernels keep the masked loads:
+// there the plain form measured 11-17% slower on the 1024^2 Laplacian basis.)
+template <bool FULL>
+__device__ __forceinline__ float4 load4(const float* __restrict__ p, int64_t idx, int64_t n) {
+  if constexpr (FULL) return *reinterpret_cast<const float4*>(p + idx);
+  else return load4_masked(p, idx, n);
+}
+
 // hpart[j * P + blk] = V[:, j]^T w over this block's 1024-element tile, j0 <= j <= j1, with
 // the tile of w staged in wl.  Each lane owns 4 consecutive elements of each 256-element
 // quarter; two basis vectors (j, j+4) per pass keep eight 16-byte loads in flight per lane.
@@ -418,6 +428,141 @@ __global__ __launch_bounds__(256) void lz_apply_project(int64_t n, float* __rest
   project_tile_vec<true>(Vb, ldv, n, i0, j0, j1, wl, hpart + (int64_t)b * M2 * P, P, blk);
 }
 
+// RPL consecutive elements idx.. of p (zero past n): one 16-, 8- or 4-byte load
+template <int RPL, bool FULL>
+__device__ __forceinline__ void load_rows(const float* __restrict__ p, int64_t idx, int64_t n, float (&r)[RPL]) {
+  if constexpr (RPL == 4) {
+    const float4 v = load4<FULL>(p, idx, n);
+    r[0] = v.x, r[1] = v.y, r[2] = v.z, r[3] = v.w;
+  } else if constexpr (RPL == 2) {
+    if (FULL || idx + 1 < n) {
+      const float2 v = *reinterpret_cast<const float2*>(p + idx);
+      r[0] = v.x, r[1] = v.y;
+    } else {
+      r[0] = idx < n ? p[idx] : 0.f;
+      r[1] = 0.f;
+    }
+  } else {
+    r[0] = (FULL || idx < n) ? p[idx] : 0.f;
+  }
+}
+
+// The same two passes as lz_apply_project with one read of the basis: the block's 1024-row
+// tile is walked in sub-tiles of SR rows; each wave takes the columns j0 + wave + 4c
+// (c < 4 JW, so up to 16 JW columns; 32 KB of LDS for the staged rows), adds their share of
+// V h for its lane's RPL = SR / 64 rows and keeps the loaded rows in LDS; after one barrier
+// every wave forms the updated w for its lane's rows (the four shares in wave order), wave 0
+// stores it, and each wave accumulates its columns' dot products from the LDS copy.  The
+// per-lane sums run across the sub-tiles; one wave sum per column at the end gives the
+// tile's partial.  The tiles below n (all but a vector's last) load without bounds masks.
+template <int SR, int JW, bool FULL>
+__device__ __forceinline__ void apply_project_tile(int64_t n, const float* __restrict__ Vb, int64_t ldv,
+                                                   float* __restrict__ w, int64_t i0, int j0, int j1,
+                                                   const float* hs, float (*Vs)[SR], float (*red)[SR],
+                                                   float* __restrict__ hp, int P, int blk) {
+  constexpr int RPL = SR / 64;
+  constexpr int NC = 4 * JW;  // column slots per wave
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // columns of this wave: j0 + wv + 4 c for c < nc
+  const int nc = j0 + wv > j1 ? 0 : (j1 - j0 - wv) / 4 + 1;
+  float acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) acc[c] = 0.f;
+  for (int st = 0; st < kTile / SR; ++st) {
+    const int64_t idx = i0 + st * SR + RPL * lane;
+    float wr[RPL], part[RPL];
+    load_rows<RPL, FULL>(w, idx, n, wr);
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) part[q] = 0.f;
+    // four columns per batch, loads clamped to the last column and issued together
+#pragma unroll
+    for (int c0 = 0; c0 < NC; c0 += 4) {
+      if (c0 >= nc) break;
+      float v[4][RPL];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        load_rows<RPL, FULL>(Vb + (int64_t)min(j0 + wv + 4 * (c0 + u), j1) * ldv, idx, n, v[u]);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        float* dst = &Vs[wv * NC + c0 + u][RPL * lane];
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) dst[q] = v[u][q];
+        const float hv = hs[min(j0 + wv + 4 * (c0 + u), j1)];
+        const float hj = c0 + u < nc ? hv : 0.f;
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) part[q] = fmaf(v[u][q], hj, part[q]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) red[wv][RPL * lane + q] = part[q];
+    __syncthreads();
+    float w1[RPL];
+#pragma unroll
+    for (int q = 0; q < RPL; ++q) {
+      const int r = RPL * lane + q;
+      w1[q] = wr[q] - (((red[0][r] + red[1][r]) + red[2][r]) + red[3][r]);
+    }
+    if (wv == 0) {
+#pragma unroll
+      for (int q = 0; q < RPL; ++q)
+        if (FULL || idx + q < n) w[idx + q] = w1[q];
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      if (c >= nc) break;
+      const float* src = &Vs[wv * NC + c][RPL * lane];
+      float d = 0.f;
+#pragma unroll
+      for (int q = 0; q < RPL; ++q) d = fmaf(src[q], w1[q], d);
+      acc[c] += d;
+    }
+    __syncthreads();  // Vs and red are rewritten by the next sub-tile
+  }
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    if (c >= nc) break;
+    const float sum = wave_sum(acc[c]);
+    if (lane == 0) hp[(int64_t)(j0 + wv + 4 * c) * P + blk] = sum;
+  }
+}
+
+template <int SR, int JW>
+__global__ __launch_bounds__(256) void lz_apply_project_lds(int64_t n, float* __restrict__ V, int64_t ldv,
+                                                            int64_t ldb, int wcol, int j0, int j1,
+                                                            const float* __restrict__ h, int M2,
+                                                            float* __restrict__ hpart, int P,
+                                                            const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ float hs[1056];
+  __shared__ __align__(16) float Vs[16 * JW][SR];
+  __shared__ __align__(16) float red[4][SR];
+  const int b = blockIdx.y, blk = blockIdx.x;
+  for (int j = threadIdx.x; j < M2 && j < 1056; j += 256) hs[j] = h[b * M2 + j];
+  __syncthreads();
+  float* Vb = V + b * ldb;
+  float* w = Vb + (int64_t)wcol * ldv;
+  const int64_t i0 = (int64_t)blk * kTile;
+  float* hp = hpart + (int64_t)b * M2 * P;
+  if (i0 + kTile <= n)
+    apply_project_tile<SR, JW, true>(n, Vb, ldv, w, i0, j0, j1, hs, Vs, red, hp, P, blk);
+  else
+    apply_project_tile<SR, JW, false>(n, Vb, ldv, w, i0, j0, j1, hs, Vs, red, hp, P, blk);
+}
+
+// w -= V[:, j0..j1] h, then the partials of V[:, j0..j1]^T w (the two CGS2 passes between
+// the reductions): one read of the basis for up to 32 columns, else lz_apply_project.
+inline void launch_apply_project(dim3 grid, hipStream_t s, int64_t n, float* V, int64_t ldv, int64_t ldb, int wcol,
+                                 int j0, int j1, const float* h, int M2, float* hpart, int P, const int* done) {
+  // measured on a 1024^2 Laplacian basis: the staged kernel wins up to 32 columns (SLQ's 32-probe
+  // bases of 30 steps: 14% per step); with more columns its smaller sub-tiles and lower occupancy
+  // lose to the two-sweep kernel, whose second sweep partly hits the caches
+  if (j1 - j0 + 1 <= 32)
+    hipLaunchKernelGGL((lz_apply_project_lds<256, 2>), grid, dim3(256), 0, s, n, V, ldv, ldb, wcol, j0, j1, h, M2,
+                       hpart, P, done);
+  else
+    hipLaunchKernelGGL(lz_apply_project, grid, dim3(256), 0, s, n, V, ldv, ldb, wcol, j0, j1, h, M2, hpart, P, done);
+}
+
 // beta_i = ||w||, errors and the loop condition (lanczos.py:251-268)
 __global__ __launch_bounds__(1024) void lz_finalize(int nb, int m, const float* __restrict__ npart, int P,
                                                     float* __restrict__ subdiag, void* state, float tol, int i) {
@@ -631,8 +776,7 @@ int lz_post_full(int64_t n, int nb, float* V, int64_t ldv, int64_t ldb, float* d
   const dim3 rgrid((unsigned)cdiv(i, 4), nb);
   hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, 1, i, sc.h, done);
   if (vec4) {
-    hipLaunchKernelGGL(lz_apply_project, grid, dim3(256), 0, s, n, V, ldv, ldb, i + 1, 1, i, (const float*)sc.h,
-                       sc.M2, sc.hpart, sc.P, done);
+    launch_apply_project(grid, s, n, V, ldv, ldb, i + 1, 1, i, (const float*)sc.h, sc.M2, sc.hpart, sc.P, done);
   } else {
     launch_apply<false>(vec4, grid, s, n, V, ldv, ldb, i + 1, 1, i, sc.h, sc.M2, sc.dpart, sc.P, done);
     launch_project<false>(vec4, grid, s, n, V, ldv, ldb, i, 1, i, sc.dpart, sc.P, sc.hpart, sc.M2, diag, subdiag, m,
@@ -845,8 +989,7 @@ int lz_post_window(int64_t n, int nb, float* V, int64_t ldv, int64_t ldb, float*
   const dim3 rgrid((unsigned)cdiv(i - jlo + 1, 4), nb);
   hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, jlo, i, sc.h, done);
   if (vec4) {
-    hipLaunchKernelGGL(lz_apply_project, grid, dim3(256), 0, s, n, V, ldv, ldb, i + 1, jlo, i, (const float*)sc.h,
-                       sc.M2, sc.hpart, sc.P, done);
+    launch_apply_project(grid, s, n, V, ldv, ldb, i + 1, jlo, i, (const float*)sc.h, sc.M2, sc.hpart, sc.P, done);
   } else {
     launch_apply<false>(vec4, grid, s, n, V, ldv, ldb, i + 1, jlo, i, sc.h, sc.M2, sc.dpart, sc.P, done);
     launch_project<false>(vec4, grid, s, n, V, ldv, ldb, i, jlo, i, sc.dpart, sc.P, sc.hpart, sc.M2, diag, subdiag,
@@ -882,8 +1025,8 @@ int ar_post_full(int64_t n, int nb, float* Q, int64_t ldv, int64_t ldb, float* H
                        sc.hpart, sc.M2, (float*)nullptr, (const float*)nullptr, m, done);
     hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, 0, idx, sc.h, done);
     if (vec4) {
-      hipLaunchKernelGGL(lz_apply_project, grid, dim3(256), 0, s, n, Q, ldv, ldb, idx + 1, 0, idx,
-                         (const float*)sc.h, sc.M2, sc.hpart, sc.P, done);
+      launch_apply_project(grid, s, n, Q, ldv, ldb, idx + 1, 0, idx, (const float*)sc.h, sc.M2, sc.hpart, sc.P,
+                           done);
     } else {
       launch_apply<false>(vec4, grid, s, n, Q, ldv, ldb, idx + 1, 0, idx, sc.h, sc.M2, sc.dpart, sc.P, done);
       launch_project<false>(vec4, grid, s, n, Q, ldv, ldb, idx, 0, idx, sc.dpart, sc.P, sc.hpart, sc.M2,

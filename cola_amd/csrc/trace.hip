@@ -193,6 +193,93 @@ __global__ __launch_bounds__(256) void trace_partials(int n, int64_t nnz, const 
   if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
+// The one-word-group pass with coalesced entry loads: each wave takes 64 consecutive rows,
+// loads their nonzeros [rowptr[r], rowptr[r + 64]) as whole 64-entry lines (column index and
+// value, up to kStreamStage entries) into its own LDS stage, and each lane then walks its
+// row's entries from LDS; rows of a chunk with more entries take trace_row's loads.  The
+// block's kStreamRows rows have their probe words in LDS (one mixed word per row), a column
+// outside them is mixed again or gathered.  Each row's entries are summed in order, rows in
+// round order, then the block sum: one partial per kStreamRows rows.
+constexpr int kStreamRounds = 2;
+constexpr int kStreamRows = 256 * kStreamRounds;
+constexpr int kStreamStage = 512;
+
+template <typename Words>
+__global__ __launch_bounds__(256) void trace_stream(int n, const int32_t* __restrict__ rowptr,
+                                                    const int32_t* __restrict__ colind,
+                                                    const float* __restrict__ val, Words words,
+                                                    double* __restrict__ part) {
+  __shared__ uint64_t wl[kStreamRows];
+  __shared__ int sc[4][kStreamStage];
+  __shared__ float sv[4][kStreamStage];
+  __shared__ double red[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r0 = blockIdx.x * kStreamRows;
+#pragma unroll
+  for (int j = 0; j < kStreamRounds; ++j) {
+    const int row = r0 + j * 256 + threadIdx.x;
+    wl[j * 256 + threadIdx.x] = row < n ? words(0, row) : 0ull;
+  }
+  __syncthreads();
+  auto word = [&](int c) -> uint64_t {
+    const int d = c - r0;
+    return (d >= 0 && d < kStreamRows) ? wl[d] : words(0, c);
+  };
+  double acc = 0.0;
+  for (int round = 0; round < kStreamRounds; ++round) {
+    const int base = r0 + round * 256 + wave * 64;
+    if (base >= n) break;  // wave-uniform: no block barrier follows
+    const int row = base + lane;
+    const int last = min(base + 63, n - 1);
+    const int rc = min(row, last);
+    const int rs = rowptr[rc], re = rowptr[rc + 1];
+    const int p0 = __shfl(rs, 0), p1 = __shfl(re, last - base);
+    const int cnt = p1 - p0;
+    const int len = row <= last ? re - rs : 0;
+    const uint64_t wr = wl[row - r0];
+    double rsum = 0.0;
+    if (cnt <= kStreamStage) {
+      if (cnt > 0) {
+        // every line's loads first (indexes clamped into the chunk), then the LDS writes
+        int cv[kStreamStage / 64];
+        float vv[kStreamStage / 64];
+#pragma unroll
+        for (int t = 0; t < kStreamStage / 64; ++t) {
+          const int k = min(p0 + t * 64 + lane, p1 - 1);
+          cv[t] = colind[k];
+          vv[t] = val[k];
+        }
+#pragma unroll
+        for (int t = 0; t < kStreamStage / 64; ++t) {
+          sc[wave][t * 64 + lane] = cv[t];
+          sv[wave][t * 64 + lane] = vv[t];
+        }
+      }
+      // the stage is this wave's: order its LDS writes before the lanes' reads
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int e0 = rs - p0;
+      for (int e = 0; e < len; ++e) {
+        const int c = sc[wave][e0 + e];
+        const float v = sv[wave][e0 + e];
+        rsum += (double)v * (double)(64 - 2 * (int)__popcll(wr ^ word(c)));
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+      for (int k = rs; k < rs + len; ++k) {
+        const int c = colind[k];
+        rsum += (double)val[k] * (double)(64 - 2 * (int)__popcll(wr ^ word(c)));
+      }
+    }
+    acc += rsum;
+  }
+  acc = block_sum<256>(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
 __global__ __launch_bounds__(256) void trace_bits_final(int nparts, const double* __restrict__ part, double scale,
                                                         double shift_n, double inv_probes, double* __restrict__ out) {
   __shared__ double red[4];
@@ -244,23 +331,39 @@ __global__ __launch_bounds__(64) void slq_kernel(int m, const double* __restrict
 }
 
 // ---------------------------------------------------------------- Ritz vectors
-constexpr int kCombine = 16;
+// out = S^T V for up to 32 columns of S per block: S is staged through LDS as fp32 in slices
+// of 128 basis rows, every thread keeps its row's 32 sums in registers, so V is read once per
+// 32 Ritz vectors and each loaded element feeds 32 FMAs.
+constexpr int kCombine = 32;
+constexpr int kCombineRows = 128;
 
 __global__ __launch_bounds__(256) void basis_combine_kernel(int64_t n, int m, int k, const float* __restrict__ V,
                                                             int64_t ldv, const double* __restrict__ S, int64_t lds_,
                                                             float* __restrict__ out, int64_t ldo) {
+  __shared__ float Ss[kCombineRows][kCombine];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int c0 = blockIdx.y * kCombine;
-  if (i >= n) return;
+  const int64_t ic = i < n ? i : n - 1;
   float acc[kCombine];
 #pragma unroll
   for (int c = 0; c < kCombine; ++c) acc[c] = 0.f;
-  for (int j = 0; j < m; ++j) {
-    const float v = V[(int64_t)j * ldv + i];
+  for (int j0 = 0; j0 < m; j0 += kCombineRows) {
+    const int rows = min(kCombineRows, m - j0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < kCombineRows * kCombine; t += blockDim.x) {
+      const int r = t / kCombine, c = t % kCombine;
+      Ss[r][c] = (r < rows && c0 + c < k) ? (float)S[(int64_t)(j0 + r) * lds_ + c0 + c] : 0.f;
+    }
+    __syncthreads();
+    const float* Vj = V + (int64_t)j0 * ldv + ic;
+#pragma unroll 4
+    for (int r = 0; r < rows; ++r) {
+      const float v = Vj[(int64_t)r * ldv];
 #pragma unroll
-    for (int c = 0; c < kCombine; ++c)
-      if (c0 + c < k) acc[c] = fmaf((float)S[(int64_t)j * lds_ + c0 + c], v, acc[c]);
+      for (int c = 0; c < kCombine; ++c) acc[c] = fmaf(Ss[r][c], v, acc[c]);
+    }
   }
+  if (i >= n) return;
 #pragma unroll
   for (int c = 0; c < kCombine; ++c)
     if (c0 + c < k) out[(int64_t)(c0 + c) * ldo + i] = acc[c];
@@ -345,10 +448,15 @@ int ck_trace_csr_rademacher_bits(const ck_op* A, int32_t ngroups, const uint64_t
     return CK_ERR_BAD_ARG;
   if (!work || work_bytes < ck_trace_workspace_bytes(A->n)) return CK_ERR_WORKSPACE;
   hipStream_t s = (hipStream_t)stream;
-  const int nparts = (int)cdiv(A->n, 256 * kTraceRowsPerThread);
+  const bool stream1 = ngroups == 1;
+  const int nparts = (int)(stream1 ? cdiv(A->n, kStreamRows) : cdiv(A->n, 256 * kTraceRowsPerThread));
   double* part = static_cast<double*>(work);
-  hipLaunchKernelGGL(trace_partials<StoredWords>, dim3(nparts), dim3(256), 0, s, A->n, A->nnz, A->rowptr,
-                     A->colind, A->values, ngroups, StoredWords{W, (int64_t)A->n}, part);
+  if (stream1)
+    hipLaunchKernelGGL(trace_stream<StoredWords>, dim3(nparts), dim3(256), 0, s, A->n, A->rowptr, A->colind, A->values,
+                       StoredWords{W, (int64_t)A->n}, part);
+  else
+    hipLaunchKernelGGL(trace_partials<StoredWords>, dim3(nparts), dim3(256), 0, s, A->n, A->nnz, A->rowptr, A->colind,
+                       A->values, ngroups, StoredWords{W, (int64_t)A->n}, part);
   hipLaunchKernelGGL(trace_bits_final, dim3(1), dim3(256), 0, s, nparts, (const double*)part, (double)A->scale,
                      (double)A->shift * (double)A->n, 1.0 / (64.0 * ngroups), out);
   CK_CHECK_LAUNCH();
@@ -361,10 +469,15 @@ int ck_trace_csr_rademacher_seeded(const ck_op* A, int32_t ngroups, uint64_t see
     return CK_ERR_BAD_ARG;
   if (!work || work_bytes < ck_trace_workspace_bytes(A->n)) return CK_ERR_WORKSPACE;
   hipStream_t s = (hipStream_t)stream;
-  const int nparts = (int)cdiv(A->n, 256 * kTraceRowsPerThread);
+  const bool stream1 = ngroups == 1;
+  const int nparts = (int)(stream1 ? cdiv(A->n, kStreamRows) : cdiv(A->n, 256 * kTraceRowsPerThread));
   double* part = static_cast<double*>(work);
-  hipLaunchKernelGGL(trace_partials<SeededWords>, dim3(nparts), dim3(256), 0, s, A->n, A->nnz, A->rowptr,
-                     A->colind, A->values, ngroups, SeededWords{seed}, part);
+  if (stream1)
+    hipLaunchKernelGGL(trace_stream<SeededWords>, dim3(nparts), dim3(256), 0, s, A->n, A->rowptr, A->colind, A->values,
+                       SeededWords{seed}, part);
+  else
+    hipLaunchKernelGGL(trace_partials<SeededWords>, dim3(nparts), dim3(256), 0, s, A->n, A->nnz, A->rowptr, A->colind,
+                       A->values, ngroups, SeededWords{seed}, part);
   hipLaunchKernelGGL(trace_bits_final, dim3(1), dim3(256), 0, s, nparts, (const double*)part, (double)A->scale,
                      (double)A->shift * (double)A->n, 1.0 / (64.0 * ngroups), out);
   CK_CHECK_LAUNCH();
