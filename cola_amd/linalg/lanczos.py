@@ -166,15 +166,15 @@ def lanczos(A, start_vector=None, max_iters=100, tol=1e-7, pbar=False, key=None,
 def tridiag_eigh_batch(d, e, vectors=True, method=None):
     """FP64 eigen-decompositions of nb tridiagonals: d (nb, k) and e (nb, >= k-1) fp32 rows.
 
-    ``method`` "bisect" (Sturm bisection + inverse iteration, ck_tridiag_eigh_bisect_f64) or
-    "ql" (implicit QL, ck_tridiag_eigh_f64); by default bisection when eigenvectors are
-    wanted (its vectors are computed in parallel, the QL rotations are serial) and QL for
-    eigenvalues with first components only.
+    ``method`` "bisect" (Sturm multisection + inverse iteration, ck_tridiag_eigh_bisect_f64,
+    the default: its eigenvalues and vectors are computed in parallel, 0.1 ms for 32
+    problems of order 30 and 0.6 ms for one of order 120; orders up to 1024) or "ql"
+    (implicit QL, ck_tridiag_eigh_f64, whose rotations are serial: 0.45 and 4.9 ms).
     Returns (evals (nb, k) f64 ascending, evecs (nb, k, k) f64 or None, first_row (nb, k) f64,
     status (nb,) int32)."""
     nb, k = d.shape
     dev = d.device
-    method = method or ("bisect" if vectors else "ql")
+    method = method or ("bisect" if k <= 1024 else "ql")  # the bisection kernel holds T in LDS
     ev = torch.empty((nb, k), dtype=torch.float64, device=dev)
     vecs = torch.empty((nb, k, k), dtype=torch.float64, device=dev) if vectors else None
     fr = torch.empty((nb, k), dtype=torch.float64, device=dev)
