@@ -692,7 +692,9 @@ def bench_learned_cg(args, dev):
                                   f"lengthscale~LogU[0.05,1], noise~LogU[1e-4,1e-1], {B} rhs each, to FP64-checked "
                                   f"{args.tol:g}",
                       "global_batch": total_rhs, "seq_len": n, "parallelism": "single GPU",
-                      "step": f"one pass of refined PCG over all {len(systems)} systems ({total_rhs} rhs)",
+                      "step": f"one pass of refined PCG over all {len(systems)} systems ({total_rhs} rhs), "
+                              f"{args.streams} systems in flight at once",
+                      "streams": args.streams,
                       "rhs_solved": head["solved"], "rhs_total": total_rhs, "inner_tol": itol,
                       "plain_refined_solves_per_s": plain_rate, "plain_refined_rhs_solved": plain["solved"],
                       "ratio_vs_plain": value / plain_rate if plain_rate > 0 else None,
@@ -757,23 +759,24 @@ def bench_sketch(sd, net, n, dev):
 def timed_refined(systems, precs, B, args, itol, reps):
     """FP64-residual refinement around FP32 (P)CG on every system: one set-up call per solver
     (it records the refinement schedule), args.warmup untimed passes, then `reps` timed passes
-    over all systems; every rhs is checked in FP64 afterwards."""
-    from cola_amd.linalg.mixed import HipMixedCG, MixedSolver
+    over all systems, args.streams systems in flight at once (solve_concurrent); every rhs is
+    checked in FP64 afterwards."""
+    from cola_amd.linalg.mixed import HipMixedCG, MixedSolver, solve_concurrent
     solvers = []
     for sd, P in zip(systems, precs):
         alg = HipMixedCG(tol=args.tol, inner_tol=itol, inner_max_iters=1000, max_refinements=args.max_refinements,
                          P=P, chunk=args.chunk)
         solvers.append(MixedSolver(sd["A"], B, alg))
-    for sv, sd in zip(solvers, systems):
-        sv.solve(sd["b"])
+    rhs = [sd["b"] for sd in systems]
+    for sv, b in zip(solvers, rhs):
+        sv.solve(b)
     for _ in range(args.warmup):
-        for sv, sd in zip(solvers, systems):
-            sv.solve(sd["b"])
+        solve_concurrent(solvers, rhs, args.streams)
     torch.cuda.synchronize()
     t = time.perf_counter()
     infos = []
     for _ in range(max(1, reps)):
-        infos = [sv.solve(sd["b"]) for sv, sd in zip(solvers, systems)]
+        infos = solve_concurrent(solvers, rhs, args.streams)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t) / max(1, reps)
     solved, steps, per = 0, 0, []
@@ -1042,6 +1045,8 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="launch the refinement schedule eagerly (no HIP graph)")
     ap.add_argument("--n", type=int, default=16384, help="points per system (learned_cg)")
     ap.add_argument("--systems", type=int, default=8, help="held-out systems (learned_cg)")
+    ap.add_argument("--streams", type=int, default=2,
+                    help="systems solved at once on their own streams (learned_cg; 1 = one after another)")
     ap.add_argument("--max-refinements", type=int, default=30, help="refinement cap (learned_cg)")
     ap.add_argument("--train-instances", type=int, default=2000)
     ap.add_argument("--train-n", type=int, default=1024)

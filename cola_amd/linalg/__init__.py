@@ -14,7 +14,7 @@ from .lanczos import HipLanczos, lanczos, lanczos_eigs
 from .learned import (LearnedPreconditioner, PivotedCholeskyPreconditioner, PointMLP, make_training_instance,
                       rbf_matrix, select_preconditioner, train)
 from .logdet import logdet, slogdet
-from .mixed import HipMixedCG, KroneckerEigInverse, mixed_cg
+from .mixed import HipMixedCG, KroneckerEigInverse, MixedSolver, mixed_cg, solve_concurrent
 from .preconditioning import NystromPrecond, NystromPrecondLazy, get_nys_approx
 from .slq import HipSLQ, stochastic_lanczos_quad
 from .trace import Exact, HipHutch, diag, hutch_trace_csr, hutchinson_diag_estimate, trace
@@ -30,7 +30,7 @@ __all__ = ["Algorithm", "Auto", "IterativeOperatorWInfo", "HipCG", "CG", "cg", "
            "HipLanczos", "Lanczos", "lanczos", "lanczos_eigs", "HipArnoldi", "Arnoldi", "arnoldi", "arnoldi_eigs",
            "eig", "eigmax", "eigmin", "get_slice", "logdet", "slogdet", "HipSLQ", "stochastic_lanczos_quad",
            "HipHutch", "Hutch", "Exact", "diag", "trace", "hutch_trace_csr", "hutchinson_diag_estimate",
-           "HipMixedCG", "KroneckerEigInverse", "mixed_cg", "LanczosUnary", "apply_unary", "log", "exp", "sqrt", "isqrt", "pow", "PRNGKey", "next_key",
+           "HipMixedCG", "KroneckerEigInverse", "MixedSolver", "mixed_cg", "solve_concurrent", "LanczosUnary", "apply_unary", "log", "exp", "sqrt", "isqrt", "pow", "PRNGKey", "next_key",
            "PointMLP", "LearnedPreconditioner", "PivotedCholeskyPreconditioner", "select_preconditioner",
            "make_training_instance", "train", "rbf_matrix",
            "HipBatchedCG", "BatchedSystems", "BatchedMixedSolver", "BatchedPrecond", "autotune",

@@ -14,6 +14,7 @@ with 64x64 symmetric factors: (Q1 x Q2 x Q3) diag(1/(lambda + s)) (Q1 x Q2 x Q3)
 applied with the same MFMA Kronecker passes as the operator (CK_OP_KRON_EIG_F32).
 """
 import ctypes
+import threading
 import time
 from dataclasses import dataclass
 
@@ -377,6 +378,54 @@ def mixed_cg(A, b, alg):
     N.require_device(b, "right-hand side")
     solver = MixedSolver(A, b.shape[1], alg)
     return solver.solve(b)
+
+
+_STREAMS = {}
+
+
+def solve_concurrent(solvers, rhs, streams=2):
+    """``[s.solve(b) for s, b in zip(solvers, rhs)]`` for independent systems, on ``streams``
+    HIP streams at once: one host thread per stream takes the systems k, k + streams, ... in
+    order.  A refinement step of one system is a large matrix product followed by a chain of
+    small latency-bound launches (vector updates, the preconditioner's core, step scalars); with
+    two systems in flight the small launches of one run beside the other's product, whose
+    bandwidth use they barely touch.  Every solver must already have been called once (the
+    first call records its refinement schedule as a graph, which is not done concurrently).
+    The streams keep the default priority and are reused across calls; the caller's stream
+    waits for all of them before this returns.  Returns the list of (x, info)."""
+    if len(solvers) != len(rhs):
+        raise ValueError("one right-hand side block per solver")
+    if streams <= 1 or len(solvers) <= 1:
+        return [sv.solve(b) for sv, b in zip(solvers, rhs)]
+    dev = rhs[0].device
+    key = (str(dev), streams)
+    if key not in _STREAMS:
+        _STREAMS[key] = [torch.cuda.Stream(device=dev) for _ in range(streams)]
+    ss = _STREAMS[key]
+    cur = torch.cuda.current_stream(dev)
+    for st in ss:
+        st.wait_stream(cur)
+    out = [None] * len(solvers)
+    errors = []
+
+    def run(k):
+        try:
+            with torch.cuda.stream(ss[k]):
+                for i in range(k, len(solvers), streams):
+                    out[i] = solvers[i].solve(rhs[i])
+        except BaseException as e:  # re-raised in the caller's thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=run, args=(k,)) for k in range(streams)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for st in ss:
+        cur.wait_stream(st)
+    if errors:
+        raise errors[0]
+    return out
 
 
 _ = (from_blocks, to_blocks, Dense, Identity, ScalarMul)
