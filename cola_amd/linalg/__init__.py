@@ -6,6 +6,7 @@ from .algorithm_base import Algorithm, Auto, IterativeOperatorWInfo
 from .arnoldi import HipArnoldi, arnoldi, arnoldi_eigs
 from .batched import BatchedMixedSolver, BatchedPrecond, BatchedSystems, HipBatchedCG, autotune
 from .cg import HipCG, cg, run_batched_cg
+from .decompositions import LU, Cholesky, cholesky, plu
 from .eig import eig, eigmax, eigmin, get_slice
 from .gmres import HipGMRES, gmres
 from .inv import inv, solve
@@ -26,7 +27,7 @@ Arnoldi = HipArnoldi
 Hutch = HipHutch
 GMRES = HipGMRES
 
-__all__ = ["Algorithm", "Auto", "IterativeOperatorWInfo", "HipCG", "CG", "cg", "HipGMRES", "GMRES", "gmres", "run_batched_cg", "inv", "solve",
+__all__ = ["Algorithm", "Auto", "IterativeOperatorWInfo", "Cholesky", "LU", "cholesky", "plu", "HipCG", "CG", "cg", "HipGMRES", "GMRES", "gmres", "run_batched_cg", "inv", "solve",
            "HipLanczos", "Lanczos", "lanczos", "lanczos_eigs", "HipArnoldi", "Arnoldi", "arnoldi", "arnoldi_eigs",
            "eig", "eigmax", "eigmin", "get_slice", "logdet", "slogdet", "HipSLQ", "stochastic_lanczos_quad",
            "HipHutch", "Hutch", "Exact", "diag", "trace", "hutch_trace_csr", "hutchinson_diag_estimate",

@@ -9,6 +9,7 @@ from ..ops import (BlockDiag, Diagonal, Identity, Kronecker, LinearOperator, Per
 from .algorithm_base import Algorithm, Auto, IterativeOperatorWInfo
 from .batched import BatchedInverse, HipBatchedCG
 from .cg import HipCG
+from .decompositions import LU, Cholesky, adjoint, cholesky, is_small, plu
 from .gmres import HipGMRES
 from .mixed import HipMixedCG
 
@@ -45,11 +46,27 @@ def inv(A: LinearOperator, alg: HipGMRES):
 
 @dispatch(precedence=-1)
 def inv(A: LinearOperator, alg: Auto):
-    """inv.py:73-92 on the engine: PSD operators go to CG and the others to GMRES at every
-    size (the reference's dense Cholesky and LU for N^2 <= 1e6 are outside the engine)."""
+    """inv.py:73-92: N^2 <= 1e6 -> a dense Cholesky (PSD) or LU factorization; larger
+    operators -> CG on the engine (PSD) or GMRES on the engine's Arnoldi kernels."""
+    small = is_small(A)
     if A.isa(PSD):
-        return inv(A, HipCG(**alg.__dict__))
-    return inv(A, HipGMRES(**alg.__dict__))
+        return inv(A, Cholesky() if small else HipCG(**alg.__dict__))
+    return inv(A, LU() if small else HipGMRES(**alg.__dict__))
+
+
+@dispatch(precedence=-1)
+def inv(A: LinearOperator, alg: Cholesky):
+    """inv.py:95-99: inv(L^*) inv(L) for A = L L^*."""
+    assert A.isa(PSD), "Cholesky only valid for PSD matrices, wrap in cola_amd.PSD if desired"
+    L = cholesky(A)
+    return inv(adjoint(L), alg) @ inv(L, alg)
+
+
+@dispatch(precedence=-1)
+def inv(A: LinearOperator, alg: LU):
+    """inv.py:101-104: inv(U) inv(L) inv(P) for A = P L U."""
+    P, L, U = plu(A)
+    return inv(U, alg) @ inv(L, alg) @ inv(P, alg)
 
 
 @dispatch(cond=lambda A, *_: A.isa(Unitary))

@@ -14,6 +14,7 @@ from ..annotations import PSD
 from ..dispatch import dispatch
 from ..ops import BlockDiag, Diagonal, Identity, Kronecker, LinearOperator, Permutation, Product, ScalarMul, Triangular
 from .algorithm_base import Algorithm, Auto
+from .decompositions import LU, Cholesky, cholesky, is_small, plu
 from .lanczos import HipLanczos
 from .slq import HipSLQ, slq_logdet
 from .trace import HipHutch, trace
@@ -33,10 +34,31 @@ def slogdet(A: LinearOperator, log_alg: Algorithm = Auto(), trace_alg: Algorithm
 
 @dispatch(precedence=-1)
 def slogdet(A: LinearOperator, log_alg: Auto, trace_alg: Algorithm):
-    """logdet.py:82-94 restricted to the engine: PSD operators use Lanczos."""
+    """logdet.py:82-94: N^2 <= 1e6 -> a dense Cholesky (PSD) or LU factorization; larger PSD
+    operators -> log(A) by Lanczos on the engine (the reference's Arnoldi branch for large
+    non-PSD operators is not built: NotImplementedError)."""
+    small = is_small(A)
+    if small:
+        return slogdet(A, Cholesky() if A.isa(PSD) else LU(), trace_alg)
     if A.isa(PSD):
         return slogdet(A, HipLanczos(**log_alg.__dict__), trace_alg)
-    raise NotImplementedError("the engine computes log-determinants of PSD operators")
+    raise NotImplementedError("log-determinants of large non-PSD operators (Arnoldi) are not built")
+
+
+@dispatch(precedence=-1)
+def slogdet(A: LinearOperator, log_alg: Cholesky, trace_alg: Algorithm):
+    """logdet.py:97-102: twice the log-determinant of the Cholesky factor."""
+    assert A.isa(PSD), "Cholesky only valid for PSD matrices, wrap in cola_amd.PSD if necessary"
+    sign, ld = slogdet(cholesky(A), log_alg, trace_alg)
+    return sign * torch.conj(torch.as_tensor(sign)), 2 * ld
+
+
+@dispatch(precedence=-1)
+def slogdet(A: LinearOperator, log_alg: LU, trace_alg: Algorithm):
+    """logdet.py:105-108: the product P L U's log-determinant (the permutation contributes
+    (1, 0), as in the reference: the sign ignores its parity)."""
+    P, L, U = plu(A)
+    return slogdet(P @ L @ U, log_alg, trace_alg)
 
 
 @dispatch(precedence=-1)

@@ -98,7 +98,10 @@ def test_inv_routing_and_cpu_refusal():
     A = ca.PSD(ca.Dense(torch.eye(4) * 2))
     Ainv = inv(A, HipCG())
     assert isinstance(Ainv, IterativeOperatorWInfo)
-    assert isinstance(inv(A, Auto(tol=1e-3)).alg, HipCG) and inv(A, Auto(tol=1e-3)).alg.tol == 1e-3
+    # Auto (inv.py:73-92): N^2 <= 1e6 -> dense Cholesky (PSD) / LU, larger -> CG / GMRES
+    big = ca.PSD(ca.Dense(torch.eye(1001) * 2))
+    assert isinstance(inv(big, Auto(tol=1e-3)).alg, HipCG) and inv(big, Auto(tol=1e-3)).alg.tol == 1e-3
+    assert torch.allclose(inv(A, Auto()) @ torch.ones(4), 0.5 * torch.ones(4))
     with pytest.raises(ValueError, match="HIP"):
         Ainv @ torch.ones(4)
     with pytest.raises(AssertionError):
@@ -223,3 +226,42 @@ def test_structural_rules_triangular_permutation_kronsum_blockdiag():
     sb, lb = slogdet(BDd)
     s_ref, l_ref = torch.linalg.slogdet(torch.diag(torch.cat([d1, d1, d2])))
     assert abs(float(lb) - float(l_ref)) < 1e-12 and float(sb) == float(s_ref)
+
+
+def test_auto_small_operators_use_dense_factorizations():
+    """inv.py:73-104 and logdet.py:82-108: below 1e6 entries Auto factorizes densely, Cholesky for
+    PSD operators and P L U otherwise; solves and log-determinants against torch in FP64, the
+    structured factors, and the reference's (1, 0) for the permutation (its parity is not in the
+    sign: an LU whose pivoting swaps an odd number of rows reports the opposite sign)."""
+    from cola_amd.linalg import LU, Cholesky, cholesky, plu, slogdet
+    g = torch.Generator().manual_seed(3)
+    M = torch.randn(60, 60, generator=g, dtype=torch.float64)
+    S = M @ M.T + 60 * torch.eye(60, dtype=torch.float64)
+    b = torch.randn(60, 2, generator=g, dtype=torch.float64)
+    A = ca.PSD(ca.Dense(S))
+    L = cholesky(A)
+    assert isinstance(L, ca.Triangular) and L.lower and torch.allclose(L.A @ L.A.T, S)
+    assert torch.allclose(inv(A) @ b, torch.linalg.solve(S, b))
+    assert torch.allclose(inv(A, Cholesky()) @ b, torch.linalg.solve(S, b))
+    s, ld = slogdet(A)
+    s0, ld0 = torch.linalg.slogdet(S)
+    assert float(s) == float(s0) and abs(float(ld) - float(ld0)) < 1e-10
+    G = ca.Dense(M)
+    P, Lg, U = plu(G)
+    assert isinstance(P, ca.Permutation) and isinstance(Lg, ca.Triangular) and not U.lower
+    assert torch.allclose((P @ Lg @ U).to_dense(), M)
+    assert torch.allclose(inv(G) @ b, torch.linalg.solve(M, b))
+    assert torch.allclose(inv(G, LU()) @ b, torch.linalg.solve(M, b))
+    s, ld = slogdet(G)
+    s0, ld0 = torch.linalg.slogdet(M)
+    assert abs(float(ld) - float(ld0)) < 1e-10
+    Pm = P.to_dense()
+    parity = float(torch.linalg.det(Pm))
+    assert float(s) == float(s0) * parity
+    # structured factors: Diagonal, Kronecker
+    D = ca.PSD(ca.Diagonal(torch.tensor([4., 9.], dtype=torch.float64)))
+    assert torch.allclose(cholesky(D).diag, torch.tensor([2., 3.], dtype=torch.float64))
+    K = ca.Kronecker(ca.PSD(ca.Dense(S[:5, :5])), ca.PSD(ca.Dense(S[5:9, 5:9])))
+    CK = cholesky(K)
+    assert isinstance(CK, ca.Kronecker)
+    assert torch.allclose((CK @ CK.to_dense().T), K.to_dense())

@@ -31,14 +31,17 @@ def nonsym(n, seed, shift=3.0):
 
 def test_gmres_easy_known_answer(dev):
     """test_gmres.py:72-88: diag(3, 4, 5), rhs of ones, 3 steps, tol 1e-8 -> (1/3, 1/4, 1/5) within 1e-6;
-    the same through inv(A, GMRES) and through inv(A, Auto) for an operator that is not PSD."""
+    the same through inv(A, GMRES); inv(A, Auto) takes GMRES for an operator that is not PSD and
+    has more than 1e6 entries (a dense LU below that, inv.py:73-92)."""
     A = ca.Dense(torch.diag(torch.tensor([3., 4., 5.])).to(dev))
     rhs = torch.ones(3, 1, device=dev)
     x, info = gmres(A, rhs, torch.zeros_like(rhs), 3, 1e-8)
     soln = torch.tensor([[1 / 3], [1 / 4], [1 / 5]])
     assert rel(x, soln) < 1e-6
     assert rel(inv(A, GMRES(max_iters=3, tol=1e-8)) @ rhs, soln) < 1e-6
-    assert isinstance(inv(A, Auto()).alg, HipGMRES)
+    assert rel(inv(A, Auto()) @ rhs, soln) < 1e-6
+    big = ca.Dense(nonsym(1001, 0, 4.0).to(dev))
+    assert isinstance(inv(big, Auto()).alg, HipGMRES)
 
 
 def test_gmres_random_spectrum_known_answer(dev):
