@@ -831,6 +831,10 @@ int cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r,
   const bool dense_xy = !fuse_kron && !fuse_wb && A->kind == CK_OP_DENSE_F32 && nrhs >= 3 && nrhs <= 8 && work &&
                         work_bytes >= dense_xy_bytes(n, nrhs) && n % 4 == 0 && A->lda % 4 == 0 && ld % 4 == 0 &&
                         (reinterpret_cast<uintptr_t>(A->a) & 15) == 0 && (reinterpret_cast<uintptr_t>(p) & 15) == 0;
+  // CSR A (rows of at most 32 entries on average): p.Ap partials from the SpMV's epilogue, into
+  // the head of the workspace like dense_xy
+  const bool csr_xy = !fuse_kron && !fuse_wb && !dense_xy && A->kind == CK_OP_CSR_F32 && work &&
+                      work_bytes >= csr_xy_bytes(n, nrhs) && (double)A->nnz <= 32.0 * (double)n;
   WbFused W{};
   if (fuse_wb) W = wb_fused_carve(work, n, nrhs, P->rank);
   for (int it = 0; it < nsteps; ++it) {
@@ -884,6 +888,11 @@ int cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r,
       np0 = dense_xy_nparts(n);
       part0 = static_cast<const float*>(work);
       st = dense_apply_xy(A, nrhs, p, ld, Ap, ld, done, static_cast<float*>(work), s);
+      if (st) return st;
+    } else if (csr_xy) {
+      np0 = csr_xy_nparts(n);
+      part0 = static_cast<const float*>(work);
+      st = csr_apply_xy(A, nrhs, p, ld, Ap, ld, done, static_cast<float*>(work), s);
       if (st) return st;
     } else {
       st = op_apply_internal(A, nrhs, p, ld, Ap, ld, work, work_bytes, done, s);

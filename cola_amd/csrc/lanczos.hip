@@ -162,8 +162,8 @@ inline void launch_scale_col(int64_t n, int nb, float* V, int64_t ldb, int64_t c
 }
 
 // Projection partials h_j = V[:,j] . w over j in [j0, j1] for one tile.
-// FIRST (Lanczos): alpha = sum(dpart), w <- w - (alpha v_i + beta v_{i-1}) first
-// and alpha is written to diag[b][i-1].
+// FIRST (Lanczos): alpha = sum(dpart) (Pd partials per probe), w <- w - (alpha v_i + beta v_{i-1})
+// first and alpha is written to diag[b][i-1].
 // elements idx..idx+3 of p, zero past n (16-byte load when all four are in range)
 __device__ __forceinline__ float4 load4_masked(const float* __restrict__ p, int64_t idx, int64_t n) {
   if (idx + 3 < n) return *reinterpret_cast<const float4*>(p + idx);
@@ -284,7 +284,7 @@ __device__ __forceinline__ float4 apply_store_vec(float* __restrict__ w, int64_t
 
 template <bool FIRST, bool VEC>
 __global__ __launch_bounds__(256) void lz_project(int64_t n, float* __restrict__ V, int64_t ldv, int64_t ldb, int i,
-                                                  int j0, int j1, const float* __restrict__ dpart, int P,
+                                                  int j0, int j1, const float* __restrict__ dpart, int Pd, int P,
                                                   float* __restrict__ hpart, int M2, float* __restrict__ diag,
                                                   const float* __restrict__ subdiag, int m,
                                                   const int* __restrict__ done) {
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(256) void lz_project(int64_t n, float* __restrict__
   float* w = Vb + (int64_t)(i + 1) * ldv;
   if (FIRST) {
     if (threadIdx.x < 64) {
-      const double a = col_total(dpart, P, (int64_t)b * P);
+      const double a = col_total(dpart, Pd, (int64_t)b * Pd);
       if (threadIdx.x == 0) {
         coef[0] = (float)a;
         coef[1] = subdiag[b * (m + 1) + (i - 1)];
@@ -760,8 +760,9 @@ int lz_pre_full(int64_t n, int nb, float* V, int64_t ldv, int64_t ldb, const flo
   return CK_OK;
 }
 
+// xy (optional): the v_i . w partials the matvec wrote (pxy per probe) instead of a tile_partials pass
 int lz_post_full(int64_t n, int nb, float* V, int64_t ldv, int64_t ldb, float* diag, float* subdiag, void* state,
-                 int m, float tol, int step, void* scratch, hipStream_t s) {
+                 int m, float tol, int step, void* scratch, hipStream_t s, const float* xy = nullptr, int pxy = 0) {
   const int i = step + 1;
   if (i > m) return CK_ERR_BAD_ARG;
   const int* done = &static_cast<ck_lz_ctrl*>(state)->done;
@@ -770,17 +771,19 @@ int lz_post_full(int64_t n, int nb, float* V, int64_t ldv, int64_t ldb, float* d
   const float* vi = V + (int64_t)i * ldv;
   dim3 grid(sc.P, nb);
   const bool vec4 = basis_vec4(V, ldv, ldb);
-  hipLaunchKernelGGL(tile_partials<0>, grid, dim3(256), 0, s, n, w, ldb, vi, ldb, sc.dpart, sc.P, done);
-  launch_project<true>(vec4, grid, s, n, V, ldv, ldb, i, 1, i, sc.dpart, sc.P, sc.hpart,
-                     sc.M2, diag, subdiag, m, done);
+  const float* dp = xy ? xy : sc.dpart;
+  const int pd = xy ? pxy : sc.P;
+  if (!xy) hipLaunchKernelGGL(tile_partials<0>, grid, dim3(256), 0, s, n, w, ldb, vi, ldb, sc.dpart, sc.P, done);
+  launch_project<true>(vec4, grid, s, n, V, ldv, ldb, i, 1, i, dp, pd, sc.P, sc.hpart, sc.M2, diag, subdiag, m,
+                       done);
   const dim3 rgrid((unsigned)cdiv(i, 4), nb);
   hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, 1, i, sc.h, done);
   if (vec4) {
     launch_apply_project(grid, s, n, V, ldv, ldb, i + 1, 1, i, (const float*)sc.h, sc.M2, sc.hpart, sc.P, done);
   } else {
     launch_apply<false>(vec4, grid, s, n, V, ldv, ldb, i + 1, 1, i, sc.h, sc.M2, sc.dpart, sc.P, done);
-    launch_project<false>(vec4, grid, s, n, V, ldv, ldb, i, 1, i, sc.dpart, sc.P, sc.hpart, sc.M2, diag, subdiag, m,
-                          done);
+    launch_project<false>(vec4, grid, s, n, V, ldv, ldb, i, 1, i, sc.dpart, sc.P, sc.P, sc.hpart, sc.M2, diag,
+                          subdiag, m, done);
   }
   hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, 1, i, sc.h, done);
   launch_apply<true>(vec4, grid, s, n, V, ldv, ldb, i + 1, 1, i, sc.h, sc.M2, sc.dpart,
@@ -974,7 +977,8 @@ int blk_orthogonalize(int64_t ld, int nb, float* V, int64_t ldv, int64_t ldb, in
 
 // lz_post_full with the re-orthogonalisation restricted to columns [jlo, i]
 int lz_post_window(int64_t n, int nb, float* V, int64_t ldv, int64_t ldb, float* diag, float* subdiag, void* state,
-                   int m, float tol, int step, int jlo, void* scratch, hipStream_t s) {
+                   int m, float tol, int step, int jlo, void* scratch, hipStream_t s, const float* xy = nullptr,
+                   int pxy = 0) {
   const int i = step + 1;
   if (i > m) return CK_ERR_BAD_ARG;
   const int* done = &static_cast<ck_lz_ctrl*>(state)->done;
@@ -983,8 +987,10 @@ int lz_post_window(int64_t n, int nb, float* V, int64_t ldv, int64_t ldb, float*
   const float* vi = V + (int64_t)i * ldv;
   dim3 grid(sc.P, nb);
   const bool vec4 = basis_vec4(V, ldv, ldb);
-  hipLaunchKernelGGL(tile_partials<0>, grid, dim3(256), 0, s, n, w, ldb, vi, ldb, sc.dpart, sc.P, done);
-  launch_project<true>(vec4, grid, s, n, V, ldv, ldb, i, jlo, i, sc.dpart, sc.P, sc.hpart, sc.M2, diag, subdiag, m,
+  const float* dp = xy ? xy : sc.dpart;
+  const int pd = xy ? pxy : sc.P;
+  if (!xy) hipLaunchKernelGGL(tile_partials<0>, grid, dim3(256), 0, s, n, w, ldb, vi, ldb, sc.dpart, sc.P, done);
+  launch_project<true>(vec4, grid, s, n, V, ldv, ldb, i, jlo, i, dp, pd, sc.P, sc.hpart, sc.M2, diag, subdiag, m,
                        done);
   const dim3 rgrid((unsigned)cdiv(i - jlo + 1, 4), nb);
   hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, jlo, i, sc.h, done);
@@ -992,8 +998,8 @@ int lz_post_window(int64_t n, int nb, float* V, int64_t ldv, int64_t ldb, float*
     launch_apply_project(grid, s, n, V, ldv, ldb, i + 1, jlo, i, (const float*)sc.h, sc.M2, sc.hpart, sc.P, done);
   } else {
     launch_apply<false>(vec4, grid, s, n, V, ldv, ldb, i + 1, jlo, i, sc.h, sc.M2, sc.dpart, sc.P, done);
-    launch_project<false>(vec4, grid, s, n, V, ldv, ldb, i, jlo, i, sc.dpart, sc.P, sc.hpart, sc.M2, diag, subdiag,
-                          m, done);
+    launch_project<false>(vec4, grid, s, n, V, ldv, ldb, i, jlo, i, sc.dpart, sc.P, sc.P, sc.hpart, sc.M2, diag,
+                          subdiag, m, done);
   }
   hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, jlo, i, sc.h, done);
   launch_apply<true>(vec4, grid, s, n, V, ldv, ldb, i + 1, jlo, i, sc.h, sc.M2, sc.dpart, sc.P, done);
@@ -1021,7 +1027,7 @@ int ar_post_full(int64_t n, int nb, float* Q, int64_t ldv, int64_t ldb, float* H
     npart = buf[(idx + 1) & 1];
   } else {
     const dim3 rgrid((unsigned)cdiv(idx + 1, 4), nb);
-    launch_project<false>(vec4, grid, s, n, Q, ldv, ldb, idx, 0, idx, sc.dpart, sc.P,
+    launch_project<false>(vec4, grid, s, n, Q, ldv, ldb, idx, 0, idx, sc.dpart, sc.P, sc.P,
                        sc.hpart, sc.M2, (float*)nullptr, (const float*)nullptr, m, done);
     hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, 0, idx, sc.h, done);
     if (vec4) {
@@ -1029,7 +1035,7 @@ int ar_post_full(int64_t n, int nb, float* Q, int64_t ldv, int64_t ldb, float* H
                            done);
     } else {
       launch_apply<false>(vec4, grid, s, n, Q, ldv, ldb, idx + 1, 0, idx, sc.h, sc.M2, sc.dpart, sc.P, done);
-      launch_project<false>(vec4, grid, s, n, Q, ldv, ldb, idx, 0, idx, sc.dpart, sc.P, sc.hpart, sc.M2,
+      launch_project<false>(vec4, grid, s, n, Q, ldv, ldb, idx, 0, idx, sc.dpart, sc.P, sc.P, sc.hpart, sc.M2,
                             (float*)nullptr, (const float*)nullptr, m, done);
     }
     hipLaunchKernelGGL(lz_reduce_h, rgrid, dim3(256), 0, s, sc.hpart, sc.P, sc.M2, 0, idx, sc.h2, done);
@@ -1090,6 +1096,43 @@ int ck_lanczos_post(int64_t n, int32_t nb, int32_t max_iters, float* V, int64_t 
   return lz_post_full(n, nb, V, ldv, ldb, diag, subdiag, state, max_iters, tol, step, scratch, (hipStream_t)stream);
 }
 
+}  // extern "C"
+
+namespace {
+
+// w = A v_i for the nb probes (vector stride ldb); where the operator's kernel has an x . y
+// epilogue (CSR with short rows, 3-factor Kronecker) it also returns the v_i . w partials it
+// wrote (*xy, *pxy per probe) so the step needs no tile_partials pass
+int lz_matvec(const ck_op* A, int nb, const float* vi, float* w, int64_t ldb, void* work, size_t work_bytes,
+              const int* done, const LzScratch& sc, const float** xy, int* pxy, hipStream_t s) {
+  *xy = nullptr;
+  *pxy = 0;
+  const int64_t n = A->n;
+  if (A->kind == CK_OP_CSR_F32 && work && work_bytes >= csr_xy_bytes(n, nb) && (double)A->nnz <= 32.0 * (double)n) {
+    if (int st = csr_apply_xy(A, nb, vi, ldb, w, ldb, done, static_cast<float*>(work), s)) return st;
+    *xy = static_cast<const float*>(work);
+    *pxy = csr_xy_nparts(n);
+    return CK_OK;
+  }
+  if (A->kind == CK_OP_KRON_F32 && A->nfactors == 3) {
+    const int np = kron_nparts(A);
+    if (np > 0 && np <= 2 * sc.P) {  // the partials go to dpart (room for nb x 2P)
+      const int st = kron_dispatch(A, nb, vi, ldb, w, ldb, work, work_bytes, done, sc.dpart, np, s);
+      if (st == CK_OK) {
+        *xy = sc.dpart;
+        *pxy = np;
+        return CK_OK;
+      }
+      if (st != CK_ERR_UNSUPPORTED) return st;
+    }
+  }
+  return op_apply_internal(A, nb, vi, ldb, w, ldb, work, work_bytes, done, s);
+}
+
+}  // namespace
+
+extern "C" {
+
 int ck_lanczos_iterate(const ck_op* A, int64_t n, int32_t nb, int32_t max_iters, float* V, int64_t ldv, int64_t ldb,
                        float* diag, float* subdiag, void* state, float tol, int32_t first_step, int32_t nsteps,
                        void* scratch, void* work, size_t work_bytes, void* stream) {
@@ -1098,15 +1141,18 @@ int ck_lanczos_iterate(const ck_op* A, int64_t n, int32_t nb, int32_t max_iters,
   if (!diag || !subdiag || !state || !scratch) return CK_ERR_BAD_ARG;
   hipStream_t s = (hipStream_t)stream;
   const int* done = &static_cast<ck_lz_ctrl*>(state)->done;
+  const LzScratch sc = scratch_of(scratch, n, nb, max_iters);
   for (int step = first_step; step < first_step + nsteps && step < max_iters; ++step) {
     const int i = step + 1;
     int st = lz_pre_full(n, nb, V, ldv, ldb, subdiag, state, max_iters, step, scratch, s);
     if (st) return st;
     // w = A v_i into column i+1, batched over the nb probes (vector stride ldb)
-    st = op_apply_internal(A, nb, V + (int64_t)i * ldv, ldb, V + (int64_t)(i + 1) * ldv, ldb, work, work_bytes, done,
-                           s);
+    const float* xy;
+    int pxy;
+    st = lz_matvec(A, nb, V + (int64_t)i * ldv, V + (int64_t)(i + 1) * ldv, ldb, work, work_bytes, done, sc, &xy,
+                   &pxy, s);
     if (st) return st;
-    st = lz_post_full(n, nb, V, ldv, ldb, diag, subdiag, state, max_iters, tol, step, scratch, s);
+    st = lz_post_full(n, nb, V, ldv, ldb, diag, subdiag, state, max_iters, tol, step, scratch, s, xy, pxy);
     if (st) return st;
   }
   return CK_OK;
@@ -1133,6 +1179,7 @@ int ck_lanczos_iterate_block(const ck_op* A, int64_t n, int32_t nb, int32_t max_
   hipStream_t s = (hipStream_t)stream;
   const int* done = &static_cast<ck_lz_ctrl*>(state)->done;
   const BlkScratch bs = blk_scratch_of(block_scratch, ld, nb);
+  const LzScratch sc = scratch_of(scratch, n, nb, max_iters);
   const int S = block;
   for (int step = first_step; step < first_step + nsteps && step < max_iters; ++step) {
     const int i = step + 1;
@@ -1143,13 +1190,15 @@ int ck_lanczos_iterate_block(const ck_op* A, int64_t n, int32_t nb, int32_t max_
       st = blk_orthogonalize(ld, nb, V, ldv, ldb, i - S + 1, S, bs, done, s);
       if (st) return st;
     }
-    st = op_apply_internal(A, nb, V + (int64_t)i * ldv, ldb, V + (int64_t)(i + 1) * ldv, ldb, work, work_bytes, done,
-                           s);
+    const float* xy;
+    int pxy;
+    st = lz_matvec(A, nb, V + (int64_t)i * ldv, V + (int64_t)(i + 1) * ldv, ldb, work, work_bytes, done, sc, &xy,
+                   &pxy, s);
     if (st) return st;
     // w (column i + 1) against its own block and the column before it
     const int c0 = (i / S) * S + 1;
     const int jlo = c0 - 1 > 1 ? c0 - 1 : 1;
-    st = lz_post_window(n, nb, V, ldv, ldb, diag, subdiag, state, max_iters, tol, step, jlo, scratch, s);
+    st = lz_post_window(n, nb, V, ldv, ldb, diag, subdiag, state, max_iters, tol, step, jlo, scratch, s, xy, pxy);
     if (st) return st;
   }
   return CK_OK;

@@ -406,23 +406,29 @@ __global__ __launch_bounds__(256) void csr_wave_per_row(int n, const int32_t* __
 // on the path or on the vector count.
 constexpr int kCsrShort = 8;
 
-template <int NB>
-__global__ __launch_bounds__(256) void csr_lane_per_row(int n, const int32_t* __restrict__ rowptr,
+// XY: also xy_part[b * gridDim.x + block] = sum over the block's rows of X[b][row] Y[b][row] (a
+// Krylov step's x . Ax from the matvec itself, instead of a separate pass over both vectors).
+// The XY launch takes 1024 rows per block, so its partials are as few as a 1024-element tile
+// pass would give: every block of the kernel that consumes them reads all of them.
+template <int NB, bool XY = false>
+__global__ __launch_bounds__(XY ? 1024 : 256) void csr_lane_per_row(int n, const int32_t* __restrict__ rowptr,
                                                         const int32_t* __restrict__ colind,
                                                         const float* __restrict__ val, const float* __restrict__ X,
                                                         int64_t ldx, float* __restrict__ Y, int64_t ldy, int nrhs,
-                                                        float scale, float shift, const int* __restrict__ done) {
+                                                        float scale, float shift, const int* __restrict__ done,
+                                                        float* __restrict__ xy_part) {
   if (done && *done) return;
   const int row = blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= n) return;
-  const int k0 = rowptr[row], k1 = rowptr[row + 1];
+  if (!XY && row >= n) return;
+  const int rc = row < n ? row : n - 1;  // XY: rows past n compute a copy of row n - 1 and store nothing
+  const int k0 = rowptr[rc], k1 = rowptr[rc + 1];
   const int len = k1 - k0;
   if (len <= kCsrShort) {
     int c[kCsrShort];
     float v[kCsrShort];
 #pragma unroll
     for (int i = 0; i < kCsrShort; ++i) {
-      c[i] = i < len ? colind[k0 + i] : row;
+      c[i] = i < len ? colind[k0 + i] : rc;
       v[i] = i < len ? val[k0 + i] : 0.f;
     }
     for (int b0 = 0; b0 < nrhs; b0 += NB) {
@@ -441,19 +447,27 @@ __global__ __launch_bounds__(256) void csr_lane_per_row(int n, const int32_t* __
         for (int i = 0; i < kCsrShort; ++i)
           if (i < len) acc = fmaf(v[i], xv[j][i], acc);
         float y = scale * acc;
-        if (shift != 0.f) y = fmaf(shift, X[(int64_t)b * ldx + row], y);
-        Y[(int64_t)b * ldy + row] = y;
+        if (shift != 0.f) y = fmaf(shift, X[(int64_t)b * ldx + rc], y);
+        if (!XY || row < n) Y[(int64_t)b * ldy + row] = y;
       }
     }
-    return;
+  } else {
+    for (int b = 0; b < nrhs; ++b) {
+      const float* xb = X + (int64_t)b * ldx;
+      float acc = 0.f;
+      for (int k = k0; k < k1; ++k) acc = fmaf(val[k], xb[colind[k]], acc);
+      float y = scale * acc;
+      if (shift != 0.f) y = fmaf(shift, xb[rc], y);
+      if (!XY || row < n) Y[(int64_t)b * ldy + row] = y;
+    }
   }
-  for (int b = 0; b < nrhs; ++b) {
-    const float* xb = X + (int64_t)b * ldx;
-    float acc = 0.f;
-    for (int k = k0; k < k1; ++k) acc = fmaf(val[k], xb[colind[k]], acc);
-    float y = scale * acc;
-    if (shift != 0.f) y = fmaf(shift, xb[row], y);
-    Y[(int64_t)b * ldy + row] = y;
+  if constexpr (XY) {
+    __shared__ float red[16];
+    for (int b = 0; b < nrhs; ++b) {
+      const float xy = row < n ? X[(int64_t)b * ldx + row] * Y[(int64_t)b * ldy + row] : 0.f;
+      const float sum = block_sum<1024>(xy, red);
+      if (threadIdx.x == 0) xy_part[(int64_t)b * gridDim.x + blockIdx.x] = sum;
+    }
   }
 }
 
@@ -470,17 +484,44 @@ int csr_dispatch(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* 
     const dim3 grid((unsigned)((n + 255) / 256)), blk(256);
     if (nrhs == 1)
       hipLaunchKernelGGL(csr_lane_per_row<1>, grid, blk, 0, s, n, op->rowptr, op->colind, op->values, X, ldx, Y, ldy,
-                         nrhs, op->scale, op->shift, done);
+                         nrhs, op->scale, op->shift, done, nullptr);
     else if (nrhs == 2)
       hipLaunchKernelGGL(csr_lane_per_row<2>, grid, blk, 0, s, n, op->rowptr, op->colind, op->values, X, ldx, Y, ldy,
-                         nrhs, op->scale, op->shift, done);
+                         nrhs, op->scale, op->shift, done, nullptr);
     else
       hipLaunchKernelGGL(csr_lane_per_row<4>, grid, blk, 0, s, n, op->rowptr, op->colind, op->values, X, ldx, Y, ldy,
-                         nrhs, op->scale, op->shift, done);
+                         nrhs, op->scale, op->shift, done, nullptr);
   }
   CK_CHECK_LAUNCH();
   return CK_OK;
 }
+
+}  // namespace
+
+namespace ck {
+
+int csr_apply_xy(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, const int* done,
+                 float* xy_part, hipStream_t s) {
+  const int n = op->n;
+  if (op->kind != CK_OP_CSR_F32 || !op->rowptr || !op->colind || !op->values || !xy_part) return CK_ERR_BAD_ARG;
+  if (n > 0 && (double)op->nnz / n > 32.0) return CK_ERR_UNSUPPORTED;  // the wave-per-row kernel has no epilogue
+  const dim3 grid((unsigned)csr_xy_nparts(n)), blk(1024);
+  if (nrhs == 1)
+    hipLaunchKernelGGL((csr_lane_per_row<1, true>), grid, blk, 0, s, n, op->rowptr, op->colind, op->values, X, ldx, Y,
+                       ldy, nrhs, op->scale, op->shift, done, xy_part);
+  else if (nrhs == 2)
+    hipLaunchKernelGGL((csr_lane_per_row<2, true>), grid, blk, 0, s, n, op->rowptr, op->colind, op->values, X, ldx, Y,
+                       ldy, nrhs, op->scale, op->shift, done, xy_part);
+  else
+    hipLaunchKernelGGL((csr_lane_per_row<4, true>), grid, blk, 0, s, n, op->rowptr, op->colind, op->values, X, ldx, Y,
+                       ldy, nrhs, op->scale, op->shift, done, xy_part);
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
+}  // namespace ck
+
+namespace {
 
 // ============================================================== diagonal / identity
 __global__ __launch_bounds__(256) void diag_kernel(int64_t n, int nrhs, const float* __restrict__ d,
@@ -762,6 +803,9 @@ size_t op_workspace_internal(const ck_op* op, int nrhs) {
     case CK_OP_DENSE_F32:
       // room for the p.Ap partials a CG step takes from the matvec epilogue (dense_apply_xy)
       return (nrhs >= 3 && nrhs <= 8) ? dense_xy_bytes(op->n, nrhs) : 0;
+    case CK_OP_CSR_F32:
+      // the x.Ax partials of csr_apply_xy (CG's p.Ap, Lanczos's v.Av)
+      return csr_xy_bytes(op->n, nrhs);
     default:
       return 0;
   }

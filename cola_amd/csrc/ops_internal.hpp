@@ -33,6 +33,13 @@ inline size_t dense_xy_bytes(int64_t n, int nrhs) { return ((size_t)4 * nrhs * d
 int dense_apply_xy(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, const int* done,
                    float* xy_part, hipStream_t s);
 
+// CSR fp32 apply (average row length <= 32) that also writes per-block partial sums of X . Y:
+// csr_xy_nparts(n) partials per rhs; CK_ERR_UNSUPPORTED for longer rows.
+inline int csr_xy_nparts(int64_t n) { return (int)((n + 1023) / 1024); }
+inline size_t csr_xy_bytes(int64_t n, int nrhs) { return ((size_t)4 * nrhs * csr_xy_nparts(n) + 255) / 256 * 256; }
+int csr_apply_xy(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, const int* done,
+                 float* xy_part, hipStream_t s);
+
 int woodbury_apply(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, void* work,
                    size_t work_bytes, const int* done, hipStream_t s);
 size_t woodbury_workspace_bytes(const ck_op* op, int nrhs);

@@ -60,6 +60,8 @@ __device__ Bounds load_problem(int m, const float* din, int64_t ldd, const float
     lo = fmin(lo, d[i] - rad);
     hi = fmax(hi, d[i] + rad);
     emax = fmax(emax, e2[i]);
+    // fmin and fmax skip a NaN: a non-finite entry makes the interval infinite instead
+    if (!isfinite(d[i]) || !isfinite(e[i])) lo = -INFINITY;
   }
   for (int o = 32; o > 0; o >>= 1) {
     lo = fmin(lo, __shfl_xor(lo, o));

@@ -202,3 +202,16 @@ def test_tridiag_bisect_matches_ql_on_lanczos_run(dev):
     sgn = np.sign(np.sum(Vq * Vb, axis=0))
     print("vector diff", np.abs(Vq * sgn - Vb)[:, ok].max(), "distinct", int(ok.sum()), flush=True)
     assert np.abs(Vq * sgn - Vb)[:, ok].max() < 1e-9
+
+
+def test_tridiag_bisect_nonfinite_input_reports_nan(dev):
+    """A NaN or infinite entry (in d or e) gives CK_ERR_NAN for that problem and leaves the
+    others as they are."""
+    rs = np.random.RandomState(5)
+    d = rs.normal(size=(3, 20)).astype(np.float32)
+    e = rs.uniform(0.1, 1.0, size=(3, 19)).astype(np.float32)
+    d[1, 7] = np.nan
+    e[2, 3] = np.inf
+    w, V, fr, st = _bisect(dev, d, e)
+    assert st[0] == 0 and st[1] == N.CK_ERR_NAN and st[2] == N.CK_ERR_NAN
+    _check_decomposition(d[0], e[0], w[0], V[0], fr[0], "finite neighbour")
