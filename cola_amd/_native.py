@@ -146,8 +146,7 @@ _PROTOS = {
     "ck_arnoldi_iterate": (ctypes.c_int, [_op, _i64, _i32, _i32, _p, _i64, _i64, _p, _p, _f32, _i32, _i32, _i32, _p,
                                           _p, _sz, _p]),
     "ck_tridiag_eigh_f64": (ctypes.c_int, [_i32, _i32, _p, _i64, _p, _i64, _p, _p, _p, _p, _p]),
-    "ck_tridiag_eigh_bisect_workspace_bytes": (_sz, [_i32, _i32]),
-    "ck_tridiag_eigh_bisect_f64": (ctypes.c_int, [_i32, _i32, _p, _i64, _p, _i64, _p, _p, _p, _p, _p, _sz, _p]),
+    "ck_tridiag_eigh_bisect_f64": (ctypes.c_int, [_i32, _i32, _p, _i64, _p, _i64, _p, _p, _p, _p, _p]),
     "ck_basis_combine": (ctypes.c_int, [_i64, _i32, _i32, _p, _i64, _p, _i64, _p, _i64, _p]),
     "ck_basis_combine_batched": (ctypes.c_int, [_i64, _i32, _i32, _p, _i64, _i64, _p, _i64, _p, _i64, _p]),
     "ck_slq_quadrature": (ctypes.c_int, [_i32, _i32, _p, _p, _p, _i32, _f64, _i32, _p, _p]),

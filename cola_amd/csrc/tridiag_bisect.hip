@@ -22,9 +22,9 @@
 //     by one thread: a member within 10 eps ||T|| of the previous one is shifted by that
 //     much, and after every solve it is orthogonalised against the cluster's earlier vectors
 //     (modified Gram-Schmidt).  This is the inverse iteration of Peters & Wilkinson with the
-//     clustering rule LAPACK's dstein documents.  One wave per 64 eigenvalues; the
-//     per-thread vectors and factors live in global scratch with the thread index fastest,
-//     so a wave touches 64 consecutive doubles per row, and loads run eight rows ahead.
+//     clustering rule LAPACK's dstein documents.  One wave per block, whose first tpb lanes
+//     take tpb eigenvalues: their factors and vectors (4 m doubles each) live in 64 KB of
+//     LDS with the lane index fastest, so tpb = 16 at m = 120 and 2 at m = 1024.
 // Outputs use ck_tridiag_eigh_f64's layout: ascending evals, evecs[j * m + c] = component j
 // of vector c (its first largest-magnitude component positive), first_row[c] = evecs[c].
 #include <float.h>
@@ -236,12 +236,25 @@ __device__ double solve_shifted(int m, const double* d, const double* e, double 
   return nrm2;
 }
 
-// one wave per 64 eigenvalues; grid (ceil(m / 64), nb)
-__global__ __launch_bounds__(64) void invit_kernel(int m, const float* __restrict__ din, int64_t ldd,
+// the per-thread factors and vectors of invit_kernel: 4 m doubles per thread in 64 KB of LDS
+constexpr int kInvitLds = 8192;
+
+// threads per block with vectors for invit_kernel: the largest power of two <= 64 whose scratch fits
+inline int invit_tpb(int m) {
+  const int fit = kInvitLds / (4 * m);
+  int t = 1;
+  while (2 * t <= fit && 2 * t <= 64) t *= 2;
+  return t;
+}
+
+// one wave per block; its first tpb lanes take the eigenvalues blockIdx.x * tpb + lane; grid
+// (ceil(m / tpb), nb)
+__global__ __launch_bounds__(64) void invit_kernel(int m, int tpb, const float* __restrict__ din, int64_t ldd,
                                                    const float* __restrict__ ein, int64_t lde,
                                                    const double* __restrict__ evals, double* __restrict__ evecs,
-                                                   double* __restrict__ first_row, double* __restrict__ work) {
+                                                   double* __restrict__ first_row) {
   __shared__ double d[kMaxM], e[kMaxM], e2[kMaxM], lam[kMaxM], red[12];
+  __shared__ double Ws[kInvitLds];
   const int b = blockIdx.y, tid = threadIdx.x;
   for (int i = tid; i < m; i += 64) lam[i] = evals[(int64_t)b * m + i];
   Bounds B = load_problem(m, din, ldd, ein, lde, b, d, e, e2, red);
@@ -249,12 +262,12 @@ __global__ __launch_bounds__(64) void invit_kernel(int m, const float* __restric
   const double tnorm = fmax(fabs(lam[0]), fabs(lam[m - 1]));
   const double ortol = 1e-3 * tnorm;
   const double tiny = DBL_EPSILON * tnorm + B.pivmin;
-  const int64_t ts = (int64_t)gridDim.x * 64;
-  const int j0 = blockIdx.x * 64 + tid;
+  const int64_t ts = tpb;
+  const int j0 = blockIdx.x * tpb + tid;
   double* Z = evecs + (int64_t)b * m * m;
-  double* W = work + (int64_t)b * 4 * m * ts + j0;
+  double* W = Ws + tid;
   double* X = W + (int64_t)3 * m * ts;
-  if (j0 >= m || (j0 > 0 && lam[j0] - lam[j0 - 1] <= ortol)) return;  // not the first of a cluster
+  if (tid >= tpb || j0 >= m || (j0 > 0 && lam[j0] - lam[j0 - 1] <= ortol)) return;  // not a cluster's first
   int j1 = j0 + 1;
   while (j1 < m && lam[j1] - lam[j1 - 1] <= ortol) ++j1;
   double prev = 0.0;
@@ -329,24 +342,15 @@ __global__ __launch_bounds__(64) void invit_kernel(int m, const float* __restric
   }
 }
 
-inline int64_t invit_threads(int32_t m) { return ((int64_t)m + 63) / 64 * 64; }
-
 }  // namespace
 
 extern "C" {
 
-size_t ck_tridiag_eigh_bisect_workspace_bytes(int32_t nb, int32_t m) {
-  if (nb < 1 || m < 1) return 0;
-  return sizeof(double) * (size_t)nb * 4 * (size_t)m * (size_t)invit_threads(m);
-}
-
 int ck_tridiag_eigh_bisect_f64(int32_t nb, int32_t m, const float* d, int64_t ldd, const float* e, int64_t lde,
-                               double* evals, double* evecs, double* first_row, int32_t* status_out, void* work,
-                               size_t work_bytes, void* stream) {
+                               double* evals, double* evecs, double* first_row, int32_t* status_out, void* stream) {
   if (nb < 1 || m < 1 || m > kMaxM || nb > 65535 || !d || !evals || ldd < m || (m > 1 && (!e || lde < m - 1)) ||
       (first_row && !evecs))
     return CK_ERR_BAD_ARG;
-  if (evecs && (!work || work_bytes < ck_tridiag_eigh_bisect_workspace_bytes(nb, m))) return CK_ERR_WORKSPACE;
   const float* ep = e ? e : d;
   hipStream_t s = (hipStream_t)stream;
   // a wave per eigenvalue while that still leaves few waves in total; 16 lanes otherwise
@@ -359,9 +363,9 @@ int ck_tridiag_eigh_bisect_f64(int32_t nb, int32_t m, const float* d, int64_t ld
   }
   CK_CHECK_LAUNCH();
   if (evecs) {
-    const dim3 grid((unsigned)(invit_threads(m) / 64), (unsigned)nb);
-    hipLaunchKernelGGL(invit_kernel, grid, dim3(64), 0, s, (int)m, d, ldd, ep, lde, evals, evecs, first_row,
-                       static_cast<double*>(work));
+    const int tpb = invit_tpb(m);
+    const dim3 grid((unsigned)((m + tpb - 1) / tpb), (unsigned)nb);
+    hipLaunchKernelGGL(invit_kernel, grid, dim3(64), 0, s, (int)m, tpb, d, ldd, ep, lde, evals, evecs, first_row);
     CK_CHECK_LAUNCH();
   }
   return CK_OK;

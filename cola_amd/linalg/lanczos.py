@@ -181,10 +181,8 @@ def tridiag_eigh_batch(d, e, vectors=True, method=None):
     st = torch.empty(nb, dtype=torch.int32, device=dev)
     if method == "bisect":
         tmp = vecs if vectors else torch.empty((nb, k, k), dtype=torch.float64, device=dev)
-        wb = N.q("ck_tridiag_eigh_bisect_workspace_bytes", nb, k)
-        work = torch.empty(wb, dtype=torch.uint8, device=dev)
         N.call("ck_tridiag_eigh_bisect_f64", nb, k, N.ptr(d), d.stride(0), N.ptr(e), e.stride(0), N.ptr(ev),
-               N.ptr(tmp), N.ptr(fr), N.ptr(st), N.ptr(work), wb, N.stream_ptr(dev))
+               N.ptr(tmp), N.ptr(fr), N.ptr(st), N.stream_ptr(dev))
     elif method == "ql":
         N.call("ck_tridiag_eigh_f64", nb, k, N.ptr(d), d.stride(0), N.ptr(e), e.stride(0), N.ptr(ev), N.ptr(vecs),
                N.ptr(fr), N.ptr(st), N.stream_ptr(dev))
