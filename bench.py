@@ -41,6 +41,24 @@ FP32_MFMA_PEAK = 157.3  # TFLOP/s
 # plus WRITE_SIZE (x1.0), the factors a 1 GiB stream copy reads on the same counters
 # (profiles/dense_matvec_16384x8_hbm_traffic.md)
 DENSE8_TRAFFIC_BYTES = (526427.07 * 2.0 + 512.42) * 1024
+# HBM bytes per symmetric 8-rhs product (dense_sym_kernel<8> + dense_sym_reduce<8>) on the same matrix,
+# same counters and factors (profiles/dense_sym_matvec_16384x8_hbm_traffic.md); None until measured
+DENSE8_SYM_TRAFFIC_BYTES = None
+
+
+def dense_product_roof(op, n, B, t):
+    """Roofline object of one dense n x n, B-rhs product of `op` timed at t seconds per apply: the
+    symmetric product (CK_OP_SYMMETRIC) reads the 128-column strips on and above the diagonal
+    (n (n + 128) / 2 entries), the full product all n^2; both read X and write Y once."""
+    sym = bool(op.flags & N.CK_OP_SYMMETRIC)
+    mat = 4 * n * (n + 128) // 2 if sym else 4 * n * n
+    nbytes = mat + 2 * 4 * n * B
+    kernel = (f"dense_sym_kernel<8> + dense_sym_reduce<8> (upper 128-column strips, {B} rhs)" if sym
+              else f"dense_lds_kernel<8,4> ({B} rhs)")
+    traffic = DENSE8_SYM_TRAFFIC_BYTES if sym else DENSE8_TRAFFIC_BYTES
+    return {"bound": "hbm", "kernel": kernel, "achieved": nbytes / t / 1e9, "peak": HBM_PEAK, "unit": "GB/s",
+            "frac": nbytes / t / 1e9 / HBM_PEAK, "traffic": traffic, "launch_us": t * 1e6,
+            "algorithmic_bytes_per_launch": nbytes}
 
 
 def rbf_factor(t, ell):
@@ -678,10 +696,8 @@ def bench_learned_cg(args, dev):
     work = torch.empty(max(wb, 4) // 4, device=dev)
     t_mv = time_kernel(lambda: N.call("ck_op_apply", op, B, N.ptr(Xb), ld, N.ptr(Yb), ld, N.ptr(work), wb,
                                       N.stream_ptr(dev)), reps=20)
-    mv_bytes = 4 * n * n + 2 * 4 * n * B
-    roof = {"bound": "hbm", "kernel": "dense_lds_kernel<8,4> (K + noise I, 8 rhs)", "achieved": mv_bytes / t_mv / 1e9,
-            "peak": HBM_PEAK, "unit": "GB/s", "frac": mv_bytes / t_mv / 1e9 / HBM_PEAK,
-            "traffic": DENSE8_TRAFFIC_BYTES, "launch_us": t_mv * 1e6, "algorithmic_bytes_per_launch": mv_bytes}
+    roof = dense_product_roof(op, n, B, t_mv)
+    roof["kernel"] += ", K + noise I"
     sketch = bench_sketch(systems[0], net, n, dev)
     out = {"metric": METRIC, "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": head["s"] * 1e3, "higher_is_better": True, "scaling": "weak",
@@ -862,14 +878,20 @@ def bench_dense_matvec(args, dev):
     reps = max(args.steps, 10)
     t = time_kernel(lambda: N.call("ck_op_apply", op, B, N.ptr(Xb), ld, N.ptr(Yb), ld, N.ptr(work), wb,
                                    N.stream_ptr(dev)), reps=reps)
-    nbytes = 4 * n * n + 2 * 4 * n * B
-    return {"metric": "dense 8-rhs matvec bandwidth (K + noise I, K 16384^2 fp32)", "value": nbytes / t / 1e9,
-            "unit": "GB/s", "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": t * 1e3, "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+    roof = dense_product_roof(op, n, B, t)
+    # the same product without the symmetry flag (the whole matrix read), for comparison
+    full = N.CkOp.from_buffer_copy(op)
+    full.flags = 0
+    t_full = time_kernel(lambda: N.call("ck_op_apply", full, B, N.ptr(Xb), ld, N.ptr(Yb), ld, N.ptr(work), wb,
+                                        N.stream_ptr(dev)), reps=reps)
+    return {"metric": "dense 8-rhs matvec products/s (K + noise I, K 16384^2 fp32)", "value": 1.0 / t,
+            "unit": "products/s", "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": t * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": f"dense_matvec: {n}^2 fp32 RBF matrix + noise I, {B} rhs",
-                       "algorithmic_bytes_per_launch": nbytes},
-            "roofline": {"bound": "hbm", "achieved": nbytes / t / 1e9, "peak": HBM_PEAK, "unit": "GB/s",
-                         "frac": nbytes / t / 1e9 / HBM_PEAK, "traffic": DENSE8_TRAFFIC_BYTES}}
+                       "algorithmic_bytes_per_launch": roof["algorithmic_bytes_per_launch"],
+                       "full_product_us": t_full * 1e6,
+                       "full_product_gbs": (4 * n * n + 2 * 4 * n * B) / t_full / 1e9},
+            "roofline": roof}
 
 
 def gp_systems(nsys, dev, bf16=True):

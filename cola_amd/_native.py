@@ -15,6 +15,7 @@ _LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "lib
  CK_ERR_BREAKDOWN, CK_ERR_NAN, CK_ERR_TIMEOUT) = range(10)
 (CK_OP_IDENTITY, CK_OP_DENSE_F32, CK_OP_DENSE_BF16, CK_OP_CSR_F32, CK_OP_KRON_F32, CK_OP_DIAG_F32,
  CK_OP_WOODBURY_INV, CK_OP_RBF_F32, CK_OP_KRON_EIG_F32, CK_OP_DENSE_BATCHED_F32) = range(10)
+CK_OP_SYMMETRIC = 1  # ck_op.flags
 CK_CG_MULT, CK_CG_TOL, CK_CG_GAMMA, CK_CG_ALPHA, CK_CG_BETA, CK_CG_RNORM, CK_CG_ERRORS = range(7)
 CTRL_BYTES = 64
 
@@ -51,7 +52,8 @@ class CkOp(ctypes.Structure):
         ("rank", ctypes.c_int32),
         ("dim", ctypes.c_int32),
         ("aux_f", ctypes.c_float),
-        ("reserved", ctypes.c_int32 * 5),
+        ("flags", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 4),
     ]
 
 
@@ -146,7 +148,8 @@ _PROTOS = {
     "ck_arnoldi_iterate": (ctypes.c_int, [_op, _i64, _i32, _i32, _p, _i64, _i64, _p, _p, _f32, _i32, _i32, _i32, _p,
                                           _p, _sz, _p]),
     "ck_tridiag_eigh_f64": (ctypes.c_int, [_i32, _i32, _p, _i64, _p, _i64, _p, _p, _p, _p, _p]),
-    "ck_tridiag_eigh_bisect_f64": (ctypes.c_int, [_i32, _i32, _p, _i64, _p, _i64, _p, _p, _p, _p, _p]),
+    "ck_tridiag_eigh_bisect_workspace_bytes": (_sz, [_i32, _i32]),
+    "ck_tridiag_eigh_bisect_f64": (ctypes.c_int, [_i32, _i32, _p, _i64, _p, _i64, _p, _p, _p, _p, _p, _sz, _p]),
     "ck_basis_combine": (ctypes.c_int, [_i64, _i32, _i32, _p, _i64, _p, _i64, _p, _i64, _p]),
     "ck_basis_combine_batched": (ctypes.c_int, [_i64, _i32, _i32, _p, _i64, _i64, _p, _i64, _p, _i64, _p]),
     "ck_slq_quadrature": (ctypes.c_int, [_i32, _i32, _p, _p, _p, _i32, _f64, _i32, _p, _p]),

@@ -843,7 +843,7 @@ int cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r,
     if (fuse_wb) {
       const float* F = static_cast<const float*>(P->a);
       const int k = P->rank;
-      st = dense_apply_xy(A, nrhs, p, ld, Ap, ld, done, W.pap, s);
+      st = dense_apply_xy(A, nrhs, p, ld, Ap, ld, done, W.pap, work, work_bytes, s);
       if (st) return st;
       if (last && x64) {
         dim3 grid(q.P, nrhs);
@@ -887,7 +887,7 @@ int cg_iterate(const ck_op* A, const ck_op* P, int32_t nrhs, float* x, float* r,
       // which reuses it, runs after the x/r update has read them)
       np0 = dense_xy_nparts(n);
       part0 = static_cast<const float*>(work);
-      st = dense_apply_xy(A, nrhs, p, ld, Ap, ld, done, static_cast<float*>(work), s);
+      st = dense_apply_xy(A, nrhs, p, ld, Ap, ld, done, static_cast<float*>(work), work, work_bytes, s);
       if (st) return st;
     } else if (csr_xy) {
       np0 = csr_xy_nparts(n);

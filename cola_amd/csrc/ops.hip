@@ -166,6 +166,7 @@ __global__ __launch_bounds__(256) void dense_bf16_kernel(int n, const uint16_t* 
 constexpr int kDenseChunk = 1024;
 constexpr int kDenseWaves = 8;
 typedef float f32x4_nt __attribute__((ext_vector_type(4)));
+typedef float f32x2_nt __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4_nt __attribute__((ext_vector_type(4)));
 
 // With astride != 0 the launch is batched over blockIdx.y: block row b multiplies its own
@@ -308,8 +309,325 @@ __global__ __launch_bounds__(64 * kDenseWaves) void dense_lds_kernel(int n, cons
   }
 }
 
+// ---------------------------------------------------------------- symmetric dense fp32
+// Y = scale * A X + shift * X for a symmetric A reading only the 128-column strips on and
+// above the diagonal: half the matrix bytes of the full product.  The upper part is cut
+// into units of 64 rows x 128 columns, strip by strip (strip J: units s = 0 .. 2J+1, rows
+// 64s .. 64s+63, columns 128J .. 128J+127; s >= 2J is the diagonal block), and the units are
+// split evenly over a persistent grid of four blocks per CU.  Each wave of a block takes
+// 16 rows of a unit (lane l: columns 2l, 2l+1 of every row):
+//   column sums (K^T x for the rows' x: the block's mirror image below the diagonal) on the
+//   VALU, accumulated over the block's consecutive units of one strip in registers and
+//   summed over the four waves when the strip ends;
+//   row sums (K x for the strip's x) on v_mfma_f32_16x16x4f32 from the wave's own 16 rows
+//   staged in LDS (lane = row l % 16, four columns at 4 (l / 16)), written per unit.
+// Diagonal units compute row sums over the whole 128 x 128 block and no column sums.
+// dense_sym_reduce then adds, per 64 rows, the row sums of every strip to the right and the
+// column sums of the rows above, in a fixed order (bit-reproducible).
+constexpr int kSymCols = 128;
+constexpr int kSymKs4 = 33;  // LDS row stride of a wave's 16 x 128 block in float4 (bank padding)
+constexpr int kSymGrid = 1024;
+constexpr int kSymNB = 8;    // rhs per pass
+
+__host__ __device__ inline int64_t sym_strip_start(int64_t J) { return J * (J + 1); }
+__host__ __device__ inline int64_t sym_units(int64_t nJ) { return sym_strip_start(nJ); }
+__host__ __device__ inline int64_t sym_lo(int64_t w, int64_t U, int G) { return w * U / G; }
+// block whose unit range holds unit u: the largest w with sym_lo(w) <= u
+__host__ __device__ inline int sym_block_of(int64_t u, int64_t U, int G) { return (int)(((u + 1) * G - 1) / U); }
+__host__ __device__ inline int sym_strip_of(int64_t u) {
+  int64_t J = (int64_t)((sqrt(4.0 * (double)u + 1.0) - 1.0) * 0.5);
+  while (sym_strip_start(J + 1) <= u) ++J;
+  while (J > 0 && sym_strip_start(J) > u) --J;
+  return (int)J;
+}
+inline int sym_grid(int64_t U) { return (int)(U < kSymGrid ? U : kSymGrid); }
+// strips one block's range can touch: interior strips hold >= 4 units each
+inline int sym_slots(int64_t U, int G) { return (int)(((U + G - 1) / G + 3) / 4 + 2); }
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// qr: units x NB x 64 row sums; cp: blocks x mslots x NB x 128 column sums
+template <int NB>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) void dense_sym_kernel(
+    int n, const float* __restrict__ A, int64_t lda, const float* __restrict__ X, int64_t ldx, int nbr,
+    float* __restrict__ qr, float* __restrict__ cp, int mslots, const int* __restrict__ done) {
+  if (done && *done) return;
+  __shared__ float4 ks4[4 * 16 * kSymKs4];
+  __shared__ float4 xs4[8 * 4 * kSymNB];  // [column block][lane / 16][rhs]: x at 4 consecutive columns
+  __shared__ float xis[4][16][kSymNB];    // per wave: x at its 16 rows, rhs fastest
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t U = sym_units(n / kSymCols);
+  const int G = gridDim.x, w = blockIdx.x;
+  const int64_t lo = sym_lo(w, U, G), hi = sym_lo(w + 1, U, G);
+  if (lo >= hi) return;
+  const int Jlo = sym_strip_of(lo);
+  float* ksw = reinterpret_cast<float*>(ks4 + wv * 16 * kSymKs4);
+  const int r16 = lane & 15, g = lane >> 4;
+  float accz[2][NB];  // column sums at columns 2 lane, 2 lane + 1
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) accz[t][b] = 0.f;
+  bool have = false;
+  int curJ = -1;
+  // the unit's rows as loaded and x at (row lane % 16, rhs lane / 16 and lane / 16 + 4); the
+  // next unit's loads are issued as soon as the rows are in LDS
+  float2 kr[16];
+  float xv[2];
+  auto load_unit = [&](int64_t u) {
+    const int J = sym_strip_of(u);
+    const int64_t s = u - sym_strip_start(J);
+    const float* base = A + (64 * s + 16 * wv) * lda + (int64_t)J * kSymCols + 2 * lane;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const f32x2_nt v = __builtin_nontemporal_load(reinterpret_cast<const f32x2_nt*>(base + i * lda));
+      kr[i] = make_float2(v.x, v.y);
+    }
+    if (s < 2 * (int64_t)J) {
+      const int64_t r = 64 * s + 16 * wv + r16;
+      const int b0 = g < nbr ? g : 0, b1 = g + 4 < nbr ? g + 4 : 0;  // clamped: loads stay unconditional
+      const float v0 = X[(int64_t)b0 * ldx + r], v1 = X[(int64_t)b1 * ldx + r];
+      xv[0] = g < nbr ? v0 : 0.f;
+      xv[1] = g + 4 < nbr ? v1 : 0.f;
+    }
+  };
+  auto flush = [&](int slot) {
+    __syncthreads();  // every wave is past its row sums: ks4 is free
+    float* red = reinterpret_cast<float*>(ks4);
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      *reinterpret_cast<float2*>(red + (wv * NB + b) * kSymCols + 2 * lane) = make_float2(accz[0][b], accz[1][b]);
+    __syncthreads();
+    if (slot < mslots) {
+      for (int e = threadIdx.x; e < NB * kSymCols; e += 256) {
+        const int b = e / kSymCols, c = e % kSymCols;
+        if (b < nbr)
+          cp[((int64_t)(w * mslots + slot) * NB + b) * kSymCols + c] =
+              ((red[(0 * NB + b) * kSymCols + c] + red[(1 * NB + b) * kSymCols + c]) +
+               red[(2 * NB + b) * kSymCols + c]) + red[(3 * NB + b) * kSymCols + c];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) accz[t][b] = 0.f;
+  };
+  load_unit(lo);
+  for (int64_t u = lo; u < hi; ++u) {
+    const int J = sym_strip_of(u);
+    const int64_t s = u - sym_strip_start(J);
+    const bool diag = s >= 2 * (int64_t)J;
+    if (have && (J != curJ || diag)) {
+      flush(curJ - Jlo);
+      have = false;
+    }
+    if (J != curJ) {
+      __syncthreads();  // the previous strip's x is no longer read
+      for (int e = threadIdx.x; e < 8 * 4 * kSymNB; e += 256) {
+        const int r = e % kSymNB, gg = (e / kSymNB) % 4, cb = e / (4 * kSymNB);
+        xs4[e] = r < nbr ? *reinterpret_cast<const float4*>(X + (int64_t)r * ldx + (int64_t)J * kSymCols + cb * 16 +
+                                                             4 * gg)
+                         : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+      __syncthreads();
+      curJ = J;
+    }
+    if (!diag) {
+      // column sums; the rows' x values go through LDS and come back wave-uniform
+      if (g < NB) xis[wv][r16][g] = xv[0];
+      if (g + 4 < NB) xis[wv][r16][g + 4] = xv[1];
+      wave_lds_sync();
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        float xr[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) xr[b] = xis[wv][i][b];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          accz[0][b] = fmaf(kr[i].x, xr[b], accz[0][b]);
+          accz[1][b] = fmaf(kr[i].y, xr[b], accz[1][b]);
+        }
+      }
+      have = true;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) *reinterpret_cast<float2*>(ksw + i * 4 * kSymKs4 + 2 * lane) = kr[i];
+    wave_lds_sync();
+    if (u + 1 < hi) load_unit(u + 1);
+    // row sums: D[row][rhs] += K[row][c] x[c][rhs] over the strip's 8 column blocks
+    floatx4 d0 = {0.f, 0.f, 0.f, 0.f}, d1 = {0.f, 0.f, 0.f, 0.f};
+    const float4* ksr = reinterpret_cast<const float4*>(ksw) + r16 * kSymKs4 + g;
+#pragma unroll
+    for (int cb = 0; cb < 8; cb += 2) {
+      const float4 a0 = ksr[cb * 4];
+      const float4 a1 = ksr[(cb + 1) * 4];
+      const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
+      const float4 b0 = r16 < NB ? xs4[(cb * 4 + g) * kSymNB + r16] : zero;
+      const float4 b1 = r16 < NB ? xs4[((cb + 1) * 4 + g) * kSymNB + r16] : zero;
+      d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.x, b0.x, d0, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.x, b1.x, d1, 0, 0, 0);
+      d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.y, b0.y, d0, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.y, b1.y, d1, 0, 0, 0);
+      d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.z, b0.z, d0, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.z, b1.z, d1, 0, 0, 0);
+      d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.w, b0.w, d0, 0, 0, 0);
+      d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.w, b1.w, d1, 0, 0, 0);
+    }
+    // lane (g, r16) holds rows 4g .. 4g+3 of the wave's 16 for rhs r16
+    if (r16 < nbr) {
+      float* q = qr + (u * NB + r16) * 64 + 16 * wv + 4 * g;
+      q[0] = d0[0] + d1[0];
+      q[1] = d0[1] + d1[1];
+      q[2] = d0[2] + d1[2];
+      q[3] = d0[3] + d1[3];
+    }
+    wave_lds_sync();  // the rows are read before the next unit overwrites them
+  }
+  if (have) flush(curJ - Jlo);
+}
+
+// one block per 64 rows (sub-tile s of strip s / 2).  Wave q sums the row sums of strips
+// Ji + q, Ji + q + 4, ... (four strips' loads per batch, all rhs at once) and the column sums
+// of every fourth block that covered strip Ji above the diagonal; then wave q takes rhs q and
+// q + 4: the four waves' sums in order, the epilogue and the X . Y partials.
+template <int NB>
+__global__ __launch_bounds__(256) void dense_sym_reduce(int n, const float* __restrict__ X, int64_t ldx,
+                                                        float* __restrict__ Y, int64_t ldy, int nbr, float scale,
+                                                        float shift, const float* __restrict__ qr,
+                                                        const float* __restrict__ cp, int mslots, int G,
+                                                        const int* __restrict__ done, float* __restrict__ xy_part) {
+  if (done && *done) return;
+  __shared__ float red[4][NB][64];
+  const int s = blockIdx.x, nJ = n / kSymCols, Ji = s / 2;
+  const int64_t U = sym_units(nJ);
+  const int il = threadIdx.x & 63, q = threadIdx.x >> 6;
+  float acc[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) acc[b] = 0.f;
+  for (int J0 = Ji + q; J0 < nJ; J0 += 16) {
+    float t[4][NB];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int J = J0 + 4 * k < nJ ? J0 + 4 * k : nJ - 1;  // clamped: the load stays unconditional
+      const float* src = qr + (sym_strip_start(J) + s) * NB * 64 + il;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) t[k][b] = src[b * 64];
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const float v0 = J0 < nJ ? t[0][b] : 0.f, v1 = J0 + 4 < nJ ? t[1][b] : 0.f;
+      const float v2 = J0 + 8 < nJ ? t[2][b] : 0.f, v3 = J0 + 12 < nJ ? t[3][b] : 0.f;
+      acc[b] += (v0 + v1) + (v2 + v3);
+    }
+  }
+  // column sums of the blocks wf + q, wf + q + 4, ... that covered strip Ji above the diagonal
+  if (Ji > 0) {
+    const int wf = sym_block_of(sym_strip_start(Ji), U, G);
+    const int wl = sym_block_of(sym_strip_start(Ji) + 2 * (int64_t)Ji - 1, U, G);
+    for (int w = wf + q; w <= wl; w += 4) {
+      const int slot = Ji - sym_strip_of(sym_lo(w, U, G));
+      const float* src =
+          cp + ((int64_t)(w * mslots + (slot < mslots ? slot : 0)) * NB) * kSymCols + 64 * (s % 2) + il;
+      float t[NB];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) t[b] = src[b * kSymCols];
+#pragma unroll
+      for (int b = 0; b < NB; ++b) acc[b] += slot < mslots ? t[b] : 0.f;
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < NB; ++b) red[q][b][il] = acc[b];
+  __syncthreads();
+  const int64_t row = 64 * (int64_t)s + il;
+  for (int b = q; b < NB; b += 4) {
+    if (b >= nbr) break;
+    const float sum = (red[0][b][il] + red[1][b][il]) + (red[2][b][il] + red[3][b][il]);
+    const float x = X[(int64_t)b * ldx + row];
+    float y = scale * sum;
+    if (shift != 0.f) y = fmaf(shift, x, y);
+    Y[(int64_t)b * ldy + row] = y;
+    if (xy_part) {
+      // two partials of 32 rows each, as the full product's epilogue writes them
+      float v = x * y;
+      v += dpp_f32<0xB1>(v);
+      v += dpp_f32<0x4E>(v);
+      v += dpp_f32<0x124>(v);
+      v += dpp_f32<0x128>(v);
+      // read the lanes before the branch: the last add must not move under the lane test
+      const float h0 = readlane_f32(v, 0) + readlane_f32(v, 16);
+      const float h1 = readlane_f32(v, 32) + readlane_f32(v, 48);
+      if (il == 0) {
+        xy_part[(int64_t)b * (n / 32) + 2 * s] = h0;
+        xy_part[(int64_t)b * (n / 32) + 2 * s + 1] = h1;
+      }
+    }
+  }
+}
+
+// scratch of the symmetric product, for up to kSymNB rhs per pass
+size_t sym_scratch_bytes(int64_t n) {
+  const int64_t U = sym_units(n / kSymCols);
+  const int G = sym_grid(U);
+  const size_t q = (size_t)U * kSymNB * 64 * sizeof(float), c = (size_t)G * sym_slots(U, G) * kSymNB * kSymCols * 4;
+  return (q + 255) / 256 * 256 + (c + 255) / 256 * 256;
+}
+
+bool sym_eligible(const ck_op* op, const float* X, int64_t ldx, size_t avail) {
+  auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
+  const int64_t n = op->n;
+  return op->kind == CK_OP_DENSE_F32 && (op->flags & CK_OP_SYMMETRIC) && n >= kSymCols && n % kSymCols == 0 &&
+         op->lda % 4 == 0 && ldx % 4 == 0 && al16(op->a) && al16(X) && avail >= sym_scratch_bytes(n);
+}
+
+// Y = op(X) through the symmetric kernels, in passes of up to eight rhs; scratch is the
+// workspace past dense_sym_offset.  xy_part (optional, nrhs <= 8): dense_xy_nparts(n)
+// partials of X . Y per rhs.
+int dense_sym_apply(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, const int* done,
+                    void* scratch, float* xy_part, hipStream_t s) {
+  const int n = op->n;
+  const int64_t U = sym_units(n / kSymCols);
+  const int G = sym_grid(U), ms = sym_slots(U, G);
+  const size_t qb = ((size_t)U * kSymNB * 64 * sizeof(float) + 255) / 256 * 256;
+  float* qr = static_cast<float*>(scratch);
+  float* cp = reinterpret_cast<float*>(static_cast<char*>(scratch) + qb);
+  const float* A = static_cast<const float*>(op->a);
+  for (int b0 = 0; b0 < nrhs; b0 += kSymNB) {
+    const int nb = nrhs - b0 < kSymNB ? nrhs - b0 : kSymNB;
+    const float* Xb = X + b0 * ldx;
+    float* Yb = Y + b0 * ldy;
+    float* xyb = xy_part && nrhs <= kSymNB ? xy_part : nullptr;
+#define CK_SYM(NB_)                                                                                              \
+  do {                                                                                                           \
+    hipLaunchKernelGGL((dense_sym_kernel<NB_>), dim3(G), dim3(256), 0, s, n, A, op->lda, Xb, ldx, nb, qr, cp,  \
+                       ms, done);                                                                                \
+    hipLaunchKernelGGL((dense_sym_reduce<NB_>), dim3(n / 64), dim3(256), 0, s, n, Xb, ldx, Yb, ldy, nb,           \
+                       op->scale, op->shift, (const float*)qr, (const float*)cp, ms, G, done, xyb);              \
+  } while (0)
+    if (nb <= 1)
+      CK_SYM(1);
+    else if (nb <= 2)
+      CK_SYM(2);
+    else if (nb <= 4)
+      CK_SYM(4);
+    else
+      CK_SYM(8);
+#undef CK_SYM
+  }
+  CK_CHECK_LAUNCH();
+  return CK_OK;
+}
+
 int dense_dispatch(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, const int* done,
-                   hipStream_t s) {
+                   void* work, size_t work_bytes, hipStream_t s) {
+  const size_t off = dense_sym_offset(op->n, nrhs);
+  if (work && work_bytes > off && sym_eligible(op, X, ldx, work_bytes - off))
+    return dense_sym_apply(op, nrhs, X, ldx, Y, ldy, done, static_cast<char*>(work) + off, nullptr, s);
   const int n = op->n;
   const bool bf16 = op->kind == CK_OP_DENSE_BF16;
   auto aligned16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
@@ -683,9 +1001,12 @@ namespace ck {
 // per-block partial sums of X . Y to xy_part (nrhs x dense_xy_nparts(n), column-major by
 // rhs); returns CK_ERR_UNSUPPORTED when the shapes fall outside that kernel
 int dense_apply_xy(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, const int* done,
-                   float* xy_part, hipStream_t s) {
+                   float* xy_part, void* work, size_t work_bytes, hipStream_t s) {
   auto aligned16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
   const int n = op->n;
+  const size_t off = dense_sym_offset(n, nrhs);
+  if (work && work_bytes > off && nrhs >= 1 && nrhs <= 8 && sym_eligible(op, X, ldx, work_bytes - off))
+    return dense_sym_apply(op, nrhs, X, ldx, Y, ldy, done, static_cast<char*>(work) + off, xy_part, s);
   if (op->kind != CK_OP_DENSE_F32 || nrhs < 3 || nrhs > 8 || n % 4 || op->lda % 4 || ldx % 4 || !aligned16(op->a) ||
       !aligned16(X))
     return CK_ERR_UNSUPPORTED;
@@ -762,7 +1083,7 @@ int op_apply_internal(const ck_op* op, int nrhs, const float* X, int64_t ldx, fl
     case CK_OP_DENSE_F32:
     case CK_OP_DENSE_BF16:
       if (!op->a || op->lda < op->n) return CK_ERR_BAD_ARG;
-      return dense_dispatch(op, nrhs, X, ldx, Y, ldy, done, s);
+      return dense_dispatch(op, nrhs, X, ldx, Y, ldy, done, work, work_bytes, s);
     case CK_OP_DENSE_BATCHED_F32: {
       // vector b multiplies matrix b (a + b * nnz elements): independent dense systems of one order
       auto aligned16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
@@ -801,6 +1122,10 @@ size_t op_workspace_internal(const ck_op* op, int nrhs) {
     case CK_OP_KRON_EIG_F32:
       return 2 * (size_t)nrhs * op->n * sizeof(float);
     case CK_OP_DENSE_F32:
+      // a symmetric operator's product reads half the matrix through scratch placed past the
+      // regions CG's fused steps use (dense_sym_offset)
+      if ((op->flags & CK_OP_SYMMETRIC) && op->n >= kSymCols && op->n % kSymCols == 0)
+        return dense_sym_offset(op->n, nrhs) + sym_scratch_bytes(op->n);
       // room for the p.Ap partials a CG step takes from the matvec epilogue (dense_apply_xy)
       return (nrhs >= 3 && nrhs <= 8) ? dense_xy_bytes(op->n, nrhs) : 0;
     case CK_OP_CSR_F32:

@@ -30,8 +30,10 @@ int kron64_eig_apply_f32(const ck_op* op, int nrhs, const float* X, int64_t ldx,
 // dense_xy_nparts(n) partials per rhs; CK_ERR_UNSUPPORTED outside that kernel's shapes.
 inline int dense_xy_nparts(int64_t n) { return (int)((n + 31) / 32); }
 inline size_t dense_xy_bytes(int64_t n, int nrhs) { return ((size_t)4 * nrhs * dense_xy_nparts(n) + 255) / 256 * 256; }
+// With work (the operator's workspace), a CK_OP_SYMMETRIC operator of order a multiple of 128
+// takes the half-reading symmetric product for 1..8 rhs.
 int dense_apply_xy(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, const int* done,
-                   float* xy_part, hipStream_t s);
+                   float* xy_part, void* work, size_t work_bytes, hipStream_t s);
 
 // CSR fp32 apply (average row length <= 32) that also writes per-block partial sums of X . Y:
 // csr_xy_nparts(n) partials per rhs; CK_ERR_UNSUPPORTED for longer rows.
@@ -47,6 +49,12 @@ size_t woodbury_workspace_bytes(const ck_op* op, int nrhs);
 // partials, per-tile F^T r partials and the core products u.
 constexpr int kWbFusedRows = 128;
 size_t wb_fused_bytes(int64_t n, int nrhs, int k);
+// Offset of the symmetric dense product's scratch in an operator workspace: past the p.Ap
+// partials and the fused Woodbury step's regions (rank <= 64) that CG keeps at the head.
+inline size_t dense_sym_offset(int64_t n, int nrhs) {
+  const size_t a = dense_xy_bytes(n, nrhs), b = wb_fused_bytes(n, nrhs, 64);
+  return ((a > b ? a : b) + 255) / 256 * 256;
+}
 int rbf_apply(const ck_op* op, int nrhs, const float* X, int64_t ldx, float* Y, int64_t ldy, const int* done,
               hipStream_t s);
 

@@ -236,38 +236,16 @@ __device__ double solve_shifted(int m, const double* d, const double* e, double 
   return nrm2;
 }
 
-// the per-thread factors and vectors of invit_kernel: 4 m doubles per thread in 64 KB of LDS
-constexpr int kInvitLds = 8192;
-
-// threads per block with vectors for invit_kernel: the largest power of two <= 64 whose scratch fits
-inline int invit_tpb(int m) {
-  const int fit = kInvitLds / (4 * m);
-  int t = 1;
-  while (2 * t <= fit && 2 * t <= 64) t *= 2;
-  return t;
-}
-
-// one wave per block; its first tpb lanes take the eigenvalues blockIdx.x * tpb + lane; grid
-// (ceil(m / tpb), nb)
-__global__ __launch_bounds__(64) void invit_kernel(int m, int tpb, const float* __restrict__ din, int64_t ldd,
-                                                   const float* __restrict__ ein, int64_t lde,
-                                                   const double* __restrict__ evals, double* __restrict__ evecs,
-                                                   double* __restrict__ first_row) {
-  __shared__ double d[kMaxM], e[kMaxM], e2[kMaxM], lam[kMaxM], red[12];
-  __shared__ double Ws[kInvitLds];
-  const int b = blockIdx.y, tid = threadIdx.x;
-  for (int i = tid; i < m; i += 64) lam[i] = evals[(int64_t)b * m + i];
-  Bounds B = load_problem(m, din, ldd, ein, lde, b, d, e, e2, red);
-  if (!isfinite(B.tnorm)) return;
-  const double tnorm = fmax(fabs(lam[0]), fabs(lam[m - 1]));
+// Inverse iteration for the eigenvalues j0, j0 + 1, ... of problem b that start a cluster at
+// j0 (the thread's own index), its factors and vector in W (row r of array a at
+// W[(a * m + r) * ts]).  Inlined into each kernel, so that W keeps its address space
+// (LDS or global) instead of going through flat accesses.
+__device__ __forceinline__ void invit_thread(int m, int b, int j0, const double* d, const double* e, const double* lam,
+                             double tnorm, double tiny, double* W, int64_t ts, double* evecs, double* first_row) {
   const double ortol = 1e-3 * tnorm;
-  const double tiny = DBL_EPSILON * tnorm + B.pivmin;
-  const int64_t ts = tpb;
-  const int j0 = blockIdx.x * tpb + tid;
   double* Z = evecs + (int64_t)b * m * m;
-  double* W = Ws + tid;
   double* X = W + (int64_t)3 * m * ts;
-  if (tid >= tpb || j0 >= m || (j0 > 0 && lam[j0] - lam[j0 - 1] <= ortol)) return;  // not a cluster's first
+  if (j0 >= m || (j0 > 0 && lam[j0] - lam[j0 - 1] <= ortol)) return;  // not a cluster's first
   int j1 = j0 + 1;
   while (j1 < m && lam[j1] - lam[j1 - 1] <= ortol) ++j1;
   double prev = 0.0;
@@ -342,12 +320,72 @@ __global__ __launch_bounds__(64) void invit_kernel(int m, int tpb, const float* 
   }
 }
 
+// Two placements of the per-thread scratch (4 m doubles: the pivot reciprocals, two
+// super-diagonals and the vector), both with the thread index fastest:
+//   invit_lds_kernel: in 64 KB of LDS, 8192 / (4 m) threads per block -- for batches whose
+//     blocks fit the CUs at once (one eigensolve of a Lanczos run);
+//   invit_global_kernel: in the caller's workspace, 64 threads per block -- for large batches
+//     (thousands of small problems), where LDS would hold too few threads per CU.
+constexpr int kInvitLds = 8192;
+
+// threads per block with vectors for invit_lds_kernel: the largest power of two <= 64 whose
+// scratch fits
+inline int invit_tpb(int m) {
+  const int fit = kInvitLds / (4 * m);
+  int t = 1;
+  while (2 * t <= fit && 2 * t <= 64) t *= 2;
+  return t;
+}
+inline int64_t invit_threads(int32_t m) { return ((int64_t)m + 63) / 64 * 64; }
+
+// one wave per block; its first tpb lanes take the eigenvalues blockIdx.x * tpb + lane; grid
+// (ceil(m / tpb), nb)
+__global__ __launch_bounds__(64) void invit_lds_kernel(int m, int tpb, const float* __restrict__ din, int64_t ldd,
+                                                       const float* __restrict__ ein, int64_t lde,
+                                                       const double* __restrict__ evals, double* __restrict__ evecs,
+                                                       double* __restrict__ first_row) {
+  __shared__ double d[kMaxM], e[kMaxM], e2[kMaxM], lam[kMaxM], red[12];
+  __shared__ double Ws[kInvitLds];
+  const int b = blockIdx.y, tid = threadIdx.x;
+  for (int i = tid; i < m; i += 64) lam[i] = evals[(int64_t)b * m + i];
+  Bounds B = load_problem(m, din, ldd, ein, lde, b, d, e, e2, red);
+  if (!isfinite(B.tnorm)) return;
+  const double tnorm = fmax(fabs(lam[0]), fabs(lam[m - 1]));
+  const double tiny = DBL_EPSILON * tnorm + B.pivmin;
+  if (tid >= tpb) return;
+  invit_thread(m, b, blockIdx.x * tpb + tid, d, e, lam, tnorm, tiny, Ws + tid, tpb, evecs, first_row);
+}
+
+// 64 threads per block; grid (ceil(m / 64), nb); work: nb x 4 m x invit_threads(m) doubles
+__global__ __launch_bounds__(64) void invit_global_kernel(int m, const float* __restrict__ din, int64_t ldd,
+                                                          const float* __restrict__ ein, int64_t lde,
+                                                          const double* __restrict__ evals,
+                                                          double* __restrict__ evecs, double* __restrict__ first_row,
+                                                          double* __restrict__ work) {
+  __shared__ double d[kMaxM], e[kMaxM], e2[kMaxM], lam[kMaxM], red[12];
+  const int b = blockIdx.y, tid = threadIdx.x;
+  for (int i = tid; i < m; i += 64) lam[i] = evals[(int64_t)b * m + i];
+  Bounds B = load_problem(m, din, ldd, ein, lde, b, d, e, e2, red);
+  if (!isfinite(B.tnorm)) return;
+  const double tnorm = fmax(fabs(lam[0]), fabs(lam[m - 1]));
+  const double tiny = DBL_EPSILON * tnorm + B.pivmin;
+  const int64_t ts = (int64_t)gridDim.x * 64;
+  const int j0 = blockIdx.x * 64 + tid;
+  invit_thread(m, b, j0, d, e, lam, tnorm, tiny, work + (int64_t)b * 4 * m * ts + j0, ts, evecs, first_row);
+}
+
 }  // namespace
 
 extern "C" {
 
+size_t ck_tridiag_eigh_bisect_workspace_bytes(int32_t nb, int32_t m) {
+  if (nb < 1 || m < 1) return 0;
+  return sizeof(double) * (size_t)nb * 4 * (size_t)m * (size_t)invit_threads(m);
+}
+
 int ck_tridiag_eigh_bisect_f64(int32_t nb, int32_t m, const float* d, int64_t ldd, const float* e, int64_t lde,
-                               double* evals, double* evecs, double* first_row, int32_t* status_out, void* stream) {
+                               double* evals, double* evecs, double* first_row, int32_t* status_out, void* work,
+                               size_t work_bytes, void* stream) {
   if (nb < 1 || m < 1 || m > kMaxM || nb > 65535 || !d || !evals || ldd < m || (m > 1 && (!e || lde < m - 1)) ||
       (first_row && !evecs))
     return CK_ERR_BAD_ARG;
@@ -363,9 +401,22 @@ int ck_tridiag_eigh_bisect_f64(int32_t nb, int32_t m, const float* d, int64_t ld
   }
   CK_CHECK_LAUNCH();
   if (evecs) {
+    // LDS scratch while its one-wave blocks (one per CU) take at most two rounds of the chip
+    // (one problem of order 1024: 512 blocks of 2 threads, 47 ms against 61 ms from the
+    // workspace); the workspace for larger batches (2000 problems of order 100: 4.7 ms
+    // against 12 ms)
     const int tpb = invit_tpb(m);
-    const dim3 grid((unsigned)((m + tpb - 1) / tpb), (unsigned)nb);
-    hipLaunchKernelGGL(invit_kernel, grid, dim3(64), 0, s, (int)m, tpb, d, ldd, ep, lde, evals, evecs, first_row);
+    const int64_t lds_blocks = (int64_t)nb * ((m + tpb - 1) / tpb);
+    const bool global = lds_blocks > 512 && work && work_bytes >= ck_tridiag_eigh_bisect_workspace_bytes(nb, m);
+    if (global) {
+      const dim3 grid((unsigned)(invit_threads(m) / 64), (unsigned)nb);
+      hipLaunchKernelGGL(invit_global_kernel, grid, dim3(64), 0, s, (int)m, d, ldd, ep, lde, evals, evecs, first_row,
+                         static_cast<double*>(work));
+    } else {
+      const dim3 grid((unsigned)((m + tpb - 1) / tpb), (unsigned)nb);
+      hipLaunchKernelGGL(invit_lds_kernel, grid, dim3(64), 0, s, (int)m, tpb, d, ldd, ep, lde, evals, evecs,
+                         first_row);
+    }
     CK_CHECK_LAUNCH();
   }
   return CK_OK;

@@ -167,22 +167,28 @@ def tridiag_eigh_batch(d, e, vectors=True, method=None):
     """FP64 eigen-decompositions of nb tridiagonals: d (nb, k) and e (nb, >= k-1) fp32 rows.
 
     ``method`` "bisect" (Sturm multisection + inverse iteration, ck_tridiag_eigh_bisect_f64,
-    the default: its eigenvalues and vectors are computed in parallel, 0.1 ms for 32
-    problems of order 30 and 0.6 ms for one of order 120; orders up to 1024) or "ql"
-    (implicit QL, ck_tridiag_eigh_f64, whose rotations are serial: 0.45 and 4.9 ms).
+    the default for up to 64 problems of order up to 1024: its eigenvalues and vectors are
+    computed in parallel, 0.1 ms for 32 problems of order 30 and 0.5 ms for one of order 120)
+    or "ql" (implicit QL, ck_tridiag_eigh_f64, whose rotations are serial: 0.45 and 4.9 ms;
+    the default for larger batches).  Inverse iteration orthogonalises the vectors of a
+    cluster (eigenvalues closer than 1e-3 ||T||) serially in one thread, O(c^2 m) for a
+    cluster of c: spectra with most values in one cluster (kernel matrices) are faster on QL.
     Returns (evals (nb, k) f64 ascending, evecs (nb, k, k) f64 or None, first_row (nb, k) f64,
     status (nb,) int32)."""
     nb, k = d.shape
     dev = d.device
-    method = method or ("bisect" if k <= 1024 else "ql")  # the bisection kernel holds T in LDS
+    method = method or ("bisect" if k <= 1024 and nb <= 64 else "ql")  # the bisection kernel holds T in LDS
     ev = torch.empty((nb, k), dtype=torch.float64, device=dev)
     vecs = torch.empty((nb, k, k), dtype=torch.float64, device=dev) if vectors else None
     fr = torch.empty((nb, k), dtype=torch.float64, device=dev)
     st = torch.empty(nb, dtype=torch.int32, device=dev)
     if method == "bisect":
         tmp = vecs if vectors else torch.empty((nb, k, k), dtype=torch.float64, device=dev)
+        # the inverse iteration's factors: LDS for a few problems, this workspace for many
+        wb = N.q("ck_tridiag_eigh_bisect_workspace_bytes", nb, k) if nb > 1 and nb * k > 512 else 0
+        work = torch.empty(wb, dtype=torch.uint8, device=dev) if wb else None
         N.call("ck_tridiag_eigh_bisect_f64", nb, k, N.ptr(d), d.stride(0), N.ptr(e), e.stride(0), N.ptr(ev),
-               N.ptr(tmp), N.ptr(fr), N.ptr(st), N.stream_ptr(dev))
+               N.ptr(tmp), N.ptr(fr), N.ptr(st), N.ptr(work), wb, N.stream_ptr(dev))
     elif method == "ql":
         N.call("ck_tridiag_eigh_f64", nb, k, N.ptr(d), d.stride(0), N.ptr(e), e.stride(0), N.ptr(ev), N.ptr(vecs),
                N.ptr(fr), N.ptr(st), N.stream_ptr(dev))

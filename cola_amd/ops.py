@@ -219,6 +219,23 @@ def _op(kind, n, scale=1.0, shift=0.0):
     return op
 
 
+# Order from which a self-adjoint dense operator is lowered with CK_OP_SYMMETRIC.  Below it
+# the matrix (<= 512 MiB) is served largely from the 256 MB Infinity Cache and the full product
+# is as fast or faster (MI355X, 16384 x 16384: 1 rhs 120 vs 177 us, 8 rhs 157 vs 173 us;
+# 4096 x 4096: 30 vs 11 us).
+SYMMETRIC_MIN_ORDER = 11776
+
+
+def _mark_symmetric(A, got):
+    """Sets CK_OP_SYMMETRIC on a dense lowering of an operator annotated self-adjoint and of
+    order >= SYMMETRIC_MIN_ORDER: the engine then reads only the blocks on and above the
+    diagonal (half the matrix bytes)."""
+    if (got is not None and A.isa(ann.SelfAdjoint) and got[0].kind == N.CK_OP_DENSE_F32
+            and got[0].n >= SYMMETRIC_MIN_ORDER):
+        got[0].flags |= N.CK_OP_SYMMETRIC
+    return got
+
+
 class Dense(LinearOperator):
     """operators.py:12-38."""
 
@@ -240,7 +257,7 @@ class Dense(LinearOperator):
             return None
         op = _op(N.CK_OP_DENSE_F32, A.shape[0])
         op.a, op.lda = A.data_ptr(), A.stride(0)
-        return op, [A]
+        return _mark_symmetric(self, (op, [A]))
 
     def to_dense(self):
         return self.A
@@ -498,7 +515,7 @@ class Product(LinearOperator):
         op, keep = got
         op.scale *= scal
         op.shift *= scal
-        return op, keep
+        return _mark_symmetric(self, (op, keep))
 
 
 class Sum(LinearOperator):
@@ -538,7 +555,7 @@ class Sum(LinearOperator):
             return _op(N.CK_OP_IDENTITY, self.shape[0], scale=shift), []
         op, keep = structured[0]
         op.shift += shift
-        return op, keep
+        return _mark_symmetric(self, (op, keep))
 
 
 class Triangular(Dense):

@@ -118,8 +118,11 @@ typedef struct ck_op {
   int32_t rank;
   int32_t dim;
   float aux_f;
-  int32_t reserved[5];
+  int32_t flags;    /* CK_OP_SYMMETRIC: S = S^T is promised (dense fp32 of order a multiple of
+                       128: the product reads only the blocks on and above the diagonal) */
+  int32_t reserved[4];
 } ck_op;
+enum { CK_OP_SYMMETRIC = 1 };
 
 /* Bytes of device scratch ck_op_apply needs for `nrhs` vectors. */
 size_t ck_op_workspace_bytes(const ck_op* op, int32_t nrhs);
@@ -424,14 +427,17 @@ int ck_arnoldi_iterate(const ck_op* A, int64_t n, int32_t nb, int32_t max_iters,
 int ck_tridiag_eigh_f64(int32_t nb, int32_t m, const float* d, int64_t ldd, const float* e, int64_t lde,
                         double* evals, double* evecs, double* first_row, int32_t* status_out, void* stream);
 /* The same eigenproblems by Sturm multisection (L lanes per eigenvalue) and inverse
- * iteration (one thread per eigenvector, its factors in LDS; eigenvalues closer than
- * 1e-3 ||T|| form a cluster whose vectors are orthogonalised against each other), for
- * lanczos.py:57 / unary.py:52 / slq.py:44, where the QL recurrence's rotations are serial.
- * Same arguments and layouts as ck_tridiag_eigh_f64, m <= 1024; first_row needs evecs; the
- * first largest-magnitude component of each vector is positive.  status_out[b] = CK_OK or
- * CK_ERR_NAN (a non-finite entry). */
+ * iteration (one thread per eigenvector; eigenvalues closer than 1e-3 ||T|| form a cluster
+ * whose vectors are orthogonalised against each other), for lanczos.py:57 / unary.py:52 /
+ * slq.py:44, where the QL recurrence's rotations are serial.  Same layouts as
+ * ck_tridiag_eigh_f64, m <= 1024; first_row needs evecs; the first largest-magnitude
+ * component of each vector is positive.  The inverse iteration keeps its factors in LDS when
+ * the batch is small and in `work` (>= ck_tridiag_eigh_bisect_workspace_bytes(nb, m), may be
+ * NULL: LDS then) for large batches.  status_out[b] = CK_OK or CK_ERR_NAN (a non-finite entry). */
+size_t ck_tridiag_eigh_bisect_workspace_bytes(int32_t nb, int32_t m);
 int ck_tridiag_eigh_bisect_f64(int32_t nb, int32_t m, const float* d, int64_t ldd, const float* e, int64_t lde,
-                               double* evals, double* evecs, double* first_row, int32_t* status_out, void* stream);
+                               double* evals, double* evecs, double* first_row, int32_t* status_out, void* work,
+                               size_t work_bytes, void* stream);
 
 /* out[c][i] = sum_j S[j][c] V[j][i] for c < k: Ritz vectors from a basis block
  * (lanczos.py:59, arnoldi.py:61).  V rows j = 0..m-1 at V + j*ldv; S (m x k) fp64

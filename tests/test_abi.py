@@ -83,7 +83,8 @@ def test_bad_arguments_rejected_without_launch(lib):
     # null pointers / bad shapes return CK_ERR_BAD_ARG before any device work
     assert lib.ck_cg_start(0, 1, None, None, None, 1, None, ctypes.c_float(1e-6), 10, None, None) == 1
     assert lib.ck_tridiag_eigh_f64(0, 5, None, 5, None, 4, None, None, None, None, None) == 1
-    assert lib.ck_tridiag_eigh_bisect_f64(1, 1025, None, 1025, None, 1024, None, None, None, None, None) == 1
+    assert lib.ck_tridiag_eigh_bisect_f64(1, 1025, None, 1025, None, 1024, None, None, None, None, None, 0, None) == 1
+    assert N.q("ck_tridiag_eigh_bisect_workspace_bytes", 2, 120) == 8 * 2 * 4 * 120 * 128  # threads: m rounded to 64
     assert lib.ck_op_apply(None, 1, None, 1, None, 1, None, 0, None) == 1
     op = N.CkOp()
     op.kind, op.n = 99, 4

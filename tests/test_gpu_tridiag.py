@@ -68,16 +68,19 @@ def test_tridiag_eigh_split_blocks(dev, m):
     assert np.abs(V.T @ V - np.eye(m)).max() < 1e-12
 
 
-def _bisect(dev, d, e):
-    """ck_tridiag_eigh_bisect_f64 on fp32 rows d (nb, m), e (nb, m-1); returns evals, evecs, first_row, status."""
+def _bisect(dev, d, e, use_work=True):
+    """ck_tridiag_eigh_bisect_f64 on fp32 rows d (nb, m), e (nb, m-1); returns evals, evecs, first_row, status.
+    With a workspace, batches of more than 512 LDS blocks take the workspace kernel."""
     nb, m = d.shape
     dd, ee = torch.from_numpy(d).to(dev), torch.from_numpy(np.ascontiguousarray(e)).to(dev)
     ev = torch.empty(nb, m, dtype=torch.float64, device=dev)
     vecs = torch.empty(nb, m, m, dtype=torch.float64, device=dev)
     fr = torch.empty(nb, m, dtype=torch.float64, device=dev)
     st = torch.full((nb,), -1, dtype=torch.int32, device=dev)
+    wb = N.q("ck_tridiag_eigh_bisect_workspace_bytes", nb, m) if use_work else 0
+    work = torch.empty(wb, dtype=torch.uint8, device=dev) if wb else None
     N.call("ck_tridiag_eigh_bisect_f64", nb, m, N.ptr(dd), m, N.ptr(ee), max(e.shape[1], 1), N.ptr(ev), N.ptr(vecs),
-           N.ptr(fr), N.ptr(st), N.stream_ptr(dev))
+           N.ptr(fr), N.ptr(st), N.ptr(work), wb, N.stream_ptr(dev))
     torch.cuda.synchronize()
     return ev.cpu().numpy(), vecs.cpu().numpy(), fr.cpu().numpy(), st.cpu().numpy()
 
@@ -102,7 +105,8 @@ def _check_decomposition(d, e, w, V, fr, what):
     assert np.array_equal(fr, V[0])
 
 
-@pytest.mark.parametrize("m,nb", [(1, 1), (2, 3), (10, 4), (30, 32), (120, 1), (120, 6), (200, 2), (700, 1), (1024, 1)])
+@pytest.mark.parametrize("m,nb", [(1, 1), (2, 3), (10, 4), (30, 32), (120, 1), (120, 6), (200, 2), (700, 1), (1024, 1),
+                                  (30, 600), (120, 64)])
 def test_tridiag_bisect_random(dev, m, nb):
     """Bisection + inverse iteration on random tridiagonals: against LAPACK, residual and
     orthogonality bars, and the same eigenvalues as the QL kernel to 1e-12."""
@@ -213,3 +217,15 @@ def test_tridiag_bisect_nonfinite_input_reports_nan(dev):
     w, V, fr, st = _bisect(dev, d, e)
     assert st[0] == 0 and st[1] == N.CK_ERR_NAN and st[2] == N.CK_ERR_NAN
     _check_decomposition(d[0], e[0], w[0], V[0], fr[0], "finite neighbour")
+
+
+def test_tridiag_bisect_large_batch_without_workspace(dev):
+    """Without a workspace a large batch stays on the LDS kernel: same bars."""
+    rs = np.random.RandomState(7)
+    m, nb = 30, 300
+    d = rs.normal(size=(nb, m)).astype(np.float32)
+    e = rs.uniform(0.1, 1.0, size=(nb, m - 1)).astype(np.float32)
+    w, V, fr, st = _bisect(dev, d, e, use_work=False)
+    assert (st == 0).all()
+    for b in range(0, nb, 7):
+        _check_decomposition(d[b], e[b], w[b], V[b], fr[b], "random")
