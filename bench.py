@@ -42,8 +42,8 @@ FP32_MFMA_PEAK = 157.3  # TFLOP/s
 # (profiles/dense_matvec_16384x8_hbm_traffic.md)
 DENSE8_TRAFFIC_BYTES = (526427.07 * 2.0 + 512.42) * 1024
 # HBM bytes per symmetric 8-rhs product (dense_sym_kernel<8> + dense_sym_reduce<8>) on the same matrix,
-# same counters and factors (profiles/dense_sym_matvec_16384x8_hbm_traffic.md); None until measured
-DENSE8_SYM_TRAFFIC_BYTES = None
+# same counters and factors (profiles/dense_sym_matvec_16384x8_hbm_traffic.md)
+DENSE8_SYM_TRAFFIC_BYTES = ((266583.10 + 19071.09) * 2.0 + (37650.49 + 512.0)) * 1024
 
 
 def dense_product_roof(op, n, B, t):
