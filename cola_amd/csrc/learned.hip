@@ -988,6 +988,84 @@ __global__ __launch_bounds__(256) void dense_f64acc_kernel(int n, const float* _
   }
 }
 
+// The same product with every load unconditional (rows and columns clamped, the extra terms
+// multiplied by zero) and the x chunk staged once per block for all of its rows: eight waves of
+// four rows share one FP64 copy of X[:, c0:c0+512) in LDS; A is streamed with non-temporal
+// float4 loads.  Requires n % 4 == 0, 16-byte aligned rows of A and X, lda % 4 == 0 and
+// ldx % 2 == 0 (the entry point keeps dense_f64acc_kernel for the rest).
+constexpr int kF64Chunk = 512;
+constexpr int kF64Waves = 8;
+typedef float f64acc_f4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(64 * kF64Waves) __attribute__((amdgpu_waves_per_eu(4))) void dense_f64acc_lds_kernel(int n, const float* __restrict__ A,
+                                                                         int64_t lda, const double* __restrict__ X,
+                                                                         int64_t ldx, double* __restrict__ Y,
+                                                                         int64_t ldy, int nb, double scale,
+                                                                         double shift) {
+  __shared__ double2 xs[kF64NB][kF64Chunk / 2];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // rows and their pointers stay scalar
+  const int row0 = blockIdx.x * (4 * kF64Waves) + wave * 4;
+  double acc[4][kF64NB];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int b = 0; b < kF64NB; ++b) acc[r][b] = 0.0;
+  const float* arow[4];
+  float rmask[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int row = row0 + r < n ? row0 + r : n - 1;
+    arow[r] = A + (int64_t)row * lda;
+    rmask[r] = row0 + r < n ? 1.f : 0.f;
+  }
+  for (int c0 = 0; c0 < n; c0 += kF64Chunk) {
+    const int cn = n - c0 < kF64Chunk ? n - c0 : kF64Chunk;  // a multiple of 4
+    __syncthreads();
+    for (int e = threadIdx.x; e < kF64NB * (kF64Chunk / 2); e += 64 * kF64Waves) {
+      const int b = e / (kF64Chunk / 2), j2 = e % (kF64Chunk / 2);
+      xs[b][j2] = (b < nb && 2 * j2 < cn) ? *reinterpret_cast<const double2*>(X + (int64_t)b * ldx + c0 + 2 * j2)
+                                          : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kF64Chunk / 256; ++q) {
+      const int j = q * 256 + lane * 4;
+      const int jc = j < cn ? j : cn - 4;  // clamped: the load stays unconditional
+      float4 a[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const f64acc_f4 v = __builtin_nontemporal_load(reinterpret_cast<const f64acc_f4*>(arow[r] + c0 + jc));
+        const float m = j < cn ? rmask[r] : 0.f;
+        a[r] = make_float4(v.x * m, v.y * m, v.z * m, v.w * m);
+      }
+#pragma unroll
+      for (int b = 0; b < kF64NB; ++b) {
+        const double2 x0 = xs[b][jc / 2], x1 = xs[b][jc / 2 + 1];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          double t = acc[r][b];
+          t = fma((double)a[r].x, x0.x, t);
+          t = fma((double)a[r].y, x0.y, t);
+          t = fma((double)a[r].z, x1.x, t);
+          t = fma((double)a[r].w, x1.y, t);
+          acc[r][b] = t;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+#pragma unroll
+    for (int b = 0; b < kF64NB; ++b) {
+      const double sum = wave_sum(acc[r][b]);
+      const int row = row0 + r;
+      if (lane == 0 && row < n && b < nb)
+        Y[(int64_t)b * ldy + row] = scale * sum + shift * X[(int64_t)b * ldx + row];
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1146,11 +1224,18 @@ int ck_dense_apply_f64acc(int64_t n, const float* A, int64_t lda, double scale, 
                           const double* X, int64_t ldx, double* Y, int64_t ldy, void* stream) {
   if (n < 1 || n > 0x7FFFFFFF || !A || lda < n || nrhs < 1 || !X || !Y || ldx < n || ldy < n) return CK_ERR_BAD_ARG;
   hipStream_t s = (hipStream_t)stream;
+  auto al16 = [](const void* q) { return (reinterpret_cast<uintptr_t>(q) & 15) == 0; };
+  const bool vec = n % 4 == 0 && lda % 4 == 0 && ldx % 2 == 0 && al16(A) && al16(X);
   const unsigned grid = (unsigned)cdiv(n, 4 * kF64Rows);
+  const unsigned grid_lds = (unsigned)cdiv(n, 4 * kF64Waves);
   for (int b0 = 0; b0 < nrhs; b0 += kF64NB) {
     const int nb = nrhs - b0 < kF64NB ? nrhs - b0 : kF64NB;
-    hipLaunchKernelGGL(dense_f64acc_kernel, dim3(grid), dim3(256), 0, s, (int)n, A, lda, X + (int64_t)b0 * ldx, ldx,
-                       Y + (int64_t)b0 * ldy, ldy, nb, scale, shift);
+    if (vec)
+      hipLaunchKernelGGL(dense_f64acc_lds_kernel, dim3(grid_lds), dim3(64 * kF64Waves), 0, s, (int)n, A, lda,
+                         X + (int64_t)b0 * ldx, ldx, Y + (int64_t)b0 * ldy, ldy, nb, scale, shift);
+    else
+      hipLaunchKernelGGL(dense_f64acc_kernel, dim3(grid), dim3(256), 0, s, (int)n, A, lda, X + (int64_t)b0 * ldx,
+                         ldx, Y + (int64_t)b0 * ldy, ldy, nb, scale, shift);
   }
   CK_CHECK_LAUNCH();
   return CK_OK;

@@ -160,14 +160,17 @@ def test_rbf_dense_matches_oracle(dev):
     assert torch.equal(Kb.cpu(), Kc.to(torch.bfloat16))
 
 
-def test_dense_f64acc(dev):
-    g = torch.Generator().manual_seed(9)
-    n, B = 1001, 11
+@pytest.mark.parametrize("n,B,ld", [(1001, 11, 1024), (1024, 11, 1024), (1028, 8, 1032), (2048, 3, 2048),
+                                    (516, 1, 520)])
+def test_dense_f64acc(dev, n, B, ld):
+    # n % 4 != 0 takes the row-per-lane kernel; the rest the LDS-staged one (chunks of 512
+    # columns, ragged last chunk and last block of rows included)
+    g = torch.Generator().manual_seed(9 + n + B)
     A = torch.randn(n, n, generator=g)
-    X = torch.randn(B, 1024, generator=g, dtype=torch.float64)
-    Y = torch.zeros(B, 1024, dtype=torch.float64, device=dev)
+    X = torch.randn(B, ld, generator=g, dtype=torch.float64)
+    Y = torch.zeros(B, ld, dtype=torch.float64, device=dev)
     Ad, Xd = A.to(dev), X.to(dev)
-    N.call("ck_dense_apply_f64acc", n, N.ptr(Ad), n, 0.5, 0.25, B, N.ptr(Xd), 1024, N.ptr(Y), 1024,
+    N.call("ck_dense_apply_f64acc", n, N.ptr(Ad), n, 0.5, 0.25, B, N.ptr(Xd), ld, N.ptr(Y), ld,
            N.stream_ptr(dev))
     ref = 0.5 * (A.double() @ X[:, :n].T).T + 0.25 * X[:, :n]
     assert torch.allclose(Y[:, :n].cpu(), ref, rtol=1e-12, atol=1e-11)
