@@ -379,8 +379,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   // next unit's loads are issued as soon as the rows are in LDS
   float2 kr[16];
   float xv[2];
-  auto load_unit = [&](int64_t u) {
-    const int J = sym_strip_of(u);
+  // units are walked in order, so their strips follow from integer compares (no square root
+  // per unit)
+  auto load_unit = [&](int64_t u, int J) {
     const int64_t s = u - sym_strip_start(J);
     const float* base = A + (64 * s + 16 * wv) * lda + (int64_t)J * kSymCols + 2 * lane;
 #pragma unroll
@@ -418,9 +419,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 #pragma unroll
       for (int b = 0; b < NB; ++b) accz[t][b] = 0.f;
   };
-  load_unit(lo);
+  int J = Jlo;
+  load_unit(lo, J);
   for (int64_t u = lo; u < hi; ++u) {
-    const int J = sym_strip_of(u);
+    while (sym_strip_start(J + 1) <= u) ++J;
     const int64_t s = u - sym_strip_start(J);
     const bool diag = s >= 2 * (int64_t)J;
     if (have && (J != curJ || diag)) {
@@ -459,7 +461,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 #pragma unroll
     for (int i = 0; i < 16; ++i) *reinterpret_cast<float2*>(ksw + i * 4 * kSymKs4 + 2 * lane) = kr[i];
     wave_lds_sync();
-    if (u + 1 < hi) load_unit(u + 1);
+    if (u + 1 < hi) load_unit(u + 1, sym_strip_start(J + 1) <= u + 1 ? J + 1 : J);
     // row sums: D[row][rhs] += K[row][c] x[c][rhs] over the strip's 8 column blocks
     floatx4 d0 = {0.f, 0.f, 0.f, 0.f}, d1 = {0.f, 0.f, 0.f, 0.f};
     const float4* ksr = reinterpret_cast<const float4*>(ksw) + r16 * kSymKs4 + g;
