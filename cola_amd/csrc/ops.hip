@@ -465,13 +465,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     // row sums: D[row][rhs] += K[row][c] x[c][rhs] over the strip's 8 column blocks
     floatx4 d0 = {0.f, 0.f, 0.f, 0.f}, d1 = {0.f, 0.f, 0.f, 0.f};
     const float4* ksr = reinterpret_cast<const float4*>(ksw) + r16 * kSymKs4 + g;
+    // column j of D depends only on column j of B: lanes r16 >= 8 read another rhs's x (an
+    // unconditional read) and fill columns that are never stored
+    const float4* xsr = xs4 + g * kSymNB + (r16 & (kSymNB - 1));
 #pragma unroll
     for (int cb = 0; cb < 8; cb += 2) {
       const float4 a0 = ksr[cb * 4];
       const float4 a1 = ksr[(cb + 1) * 4];
-      const float4 zero = make_float4(0.f, 0.f, 0.f, 0.f);
-      const float4 b0 = r16 < NB ? xs4[(cb * 4 + g) * kSymNB + r16] : zero;
-      const float4 b1 = r16 < NB ? xs4[((cb + 1) * 4 + g) * kSymNB + r16] : zero;
+      const float4 b0 = xsr[cb * 4 * kSymNB];
+      const float4 b1 = xsr[(cb + 1) * 4 * kSymNB];
       d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.x, b0.x, d0, 0, 0, 0);
       d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.x, b1.x, d1, 0, 0, 0);
       d0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0.y, b0.y, d0, 0, 0, 0);
@@ -482,13 +484,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
       d1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1.w, b1.w, d1, 0, 0, 0);
     }
     // lane (g, r16) holds rows 4g .. 4g+3 of the wave's 16 for rhs r16
-    if (r16 < nbr) {
-      float* q = qr + (u * NB + r16) * 64 + 16 * wv + 4 * g;
-      q[0] = d0[0] + d1[0];
-      q[1] = d0[1] + d1[1];
-      q[2] = d0[2] + d1[2];
-      q[3] = d0[3] + d1[3];
-    }
+    if (r16 < nbr)
+      *reinterpret_cast<float4*>(qr + (u * NB + r16) * 64 + 16 * wv + 4 * g) =
+          make_float4(d0[0] + d1[0], d0[1] + d1[1], d0[2] + d1[2], d0[3] + d1[3]);
     wave_lds_sync();  // the rows are read before the next unit overwrites them
   }
   if (have) flush(curJ - Jlo);
