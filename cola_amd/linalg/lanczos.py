@@ -163,6 +163,27 @@ def lanczos(A, start_vector=None, max_iters=100, tol=1e-7, pbar=False, key=None,
     return Q, Tridiagonal(alpha, beta, alpha), run.info
 
 
+# clusters (eigenvalues closer than 1e-3 ||T||) larger than this send the automatic choice to QL
+CLUSTER_LIMIT = 32
+
+
+def largest_cluster(ev):
+    """Size of the largest run of ascending eigenvalues (rows of ev) whose neighbours are closer
+    than 1e-3 max(|first|, |last|) -- the clusters ck_tridiag_eigh_bisect_f64 orthogonalises
+    in one thread."""
+    ev = np.atleast_2d(np.asarray(ev, dtype=np.float64))
+    best = 1 if ev.shape[1] else 0
+    for row in ev:
+        if row.size < 2 or not np.all(np.isfinite(row)):
+            continue
+        tol = 1e-3 * max(abs(row[0]), abs(row[-1]))
+        run = 1
+        for close in np.diff(row) <= tol:
+            run = run + 1 if close else 1
+            best = max(best, run)
+    return best
+
+
 def tridiag_eigh_batch(d, e, vectors=True, method=None):
     """FP64 eigen-decompositions of nb tridiagonals: d (nb, k) and e (nb, >= k-1) fp32 rows.
 
@@ -172,26 +193,40 @@ def tridiag_eigh_batch(d, e, vectors=True, method=None):
     or "ql" (implicit QL, ck_tridiag_eigh_f64, whose rotations are serial: 0.45 and 4.9 ms;
     the default for larger batches).  Inverse iteration orthogonalises the vectors of a
     cluster (eigenvalues closer than 1e-3 ||T||) serially in one thread, O(c^2 m) for a
-    cluster of c: spectra with most values in one cluster (kernel matrices) are faster on QL.
+    cluster of c: spectra with most values in one cluster (kernel matrices) are faster on QL,
+    so the automatic choice computes the eigenvalues first when vectors are asked for and
+    takes QL if a cluster holds more than CLUSTER_LIMIT of them.
     Returns (evals (nb, k) f64 ascending, evecs (nb, k, k) f64 or None, first_row (nb, k) f64,
     status (nb,) int32)."""
     nb, k = d.shape
     dev = d.device
-    method = method or ("bisect" if k <= 1024 and nb <= 64 else "ql")  # the bisection kernel holds T in LDS
+    d = d if d.stride(-1) == 1 else d.contiguous()
+    e = e if e.stride(-1) == 1 else e.contiguous()
+    # a single row's stride(0) is not a leading dimension (torch may report 0 or 1 for it)
+    ldd = d.stride(0) if nb > 1 else k
+    lde = e.stride(0) if nb > 1 else max(e.shape[1], 1)
     ev = torch.empty((nb, k), dtype=torch.float64, device=dev)
     vecs = torch.empty((nb, k, k), dtype=torch.float64, device=dev) if vectors else None
     fr = torch.empty((nb, k), dtype=torch.float64, device=dev)
     st = torch.empty(nb, dtype=torch.int32, device=dev)
+    if method is None:
+        method = "bisect" if k <= 1024 and nb <= 64 else "ql"  # the bisection kernel holds T in LDS
+        if method == "bisect" and vectors and k > CLUSTER_LIMIT:
+            # eigenvalues first (one multisection pass): a spectrum with a large cluster goes to QL
+            N.call("ck_tridiag_eigh_bisect_f64", nb, k, N.ptr(d), ldd, N.ptr(e), lde, N.ptr(ev), None, None,
+                   N.ptr(st), None, 0, N.stream_ptr(dev))
+            if largest_cluster(ev.cpu().numpy()) > CLUSTER_LIMIT:
+                method = "ql"
     if method == "bisect":
         tmp = vecs if vectors else torch.empty((nb, k, k), dtype=torch.float64, device=dev)
         # the inverse iteration's factors: LDS for a few problems, this workspace for many
         wb = N.q("ck_tridiag_eigh_bisect_workspace_bytes", nb, k) if nb > 1 and nb * k > 512 else 0
         work = torch.empty(wb, dtype=torch.uint8, device=dev) if wb else None
-        N.call("ck_tridiag_eigh_bisect_f64", nb, k, N.ptr(d), d.stride(0), N.ptr(e), e.stride(0), N.ptr(ev),
-               N.ptr(tmp), N.ptr(fr), N.ptr(st), N.ptr(work), wb, N.stream_ptr(dev))
+        N.call("ck_tridiag_eigh_bisect_f64", nb, k, N.ptr(d), ldd, N.ptr(e), lde, N.ptr(ev), N.ptr(tmp), N.ptr(fr),
+               N.ptr(st), N.ptr(work), wb, N.stream_ptr(dev))
     elif method == "ql":
-        N.call("ck_tridiag_eigh_f64", nb, k, N.ptr(d), d.stride(0), N.ptr(e), e.stride(0), N.ptr(ev), N.ptr(vecs),
-               N.ptr(fr), N.ptr(st), N.stream_ptr(dev))
+        N.call("ck_tridiag_eigh_f64", nb, k, N.ptr(d), ldd, N.ptr(e), lde, N.ptr(ev), N.ptr(vecs), N.ptr(fr),
+               N.ptr(st), N.stream_ptr(dev))
     else:
         raise ValueError(f"unknown tridiagonal eigensolver {method!r}")
     return ev, vecs, fr, st

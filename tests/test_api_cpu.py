@@ -265,3 +265,16 @@ def test_auto_small_operators_use_dense_factorizations():
     CK = cholesky(K)
     assert isinstance(CK, ca.Kronecker)
     assert torch.allclose((CK @ CK.to_dense().T), K.to_dense())
+
+
+def test_largest_cluster_rule():
+    # the bisection solver's cluster rule: neighbours closer than 1e-3 max(|first|, |last|)
+    from cola_amd.linalg.lanczos import largest_cluster
+    assert largest_cluster(np.array([[0.0, 1.0, 2.0, 3.0]])) == 1
+    ev = np.linspace(0.0, 1000.0, 53)  # gaps 19.2 > 1e-3 * 1000
+    assert largest_cluster(ev[None, :]) == 1
+    near = np.concatenate([np.linspace(0.0, 0.5, 40), [10.0, 100.0, 1000.0]])  # gaps 0.0128 < 1.0
+    assert largest_cluster(near[None, :]) == 40
+    tail = np.concatenate([1e-6 * np.arange(50), [3.0, 700.0, 1000.0]])  # 50 values within 1e-3 * 1000
+    assert largest_cluster(tail[None, :]) == 50
+    assert largest_cluster(np.stack([ev, tail])) == 50

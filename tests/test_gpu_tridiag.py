@@ -229,3 +229,39 @@ def test_tridiag_bisect_large_batch_without_workspace(dev):
     assert (st == 0).all()
     for b in range(0, nb, 7):
         _check_decomposition(d[b], e[b], w[b], V[b], fr[b], "random")
+
+
+def test_tridiag_auto_method_on_a_clustered_spectrum(dev):
+    """The Lanczos tridiagonal of an RBF kernel matrix: most Ritz values sit in one cluster, so
+    the automatic choice takes QL; eigenvalues and vectors pass the same bars."""
+    from cola_amd.linalg.lanczos import CLUSTER_LIMIT, largest_cluster, tridiag_eigh_batch
+    rs = np.random.RandomState(3)
+    pts = rs.uniform(size=(400, 3))
+    K = np.exp(-((pts[:, None, :] - pts[None, :, :]) ** 2).sum(-1) / (2 * 0.5 ** 2))
+    m = 100
+    # Lanczos in FP64 with full re-orthogonalisation on the host: T's diagonals
+    Q = np.zeros((400, m + 1))
+    q = rs.normal(size=400)
+    Q[:, 0] = q / np.linalg.norm(q)
+    d, e = np.zeros(m), np.zeros(m)
+    for j in range(m):
+        w = K @ Q[:, j]
+        d[j] = Q[:, j] @ w
+        w -= Q[:, :j + 1] @ (Q[:, :j + 1].T @ w)
+        w -= Q[:, :j + 1] @ (Q[:, :j + 1].T @ w)
+        e[j] = np.linalg.norm(w)
+        Q[:, j + 1] = w / e[j]
+    d32, e32 = d.astype(np.float32)[None, :], e[:m - 1].astype(np.float32)[None, :]
+    assert largest_cluster(np.linalg.eigvalsh(np.diag(d32[0].astype(np.float64)) + np.diag(e32[0].astype(np.float64), 1)
+                                              + np.diag(e32[0].astype(np.float64), -1))[None, :]) > CLUSTER_LIMIT
+    ev, V, fr, st = tridiag_eigh_batch(torch.from_numpy(d32).to(dev), torch.from_numpy(e32).to(dev), vectors=True)
+    assert int(st.max()) == 0
+    # the QL bars (its vectors follow no sign rule): eigenvalues, residual, orthogonality
+    w, Vn, frn = ev[0].cpu().numpy(), V[0].cpu().numpy(), fr[0].cpu().numpy()
+    T = np.diag(d32[0].astype(np.float64)) + np.diag(e32[0].astype(np.float64), 1) + np.diag(e32[0].astype(np.float64), -1)
+    wl = np.linalg.eigvalsh(T)
+    tn = np.abs(wl).max()
+    assert np.abs(w - wl).max() <= 1e-12 * tn
+    assert np.abs(T @ Vn - Vn * w[None, :]).max() / tn < 1e-11
+    assert np.abs(Vn.T @ Vn - np.eye(m)).max() < 1e-11
+    assert np.allclose(frn, Vn[0], rtol=0, atol=1e-14)
