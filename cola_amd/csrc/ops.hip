@@ -314,7 +314,7 @@ __global__ __launch_bounds__(64 * kDenseWaves) void dense_lds_kernel(int n, cons
 // above the diagonal: half the matrix bytes of the full product.  The upper part is cut
 // into units of 64 rows x 128 columns, strip by strip (strip J: units s = 0 .. 2J+1, rows
 // 64s .. 64s+63, columns 128J .. 128J+127; s >= 2J is the diagonal block), and the units are
-// split evenly over a persistent grid of four blocks per CU.  Each wave of a block takes
+// split evenly over a persistent grid of three blocks per CU.  Each wave of a block takes
 // 16 rows of a unit (lane l: columns 2l, 2l+1 of every row):
 //   column sums (K^T x for the rows' x: the block's mirror image below the diagonal) on the
 //   VALU, accumulated over the block's consecutive units of one strip in registers and
@@ -326,7 +326,9 @@ __global__ __launch_bounds__(64 * kDenseWaves) void dense_lds_kernel(int n, cons
 // column sums of the rows above, in a fixed order (bit-reproducible).
 constexpr int kSymCols = 128;
 constexpr int kSymKs4 = 33;  // LDS row stride of a wave's 16 x 128 block in float4 (bank padding)
-constexpr int kSymGrid = 1024;
+// persistent blocks: three per CU (a fourth block's 39.9 KB of LDS does not fit beside them;
+// a grid sized for four left a tail of a quarter of the blocks: 143 against 126 us at 8 rhs)
+constexpr int kSymGrid = 768;
 constexpr int kSymNB = 8;    // rhs per pass
 
 __host__ __device__ inline int64_t sym_strip_start(int64_t J) { return J * (J + 1); }

@@ -221,7 +221,7 @@ def _op(kind, n, scale=1.0, shift=0.0):
 
 # Order from which a self-adjoint dense operator is lowered with CK_OP_SYMMETRIC.  Below it
 # the matrix (<= 512 MiB) is served largely from the 256 MB Infinity Cache and the full product
-# is as fast or faster (MI355X, 16384 x 16384: 1 rhs 111 vs 178 us, 8 rhs 145 vs 173 us;
+# is as fast or faster (MI355X, 16384 x 16384: 1 rhs 109 vs 177 us, 8 rhs 126 vs 170 us;
 # 4096 x 4096: 28 vs 11 us).
 SYMMETRIC_MIN_ORDER = 11776
 
