@@ -391,13 +391,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
       const f32x2_nt v = __builtin_nontemporal_load(reinterpret_cast<const f32x2_nt*>(base + i * lda));
       kr[i] = make_float2(v.x, v.y);
     }
-    if (s < 2 * (int64_t)J) {
-      const int64_t r = 64 * s + 16 * wv + r16;
-      const int b0 = g < nbr ? g : 0, b1 = g + 4 < nbr ? g + 4 : 0;  // clamped: loads stay unconditional
-      const float v0 = X[(int64_t)b0 * ldx + r], v1 = X[(int64_t)b1 * ldx + r];
-      xv[0] = g < nbr ? v0 : 0.f;
-      xv[1] = g + 4 < nbr ? v1 : 0.f;
-    }
+    // x at the rows, loaded for every unit (diagonal ones do not use it) and masked where it is
+    // used: a load behind a branch, or a select right after it, makes the wave wait for all of
+    // its loads -- the next unit's rows included -- before the row-sum phase
+    const int64_t r = 64 * s + 16 * wv + r16;
+    const int b0 = g < nbr ? g : 0, b1 = g + 4 < nbr ? g + 4 : 0;  // clamped rhs
+    xv[0] = X[(int64_t)b0 * ldx + r];
+    xv[1] = X[(int64_t)b1 * ldx + r];
   };
   auto flush = [&](int slot) {
     __syncthreads();  // every wave is past its row sums: ks4 is free
@@ -444,8 +444,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     }
     if (!diag) {
       // column sums; the rows' x values go through LDS and come back wave-uniform
-      if (g < NB) xis[wv][r16][g] = xv[0];
-      if (g + 4 < NB) xis[wv][r16][g + 4] = xv[1];
+      if (g < NB) xis[wv][r16][g] = g < nbr ? xv[0] : 0.f;
+      if (g + 4 < NB) xis[wv][r16][g + 4] = g + 4 < nbr ? xv[1] : 0.f;
       wave_lds_sync();
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
