@@ -494,37 +494,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   if (have) flush(curJ - Jlo);
 }
 
-// one block per 64 rows (sub-tile s of strip s / 2).  Wave q sums the row sums of strips
-// Ji + q, Ji + q + 4, ... (four strips' loads per batch, all rhs at once) and the column sums
-// of every fourth block that covered strip Ji above the diagonal; then wave q takes rhs q and
-// q + 4: the four waves' sums in order, the epilogue and the X . Y partials.
+// one block per 64 rows (sub-tile s of strip s / 2).  Wave q of eight sums the row sums of
+// strips Ji + q, Ji + q + 8, ... (four strips' loads per batch, all rhs at once) and the column
+// sums of every eighth block that covered strip Ji above the diagonal; then wave q takes rhs q:
+// the eight waves' sums in order, the epilogue and the X . Y partials.
+constexpr int kSymRedWaves = 8;
 template <int NB>
-__global__ __launch_bounds__(256) void dense_sym_reduce(int n, const float* __restrict__ X, int64_t ldx,
+__global__ __launch_bounds__(64 * kSymRedWaves) void dense_sym_reduce(int n, const float* __restrict__ X, int64_t ldx,
                                                         float* __restrict__ Y, int64_t ldy, int nbr, float scale,
                                                         float shift, const float* __restrict__ qr,
                                                         const float* __restrict__ cp, int mslots, int G,
                                                         const int* __restrict__ done, float* __restrict__ xy_part) {
   if (done && *done) return;
-  __shared__ float red[4][NB][64];
+  __shared__ float red[kSymRedWaves][NB][64];
   const int s = blockIdx.x, nJ = n / kSymCols, Ji = s / 2;
   const int64_t U = sym_units(nJ);
   const int il = threadIdx.x & 63, q = threadIdx.x >> 6;
   float acc[NB];
 #pragma unroll
   for (int b = 0; b < NB; ++b) acc[b] = 0.f;
-  for (int J0 = Ji + q; J0 < nJ; J0 += 16) {
+  for (int J0 = Ji + q; J0 < nJ; J0 += 4 * kSymRedWaves) {
     float t[4][NB];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const int J = J0 + 4 * k < nJ ? J0 + 4 * k : nJ - 1;  // clamped: the load stays unconditional
+      const int J = J0 + kSymRedWaves * k < nJ ? J0 + kSymRedWaves * k : nJ - 1;  // clamped: unconditional load
       const float* src = qr + (sym_strip_start(J) + s) * NB * 64 + il;
 #pragma unroll
       for (int b = 0; b < NB; ++b) t[k][b] = src[b * 64];
     }
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      const float v0 = J0 < nJ ? t[0][b] : 0.f, v1 = J0 + 4 < nJ ? t[1][b] : 0.f;
-      const float v2 = J0 + 8 < nJ ? t[2][b] : 0.f, v3 = J0 + 12 < nJ ? t[3][b] : 0.f;
+      const float v0 = J0 < nJ ? t[0][b] : 0.f, v1 = J0 + kSymRedWaves < nJ ? t[1][b] : 0.f;
+      const float v2 = J0 + 2 * kSymRedWaves < nJ ? t[2][b] : 0.f;
+      const float v3 = J0 + 3 * kSymRedWaves < nJ ? t[3][b] : 0.f;
       acc[b] += (v0 + v1) + (v2 + v3);
     }
   }
@@ -532,7 +534,7 @@ __global__ __launch_bounds__(256) void dense_sym_reduce(int n, const float* __re
   if (Ji > 0) {
     const int wf = sym_block_of(sym_strip_start(Ji), U, G);
     const int wl = sym_block_of(sym_strip_start(Ji) + 2 * (int64_t)Ji - 1, U, G);
-    for (int w = wf + q; w <= wl; w += 4) {
+    for (int w = wf + q; w <= wl; w += kSymRedWaves) {
       const int slot = Ji - sym_strip_of(sym_lo(w, U, G));
       const float* src =
           cp + ((int64_t)(w * mslots + (slot < mslots ? slot : 0)) * NB) * kSymCols + 64 * (s % 2) + il;
@@ -547,9 +549,11 @@ __global__ __launch_bounds__(256) void dense_sym_reduce(int n, const float* __re
   for (int b = 0; b < NB; ++b) red[q][b][il] = acc[b];
   __syncthreads();
   const int64_t row = 64 * (int64_t)s + il;
-  for (int b = q; b < NB; b += 4) {
+  for (int b = q; b < NB; b += kSymRedWaves) {
     if (b >= nbr) break;
-    const float sum = (red[0][b][il] + red[1][b][il]) + (red[2][b][il] + red[3][b][il]);
+    float sum = 0.f;
+#pragma unroll
+    for (int w = 0; w < kSymRedWaves; ++w) sum += red[w][b][il];
     const float x = X[(int64_t)b * ldx + row];
     float y = scale * sum;
     if (shift != 0.f) y = fmaf(shift, x, y);
@@ -608,7 +612,7 @@ int dense_sym_apply(const ck_op* op, int nrhs, const float* X, int64_t ldx, floa
   do {                                                                                                           \
     hipLaunchKernelGGL((dense_sym_kernel<NB_>), dim3(G), dim3(256), 0, s, n, A, op->lda, Xb, ldx, nb, qr, cp,  \
                        ms, done);                                                                                \
-    hipLaunchKernelGGL((dense_sym_reduce<NB_>), dim3(n / 64), dim3(256), 0, s, n, Xb, ldx, Yb, ldy, nb,           \
+    hipLaunchKernelGGL((dense_sym_reduce<NB_>), dim3(n / 64), dim3(64 * kSymRedWaves), 0, s, n, Xb, ldx, Yb, ldy, nb, \
                        op->scale, op->shift, (const float*)qr, (const float*)cp, ms, G, done, xyb);              \
   } while (0)
     if (nb <= 1)
