@@ -107,3 +107,19 @@ def test_struct_layout_matches_header(tmp_path, cname, pystruct):
     out = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     assert out[0] == ctypes.sizeof(pystruct)
     assert out[1:] == [getattr(pystruct, f).offset for f in fields]
+
+
+def test_symmetric_flag_workspace():
+    """ck_op.flags = CK_OP_SYMMETRIC adds the half-reading product's scratch to the operator
+    workspace for orders that are a multiple of 128, and nothing otherwise."""
+    N.load()
+
+    def ws(n, flags, nrhs=8):
+        op = N.CkOp()
+        op.kind, op.n, op.scale, op.shift, op.flags = N.CK_OP_DENSE_F32, n, 1.0, 0.0, flags
+        return N.q("ck_op_workspace_bytes", op, nrhs)
+
+    assert ws(16384, N.CK_OP_SYMMETRIC) > ws(16384, 0) + 32 * 2**20  # row and column partial sums
+    assert ws(1000, N.CK_OP_SYMMETRIC) == ws(1000, 0)  # not a multiple of 128: the full product
+    assert ws(128, N.CK_OP_SYMMETRIC) > ws(128, 0)
+    assert ws(16384, N.CK_OP_SYMMETRIC, 1) > 0
